@@ -1,0 +1,250 @@
+"""ctypes bindings for the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker.  Two builds are bound:
+
+* ``rest`` -- oracle/liboracle.so, the project's plain-C restatement of
+  src/SWD/surfdisp96.f, sregn96.f90, slegn96.f90, surfdisp.cpp and the
+  SurfTime glue of SurfaceWave.cpp;
+* ``ref``  -- oracle/_ref/libswdref.so, the same glue bound to the reference
+  Fortran/C++ compiled from its own sources (present only where it was built).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_REST = os.path.join(HERE, "liboracle.so")
+LIB_REF = os.path.join(HERE, "_ref", "libswdref.so")
+
+_f = np.float32
+_d = np.float64
+_fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+WAVES = {"Rc": 0, "Rg": 1, "Lc": 2, "Lg": 3}
+
+
+class _Tables(C.Structure):
+    _fields_ = [("t", C.c_void_p * 4), ("ntmax", C.c_int * 4), ("nmode", C.c_int * 4)]
+
+
+class _Batch(C.Structure):
+    _fields_ = [("ncol", C.c_int), ("layer_off", C.c_void_p), ("thk", C.c_void_p),
+                ("vp", C.c_void_p), ("vs", C.c_void_p), ("rho", C.c_void_p),
+                ("per_off", C.c_void_p), ("periods", C.c_void_p),
+                ("nmode", C.c_void_p), ("group", C.c_int)]
+
+
+class _BatchOut(C.Structure):
+    _fields_ = [("ent_off", C.c_void_p), ("kern_off", C.c_void_p), ("c", C.c_void_p),
+                ("u", C.c_void_p)] + [(n, C.c_void_p) for n in
+                                       ("dcda", "dcdb", "dcdr", "dcdh",
+                                        "duda", "dudb", "dudr", "dudh")] + \
+               [("status", C.c_void_p)]
+
+
+def available(kind="rest"):
+    return os.path.exists(LIB_REST if kind == "rest" else LIB_REF)
+
+
+class Oracle:
+    """One loaded oracle build; ``kind`` is 'rest' or 'ref'."""
+
+    def __init__(self, kind="rest"):
+        self.kind = kind
+        path = LIB_REST if kind == "rest" else LIB_REF
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"oracle library {path} is not built")
+        self.lib = L = C.CDLL(path)
+        L.or_surfdisp96.argtypes = [_fp, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_longlong)]
+        L.or_sregn96.argtypes = [_fp] * 4 + [C.c_int, _dp, _dp, _dp] + [_dp] * 8 + [C.c_int]
+        L.or_sregnpu.argtypes = [_fp] * 4 + [C.c_int, _dp, _dp, _dp] + [_dp] * 4 + \
+                                [_dp] * 4 + [_dp] * 8 + [C.c_int]
+        L.or_slegn96.argtypes = [_fp] * 3 + [C.c_int, _dp, _dp, _dp] + [_dp] * 5 + [C.c_int]
+        L.or_slegnpu.argtypes = [_fp] * 3 + [C.c_int, _dp, _dp, _dp, _dp, _dp] + \
+                                [_dp] * 4 + [_dp] * 6 + [C.c_int]
+        L.or_surfdisp.argtypes = [_fp] * 4 + [C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_int,
+                                              C.c_int, C.c_int]
+        L.or_surfdisp.restype = C.c_int
+        L.or_empirical_relation.argtypes = [C.c_float, C.POINTER(C.c_float),
+                                            C.POINTER(C.c_float)]
+        L.or_empirical_deriv.argtypes = [C.c_float, C.c_float, C.POINTER(C.c_float),
+                                         C.POINTER(C.c_float)]
+        L.or_grid2layer.argtypes = [_fp] * 4 + [C.c_int, C.c_int] + [_fp] * 4
+        L.or_grid2layer.restype = C.c_int
+        L.or_param_convert.argtypes = [_fp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.or_get_1d_kernel.argtypes = [C.c_void_p, _fp, C.c_int, C.c_int, C.c_int, _fp,
+                                       C.POINTER(_Tables), _dp, _dp, _dp, C.c_int]
+        L.or_get_1d_kernel.restype = C.c_int
+        L.or_get_2d_map.argtypes = [C.c_void_p, _fp, C.c_int, C.c_int, C.c_int, _fp,
+                                    C.POINTER(_Tables), _dp, _dp, C.c_int]
+        L.or_get_2d_map.restype = C.c_int
+        L.or_batch_offsets.argtypes = [C.POINTER(_Batch), C.c_void_p, C.c_void_p]
+        L.or_batch_solve.argtypes = [C.c_void_p, C.POINTER(_Batch), C.POINTER(_BatchOut),
+                                     C.c_int]
+        L.or_restatement_backend.restype = C.c_void_p
+        if kind == "ref":
+            L.ref_backend.restype = C.c_void_p
+            self.backend = L.ref_backend()
+            L.ref_surfdisp96.argtypes = [_fp, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                         C.POINTER(C.c_int)]
+        else:
+            self.backend = L.or_restatement_backend()
+
+    # ---------------- per-column ABI ----------------
+    def surfdisp96(self, thk, vp, vs, rho, t, iflsph=1, iwave=2, mode=1, igr=0):
+        thk, vp, vs, rho = (np.ascontiguousarray(a, _f) for a in (thk, vp, vs, rho))
+        t = np.ascontiguousarray(t, _d)
+        cg = np.zeros(len(t), _d)
+        ierr = C.c_int(0)
+        nev = C.c_longlong(0)
+        if self.kind == "ref":
+            self.lib.ref_surfdisp96(thk, vp, vs, rho, len(thk), iflsph, iwave, mode, igr,
+                                    len(t), t, cg, C.byref(ierr))
+        else:
+            self.lib.or_surfdisp96(thk, vp, vs, rho, len(thk), iflsph, iwave, mode, igr,
+                                   len(t), t, cg, C.byref(ierr), C.byref(nev))
+        return cg, ierr.value, nev.value
+
+    def surfdisp(self, thk, vp, vs, rho, t, wave="Rc", mode=0, sphere=True, keep_flat=True):
+        thk, vp, vs, rho = (np.ascontiguousarray(a, _f) for a in (thk, vp, vs, rho))
+        t = np.ascontiguousarray(t, _d)
+        cg = np.zeros(len(t), _d)
+        if self.kind == "ref":
+            raise NotImplementedError("the reference wrapper exit(1)s; use surfdisp96")
+        rc = self.lib.or_surfdisp(thk, vp, vs, rho, len(thk), t, cg, len(t), WAVES[wave],
+                                  mode, int(sphere), int(keep_flat))
+        return cg, rc
+
+    def _be(self):
+        return C.cast(self.backend, C.c_void_p)
+
+    def sregn96(self, thk, vp, vs, rho, t, cp, iflsph=1):
+        n = len(thk)
+        thk, vp, vs, rho = (np.ascontiguousarray(a, _f) for a in (thk, vp, vs, rho))
+        out = [np.zeros(n, _d) for _ in range(8)]
+        tt = np.array([t], _d)
+        c = np.array([cp], _d)
+        u = np.zeros(1, _d)
+        fn = self.lib.or_sregn96 if self.kind == "rest" else None
+        if fn is None:
+            return self._ref_call("sregn96", thk, vp, vs, rho, tt, c, u, out, iflsph)
+        fn(thk, vp, vs, rho, n, tt, c, u, *out, iflsph)
+        return c[0], u[0], dict(zip(("ur", "uz", "tr", "tz", "dcda", "dcdb", "dcdh", "dcdr"), out))
+
+    def _ref_call(self, *a):
+        raise NotImplementedError("per-column reference calls go through the glue")
+
+    # ---------------- SurfTime glue ----------------
+    @staticmethod
+    def make_tables(tabs):
+        """tabs: dict wave -> 2-D array (ntmax, nmode) padded with -1."""
+        tb = _Tables()
+        keep = []
+        for w, i in WAVES.items():
+            a = tabs.get(w)
+            if a is None or np.size(a) == 0:
+                tb.t[i] = None
+                tb.ntmax[i] = 0
+                tb.nmode[i] = 0
+                continue
+            a = np.asfortranarray(np.asarray(a, _d))
+            keep.append(a)
+            tb.t[i] = a.ctypes.data
+            tb.ntmax[i] = a.shape[0]
+            tb.nmode[i] = a.shape[1]
+        return tb, keep
+
+    @staticmethod
+    def tables_kmax(tabs):
+        k = 0
+        for w in WAVES:
+            a = tabs.get(w)
+            if a is None:
+                continue
+            a = np.asarray(a)
+            for m in range(a.shape[1]):
+                col = a[:, m]
+                n = 0
+                while n < len(col) and col[n] != -1:
+                    n += 1
+                k += n
+        return k
+
+    def get_1d_kernel(self, vs, dep, tabs, nthreads=1):
+        """vs: (nx,ny,nz) array; returns vc (nxy,kmax), vout, kernel (nxy,nz,kmax),
+        all Fortran-ordered like the reference's Eigen arrays, and the error count."""
+        nx, ny, nz = vs.shape
+        vsf = np.ascontiguousarray(np.asarray(vs, _f).ravel(order="F"))
+        tb, keep = self.make_tables(tabs)
+        kmax = self.tables_kmax(tabs)
+        vc = np.zeros(nx * ny * kmax, _d)
+        vout = np.zeros(nx * ny * kmax, _d)
+        ker = np.zeros(nx * ny * nz * kmax, _d)
+        nerr = self.lib.or_get_1d_kernel(self._be(), vsf, nx, ny, nz,
+                                         np.ascontiguousarray(dep, _f), C.byref(tb),
+                                         vc, vout, ker, nthreads)
+        del keep
+        return (vc.reshape((nx * ny, kmax), order="F"), vout.reshape((nx * ny, kmax), order="F"),
+                ker.reshape((nx * ny, nz, kmax), order="F"), nerr)
+
+    def get_2d_map(self, vs, dep, tabs, nthreads=1):
+        nx, ny, nz = vs.shape
+        vsf = np.ascontiguousarray(np.asarray(vs, _f).ravel(order="F"))
+        tb, keep = self.make_tables(tabs)
+        kmax = self.tables_kmax(tabs)
+        vc = np.zeros(nx * ny * kmax, _d)
+        vout = np.zeros(nx * ny * kmax, _d)
+        nerr = self.lib.or_get_2d_map(self._be(), vsf, nx, ny, nz,
+                                      np.ascontiguousarray(dep, _f), C.byref(tb), vc, vout,
+                                      nthreads)
+        del keep
+        return vc.reshape((nx * ny, kmax), order="F"), vout.reshape((nx * ny, kmax), order="F"), nerr
+
+    # ---------------- layer-stack batch ----------------
+    def batch_solve(self, layer_off, thk, vp, vs, rho, per_off, periods, nmode, group,
+                    nthreads=1, kernels=True):
+        layer_off = np.ascontiguousarray(layer_off, np.int32)
+        per_off = np.ascontiguousarray(per_off, np.int32)
+        nmode = np.ascontiguousarray(nmode, np.int32)
+        arrs = [np.ascontiguousarray(a, _f) for a in (thk, vp, vs, rho)]
+        periods = np.ascontiguousarray(periods, _d)
+        ncol = len(layer_off) - 1
+        b = _Batch(ncol, layer_off.ctypes.data, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                   arrs[2].ctypes.data, arrs[3].ctypes.data, per_off.ctypes.data,
+                   periods.ctypes.data, nmode.ctypes.data, int(group))
+        ent_off = np.zeros(ncol + 1, np.int64)
+        kern_off = np.zeros(ncol + 1, np.int64)
+        self.lib.or_batch_offsets(C.byref(b), ent_off.ctypes.data, kern_off.ctypes.data)
+        ne, nk = int(ent_off[-1]), int(kern_off[-1])
+        res = {"c": np.zeros(ne, _d), "u": np.zeros(ne, _d), "status": np.zeros(ne, np.int32),
+               "ent_off": ent_off, "kern_off": kern_off}
+        names = ("dcda", "dcdb", "dcdr", "dcdh", "duda", "dudb", "dudr", "dudh")
+        for n in names:
+            res[n] = np.zeros(nk if kernels else 0, _d)
+        o = _BatchOut(ent_off.ctypes.data, kern_off.ctypes.data, res["c"].ctypes.data,
+                      res["u"].ctypes.data,
+                      *[(res[n].ctypes.data if kernels else None) for n in names],
+                      res["status"].ctypes.data)
+        self.lib.or_batch_solve(self._be(), C.byref(b), C.byref(o), nthreads)
+        return res
+
+
+def empirical_relation(vs):
+    """Brocher Vp, rho for an array of Vs (float32), empirical.cpp:9-18."""
+    o = Oracle("rest")
+    vs = np.asarray(vs, _f)
+    vp = np.zeros_like(vs)
+    rho = np.zeros_like(vs)
+    a, b = C.c_float(), C.c_float()
+    for i, v in enumerate(vs.ravel()):
+        o.lib.or_empirical_relation(float(v), C.byref(a), C.byref(b))
+        vp.ravel()[i] = a.value
+        rho.ravel()[i] = b.value
+    return vp, rho

@@ -1,0 +1,150 @@
+/*
+ * swd_oracle.h -- CPU restatement of JDSurfG's surface-wave dispersion and
+ * depth-kernel path.  TEST INFRASTRUCTURE ONLY: the product (jdsurfg_amd/)
+ * never links, loads or calls anything under oracle/.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the
+ * checker.
+ *
+ * Every function mirrors one reference routine (cited file:line, relative
+ * to the JDSurfG source tree) including its single-precision quirks, so that
+ * the plain-method outputs agree with the Fortran build to float rounding.
+ */
+#ifndef JDSG_SWD_ORACLE_H
+#define JDSG_SWD_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- per-column numerics (same argument meaning as src/SWD/swd.hpp) ---- */
+void or_surfdisp96(const float *thkm, const float *vpm, const float *vsm,
+                   const float *rhom, int nlayer, int iflsph, int iwave,
+                   int mode, int igr, int kmax, const double *t, double *cg,
+                   int *ierr, long long *nevals);
+void or_sregn96(const float *thk, const float *vp, const float *vs,
+                const float *rhom, int nlayer, const double *t, double *cp,
+                double *cg, double *dispu, double *dispw, double *stressu,
+                double *stressw, double *dc2da, double *dc2db, double *dc2dh,
+                double *dc2dr, int iflsph);
+void or_sregnpu(const float *thk, const float *vp, const float *vs,
+                const float *rhom, int nlayer, const double *t, double *cp,
+                double *cg, double *dispu, double *dispw, double *stressu,
+                double *stressw, double *t1, double *cp1, double *t2,
+                double *cp2, double *dc2da, double *dc2db, double *dc2dh,
+                double *dc2dr, double *du2da, double *du2db, double *du2dh,
+                double *du2dr, int iflsph);
+void or_slegn96(const float *thk, const float *vs, const float *rhom,
+                int nlayer, const double *t, double *cp, double *cg,
+                double *disp, double *stress, double *dc2db, double *dc2dh,
+                double *dc2dr, int iflsph);
+void or_slegnpu(const float *thk, const float *vs, const float *rhom,
+                int nlayer, const double *t, double *cp, double *cg,
+                double *disp, double *stress, double *t1, double *cp1,
+                double *t2, double *cp2, double *dc2db, double *dc2dh,
+                double *dc2dr, double *du2db, double *du2dh, double *du2dr,
+                int iflsph);
+
+/* surfdisp.cpp:64-116 with swdtp coded 0=Rc 1=Rg 2=Lc 3=Lg; returns 0, or 1
+ * where the reference would print the model and exit(1). */
+int or_surfdisp(const float *thk, const float *vp, const float *vs,
+                const float *rho, int nlayer, const double *t, double *cg,
+                int kmax, int swdtp, int mode, int sphere, int keep_flat);
+
+/* empirical.cpp:9-34 */
+void or_empirical_relation(float vs, float *vp, float *rho);
+void or_empirical_deriv(float vp, float vs, float *drda, float *dadb);
+
+/* ---- SurfTime glue (SurfaceWave.cpp) ---- */
+int or_grid2layer(const float *dep, const float *vp, const float *vs,
+                  const float *rho, int nz, int sublayers, float *rthk,
+                  float *rvp, float *rvs, float *rrho);
+void or_param_convert(const float *vs, int nz, int sublayer, int rmax,
+                      const double *rdcdm, double *dcdm);
+
+/* Period tables: t{Rc,Rg,Lc,Lg} are column-major (ntmax x nmode) arrays padded
+ * with -1 (SurfaceWave.cpp:157-169); a table with nmode==0 is absent. */
+typedef struct {
+  const double *t[4];
+  int ntmax[4];
+  int nmode[4];
+} or_tables;
+
+/* Numerics backend for the glue: the restatement above, or the compiled
+ * reference (oracle/_ref).  Function pointers keep one glue for both. */
+typedef struct {
+  int (*surfdisp)(const float *, const float *, const float *, const float *,
+                  int, const double *, double *, int, int, int, int, int);
+  void (*sregn96)(const float *, const float *, const float *, const float *,
+                  int, const double *, double *, double *, double *, double *,
+                  double *, double *, double *, double *, double *, double *,
+                  int);
+  void (*sregnpu)(const float *, const float *, const float *, const float *,
+                  int, const double *, double *, double *, double *, double *,
+                  double *, double *, double *, double *, double *, double *,
+                  double *, double *, double *, double *, double *, double *,
+                  double *, double *, int);
+  void (*slegn96)(const float *, const float *, const float *, int,
+                  const double *, double *, double *, double *, double *,
+                  double *, double *, double *, int);
+  void (*slegnpu)(const float *, const float *, const float *, int,
+                  const double *, double *, double *, double *, double *,
+                  double *, double *, double *, double *, double *, double *,
+                  double *, double *, double *, double *, int);
+} or_backend;
+
+const or_backend *or_restatement_backend(void);
+
+int or_kernel1d(const or_backend *be, const float *vs, const float *vp,
+                const float *rho, const float *dep, int nz,
+                const or_tables *tb, double *vc, double *vout, double *kvs,
+                double *kvp, double *krho);
+int or_disp1d(const or_backend *be, const float *vs, const float *vp,
+              const float *rho, const float *dep, int nz, const or_tables *tb,
+              double *vc, double *vout);
+/* SurfaceWave.cpp:420-502; vs is (nx,ny,nz) column-major, vc/vout are
+ * (nx*ny,kmax) and kernel (nx*ny,nz,kmax), all column-major.  nthreads<=0
+ * means serial.  Returns number of columns that hit the exit(1) path. */
+int or_get_1d_kernel(const or_backend *be, const float *vs, int nx, int ny,
+                     int nz, const float *dep, const or_tables *tb,
+                     double *vc, double *vout, double *kernel, int nthreads);
+int or_get_2d_map(const or_backend *be, const float *vs, int nx, int ny,
+                  int nz, const float *dep, const or_tables *tb, double *vc,
+                  double *vout, int nthreads);
+
+/* ---- layer-stack batch, the unit the GPU path measures ----
+ * Column c has N_c = layer_off[c+1]-layer_off[c] layers (last = half-space)
+ * and K_c = per_off[c+1]-per_off[c] ascending periods.  For each wave
+ * w (0=Rayleigh,1=Love) and mode m < nmode[2c+w]: with group!=0 the Rg/Lg
+ * path of kernel1D (three chains, sregnpu_/slegnpu_), else the Rc/Lc path
+ * (one chain, sregn96_/slegn96_).  Entry (c,w,m,k) is written at
+ * ent_off[c] + (w*nmode[2c] + m)*K_c + k (for w=1 the R modes come first);
+ * kernels at that entry index times N_c plus layer, offsets from kern_off.
+ * status: 0 ok, 1 no root at T, 2 no root at T*(1+-0.005), 4 the
+ * reference's exit(1) path. */
+typedef struct {
+  int ncol;
+  const int *layer_off;
+  const float *thk, *vp, *vs, *rho;
+  const int *per_off;
+  const double *periods;
+  const int *nmode;
+  int group;
+} or_batch;
+
+typedef struct {
+  const long long *ent_off;
+  const long long *kern_off;
+  double *c, *u;
+  double *dcda, *dcdb, *dcdr, *dcdh;
+  double *duda, *dudb, *dudr, *dudh;
+  int *status;
+} or_batch_out;
+
+void or_batch_offsets(const or_batch *b, long long *ent_off,
+                      long long *kern_off);
+void or_batch_solve(const or_backend *be, const or_batch *b,
+                    const or_batch_out *o, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
