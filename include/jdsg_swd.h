@@ -1,0 +1,100 @@
+/*
+ * jdsg_swd.h -- C ABI of the MI355X dispersion / depth-kernel library
+ * (libjdsg_swd.so).  Plain pointers and sizes only.
+ *
+ * Three levels, matching the reference's boundaries:
+ *   1. the per-column Fortran ABI of src/SWD/swd.hpp (same symbols, same
+ *      argument meaning), executed on the GPU;
+ *   2. the surfdisp/groupvel wrappers of src/SWD/surfdisp.cpp as C calls that
+ *      return an error code where the reference calls exit(1);
+ *   3. the batched forms: a layer-stack plan (the per-column call sequence of
+ *      SurfaceWave.cpp:266-412 kernel1D for a whole batch) and the SurfTime
+ *      grid entry points (SurfaceWave.cpp:420-502).
+ * All functions return 0 on success; jdsg_last_error() describes a failure.
+ */
+#ifndef JDSG_SWD_H
+#define JDSG_SWD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ level 1
+ * Replaces sregn96_ (swd.hpp:19-23, sregn96.f90:1631).  cp is in/out: flat
+ * phase velocity in, spherical out when iflsph=1. */
+void sregn96_(const float *thk, const float *vp, const float *vs, const float *rhom,
+              int nlayer, const double *t, double *cp, double *cg, double *dispu,
+              double *dispw, double *stressu, double *stressw, double *dc2da,
+              double *dc2db, double *dc2dh, double *dc2dr, int iflsph);
+/* Replaces slegn96_ (swd.hpp:36-39, slegn96.f90:672). */
+void slegn96_(const float *thk, const float *vs, const float *rhom, int nlayer,
+              const double *t, double *cp, double *cg, double *disp, double *stress,
+              double *dc2db, double *dc2dh, double *dc2dr, int iflsph);
+/* Replaces surfdisp96_ (swd.hpp:52-54, surfdisp96.f:57). mode is 1-based,
+ * iwave 1 Love / 2 Rayleigh, igr>0 the finite-difference group velocity. */
+void surfdisp96_(const float *thkm, const float *vpm, const float *vsm,
+                 const float *rhom, int nlayer, int iflsph, int iwave, int mode,
+                 int igr, int kmax, const double *t, double *cg, int *ierr);
+/* Replaces slegnpu_ (swd.hpp:69-74, slegn96.f90:785). */
+void slegnpu_(float *thk, float *vs, float *rhom, int nlayer, const double *t,
+              double *cp, double *cg, double *disp, double *stress, double *t1,
+              double *cp1, double *t2, double *cp2, double *dc2db, double *dc2dh,
+              double *dc2dr, double *du2db, double *du2dh, double *du2dr, int iflsph);
+/* Replaces sregnpu_ (swd.hpp:89-95, sregn96.f90:1737). */
+void sregnpu_(const float *thk, const float *vp, const float *vs, const float *rhom,
+              int nlayer, const double *t, double *cp, double *cg, double *dispu,
+              double *dispw, double *stressu, double *stressw, double *t1,
+              double *cp1, double *t2, double *cp2, double *dc2da, double *dc2db,
+              double *dc2dh, double *dc2dr, double *du2da, double *du2db,
+              double *du2dh, double *du2dr, int iflsph);
+
+/* ------------------------------------------------------------------ level 2
+ * surfdisp (surfdisp.cpp:64-116, swd.hpp:99-102) with the wave type coded
+ * 0=Rc 1=Rg 2=Lc 3=Lg; returns 1 where the reference exit(1)s. */
+int jdsg_surfdisp(const float *thk, const float *vp, const float *vs, const float *rho,
+                  int nlayer, const double *t, double *cg, int kmax, int wavetype,
+                  int mode, int sphere, int keep_flat);
+
+/* ------------------------------------------------------------------ level 3
+ * Layer-stack batch plan.  Host arrays (copied at creation):
+ *   layer_off[ncol+1]  layer offsets (column c has layer_off[c+1]-layer_off[c]
+ *                      entries, the last the half-space);
+ *   per_off[ncol+1]    period offsets (ascending periods per column);
+ *   nmode[2*ncol]      Rayleigh and Love mode counts per column.
+ * group=1 gives the Rg/Lg form of kernel1D (three root chains, U kernels);
+ * group=0 the Rc/Lc form.  iflsph=1 is the spherical Earth JDSurfG uses.
+ * layout 0 ("grid", uniform batches only): kernel value of entry-local index
+ *   j (= (wave*nR + mode)*K + k) at layer i of column c is at
+ *   [(j*N + i)*ncol + c], the column-fastest order of SurfTime's dmat3;
+ * layout 1 ("csr"): at kern_off[c] + j*N + i.
+ * Entry (c, j) scalar outputs are at ent_off[c] + j. */
+typedef struct jdsg_plan jdsg_plan;
+int jdsg_plan_create(int ncol, const int *layer_off, const int *per_off, const int *nmode,
+                     int group, int layout, int iflsph, jdsg_plan **plan);
+int jdsg_plan_sizes(const jdsg_plan *plan, long long *nent, long long *nkern, int *nmax,
+                    int *kmax);
+int jdsg_plan_offsets(const jdsg_plan *plan, long long *ent_off, long long *kern_off);
+/* Launch the solve on `stream` (a hipStream_t, may be null).  Every pointer
+ * is device memory; kernel outputs may be null to skip them.  status per
+ * entry: 0 ok, 1 no root, 2 no root at T(1+-0.005), 4 reference exit(1) path.
+ * Allocates nothing and does not synchronise, so it may be graph-captured. */
+int jdsg_plan_execute(jdsg_plan *plan, const float *thk, const float *vp, const float *vs,
+                      const float *rho, const double *periods, double *c, double *u,
+                      int *status, double *dcda, double *dcdb, double *dcdr, double *dcdh,
+                      double *duda, double *dudb, double *dudr, double *dudh, void *stream);
+/* Per-kernel launch tuning: waves for the chain and eigen launches (0 keeps
+ * the default). */
+int jdsg_plan_set_waves(jdsg_plan *plan, int chain_waves, int eigen_waves);
+void jdsg_plan_destroy(jdsg_plan *plan);
+/* Number of period-equation evaluations the last execute performed
+ * (synchronises; for the flop model). */
+int jdsg_plan_eval_count(const jdsg_plan *plan, long long *nevals);
+/* Per-stage milliseconds (prep, chains, eigen Rayleigh, eigen Love) of the
+ * last execute, when JDSG_TIMING=1 was set at plan creation. */
+int jdsg_plan_timings(const jdsg_plan *plan, float *ms4);
+
+const char *jdsg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

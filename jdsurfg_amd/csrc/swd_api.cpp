@@ -1,0 +1,575 @@
+// swd_api.cpp -- C ABI (include/jdsg_swd.h): batch plans and the reference's
+// per-column entry points, all executed by the HIP kernels of this library.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/jdsg_swd.h"
+#include "swd_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &m) {
+  g_err = m;
+  return 1;
+}
+
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) return fail(std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+int default_waves() {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) ncu = p.multiProcessorCount;
+  }
+  return ncu * 16;
+}
+
+}  // namespace
+
+struct jdsg_plan {
+  int ncol = 0, ncol_pad = 0, nmax = 0, kmax = 0, kpad = 0;
+  int group = 0, layout = 0, iflsph = 1, nvar = 1;
+  long long nent = 0, nkern = 0, nchain = 0, nitem[2] = {0, 0};
+  bool uniform = false;
+  int uK = 0, unR = 0, unL = 0;
+  std::vector<long long> ent_off, kern_off;
+  std::vector<void *> allocs;
+  // device
+  int *layer_off_d = nullptr, *per_off_d = nullptr, *nmode_d = nullptr;
+  long long *ent_off_d = nullptr, *kern_off_d = nullptr, *items_d[2] = {nullptr, nullptr};
+  PrepOut prep{};
+  ColumnModel cm{};
+  double *cflat = nullptr, *cscr = nullptr, *escr = nullptr;
+  int *cstat = nullptr, *ierr = nullptr;
+  unsigned *counters = nullptr;
+  unsigned long long *nevals = nullptr;
+  int chain_waves = 0, eigen_waves = 0;
+  // explicit per-entry periods for the eigen stage (per-column ABI only)
+  double *tv = nullptr;
+  double *eig[4] = {nullptr, nullptr, nullptr, nullptr};
+  // timing
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  float last_ms[4] = {0, 0, 0, 0};
+
+  template <class T>
+  int dalloc(T **p, size_t n) {
+    if (n == 0) n = 1;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+  }
+  ~jdsg_plan() {
+    for (void *p : allocs) (void)hipFree(p);
+    if (timing)
+      for (auto &e : ev) (void)hipEventDestroy(e);
+  }
+  int size_scratch() {
+    if (cscr) {
+      (void)hipFree(cscr);
+      allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)cscr));
+      cscr = nullptr;
+    }
+    if (escr) {
+      (void)hipFree(escr);
+      allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)escr));
+      escr = nullptr;
+    }
+    if (dalloc(&cscr, (size_t)chain_waves * 64 * 2 * kpad)) return 1;
+    if (dalloc(&escr, (size_t)eigen_waves * 64 * 18 * nmax)) return 1;
+    return 0;
+  }
+};
+
+extern "C" const char *jdsg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_off,
+                                const int *nmode, int group, int layout, int iflsph,
+                                jdsg_plan **out) {
+  *out = nullptr;
+  if (ncol <= 0) return fail("ncol must be positive");
+  if (layer_off[0] != 0 || per_off[0] != 0) return fail("offsets must start at 0");
+  jdsg_plan *P = new jdsg_plan();
+  P->ncol = ncol;
+  P->ncol_pad = (ncol + 63) / 64 * 64;
+  P->group = group ? 1 : 0;
+  P->nvar = group ? 3 : 1;
+  P->layout = layout;
+  P->iflsph = iflsph;
+  P->ent_off.resize(ncol + 1);
+  P->kern_off.resize(ncol + 1);
+  long long e = 0, q = 0;
+  int N0 = layer_off[1] - layer_off[0], K0 = per_off[1] - per_off[0];
+  bool uni = true;
+  for (int c = 0; c < ncol; ++c) {
+    int N = layer_off[c + 1] - layer_off[c];
+    int K = per_off[c + 1] - per_off[c];
+    int nR = nmode[2 * c], nL = nmode[2 * c + 1];
+    if (N < 2) { delete P; return fail("every column needs at least one layer and a half-space"); }
+    if (K < 0 || nR < 0 || nL < 0) { delete P; return fail("negative period or mode count"); }
+    if (nR > JDSG_MAXMODE || nL > JDSG_MAXMODE) { delete P; return fail("at most 8 modes per wave"); }
+    P->nmax = std::max(P->nmax, N);
+    P->kmax = std::max(P->kmax, K);
+    P->ent_off[c] = e;
+    P->kern_off[c] = q;
+    long long ne = (long long)(nR + nL) * K;
+    e += ne;
+    q += ne * N;
+    P->nitem[0] += (long long)nR * K;
+    P->nitem[1] += (long long)nL * K;
+    if (N != N0 || K != K0 || nR != nmode[0] || nL != nmode[1]) uni = false;
+  }
+  P->ent_off[ncol] = e;
+  P->kern_off[ncol] = q;
+  P->nent = e;
+  P->nkern = q;
+  P->uniform = uni;
+  P->uK = K0;
+  P->unR = nmode[0];
+  P->unL = nmode[1];
+  if (layout == 0 && !uni) { delete P; return fail("grid layout needs a uniform batch"); }
+  P->kpad = std::max(1, P->kmax);
+  P->nchain = (long long)ncol * 2 * P->nvar;
+  int dw = default_waves();
+  P->chain_waves = (int)std::min<long long>(dw, (P->nchain + 63) / 64);
+  P->eigen_waves = (int)std::min<long long>(dw, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
+  P->chain_waves = std::max(1, P->chain_waves);
+  P->eigen_waves = std::max(1, P->eigen_waves);
+  // device descriptors
+  size_t cp = (size_t)P->ncol_pad * P->nmax;
+  if (P->dalloc(&P->layer_off_d, ncol + 1) || P->dalloc(&P->per_off_d, ncol + 1) ||
+      P->dalloc(&P->nmode_d, 2 * ncol) || P->dalloc(&P->ent_off_d, ncol + 1) ||
+      P->dalloc(&P->kern_off_d, ncol + 1) || P->dalloc(&P->prep.chainR, cp) ||
+      P->dalloc(&P->prep.chainL, cp) || P->dalloc(&P->prep.cc, ncol) ||
+      P->dalloc(&P->prep.betmx, ncol) || P->dalloc(&P->prep.llw, ncol) ||
+      P->dalloc(&P->prep.nlay, ncol) || P->dalloc(&P->prep.zd, cp) ||
+      P->dalloc(&P->prep.za, cp) || P->dalloc(&P->prep.zb, cp) ||
+      P->dalloc(&P->prep.rho0, cp) || P->dalloc(&P->prep.rtpR, cp) ||
+      P->dalloc(&P->prep.rtpL, cp) || P->dalloc(&P->prep.vtp, cp) ||
+      P->dalloc(&P->prep.dtp, cp) || P->dalloc(&P->cflat, (size_t)P->nvar * P->nent) ||
+      P->dalloc(&P->cstat, (size_t)P->nvar * P->nent) || P->dalloc(&P->ierr, P->nchain) ||
+      P->dalloc(&P->counters, 4) || P->dalloc(&P->nevals, 2) || P->size_scratch()) {
+    delete P;
+    return 1;
+  }
+  std::vector<int> lo(layer_off, layer_off + ncol + 1), po(per_off, per_off + ncol + 1),
+      nm(nmode, nmode + 2 * ncol);
+  hipError_t err = hipSuccess;
+  err = hipMemcpy(P->layer_off_d, lo.data(), sizeof(int) * (ncol + 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(P->per_off_d, po.data(), sizeof(int) * (ncol + 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(P->nmode_d, nm.data(), sizeof(int) * 2 * ncol, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(P->ent_off_d, P->ent_off.data(), sizeof(long long) * (ncol + 1),
+                    hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(P->kern_off_d, P->kern_off.data(), sizeof(long long) * (ncol + 1),
+                    hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    delete P;
+    return fail(std::string("plan upload: ") + hipGetErrorString(err));
+  }
+  if (!uni) {
+    for (int w = 0; w < 2; ++w) {
+      std::vector<long long> items;
+      items.reserve(P->nitem[w]);
+      for (int c = 0; c < ncol; ++c) {
+        int K = per_off[c + 1] - per_off[c];
+        int nR = nmode[2 * c], nL = nmode[2 * c + 1];
+        int M = w == 0 ? nR : nL, off = w == 0 ? 0 : nR;
+        for (int m = 0; m < M; ++m)
+          for (int k = 0; k < K; ++k) items.push_back(P->ent_off[c] + (long long)(off + m) * K + k);
+      }
+      if (P->dalloc(&P->items_d[w], items.size())) { delete P; return 1; }
+      if (!items.empty() &&
+          hipMemcpy(P->items_d[w], items.data(), sizeof(long long) * items.size(),
+                    hipMemcpyHostToDevice) != hipSuccess) {
+        delete P;
+        return fail("item upload failed");
+      }
+    }
+  }
+  P->cm.ncol = ncol;
+  P->cm.ncol_pad = P->ncol_pad;
+  P->cm.nmax = P->nmax;
+  P->cm.nlay = P->prep.nlay;
+  P->cm.chainR = P->prep.chainR;
+  P->cm.chainL = P->prep.chainL;
+  P->cm.cc = P->prep.cc;
+  P->cm.betmx = P->prep.betmx;
+  P->cm.llw = P->prep.llw;
+  P->cm.zd = P->prep.zd;
+  P->cm.za = P->prep.za;
+  P->cm.zb = P->prep.zb;
+  P->cm.rho0 = P->prep.rho0;
+  P->cm.rtpR = P->prep.rtpR;
+  P->cm.rtpL = P->prep.rtpL;
+  P->cm.vtp = P->prep.vtp;
+  P->cm.dtp = P->prep.dtp;
+  const char *tenv = getenv("JDSG_TIMING");
+  if (tenv && tenv[0] == '1') {
+    P->timing = true;
+    for (auto &ev : P->ev) (void)hipEventCreate(&ev);
+  }
+  *out = P;
+  return 0;
+}
+
+extern "C" int jdsg_plan_sizes(const jdsg_plan *P, long long *nent, long long *nkern,
+                               int *nmax, int *kmax) {
+  if (nent) *nent = P->nent;
+  if (nkern) *nkern = P->nkern;
+  if (nmax) *nmax = P->nmax;
+  if (kmax) *kmax = P->kmax;
+  return 0;
+}
+
+extern "C" int jdsg_plan_offsets(const jdsg_plan *P, long long *ent_off, long long *kern_off) {
+  if (ent_off) memcpy(ent_off, P->ent_off.data(), sizeof(long long) * (P->ncol + 1));
+  if (kern_off) memcpy(kern_off, P->kern_off.data(), sizeof(long long) * (P->ncol + 1));
+  return 0;
+}
+
+extern "C" int jdsg_plan_set_waves(jdsg_plan *P, int cw, int ew) {
+  if (cw > 0) P->chain_waves = cw;
+  if (ew > 0) P->eigen_waves = ew;
+  return P->size_scratch();
+}
+
+extern "C" void jdsg_plan_destroy(jdsg_plan *P) { delete P; }
+
+extern "C" int jdsg_plan_eval_count(const jdsg_plan *P, long long *out) {
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpy(&v, P->nevals, sizeof(v), hipMemcpyDeviceToHost));
+  *out = (long long)v;
+  return 0;
+}
+
+extern "C" int jdsg_plan_timings(const jdsg_plan *P, float *ms4) {
+  for (int i = 0; i < 4; ++i) ms4[i] = P->last_ms[i];
+  return P->timing ? 0 : fail("timing disabled (set JDSG_TIMING=1 before plan creation)");
+}
+
+namespace {
+
+// Prepare + chains.  igr/retry as in surfdisp96 / surfdisp.
+int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
+              const float *rho, const double *periods, int igr, int retry, hipStream_t st) {
+  if (P->timing) HIPCHK(hipEventRecord(P->ev[0], st));
+  jdsg_launch_prep(P->ncol, P->ncol_pad, P->layer_off_d, thk, vp, vs, rho, P->iflsph, P->prep,
+                   st);
+  jdsg_launch_fill_u32(P->counters, 4, 0u, st);
+  jdsg_launch_fill_u32((unsigned *)P->nevals, 4, 0u, st);
+  jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, st);
+  if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
+  ChainArgs A{};
+  A.cm = P->cm;
+  A.ent_off = P->ent_off_d;
+  A.per_off = P->per_off_d;
+  A.periods = periods;
+  A.nmode = P->nmode_d;
+  A.nvar = P->nvar;
+  A.igr = igr;
+  A.retry = retry;
+  A.nent = P->nent;
+  A.kmax_pad = P->kpad;
+  A.cflat = P->cflat;
+  A.cstat = P->cstat;
+  A.ierr_out = P->ierr;
+  A.cscr = P->cscr;
+  A.counter = P->counters + 0;
+  A.nchain = P->nchain;
+  A.nevals = P->nevals;
+  jdsg_launch_chains(A, P->chain_waves, st);
+  HIPCHK(hipGetLastError());
+  if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
+  return 0;
+}
+
+int run_eigen(jdsg_plan *P, const double *periods, double *c, double *u, int *status,
+              double *const dc[4], double *const du[4], hipStream_t st) {
+  for (int w = 0; w < 2; ++w) {
+    if (P->nitem[w] == 0) {
+      if (P->timing) HIPCHK(hipEventRecord(P->ev[3 + w], st));
+      continue;
+    }
+    EigenArgs E{};
+    E.cm = P->cm;
+    E.ent_off = P->ent_off_d;
+    E.per_off = P->per_off_d;
+    E.periods = periods;
+    E.nmode = P->nmode_d;
+    E.wave = w;
+    E.group = P->group;
+    E.iflsph = P->iflsph;
+    E.nent = P->nent;
+    E.cflat = P->cflat;
+    E.cstat = P->cstat;
+    E.nitem = P->nitem[w];
+    E.item_ent = P->uniform ? nullptr : P->items_d[w];
+    E.grid_K = P->uK;
+    E.grid_nR = P->unR;
+    E.grid_nL = P->unL;
+    E.c_out = c;
+    E.u_out = u;
+    E.status = status;
+    for (int a = 0; a < 4; ++a) {
+      E.dc[a] = dc[a];
+      E.du[a] = du[a];
+      E.eig[a] = P->eig[a];
+    }
+    E.layout = P->layout;
+    E.kern_off = P->kern_off_d;
+    E.scr = P->escr;
+    E.nmax = P->nmax;
+    E.counter = P->counters + 1 + w;
+    E.tv = P->tv;
+    int waves = (int)std::min<long long>(P->eigen_waves, (P->nitem[w] + 63) / 64);
+    jdsg_launch_eigen(E, waves, st);
+    HIPCHK(hipGetLastError());
+    if (P->timing) HIPCHK(hipEventRecord(P->ev[3 + w], st));
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int jdsg_plan_execute(jdsg_plan *P, const float *thk, const float *vp,
+                                 const float *vs, const float *rho, const double *periods,
+                                 double *c, double *u, int *status, double *dcda, double *dcdb,
+                                 double *dcdr, double *dcdh, double *duda, double *dudb,
+                                 double *dudr, double *dudh, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!thk || !vp || !vs || !rho || !periods || !c || !u || !status)
+    return fail("null input or scalar output pointer");
+  if (run_front(P, thk, vp, vs, rho, periods, 0, 1, st)) return 1;
+  double *dc[4] = {dcda, dcdb, dcdr, dcdh};
+  double *du[4] = {duda, dudb, dudr, dudh};
+  if (run_eigen(P, periods, c, u, status, dc, du, st)) return 1;
+  if (P->timing) {
+    HIPCHK(hipEventSynchronize(P->ev[4]));
+    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
+  }
+  return 0;
+}
+
+// ============================================================================
+// Per-column reference ABI (host arrays), each call one small plan.
+namespace {
+
+struct DevBuf {
+  std::vector<void *> p;
+  ~DevBuf() {
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <class T>
+  T *up(const T *h, size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    if (h && n) (void)hipMemcpy(q, h, sizeof(T) * n, hipMemcpyHostToDevice);
+    return (T *)q;
+  }
+};
+
+void report(const char *where) {
+  fprintf(stderr, "libjdsg_swd: %s failed: %s\n", where, g_err.c_str());
+}
+
+// One column, chains only: returns flat roots for modes 1..mode (mode-major)
+int column_chains(const float *thk, const float *vp, const float *vs, const float *rho, int N,
+                  int iflsph, int iwave, int mode, int igr, int retry, int kmax,
+                  const double *t, std::vector<double> &cg_all, std::vector<int> &st_all,
+                  int *ierr) {
+  int lo[2] = {0, N}, po[2] = {0, kmax};
+  int nm[2] = {iwave == 2 ? mode : 0, iwave == 1 ? mode : 0};
+  jdsg_plan *P = nullptr;
+  if (jdsg_plan_create(1, lo, po, nm, 0, 1, iflsph, &P)) return 1;
+  DevBuf B;
+  float *dthk = B.up(thk, N), *dvp = B.up(vp, N), *dvs = B.up(vs, N), *drho = B.up(rho, N);
+  double *dt = B.up(t, kmax);
+  int rc = run_front(P, dthk, dvp, dvs, drho, dt, igr, retry, nullptr);
+  if (!rc) {
+    cg_all.assign((size_t)mode * kmax, 0.0);
+    st_all.assign((size_t)mode * kmax, 0);
+    hipError_t e = hipMemcpy(cg_all.data(), P->cflat, sizeof(double) * mode * kmax,
+                             hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(st_all.data(), P->cstat, sizeof(int) * mode * kmax, hipMemcpyDeviceToHost);
+    int ie = 0;
+    if (e == hipSuccess) e = hipMemcpy(&ie, P->ierr, sizeof(int), hipMemcpyDeviceToHost);
+    if (ierr) *ierr = ie;
+    if (e != hipSuccess) rc = fail(std::string("column copy: ") + hipGetErrorString(e));
+  }
+  jdsg_plan_destroy(P);
+  return rc;
+}
+
+// One column, eigen stage from given flat roots.  nv = 1 (sregn96/slegn96)
+// or 3 (sregnpu/slegnpu, explicit periods).
+int column_eigen(int wave, const float *thk, const float *vp, const float *vs, const float *rho,
+                 int N, int iflsph, const double *tv, const double *cv, int nv, double *cp,
+                 double *cg, double *eig[4], double *dc[4], double *du[4]) {
+  int lo[2] = {0, N}, po[2] = {0, 1};
+  int nm[2] = {wave == 0 ? 1 : 0, wave == 1 ? 1 : 0};
+  jdsg_plan *P = nullptr;
+  if (jdsg_plan_create(1, lo, po, nm, nv == 3, 1, iflsph, &P)) return 1;
+  DevBuf B;
+  float *dthk = B.up(thk, N), *dvp = B.up(vp ? vp : vs, N), *dvs = B.up(vs, N),
+        *drho = B.up(rho, N);
+  double *dt = B.up(tv, 1);
+  double *dtv = B.up(tv, 3);
+  double *dout = B.up<double>(nullptr, 2 + 12 * (size_t)N);
+  int *dst = B.up<int>(nullptr, 1);
+  int rc = 0;
+  if (!dthk || !dvp || !dvs || !drho || !dt || !dout || !dst) rc = fail("device allocation");
+  if (!rc) {
+    for (int a = 0; a < 4; ++a) P->eig[a] = dout + 2 + (size_t)(8 + a) * N;
+    P->tv = (nv == 3) ? dtv : nullptr;
+    jdsg_launch_prep(1, P->ncol_pad, P->layer_off_d, dthk, dvp, dvs, drho, iflsph, P->prep,
+                     nullptr);
+    jdsg_launch_fill_u32(P->counters, 4, 0u, nullptr);
+    jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, nullptr);
+    for (int v = 0; v < nv; ++v)
+      (void)hipMemcpy(P->cflat + v, cv + v, sizeof(double), hipMemcpyHostToDevice);
+    double *ddc[4], *ddu[4];
+    for (int a = 0; a < 4; ++a) {
+      ddc[a] = dout + 2 + (size_t)a * N;
+      ddu[a] = dout + 2 + (size_t)(4 + a) * N;
+    }
+    rc = run_eigen(P, dt, dout, dout + 1, dst, ddc, ddu, nullptr);
+    if (!rc) {
+      std::vector<double> h(2 + 12 * (size_t)N);
+      hipError_t e = hipMemcpy(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = fail("copy back");
+      *cp = h[0];
+      *cg = h[1];
+      for (int a = 0; a < 4; ++a) {
+        if (dc[a]) memcpy(dc[a], &h[2 + a * N], sizeof(double) * N);
+        if (du[a]) memcpy(du[a], &h[2 + (4 + a) * N], sizeof(double) * N);
+        if (eig[a]) memcpy(eig[a], &h[2 + (8 + a) * N], sizeof(double) * N);
+      }
+    }
+  }
+  jdsg_plan_destroy(P);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" void surfdisp96_(const float *thkm, const float *vpm, const float *vsm,
+                            const float *rhom, int nlayer, int iflsph, int iwave, int mode,
+                            int igr, int kmax, const double *t, double *cg, int *ierr) {
+  std::vector<double> all;
+  std::vector<int> st;
+  if (kmax <= 0 || mode <= 0) {
+    *ierr = 0;
+    return;
+  }
+  if (column_chains(thkm, vpm, vsm, rhom, nlayer, iflsph, iwave, mode, igr, 0, kmax, t, all, st,
+                    ierr)) {
+    report("surfdisp96_");
+    for (int i = 0; i < kmax; ++i) cg[i] = 0.0;
+    *ierr = 1;
+    return;
+  }
+  memcpy(cg, all.data() + (size_t)(mode - 1) * kmax, sizeof(double) * kmax);
+}
+
+extern "C" int jdsg_surfdisp(const float *thk, const float *vp, const float *vs,
+                             const float *rho, int nlayer, const double *t, double *cg,
+                             int kmax, int wavetype, int mode, int sphere, int keep_flat) {
+  if (wavetype < 0 || wavetype > 3) return fail("wavetype must be 0..3 (Rc,Rg,Lc,Lg)");
+  int iwave = wavetype >= 2 ? 1 : 2;
+  int igr = (wavetype == 1 || wavetype == 3) ? 1 : 0;
+  std::vector<double> all;
+  std::vector<int> st;
+  int ierr = 0;
+  if (kmax <= 0) return 0;
+  if (column_chains(thk, vp, vs, rho, nlayer, sphere ? 1 : 0, iwave, mode + 1, igr, 1, kmax, t,
+                    all, st, &ierr))
+    return 1;
+  memcpy(cg, all.data() + (size_t)mode * kmax, sizeof(double) * kmax);
+  for (int i = 0; i < kmax; ++i)
+    if (st[(size_t)mode * kmax + i] & JDSG_EXIT_PATH)
+      return fail("cannot find dispersion (the reference exits here)");
+  if (sphere && !keep_flat) {
+    const double ar = 6371.0;
+    for (int i = 0; i < kmax; ++i) {
+      double omega = 2.0 * M_PI / t[i];
+      double tm = (wavetype >= 2) ? 1. + pow(1.5 * cg[i] / (ar * omega), 2)
+                                  : 1. + pow(0.5 * cg[i] / (ar * omega), 2);
+      tm = sqrt(tm);
+      cg[i] = (wavetype == 0 || wavetype == 2) ? cg[i] / tm : cg[i] * tm;
+    }
+  }
+  return 0;
+}
+
+extern "C" void sregn96_(const float *thk, const float *vp, const float *vs, const float *rhom,
+                         int nlayer, const double *t, double *cp, double *cg, double *dispu,
+                         double *dispw, double *stressu, double *stressw, double *dc2da,
+                         double *dc2db, double *dc2dh, double *dc2dr, int iflsph) {
+  double *eig[4] = {dispu, dispw, stressu, stressw};
+  double *dc[4] = {dc2da, dc2db, dc2dr, dc2dh};
+  double *du[4] = {nullptr, nullptr, nullptr, nullptr};
+  double tv[3] = {*t, 0, 0}, cv[3] = {*cp, 0, 0};
+  if (column_eigen(0, thk, vp, vs, rhom, nlayer, iflsph, tv, cv, 1, cp, cg, eig, dc, du))
+    report("sregn96_");
+}
+
+extern "C" void slegn96_(const float *thk, const float *vs, const float *rhom, int nlayer,
+                         const double *t, double *cp, double *cg, double *disp, double *stress,
+                         double *dc2db, double *dc2dh, double *dc2dr, int iflsph) {
+  double *eig[4] = {disp, nullptr, nullptr, nullptr};
+  double *dc[4] = {nullptr, dc2db, dc2dr, dc2dh};
+  double *du[4] = {nullptr, nullptr, nullptr, nullptr};
+  double tv[3] = {*t, 0, 0}, cv[3] = {*cp, 0, 0};
+  if (column_eigen(1, thk, nullptr, vs, rhom, nlayer, iflsph, tv, cv, 1, cp, cg, eig, dc, du))
+    report("slegn96_");
+  // slegn96.f90:770 copies the displacement into `stress`
+  if (stress && disp) memcpy(stress, disp, sizeof(double) * nlayer);
+}
+
+extern "C" void sregnpu_(const float *thk, const float *vp, const float *vs, const float *rhom,
+                         int nlayer, const double *t, double *cp, double *cg, double *dispu,
+                         double *dispw, double *stressu, double *stressw, double *t1,
+                         double *cp1, double *t2, double *cp2, double *dc2da, double *dc2db,
+                         double *dc2dh, double *dc2dr, double *du2da, double *du2db,
+                         double *du2dh, double *du2dr, int iflsph) {
+  double *eig[4] = {dispu, dispw, stressu, stressw};
+  double *dc[4] = {dc2da, dc2db, dc2dr, dc2dh};
+  double *du[4] = {du2da, du2db, du2dr, du2dh};
+  double tv[3] = {*t, *t1, *t2}, cv[3] = {*cp, *cp1, *cp2};
+  if (column_eigen(0, thk, vp, vs, rhom, nlayer, iflsph, tv, cv, 3, cp, cg, eig, dc, du))
+    report("sregnpu_");
+}
+
+extern "C" void slegnpu_(float *thk, float *vs, float *rhom, int nlayer, const double *t,
+                         double *cp, double *cg, double *disp, double *stress, double *t1,
+                         double *cp1, double *t2, double *cp2, double *dc2db, double *dc2dh,
+                         double *dc2dr, double *du2db, double *du2dh, double *du2dr,
+                         int iflsph) {
+  double *eig[4] = {disp, stress, nullptr, nullptr};
+  double *dc[4] = {nullptr, dc2db, dc2dr, dc2dh};
+  double *du[4] = {nullptr, du2db, du2dr, du2dh};
+  double tv[3] = {*t, *t1, *t2}, cv[3] = {*cp, *cp1, *cp2};
+  if (column_eigen(1, thk, nullptr, vs, rhom, nlayer, iflsph, tv, cv, 3, cp, cg, eig, dc, du))
+    report("slegnpu_");
+}

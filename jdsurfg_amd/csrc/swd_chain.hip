@@ -1,0 +1,830 @@
+// swd_chain.hip -- per-column model preparation and the period-equation root
+// chains (the CDNA4 form of src/SWD/surfdisp96.f).
+//
+// One lane owns one chain: a (column, wave type, period scale) triple whose
+// modes are solved in the reference's sequential order (period k starts from
+// the root at k-1, mode iq is floored by mode iq-1, surfdisp96.f:262-281).
+// The chain runs as an explicit state machine that stops exactly where the
+// reference calls dltar (getsol scan, nevill halving / Neville steps), so
+// every loop trip of the kernel is one period-equation evaluation for every
+// live lane: lanes in different phases never serialise the layer recursion.
+// Lanes whose chain is finished fetch the next chain from a global counter.
+//
+// Arithmetic follows the Fortran: float model arrays and start values,
+// double recursion, sngl() of the returned root.  FP contraction is off so
+// the recursion rounds like the reference build.
+#include "swd_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr double kTwoPi = 2.0 * 3.141592653589793;
+
+JDSG_DEV float powm5f(float x) {
+  float x2 = x * x, x4 = x2 * x2, x5 = x4 * x;
+  return 1.0f / x5;
+}
+JDSG_DEV double powm5d(double x) {
+  double x2 = x * x, x4 = x2 * x2, x5 = x4 * x;
+  return 1.0 / x5;
+}
+
+// surfdisp96.f:377-398 gtsolh, single precision
+JDSG_DEV float gtsolh(float a, float b) {
+  float c = 0.95f * b;
+  for (int i = 0; i < 5; ++i) {
+    float gamma = b / a;
+    float kappa = c / b;
+    float k2 = kappa * kappa;
+    float gk = gamma * kappa;
+    float gk2 = gk * gk;
+    float fac1 = sqrtf(1.0f - gk2);
+    float fac2 = sqrtf(1.0f - k2);
+    float tk = 2.0f - k2;
+    float fr = tk * tk - 4.0f * fac1 * fac2;
+    float frp = -(4.0f * (2.0f - k2) * kappa) + 4.0f * fac2 * gamma * gamma * kappa / fac1 +
+                4.0f * fac1 * kappa / fac2;
+    frp = frp / b;
+    c = c - fr / frp;
+  }
+  return c;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Column preparation: one thread per column.
+//   * surfdisp96 `sphere` (surfdisp96.f:499-571, R = 6370 km, float storage)
+//     and the starting value / betmx scan (surfdisp96.f:148-221);
+//   * sregn96/slegn96 `bldsph` factors (R = 6371 km, double).
+
+
+extern "C" __global__ void __launch_bounds__(256)
+jdsg_prep_columns(int ncol, int ncol_pad, const int *__restrict__ layer_off,
+                  const float *__restrict__ thk, const float *__restrict__ vp,
+                  const float *__restrict__ vs, const float *__restrict__ rho,
+                  int iflsph, PrepOut o) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  int l0 = layer_off[col];
+  int N = layer_off[col + 1] - l0;
+  o.nlay[col] = N;
+  o.llw[col] = (vs[l0] <= 0.0f) ? 2 : 1;
+  // ---- root-chain model
+  {
+    const double ar = 6370.0;
+    double dr = 0.0, r0 = ar;
+    float betmn = 1.e20f, betmx = -1.e20f;
+    int jmn = 0, jsol = 1;
+    float amn = 0.f, bmn = 0.f;
+    for (int i = 0; i < N; ++i) {
+      float d = thk[l0 + i], a = vp[l0 + i], b = vs[l0 + i], rh = rho[l0 + i];
+      float rR = rh, rL = rh;
+      if (iflsph == 1) {
+        float di = (i == N - 1) ? 1.0f : d;
+        dr = dr + (double)di;
+        double r1 = ar - dr;
+        double z0 = ar * log(ar / r0);
+        double z1 = ar * log(ar / r1);
+        d = (float)(z1 - z0);
+        double tmp = (ar + ar) / (r0 + r1);
+        a = (float)(a * tmp);
+        b = (float)(b * tmp);
+        float btp = (float)tmp;
+        r0 = r1;
+        rL = rh * powm5f(btp);
+        rR = rh * powf(btp, -2.275f);
+      }
+      if (i == N - 1) d = 0.0f;
+      size_t q = (size_t)i * ncol_pad + col;
+      o.chainR[q] = make_float4(d, a, b, rR);
+      o.chainL[q] = make_float4(d, a, b, rL);
+      if (b > 0.01f && b < betmn) {
+        betmn = b; jmn = i; jsol = 1; amn = a; bmn = b;
+      } else if (b <= 0.01f && a < betmn) {
+        betmn = a; jmn = i; jsol = 0;
+      }
+      if (b > betmx) betmx = b;
+    }
+    (void)jmn;
+    float cc1 = (jsol == 0) ? betmn : gtsolh(amn, bmn);
+    cc1 = 0.95f * cc1;
+    cc1 = 0.90f * cc1;
+    o.cc[col] = cc1;
+    o.betmx[col] = betmx;
+  }
+  // ---- eigen-stage model (sregn96.f90:130-181, slegn96.f90:107-151)
+  {
+    const double ar = 6371.0;
+    double dr = 0.0, r0 = ar;
+    for (int i = 0; i < N; ++i) {
+      size_t q = (size_t)i * ncol_pad + col;
+      double zd = (double)thk[l0 + i];
+      double za = (double)vp[l0 + i], zb = (double)vs[l0 + i];
+      double vtp = 1.0, dtp = 1.0, rR = 1.0, rL = 1.0;
+      if (iflsph > 0) {
+        dr = dr + ((i == N - 1) ? 1.0 : zd);
+        double r1 = ar - dr;
+        double z0 = ar * log(ar / r0);
+        double z1 = ar * log(ar / r1);
+        double tmp = (2.0 * ar) / (r0 + r1);
+        vtp = tmp;
+        rR = pow(tmp, (double)(-2.275f));
+        rL = powm5d(tmp);
+        dtp = ar / r0;
+        za = za * tmp;
+        zb = zb * tmp;
+        zd = z1 - z0;
+        r0 = r1;
+      }
+      if (i == N - 1) zd = 0.0;
+      o.zd[q] = zd;
+      o.za[q] = za;
+      o.zb[q] = zb;
+      o.rho0[q] = (double)rho[l0 + i];
+      o.rtpR[q] = rR;
+      o.rtpL[q] = rL;
+      o.vtp[q] = vtp;
+      o.dtp[q] = dtp;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Period equations (surfdisp96.f:734-898), model read from the float4 stack.
+
+namespace {
+
+// dltar1: Love (Haskell SH) from the half-space to the surface
+JDSG_DEV double dltar1(const float4 *__restrict__ st, int stride, int mmax, int llw,
+                       double wvno, double omega) {
+  float4 h = st[(size_t)(mmax - 1) * stride];
+  double beta1 = (double)h.z;
+  double rho1 = (double)h.w;
+  double xkb = omega / beta1;
+  double wvnop = wvno + xkb;
+  double wvnom = fabs(wvno - xkb);
+  double rb = sqrt(wvnop * wvnom);
+  double e1 = rho1 * rb;
+  double e2 = 1.0 / (beta1 * beta1);
+  for (int m = mmax - 2; m >= llw - 1; --m) {
+    float4 L = st[(size_t)m * stride];
+    beta1 = (double)L.z;
+    rho1 = (double)L.w;
+    double xmu = rho1 * beta1 * beta1;
+    xkb = omega / beta1;
+    wvnop = wvno + xkb;
+    wvnom = fabs(wvno - xkb);
+    rb = sqrt(wvnop * wvnom);
+    double dm = (double)L.x;
+    double q = dm * rb;
+    double sinq, cosq, y, z;
+    if (wvno < xkb) {
+      sincos(q, &sinq, &cosq);
+      y = sinq / rb;
+      z = -rb * sinq;
+    } else if (wvno == xkb) {
+      cosq = 1.0;
+      y = dm;
+      z = 0.0;
+    } else {
+      double fac = (q < 16) ? exp(-2.0 * q) : 0.0;
+      cosq = (1.0 + fac) * 0.5;
+      sinq = (1.0 - fac) * 0.5;
+      y = sinq / rb;
+      z = rb * sinq;
+    }
+    double e10 = e1 * cosq + e2 * xmu * z;
+    double e20 = e1 * y / xmu + e2 * cosq;
+    double xnor = fabs(e10);
+    double ynor = fabs(e20);
+    if (ynor > xnor) xnor = ynor;
+    if (xnor < 1.0e-40) xnor = 1.0;
+    e1 = e10 / xnor;
+    e2 = e20 / xnor;
+  }
+  return e1;
+}
+
+// var (surfdisp96.f:901-1018): one of sin/cos, (cosh,sinh)*exp(-p), or the
+// coincident case for each of P and S
+JDSG_DEV void var_ps(double p, double ra, double wvno, double xka, double dpth,
+                     double &cosp, double &w, double &x, double &pex) {
+  pex = 0.0;
+  if (wvno < xka) {
+    double sinp;
+    sincos(p, &sinp, &cosp);
+    w = sinp / ra;
+    x = -ra * sinp;
+  } else if (wvno == xka) {
+    cosp = 1.0;
+    w = dpth;
+    x = 0.0;
+  } else {
+    pex = p;
+    double fac = (p < 16) ? exp(-2.0 * p) : 0.0;
+    cosp = (1.0 + fac) * 0.5;
+    double sinp = (1.0 - fac) * 0.5;
+    w = sinp / ra;
+    x = ra * sinp;
+  }
+}
+
+// dltar4: Rayleigh, Dunkin compound matrices from the half-space upward
+JDSG_DEV double dltar4(const float4 *__restrict__ st, int stride, int mmax, int llw,
+                       double wvno, double omga) {
+  double omega = omga;
+  if (omega < 1.0e-4) omega = 1.0e-4;
+  double wvno2 = wvno * wvno;
+  float4 h = st[(size_t)(mmax - 1) * stride];
+  double xka = omega / (double)h.y;
+  double xkb = omega / (double)h.z;
+  double wvnop = wvno + xka;
+  double wvnom = fabs(wvno - xka);
+  double ra = sqrt(wvnop * wvnom);
+  wvnop = wvno + xkb;
+  wvnom = fabs(wvno - xkb);
+  double rb = sqrt(wvnop * wvnom);
+  double t = (double)h.z / omega;
+  double gammk = 2.0 * t * t;
+  double gam = gammk * wvno2;
+  double gamm1 = gam - 1.0;
+  double rho1 = (double)h.w;
+  double e0 = rho1 * rho1 * (gamm1 * gamm1 - gam * gammk * ra * rb);
+  double e1 = -rho1 * ra;
+  double e2 = rho1 * (gamm1 - gammk * ra * rb);
+  double e3 = rho1 * rb;
+  double e4 = wvno2 - ra * rb;
+  for (int m = mmax - 2; m >= llw - 1; --m) {
+    float4 L = st[(size_t)m * stride];
+    xka = omega / (double)L.y;
+    xkb = omega / (double)L.z;
+    t = (double)L.z / omega;
+    gammk = 2.0 * t * t;
+    gam = gammk * wvno2;
+    wvnop = wvno + xka;
+    wvnom = fabs(wvno - xka);
+    ra = sqrt(wvnop * wvnom);
+    wvnop = wvno + xkb;
+    wvnom = fabs(wvno - xkb);
+    rb = sqrt(wvnop * wvnom);
+    double dpth = (double)L.x;
+    rho1 = (double)L.w;
+    double p = ra * dpth;
+    double q = rb * dpth;
+    double cosp, w, x, pex, cosq, y, z, sex;
+    var_ps(p, ra, wvno, xka, dpth, cosp, w, x, pex);
+    var_ps(q, rb, wvno, xkb, dpth, cosq, y, z, sex);
+    double exa = pex + sex;
+    double a0 = (exa < 60.0) ? exp(-exa) : 0.0;
+    double cpcq = cosp * cosq, cpy = cosp * y, cpz = cosp * z, cqw = cosq * w,
+           cqx = cosq * x, xy = x * y, xz = x * z, wy = w * y, wz = w * z;
+    // dnka (surfdisp96.f:1051-1094)
+    const double two = 2.0;
+    double gamm1_ = gam - 1.0;
+    double twgm1 = gam + gamm1_;
+    double gmgmk = gam * gammk;
+    double gmgm1 = gam * gamm1_;
+    double gm1sq = gamm1_ * gamm1_;
+    double rho2 = rho1 * rho1;
+    double a0pq = a0 - cpcq;
+    double c00 = cpcq - two * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
+    double c01 = (wvno2 * cpy - cqx) / rho1;
+    double c02 = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1_ * wy) / rho1;
+    double c03 = (cpz - wvno2 * cqw) / rho1;
+    double c04 = -(two * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) / rho2;
+    double c10 = (gmgmk * cpz - gm1sq * cqw) * rho1;
+    double c11 = cpcq;
+    double c12 = gammk * cpz - gamm1_ * cqw;
+    double c13 = -wz;
+    double c14 = c03;
+    double c30 = (gm1sq * cpy - gmgmk * cqx) * rho1;
+    double c31 = -xy;
+    double c32 = gamm1_ * cpy - gammk * cqx;
+    double c33 = c11;
+    double c34 = c01;
+    double c40 = -(two * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2;
+    double c41 = c30;
+    double c42 = -(gammk * gamm1_ * twgm1 * a0pq + gam * gammk * gammk * xz +
+                   gamm1_ * gm1sq * wy) * rho1;
+    double c43 = c10;
+    double c44 = c00;
+    double tt = -two * wvno2;
+    double c20 = tt * c42;
+    double c21 = tt * c32;
+    double c22 = a0 + two * (cpcq - c00);
+    double c23 = tt * c12;
+    double c24 = tt * c02;
+    // ee(i) = sum_j e(j) ca(j,i), in j order
+    double ee0 = 0.0, ee1 = 0.0, ee2 = 0.0, ee3 = 0.0, ee4 = 0.0;
+    ee0 = ee0 + e0 * c00; ee0 = ee0 + e1 * c10; ee0 = ee0 + e2 * c20; ee0 = ee0 + e3 * c30; ee0 = ee0 + e4 * c40;
+    ee1 = ee1 + e0 * c01; ee1 = ee1 + e1 * c11; ee1 = ee1 + e2 * c21; ee1 = ee1 + e3 * c31; ee1 = ee1 + e4 * c41;
+    ee2 = ee2 + e0 * c02; ee2 = ee2 + e1 * c12; ee2 = ee2 + e2 * c22; ee2 = ee2 + e3 * c32; ee2 = ee2 + e4 * c42;
+    ee3 = ee3 + e0 * c03; ee3 = ee3 + e1 * c13; ee3 = ee3 + e2 * c23; ee3 = ee3 + e3 * c33; ee3 = ee3 + e4 * c43;
+    ee4 = ee4 + e0 * c04; ee4 = ee4 + e1 * c14; ee4 = ee4 + e2 * c24; ee4 = ee4 + e3 * c34; ee4 = ee4 + e4 * c44;
+    // normc (surfdisp96.f:1022-1047)
+    double t1 = 0.0;
+    if (fabs(ee0) > t1) t1 = fabs(ee0);
+    if (fabs(ee1) > t1) t1 = fabs(ee1);
+    if (fabs(ee2) > t1) t1 = fabs(ee2);
+    if (fabs(ee3) > t1) t1 = fabs(ee3);
+    if (fabs(ee4) > t1) t1 = fabs(ee4);
+    if (t1 < 1.0e-40) t1 = 1.0;
+    e0 = ee0 / t1;
+    e1 = ee1 / t1;
+    e2 = ee2 / t1;
+    e3 = ee3 / t1;
+    e4 = ee4 / t1;
+  }
+  if (llw != 1) {
+    float4 L = st[0];
+    xka = omega / (double)L.y;
+    wvnop = wvno + xka;
+    wvnom = fabs(wvno - xka);
+    ra = sqrt(wvnop * wvnom);
+    double dpth = (double)L.x;
+    rho1 = (double)L.w;
+    double p = ra * dpth;
+    double cosp, w, x, pex;
+    var_ps(p, ra, wvno, xka, dpth, cosp, w, x, pex);
+    double w0 = -rho1 * w;
+    return cosp * e0 + w0 * e1;
+  }
+  return e0;
+}
+
+// ----------------------------------------------------------------------------
+// Chain state machine.
+enum Phase : int {
+  PH_START_K = 0,
+  PH_GS_FIRST,
+  PH_SCAN_NEXT,
+  PH_GS_SCAN,
+  PH_NV_INIT,
+  PH_NV_LOOP,
+  PH_NV_TOPHALF,
+  PH_NV_BODY,
+  PH_NV_END,
+  PH_RETRY_NEXT,
+  PH_DONE
+};
+
+struct ChainState {
+  // job
+  int col, wave, variant, K, M;
+  long long e0;        // entry index of (mode 0, period 0) of this wave
+  double tscale;
+  // constants
+  double cc, cm, dc, onea, one;
+  float betmx;
+  // driver
+  int iq, k, ift, ierr, iverb, sub, Mrun, Krun, kbase, retry;
+  unsigned need, exitmask;
+  double t1, omega;
+  float t1a, t1b;
+  // getsol
+  double c1, c2, clow, del1, del2, del1st;
+  int idir, ifirst;
+  // nevill
+  double c3, del3;
+  int nev, nctrl, m;
+  double x[11], y[11];
+  int phase;
+  double ec;  // requested evaluation point
+};
+
+
+
+JDSG_DEV double period_at(const ChainArgs &A, const ChainState &S, int kk) {
+  double T = A.periods[A.per_off[S.col] + kk];
+  return (S.variant == 0) ? T : T * S.tscale;
+}
+
+JDSG_DEV void write_out(const ChainArgs &A, const ChainState &S, int mode, int kk, double v) {
+  long long e = S.e0 + (long long)mode * S.K + kk;
+  A.cflat[(long long)S.variant * A.nent + e] = v;
+}
+JDSG_DEV double read_out(const ChainArgs &A, const ChainState &S, int mode, int kk) {
+  long long e = S.e0 + (long long)mode * S.K + kk;
+  return A.cflat[(long long)S.variant * A.nent + e];
+}
+
+// Start (or restart) the chain proper over Krun periods from kbase.
+JDSG_DEV void begin_pass(ChainState &S, double *cs, int kpad) {
+  S.iq = 1;
+  S.k = 0;
+  S.ift = 998;
+  S.ierr = 0;
+  S.iverb = 0;
+  for (int i = 0; i < S.Krun; ++i) {
+    cs[i] = 0.0;
+    cs[kpad + i] = 0.0;
+  }
+  S.phase = PH_START_K;
+}
+
+// Returns true with S.ec set when a period-equation value is needed at
+// phase velocity S.ec (period S.t1); false when the chain is complete.
+JDSG_DEV bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
+  const int kpad = A.kmax_pad;
+  for (;;) {
+    switch (S.phase) {
+      case PH_START_K: {
+        if (S.iq > S.Mrun) {
+          if (!S.retry) {
+            if (A.retry && S.ierr) {
+              S.retry = 1;
+              S.kbase = -1;
+              S.phase = PH_RETRY_NEXT;
+              continue;
+            }
+            return false;
+          }
+          // end of one single-period retry pass
+          S.phase = PH_RETRY_NEXT;
+          if (S.ierr) S.exitmask |= S.need;
+          continue;
+        }
+        if (S.k >= S.Krun) {
+          S.iq++;
+          S.k = 0;
+          continue;
+        }
+        int k0 = S.k;
+        if (k0 >= S.ift) goto fail_k;
+        {
+          double t1 = period_at(A, S, S.kbase + k0);
+          if (A.igr > 0) {
+            const float h = 0.005f;
+            S.t1a = (float)(t1 / (double)(1.0f + h));
+            S.t1b = (float)(t1 / (double)(1.0f - h));
+            t1 = (double)S.t1a;
+          } else {
+            S.t1a = (float)t1;
+          }
+          S.t1 = t1;
+          if (k0 == 0 && S.iq == 1) {
+            S.c1 = S.cc; S.clow = S.cc; S.ifirst = 1;
+          } else if (k0 == 0 && S.iq > 1) {
+            S.c1 = cs[0] + S.one * S.dc; S.clow = S.c1; S.ifirst = 1;
+          } else if (k0 > 0 && S.iq > 1) {
+            S.ifirst = 0;
+            S.clow = cs[k0] + S.one * S.dc;
+            S.c1 = cs[k0 - 1];
+            if (S.c1 < S.clow) S.c1 = S.clow;
+          } else {
+            S.ifirst = 0;
+            S.c1 = cs[k0 - 1] - S.onea * S.dc;
+            S.clow = S.cm;
+          }
+          S.sub = 1;
+          S.omega = kTwoPi / S.t1;
+          S.ec = S.c1;
+          S.phase = PH_GS_FIRST;
+          return true;
+        }
+      }
+      case PH_GS_FIRST: {
+        S.del1 = F;
+        if (S.ifirst == 1) S.del1st = S.del1;
+        double plmn = sign1(S.del1st) * sign1(S.del1);
+        S.idir = (S.ifirst == 1 || plmn >= 0.0) ? 1 : -1;
+        S.phase = PH_SCAN_NEXT;
+        continue;
+      }
+      case PH_SCAN_NEXT: {
+        S.c2 = (S.idir > 0) ? S.c1 + S.dc : S.c1 - S.dc;
+        if (S.c2 <= S.clow) {
+          S.idir = 1;
+          S.c1 = S.clow;
+          continue;
+        }
+        S.ec = S.c2;
+        S.phase = PH_GS_SCAN;
+        return true;
+      }
+      case PH_GS_SCAN: {
+        S.del2 = F;
+        if (sign1(S.del1) != sign1(S.del2)) {
+          S.c3 = 0.5 * (S.c1 + S.c2);
+          S.ec = S.c3;
+          S.phase = PH_NV_INIT;
+          return true;
+        }
+        S.c1 = S.c2;
+        S.del1 = S.del2;
+        if (S.c1 < S.cm || S.c1 >= ((double)S.betmx + S.dc)) goto getsol_fail;
+        S.phase = PH_SCAN_NEXT;
+        continue;
+      }
+      case PH_NV_INIT: {
+        S.del3 = F;
+        S.nev = 1;
+        S.nctrl = 1;
+        S.m = 1;
+        S.phase = PH_NV_LOOP;
+        continue;
+      }
+      case PH_NV_LOOP: {
+        S.nctrl++;
+        if (S.nctrl >= 100) goto nv_done;
+        if (S.c3 < fmin(S.c1, S.c2) || S.c3 > fmax(S.c1, S.c2)) {
+          S.nev = 0;
+          S.c3 = 0.5 * (S.c1 + S.c2);
+          S.ec = S.c3;
+          S.phase = PH_NV_TOPHALF;
+          return true;
+        }
+        S.phase = PH_NV_BODY;
+        continue;
+      }
+      case PH_NV_TOPHALF: {
+        S.del3 = F;
+        S.phase = PH_NV_BODY;
+        continue;
+      }
+      case PH_NV_BODY: {
+        double s13 = S.del1 - S.del3;
+        double s32 = S.del3 - S.del2;
+        if (sign1(S.del3) * sign1(S.del1) < 0.0) {
+          S.c2 = S.c3;
+          S.del2 = S.del3;
+        } else {
+          S.c1 = S.c3;
+          S.del1 = S.del3;
+        }
+        if (fabs(S.c1 - S.c2) <= 1.0e-6 * S.c1) goto nv_done;
+        if (sign1(s13) != sign1(s32)) S.nev = 0;
+        double ss1 = fabs(S.del1);
+        double s1 = (double)0.01f * ss1;
+        double ss2 = fabs(S.del2);
+        double s2 = (double)0.01f * ss2;
+        if (s1 > ss2 || s2 > ss1 || S.nev == 0) {
+          S.c3 = 0.5 * (S.c1 + S.c2);
+          S.nev = 1;
+          S.m = 1;
+          S.ec = S.c3;
+          S.phase = PH_NV_END;
+          return true;
+        }
+        if (S.nev == 2) {
+#pragma unroll
+          for (int j = 1; j <= 10; ++j)
+            if (j == S.m) { S.x[j] = S.c3; S.y[j] = S.del3; }
+        } else {
+          S.x[0] = S.c1; S.y[0] = S.del1;
+          S.x[1] = S.c2; S.y[1] = S.del2;
+          S.m = 1;
+        }
+        // Neville on x(y), newest point at index m (surfdisp96.f:660-668)
+        double ym = 0.0;
+#pragma unroll
+        for (int j = 1; j <= 10; ++j)
+          if (j == S.m) ym = S.y[j];
+        bool bad = false;
+#pragma unroll
+        for (int j = 9; j >= 0; --j) {
+          if (j < S.m && !bad) {
+            double denom = ym - S.y[j];
+            if (fabs(denom) < 1.0e-10 * fabs(ym)) {
+              bad = true;
+            } else {
+              S.x[j] = (-S.y[j] * S.x[j + 1] + ym * S.x[j]) / denom;
+            }
+          }
+        }
+        if (bad) {
+          S.c3 = 0.5 * (S.c1 + S.c2);
+          S.nev = 1;
+          S.m = 1;
+        } else {
+          S.c3 = S.x[0];
+          S.nev = 2;
+          S.m = S.m + 1;
+          if (S.m > 10) S.m = 10;
+        }
+        S.ec = S.c3;
+        S.phase = PH_NV_END;
+        return true;
+      }
+      case PH_NV_END: {
+        S.del3 = F;
+        S.phase = PH_NV_LOOP;
+        continue;
+      }
+      case PH_RETRY_NEXT: {
+        // surfdisp.cpp:95-109: each missing period is re-solved alone; a
+        // period still missing is where the reference exit(1)s.
+        for (;;) {
+          S.kbase++;
+          if (S.kbase >= S.K) { S.phase = PH_DONE; return false; }
+          unsigned need = 0;
+          for (int md = 0; md < S.M; ++md)
+            if (read_out(A, S, md, S.kbase) == 0.0) need |= 1u << md;
+          if (need) {
+            S.need = need;
+            S.Mrun = 32 - __builtin_clz(need);
+            S.Krun = 1;
+            begin_pass(S, cs, kpad);
+            break;
+          }
+        }
+        continue;
+      }
+      default:
+        return false;
+    }
+
+  getsol_fail:
+    if (S.sub == 2) {
+      S.c1 = cs[S.k];
+      cs[kpad + S.k] = S.c1;
+      goto record;
+    }
+    goto fail_k;
+
+  nv_done:
+    S.c1 = S.c3;
+    if (S.c1 > (double)S.betmx) goto getsol_fail;
+    // getsol success
+    if (S.sub == 1) {
+      cs[S.k] = S.c1;
+      if (A.igr > 0) {
+        S.t1 = (double)S.t1b;
+        S.ifirst = 0;
+        S.clow = cs[kpad + S.k] + S.one * S.dc;
+        S.c1 = S.c1 - S.onea * S.dc;
+        S.sub = 2;
+        S.omega = kTwoPi / S.t1;
+        S.ec = S.c1;
+        S.phase = PH_GS_FIRST;
+        return true;
+      }
+      S.c1 = 0.0;
+      goto record;
+    }
+    cs[kpad + S.k] = S.c1;
+    goto record;
+
+  record : {
+    float cc0 = (float)cs[S.k];
+    float ccg = (float)S.c1;
+    double v;
+    if (A.igr == 0) {
+      v = (double)cc0;
+    } else {
+      float gvel = (1 / S.t1a - 1 / S.t1b) / (1 / (S.t1a * cc0) - 1 / (S.t1b * ccg));
+      v = (double)gvel;
+    }
+    int md = S.iq - 1;
+    if (!S.retry) {
+      write_out(A, S, md, S.k, v);
+    } else if (S.need & (1u << md)) {
+      write_out(A, S, md, S.kbase, v);
+      if (v == 0.0) S.exitmask |= 1u << md;
+    }
+    S.k++;
+    S.phase = PH_START_K;
+    continue;
+  }
+
+  fail_k : {
+    if (S.iq <= 1 && S.iverb == 0) {
+      S.iverb = 1;
+      S.ierr = 1;
+    }
+    int k0 = S.k;
+    S.ift = k0;
+    int md = S.iq - 1;
+    if (!S.retry) {
+      for (int i = k0; i < S.Krun; ++i) write_out(A, S, md, i, 0.0);
+    } else if (S.need & (1u << md)) {
+      S.exitmask |= 1u << md;
+    }
+    S.iq++;
+    S.k = 0;
+    S.phase = PH_START_K;
+    continue;
+  }
+  }
+}
+
+// Decode chain id -> job; returns false when the chain has no work.
+JDSG_DEV bool setup_chain(ChainState &S, const ChainArgs &A, long long g, double *cs) {
+  int ncol = A.cm.ncol;
+  int col = (int)(g % ncol);
+  long long r = g / ncol;
+  int variant = (int)(r % A.nvar);
+  int w = (int)(r / A.nvar);  // 0 Rayleigh, 1 Love
+  int nR = A.nmode[2 * col], nL = A.nmode[2 * col + 1];
+  int M = (w == 0) ? nR : nL;
+  int K = A.per_off[col + 1] - A.per_off[col];
+  S.col = col;
+  S.variant = variant;
+  S.wave = (w == 0) ? 2 : 1;
+  S.K = K;
+  S.M = M;
+  if (M <= 0 || K <= 0) return false;
+  S.e0 = A.ent_off[col] + (long long)((w == 0) ? 0 : nR) * K;
+  S.tscale = (variant == 1) ? (1.0 + 0.5 * 0.01) : (1.0 - 0.5 * 0.01);
+  S.cc = (double)A.cm.cc[col];
+  S.cm = S.cc;
+  S.dc = fabs((double)0.005f);
+  S.onea = (double)1.5f;
+  S.one = 1.0e-2;
+  S.betmx = A.cm.betmx[col];
+  S.del1st = 0.0;
+  S.retry = 0;
+  S.exitmask = 0;
+  S.need = 0;
+  S.Mrun = M;
+  S.Krun = K;
+  S.kbase = 0;
+  begin_pass(S, cs, A.kmax_pad);
+  return true;
+}
+
+JDSG_DEV void finish_chain(ChainState &S, const ChainArgs &A, long long g) {
+  if (S.exitmask) {
+    for (int md = 0; md < S.M; ++md) {
+      if (!(S.exitmask & (1u << md))) continue;
+      for (int kk = 0; kk < S.K; ++kk) {
+        long long e = S.e0 + (long long)md * S.K + kk;
+        A.cstat[(long long)S.variant * A.nent + e] = JDSG_EXIT_PATH;
+      }
+    }
+  }
+  if (A.ierr_out) A.ierr_out[g] = S.ierr;
+}
+
+}  // namespace
+
+// Persistent chain kernel: lane-level work fetching.  Every trip of the main
+// loop evaluates one period equation per live lane.
+extern "C" __global__ void __launch_bounds__(64)
+jdsg_root_chains(ChainArgs A) {
+  const long long nlanes = (long long)gridDim.x * blockDim.x;
+  const long long lane_id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double *cs = A.cscr + lane_id * 2 * A.kmax_pad;
+  unsigned long long nev = 0;
+  ChainState S;
+  long long g = lane_id;
+  bool live = false;
+  // find first chain with work
+  while (g < A.nchain) {
+    if (setup_chain(S, A, g, cs)) { live = true; break; }
+    if (A.ierr_out) A.ierr_out[g] = 0;
+    g = nlanes + (long long)atomicAdd(A.counter, 1u);
+  }
+  double F = 0.0;
+  bool need = live ? advance(S, A, cs, F) : false;
+  while (live) {
+    if (need) {
+      const float4 *st = (S.wave == 2 ? A.cm.chainR : A.cm.chainL) + S.col;
+      int mmax = A.cm.nlay[S.col];
+      int llw = A.cm.llw[S.col];
+      double wvno = S.omega / S.ec;
+      F = (S.wave == 2) ? dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega)
+                        : dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
+      ++nev;
+      need = advance(S, A, cs, F);
+      continue;
+    }
+    finish_chain(S, A, g);
+    live = false;
+    for (;;) {
+      g = nlanes + (long long)atomicAdd(A.counter, 1u);
+      if (g >= A.nchain) break;
+      if (setup_chain(S, A, g, cs)) { live = true; break; }
+      if (A.ierr_out) A.ierr_out[g] = 0;
+    }
+    if (live) need = advance(S, A, cs, 0.0);
+  }
+  if (A.nevals && nev) atomicAdd(A.nevals, nev);
+}
+
+// ----------------------------------------------------------------------------
+// host launchers
+void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float *thk,
+                      const float *vp, const float *vs, const float *rho, int iflsph,
+                      const PrepOut &o, hipStream_t st) {
+  int blocks = (ncol + 255) / 256;
+  hipLaunchKernelGGL(jdsg_prep_columns, dim3(blocks), dim3(256), 0, st, ncol, ncol_pad,
+                     layer_off, thk, vp, vs, rho, iflsph, o);
+}
+
+void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st) {
+  hipLaunchKernelGGL(jdsg_root_chains, dim3(nwaves), dim3(64), 0, st, A);
+}
+
+__global__ void jdsg_fill_u32(unsigned *p, long long n, unsigned v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void jdsg_launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t st) {
+  if (n <= 0) return;
+  long long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(jdsg_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+}

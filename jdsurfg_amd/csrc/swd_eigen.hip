@@ -1,0 +1,1191 @@
+// swd_eigen.hip -- eigenfunctions, energy integrals, group velocity and
+// layer sensitivity kernels on CDNA4 (the GPU form of src/SWD/sregn96.f90
+// and slegn96.f90, including the sregnpu/slegnpu group-kernel assembly).
+//
+// One lane owns one entry (column, wave, mode, period).  Rayleigh: the
+// Dunkin up-sweep (sregn96.f90:401-489) is stored per layer in a lane-strided
+// scratch area; the Haskell down-sweep (989-1059) then runs top-down fused
+// with the eigenfunction assembly (svfunc :193), the energy integrals
+// (energy :1061, intijr :1198 -- E/E^-1 and the layer functions are formed
+// once per layer instead of once per integral) and the dc/dh terms (getdcdh
+// :1430).  With group kernels the lane runs the three periods T, T(1+.005),
+// T(1-.005) and assembles dU/dm with the reference's formula, which reads the
+// dc/dm left by the T(1-.005) pass (sregn96.f90:1831-1834).
+//
+// Outputs per entry: spherical c and U, and per layer dc/d{a,b,rho,h} and
+// dU/d{a,b,rho,h} (h: layer thickness, half-space 0), at
+//   base(entry) + layer * lstride.
+#include "swd_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+
+
+constexpr double kPiF = (double)3.1415926535898f;  // REAL literal
+
+JDSG_DEV double powm5d(double x) {
+  double x2 = x * x, x4 = x2 * x2, x5 = x4 * x;
+  return 1.0 / x5;
+}
+
+// ------------------------------------------------------------ Rayleigh parts
+struct Layer {
+  double za, zb, zd, zrho;
+  int iwat;
+};
+
+// sregn96.f90:827-911 varsv
+JDSG_DEV void varsv(double pr, double pi, double qr, double qi, cplx rp, cplx rsv, int iwat,
+                    double zd, double &cosp, double &cosq, double &rsinp, double &rsinq,
+                    double &sinpr, double &sinqr, double &pex, double &svex) {
+  const double tiny = (double)1.0e-5f;
+  pex = pr;
+  svex = 0.0;
+  double s, c;
+  sincos(pi, &s, &c);
+  cplx epp = cmk(c, s) / 2.0;
+  cplx epm = cconj(epp);
+  double pfac = (pr < 30.) ? exp(-2. * pr) : 0.0;
+  cosp = (epp + pfac * epm).re;
+  cplx sinp = epp - pfac * epm;
+  rsinp = (rp * sinp).re;
+  if (fabs(pr) < tiny && cabs_(rp) < tiny)
+    sinpr = zd;
+  else
+    sinpr = (sinp / rp).re;
+  if (iwat == 1) {
+    cosq = 1.0;
+    rsinq = 0.0;
+    sinqr = 0.0;
+    return;
+  }
+  svex = qr;
+  sincos(qi, &s, &c);
+  cplx eqp = cmk(c, s) / 2.0;
+  cplx eqm = cconj(eqp);
+  double svfac = (qr < 30.) ? exp(-2. * qr) : 0.0;
+  cosq = (eqp + svfac * eqm).re;
+  cplx sinq = eqp - svfac * eqm;
+  rsinq = (rsv * sinq).re;
+  if (fabs(qr) < tiny && cabs_(rsv) < tiny)
+    sinqr = zd;
+  else
+    sinqr = (sinq / rsv).re;
+}
+
+JDSG_DEV void layer_trig(const Layer &L, double omega, double wvno2, double &cosp,
+                         double &cossv, double &rsinp, double &rsinsv, double &sinpr,
+                         double &sinsvr, double &pex, double &svex) {
+  double xka = omega / L.za;
+  double xkb = (L.zb > 0.0) ? omega / L.zb : 0.0;
+  cplx rp = csqrt_real(wvno2 - xka * xka);
+  cplx rsv = csqrt_real(wvno2 - xkb * xkb);
+  cplx p = rp * L.zd, q = rsv * L.zd;
+  varsv(p.re, p.im, q.re, q.im, rp, rsv, L.iwat, L.zd, cosp, cossv, rsinp, rsinsv, sinpr,
+        sinsvr, pex, svex);
+}
+
+// sregn96.f90:491-647 dnka; returns the 5x5 matrix row-major in ca
+JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, double cossv,
+                     double rsinsv, double sinsvr, float rho, float b, int iwat, double ex,
+                     double exa, double wvno, double wvno2, double om2) {
+  if (iwat == 1) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) ca[i][j] = 0.0;
+    double dfac = (ex > 35.0) ? 0.0 : exp(-ex);
+    ca[2][2] = dfac;
+    ca[0][0] = cosp;
+    ca[4][4] = cosp;
+    ca[0][1] = -rsinp / ((double)rho * om2);
+    ca[1][0] = -(double)rho * sinpr * om2;
+    ca[1][1] = cosp;
+    ca[3][3] = cosp;
+    ca[3][4] = ca[0][1];
+    ca[4][3] = ca[1][0];
+    return;
+  }
+  double a0 = (exa < 60.0) ? exp(-exa) : 0.0;
+  double cpcq = cosp * cossv, cpy = cosp * sinsvr, cpz = cosp * rsinsv, cqw = cossv * sinpr,
+         cqx = cossv * rsinp, xy = rsinp * sinsvr, xz = rsinp * rsinsv, wy = sinpr * sinsvr,
+         wz = sinpr * rsinsv;
+  float rho2 = rho * rho;
+  float bb2 = 2.0f * b * b;
+  double gam = (double)bb2 * wvno2 / om2;
+  double gam2 = gam * gam;
+  double gamm1 = gam - 1.;
+  double gamm2 = gamm1 * gamm1;
+  double cqww2 = cqw * wvno2;
+  double cqxw2 = cqx / wvno2;
+  double gg1 = gam * gamm1;
+  double a0c = 2.0 * (a0 - cpcq);
+  double xz2 = xz / wvno2;
+  double gxz2 = gam * xz2;
+  double g2xz2 = gam2 * xz2;
+  double a0cgg1 = a0c * (gam + gamm1);
+  double wy2 = wy * wvno2;
+  double g2wy2 = gamm2 * wy2;
+  double g1wy2 = gamm1 * wy2;
+  double rom = (double)rho * om2;
+  double temp = a0c * gg1 + g2xz2 + g2wy2;
+  ca[2][2] = a0 + temp + temp;
+  ca[0][0] = cpcq - temp;
+  ca[0][1] = (-cqx + wvno2 * cpy) / rom;
+  temp = 0.5 * a0cgg1 + gxz2 + g1wy2;
+  ca[0][2] = wvno * temp / rom;
+  ca[0][3] = (-cqww2 + cpz) / rom;
+  temp = wvno2 * (a0c + wy2) + xz;
+  ca[0][4] = -temp / ((double)rho2 * om2 * om2);
+  ca[1][0] = (-gamm2 * cqw + gam2 * cpz / wvno2) * (double)rho * om2;
+  ca[1][1] = cpcq;
+  ca[1][2] = (gamm1 * cqww2 - gam * cpz) / wvno;
+  ca[1][3] = -wz;
+  ca[1][4] = ca[0][3];
+  temp = 0.5 * a0cgg1 * gg1 + gam2 * gxz2 + gamm2 * g1wy2;
+  ca[2][0] = -2.0 * temp * (double)rho * om2 / wvno;
+  ca[2][1] = -wvno * (gam * cqxw2 - gamm1 * cpy) * 2.0;
+  ca[2][3] = -2.0 * ca[1][2];
+  ca[2][4] = -2.0 * ca[0][2];
+  ca[3][0] = (-gam2 * cqxw2 + gamm2 * cpy) * (double)rho * om2;
+  ca[3][1] = -xy;
+  ca[3][2] = -ca[2][1] / 2.0;
+  ca[3][3] = ca[1][1];
+  ca[3][4] = ca[0][1];
+  temp = gamm2 * (a0c * gam2 + g2wy2) + gam2 * g2xz2;
+  ca[4][0] = -(double)rho2 * om2 * om2 * temp / wvno2;
+  ca[4][1] = ca[3][0];
+  ca[4][2] = -ca[2][0] / 2.0;
+  ca[4][3] = ca[1][0];
+  ca[4][4] = ca[0][0];
+}
+
+// sregn96.f90:913-987 hska
+JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, double tcossv,
+                   double trsinsv, double tsinsvr, float rho, float b, int iwat, double pex,
+                   double svex, double wvno, double wvno2, double om2) {
+  if (iwat == 1) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) AA[i][j] = 0.0;
+    double dfac = (pex > 35.0) ? 0.0 : exp(-pex);
+    AA[0][0] = dfac;
+    AA[3][3] = dfac;
+    AA[1][1] = cosp;
+    AA[2][2] = cosp;
+    AA[1][2] = -rsinp / ((double)rho * om2);
+    AA[2][1] = -(double)rho * om2 * sinpr;
+    return;
+  }
+  double dfac = ((pex - svex) > 70.0) ? 0.0 : exp(svex - pex);
+  double cossv = dfac * tcossv;
+  double rsinsv = dfac * trsinsv;
+  double sinsvr = dfac * tsinsvr;
+  float bb2 = 2.0f * b * b;
+  double gam = (double)bb2 * wvno2 / om2;
+  double gamm1 = gam - 1.0;
+  double rom = (double)rho * om2;
+  AA[0][0] = cossv + gam * (cosp - cossv);
+  AA[0][1] = -wvno * gamm1 * sinpr + gam * rsinsv / wvno;
+  AA[0][2] = -wvno * (cosp - cossv) / rom;
+  AA[0][3] = (wvno2 * sinpr - rsinsv) / rom;
+  AA[1][0] = gam * rsinp / wvno - wvno * gamm1 * sinsvr;
+  AA[1][1] = cosp - gam * (cosp - cossv);
+  AA[1][2] = (-rsinp + wvno2 * sinsvr) / rom;
+  AA[1][3] = -AA[0][2];
+  AA[2][0] = rom * gam * gamm1 * (cosp - cossv) / wvno;
+  AA[2][1] = rom * (-gamm1 * gamm1 * sinpr + gam * gam * rsinsv / wvno2);
+  AA[2][2] = AA[1][1];
+  AA[2][3] = -AA[0][1];
+  AA[3][0] = rom * (gam * gam * rsinp / wvno2 - gamm1 * gamm1 * sinsvr);
+  AA[3][1] = -AA[2][0];
+  AA[3][2] = -AA[1][0];
+  AA[3][3] = AA[0][0];
+}
+
+// sregn96.f90:649-825 evalg (jbdry = 0)
+struct EMat {
+  cplx e[4][4], ei[4][4], ra, rb;
+};
+JDSG_DEV void evalg0(const Layer &L, bool allfluid, double wvno, double om, double om2,
+                     double wvno2, EMat &M, double *gbr_re) {
+  double xka = om / L.za;
+  double xkb = (L.zb > 0.0) ? om / L.zb : 0.0;
+  cplx ra = csqrt_real(wvno2 - xka * xka);
+  cplx rb = csqrt_real(wvno2 - xkb * xkb);
+  M.ra = ra;
+  M.rb = rb;
+  double gam = L.zb * wvno / om;
+  gam = 2.0 * (gam * gam);
+  double gamm1 = gam - 1.0;
+  double zr = L.zrho;
+  if (L.iwat == 0) {
+    M.e[0][0] = cmk(wvno, 0); M.e[0][1] = rb; M.e[0][2] = cmk(wvno, 0); M.e[0][3] = -rb;
+    M.e[1][0] = ra; M.e[1][1] = cmk(wvno, 0); M.e[1][2] = -ra; M.e[1][3] = cmk(wvno, 0);
+    M.e[2][0] = cmk(zr * om2 * gamm1, 0);
+    M.e[2][1] = zr * om2 * gam * rb / wvno;
+    M.e[2][2] = cmk(zr * om2 * gamm1, 0);
+    M.e[2][3] = -zr * om2 * gam * rb / wvno;
+    M.e[3][0] = zr * om2 * gam * ra / wvno;
+    M.e[3][1] = cmk(zr * om2 * gamm1, 0);
+    M.e[3][2] = -zr * om2 * gam * ra / wvno;
+    M.e[3][3] = cmk(zr * om2 * gamm1, 0);
+    M.ei[0][0] = cmk(0.5 * gam / wvno, 0);
+    M.ei[0][1] = -0.5 * gamm1 / ra;
+    M.ei[0][2] = cmk(-0.5 / (zr * om2), 0);
+    M.ei[0][3] = 0.5 * wvno / (zr * om2 * ra);
+    M.ei[1][0] = -0.5 * gamm1 / rb;
+    M.ei[1][1] = cmk(0.5 * gam / wvno, 0);
+    M.ei[1][2] = 0.5 * wvno / (zr * om2 * rb);
+    M.ei[1][3] = cmk(-0.5 / (zr * om2), 0);
+    M.ei[2][0] = cmk(0.5 * gam / wvno, 0);
+    M.ei[2][1] = 0.5 * gamm1 / ra;
+    M.ei[2][2] = cmk(-0.5 / (zr * om2), 0);
+    M.ei[2][3] = -0.5 * wvno / (zr * om2 * ra);
+    M.ei[3][0] = 0.5 * gamm1 / rb;
+    M.ei[3][1] = cmk(0.5 * gam / wvno, 0);
+    M.ei[3][2] = -0.5 * wvno / (zr * om2 * rb);
+    M.ei[3][3] = cmk(-0.5 / (zr * om2), 0);
+    if (gbr_re) {
+      cplx g[5];
+      g[0] = (zr * zr) * om2 * om2 * (-gam * gam * ra * rb + wvno2 * gamm1 * gamm1);
+      g[1] = -zr * (wvno2 * ra) * om2;
+      g[2] = -zr * (-gam * ra * rb + wvno2 * gamm1) * om2 * wvno;
+      g[3] = zr * (wvno2 * rb) * om2;
+      g[4] = wvno2 * (wvno2 - ra * rb);
+      cplx den = -zr * zr * om2 * om2 * wvno2 * ra * rb;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) gbr_re[i] = ((0.25 * g[i]) / den).re;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) M.e[i][j] = M.ei[i][j] = cmk(0, 0);
+    M.e[0][0] = ra;
+    M.e[0][1] = -ra;
+    M.e[1][0] = cmk(-zr * om2, 0);
+    M.e[1][1] = cmk(-zr * om2, 0);
+    M.ei[0][0] = 0.5 / ra;
+    M.ei[0][1] = cmk(-0.5 / (zr * om2), 0);
+    M.ei[1][0] = -0.5 / ra;
+    M.ei[1][1] = cmk(-0.5 / (zr * om2), 0);
+    if (gbr_re) {
+      if (allfluid) {
+        gbr_re[0] = (0.5 / ra).re;
+        gbr_re[1] = (cmk(0.5, 0.0) / cmk(-zr * om2, 0)).re;
+        gbr_re[2] = gbr_re[3] = gbr_re[4] = 0.0;
+      } else {
+        gbr_re[0] = gbr_re[1] = gbr_re[2] = 0.0;
+        gbr_re[3] = ((0.5 * zr * om2) / ra).re;
+        gbr_re[4] = -0.5;
+      }
+    }
+  }
+}
+
+// sregn96.f90:1319-1397
+JDSG_DEV cplx ffunc(cplx nub, double dm) {
+  if (cabs_(nub) < 1.0e-08) return cmk(dm, 0);
+  cplx argcd = nub * dm;
+  cplx exqq = (argcd.re < 40.0) ? cexp_(-2.0 * argcd) : cmk(0, 0);
+  return (1.0 - exqq) / (2.0 * nub);
+}
+JDSG_DEV cplx gfunc(cplx nub, double dm) {
+  cplx argcd = nub * dm;
+  if (argcd.re < 75) return cexp_(-argcd) * dm;
+  return cmk(0, 0);
+}
+JDSG_DEV cplx h1func(cplx nua, cplx nub, double dm) {
+  if (cabs_(nub + nua) < 1.0e-08) return cmk(dm, 0);
+  cplx argcd = (nua + nub) * dm;
+  cplx exqq = (argcd.re < 40.0) ? cexp_(-argcd) : cmk(0, 0);
+  return (1.0 - exqq) / (nub + nua);
+}
+JDSG_DEV cplx h2func(cplx nua, cplx nub, double dm) {
+  if (cabs_(nub - nua) < 1.0e-08) return cmk(dm, 0);
+  cplx argcd = nua * dm;
+  cplx exqp = (argcd.re < 40.0) ? cexp_(-argcd) : cmk(0, 0);
+  argcd = nub * dm;
+  cplx exqq = (argcd.re < 40.0) ? cexp_(-argcd) : cmk(0, 0);
+  return (exqq - exqp) / (nua - nub);
+}
+
+struct U4 {
+  double ur, uz, tz, tr;
+};
+
+// sregn96.f90:1198-1317 intijr for a solid internal layer (typelyr 0)
+template <int I, int J>
+JDSG_DEV double int_solid(const EMat &M, cplx kmpu, cplx kmsu, cplx km1pd, cplx km1sd,
+                          cplx FA, cplx GA, cplx FB, cplx GB, cplx H1, cplx H2) {
+  const cplx(*e)[4] = M.e;
+  cplx c = e[I][0] * e[J][0] * kmpu * kmpu * FA + e[I][2] * e[J][2] * km1pd * km1pd * FA +
+           e[I][1] * e[J][1] * kmsu * kmsu * FB + e[I][3] * e[J][3] * km1sd * km1sd * FB +
+           H1 * ((e[I][0] * e[J][1] + e[I][1] * e[J][0]) * kmpu * kmsu +
+                 (e[I][2] * e[J][3] + e[I][3] * e[J][2]) * km1pd * km1sd) +
+           H2 * ((e[I][0] * e[J][3] + e[I][3] * e[J][0]) * kmpu * km1sd +
+                 (e[I][1] * e[J][2] + e[I][2] * e[J][1]) * km1pd * kmsu) +
+           GA * (e[I][0] * e[J][2] + e[I][2] * e[J][0]) * kmpu * km1pd +
+           GB * (e[I][1] * e[J][3] + e[I][3] * e[J][1]) * kmsu * km1sd;
+  return c.re;
+}
+// lower half-space (typelyr 1)
+template <int I, int J>
+JDSG_DEV double int_half(const EMat &M, cplx km1pd, cplx km1sd) {
+  const cplx(*e)[4] = M.e;
+  cplx c = e[I][2] * e[J][2] * km1pd * km1pd / (2.0 * M.ra) +
+           (e[I][2] * e[J][3] + e[I][3] * e[J][2]) * km1pd * km1sd / (M.ra + M.rb) +
+           e[I][3] * e[J][3] * km1sd * km1sd / (2.0 * M.rb);
+  return c.re;
+}
+template <int I, int J>
+JDSG_DEV double int_fluid(const EMat &M, int typelyr, cplx kmpu, cplx km1pd, cplx FA,
+                          cplx GA) {
+  const cplx(*e)[4] = M.e;
+  cplx c;
+  if (typelyr == 0)
+    c = e[I][0] * e[J][0] * kmpu * kmpu * FA +
+        (e[I][0] * e[J][1] + e[I][1] * e[J][0]) * kmpu * km1pd * GA +
+        e[I][1] * e[J][1] * km1pd * km1pd * FA;
+  else
+    c = e[I][1] * e[J][1] * km1pd * km1pd / (2.0 * M.ra);
+  return c.re;
+}
+
+struct EnergySums {
+  double s0, s1, s2, s3;
+};
+
+// energy contribution of layer m (sregn96.f90:1061-1161 loop body).  Writes
+// the unscaled dc/da, dc/db, dc/drho of layer m to ka/kb/kr.
+JDSG_DEV void layer_energy(const Layer &L, bool allfluid, int typelyr, const U4 &u0,
+                           const U4 &u1, double om, double om2, double wvno, double wvno2,
+                           double c, EnergySums &S, double &ka, double &kb, double &kr,
+                           double &ur_fluid) {
+  EMat M;
+  evalg0(L, allfluid, wvno, om, om2, wvno2, M, nullptr);
+  double ah = L.za, av = L.za, bv = L.zb, rho = L.zrho, eta = 1.0;
+  double TA = L.zrho * L.za * L.za, TC = TA;
+  if (L.iwat == 1) {
+    double TN = 0.0;
+    double TF = TA - 2. * TN;
+    double a12 = -(wvno * wvno - om * om / (ah * ah)) / (rho * om * om);
+    cplx km1pd = M.ei[1][0] * u0.uz + M.ei[1][1] * u0.tz;
+    cplx kmpu = cmk(0, 0), FA = cmk(0, 0), GA = cmk(0, 0);
+    if (typelyr == 0) {
+      kmpu = M.ei[0][0] * u1.uz + M.ei[0][1] * u1.tz;
+      FA = ffunc(M.ra, L.zd);
+      GA = gfunc(M.ra, L.zd);
+    }
+    double INT11 = int_fluid<0, 0>(M, typelyr, kmpu, km1pd, FA, GA);
+    double INT22 = int_fluid<1, 1>(M, typelyr, kmpu, km1pd, FA, GA);
+    double wr = wvno / (rho * om2);
+    double URUR = INT22 * (wr * wr);
+    double UZUZ = INT11;
+    double URDUZ = -(wvno / (rho * om2)) * a12 * INT22;
+    double DUZDUZ = a12 * a12 * INT22;
+    S.s0 = S.s0 + rho * (URUR + UZUZ);
+    S.s1 = S.s1 + TA * URUR;
+    S.s2 = S.s2 - TF * URDUZ;
+    S.s3 = S.s3 + TC * DUZDUZ;
+    double facah = rho * ah * (URUR - 2. * eta * URDUZ / wvno);
+    double facav = rho * av * DUZDUZ / wvno2;
+    ka = facah + facav;
+    kb = 0.0;
+    double facr = -0.5 * c * c * (URUR + UZUZ);
+    kr = 0.5 * (av * facav + ah * facah) / rho + facr;
+    ur_fluid = -wvno * u0.tz / (rho * om2);
+    return;
+  }
+  double TL = L.zrho * L.zb * L.zb, TN = TL;
+  double TF = TA - 2. * TN;
+  double a12 = -wvno, a14 = 1.0 / TL, a21 = wvno * TF / TC, a23 = 1.0 / TC;
+  const cplx(*ei)[4] = M.ei;
+  cplx km1pd = ei[2][0] * u0.ur + ei[2][1] * u0.uz + ei[2][2] * u0.tz + ei[2][3] * u0.tr;
+  cplx km1sd = ei[3][0] * u0.ur + ei[3][1] * u0.uz + ei[3][2] * u0.tz + ei[3][3] * u0.tr;
+  double INT11, INT13, INT22, INT24, INT33, INT44;
+  if (typelyr == 0) {
+    cplx kmpu = ei[0][0] * u1.ur + ei[0][1] * u1.uz + ei[0][2] * u1.tz + ei[0][3] * u1.tr;
+    cplx kmsu = ei[1][0] * u1.ur + ei[1][1] * u1.uz + ei[1][2] * u1.tz + ei[1][3] * u1.tr;
+    cplx FA = ffunc(M.ra, L.zd);
+    cplx GA = gfunc(M.ra, L.zd);
+    cplx FB = ffunc(M.rb, L.zd);
+    cplx GB = gfunc(M.rb, L.zd);
+    cplx H1 = h1func(M.ra, M.rb, L.zd);
+    cplx H2 = h2func(M.ra, M.rb, L.zd);
+    INT11 = int_solid<0, 0>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT13 = int_solid<0, 2>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT22 = int_solid<1, 1>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT24 = int_solid<1, 3>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT33 = int_solid<2, 2>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT44 = int_solid<3, 3>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+  } else {
+    INT11 = int_half<0, 0>(M, km1pd, km1sd);
+    INT13 = int_half<0, 2>(M, km1pd, km1sd);
+    INT22 = int_half<1, 1>(M, km1pd, km1sd);
+    INT24 = int_half<1, 3>(M, km1pd, km1sd);
+    INT33 = int_half<2, 2>(M, km1pd, km1sd);
+    INT44 = int_half<3, 3>(M, km1pd, km1sd);
+  }
+  double URUR = INT11;
+  double UZUZ = INT22;
+  double DURDUR = a12 * a12 * INT22 + 2. * a12 * a14 * INT24 + a14 * a14 * INT44;
+  double DUZDUZ = a21 * a21 * INT11 + 2. * a21 * a23 * INT13 + a23 * a23 * INT33;
+  double URDUZ = a21 * INT11 + a23 * INT13;
+  double UZDUR = a12 * INT22 + a14 * INT24;
+  S.s0 = S.s0 + rho * (URUR + UZUZ);
+  S.s1 = S.s1 + TL * UZUZ + TA * URUR;
+  S.s2 = S.s2 + TL * UZDUR - TF * URDUZ;
+  S.s3 = S.s3 + TL * DURDUR + TC * DUZDUZ;
+  double facah = rho * ah * (URUR - 2. * eta * URDUZ / wvno);
+  double facav = rho * av * DUZDUZ / wvno2;
+  double facbh = 0.0;
+  double facbv = rho * bv * (UZUZ + 2. * UZDUR / wvno + DURDUR / wvno2 + 4. * eta * URDUZ / wvno);
+  ka = facah + facav;
+  kb = facbv + facbh;
+  double facr = -0.5 * c * c * (URUR + UZUZ);
+  kr = 0.5 * (av * facav + ah * facah + bv * facbv) / rho + facr;
+  ur_fluid = u0.ur;
+}
+
+// sregn96.f90:1430-1529 getdcdh, sum of the six terms (before fac)
+JDSG_DEV double dcdh_terms(int m, const Layer &Lm, const Layer &Lp, double xmu, double xlam,
+                           double xmup, double xlamp, const U4 &u, double om2, double wvno,
+                           double wvno2) {
+  double tuz = u.uz, ttz = u.tz, ttr = u.tr, tur;
+  if (Lm.iwat == 1)
+    tur = -wvno * ttz / (Lm.zrho * om2);
+  else
+    tur = u.ur;
+  double gfac1, gfac2, gfac3, gfac4, gfac5, gfac6;
+  if (m == 0) {
+    double drho = Lm.zrho - 0.0;
+    double dmu = xmu - 0.0;
+    double dlm = xlam - 0.0;
+    double dl2mu = dlm + dmu + dmu;
+    double xl2mp = xlam + xmu + xmu;
+    double duzdzp = (ttz + wvno * xlam * tur) / xl2mp;
+    double durdzp = (Lm.iwat == 1) ? wvno * tuz : (ttr / xmu) - wvno * tuz;
+    double drur2 = tur * tur * drho;
+    double dlur2 = tur * tur * dl2mu;
+    gfac1 = om2 * drho * (tuz * tuz);
+    gfac2 = om2 * drur2;
+    gfac3 = -wvno2 * dmu * (tuz * tuz);
+    gfac4 = -wvno2 * dlur2;
+    gfac5 = (xl2mp * (duzdzp * duzdzp));
+    gfac6 = (xmu * (durdzp * durdzp));
+  } else {
+    double drho = Lm.zrho - Lp.zrho;
+    double dmu = xmu - xmup;
+    double dlm = xlam - xlamp;
+    double dl2mu = dlm + dmu + dmu;
+    double xl2mp = xlam + xmu + xmu;
+    double xl2mm = xlamp + xmup + xmup;
+    double duzdzp = (ttz + wvno * xlam * tur) / xl2mp;
+    double durdzp = (xmu == 0.0) ? wvno * tuz : (ttr / xmu) - wvno * tuz;
+    double durdzm = (xmup == 0.0) ? wvno * tuz : (ttr / xmup) - wvno * tuz;
+    double drur2, dlur2, duzdzm;
+    if (Lp.iwat == 1 && Lm.iwat == 0) {
+      double URB = -wvno * u.tz / (Lp.zrho * om2);
+      drur2 = tur * tur * Lm.zrho - URB * URB * Lp.zrho;
+      dlur2 = tur * tur * xl2mp - URB * URB * xl2mm;
+      duzdzm = (ttz + wvno * xlamp * URB) / xlamp;
+    } else if (Lp.iwat == 1 && Lm.iwat == 1) {
+      double URB = -wvno * u.tz / (Lp.zrho * om2);
+      drur2 = tur * tur * Lm.zrho - URB * URB * Lp.zrho;
+      dlur2 = tur * tur * xl2mp - URB * URB * xl2mm;
+      duzdzm = (ttz + wvno * xlamp * URB) / xl2mm;
+    } else {
+      drur2 = tur * tur * drho;
+      dlur2 = tur * tur * dl2mu;
+      duzdzm = (ttz + wvno * xlamp * tur) / xl2mm;
+    }
+    gfac1 = om2 * drho * (tuz * tuz);
+    gfac2 = om2 * drur2;
+    gfac3 = -wvno2 * dmu * (tuz * tuz);
+    gfac4 = -wvno2 * dlur2;
+    gfac5 = (xl2mp * (duzdzp * duzdzp) - xl2mm * (duzdzm * duzdzm));
+    gfac6 = (xmu * (durdzp * durdzp) - xmup * (durdzm * durdzm));
+  }
+  return gfac1 + gfac2 + gfac3 + gfac4 + gfac5 + gfac6;
+}
+
+// Item decoding: item -> (col, mode, k) of this wave
+JDSG_DEV void decode_item(const EigenArgs &A, long long it, int &col, int &mode, int &k,
+                          long long &e) {
+  int ncol = A.cm.ncol;
+  if (A.item_ent == nullptr) {
+    col = (int)(it % ncol);
+    long long r = it / ncol;
+    k = (int)(r % A.grid_K);
+    mode = (int)(r / A.grid_K);
+    int woff = (A.wave == 0) ? 0 : A.grid_nR;
+    e = A.ent_off[col] + (long long)(woff + mode) * A.grid_K + k;
+    return;
+  }
+  e = A.item_ent[it];
+  // column by binary search in ent_off
+  int lo = 0, hi = ncol - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (A.ent_off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  col = lo;
+  int K = A.per_off[col + 1] - A.per_off[col];
+  long long el = e - A.ent_off[col];
+  int wm = (int)(el / K);
+  k = (int)(el % K);
+  mode = (A.wave == 0) ? wm : wm - A.nmode[2 * col];
+}
+
+JDSG_DEV Layer load_layer(const EigenArgs &A, int col, int i, bool rayleigh) {
+  size_t q = (size_t)i * A.cm.ncol_pad + col;
+  Layer L;
+  L.za = A.cm.za[q];
+  L.zb = A.cm.zb[q];
+  L.zd = A.cm.zd[q];
+  double rt = (A.iflsph > 0) ? (rayleigh ? A.cm.rtpR[q] : A.cm.rtpL[q]) : 1.0;
+  L.zrho = (A.iflsph > 0) ? A.cm.rho0[q] * rt : A.cm.rho0[q];
+  L.iwat = rayleigh ? (L.zb > 0.0 ? 0 : 1) : 0;
+  return L;
+}
+
+// One Rayleigh pass (svfunc + energy).  Raw per-layer kernels go to
+// dst[a,b,r,h] at dst_base + layer*dst_stride (h holds the getdcdh sum before
+// fac).  Returns ugr, sumi0 and fac through the references.
+struct Dst {
+  double *p[4];
+  long long base, stride;
+};
+
+JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, double wvno,
+                            double *scr, long long sstride, const Dst &D, double &ugr,
+                            double &sumi0, double &fac, double *eig, long long ebase) {
+  double om2 = omega * omega, wvno2 = wvno * wvno;
+  bool allfluid = true;
+  int jwat = 0;
+  bool topwater = true;
+  for (int i = 0; i < N; ++i) {
+    double zb = A.cm.zb[(size_t)i * A.cm.ncol_pad + col];
+    if (zb > 0.0) allfluid = false;
+    if (topwater) {
+      if (zb > 0.0) topwater = false; else jwat = i + 1;
+    }
+  }
+  if (allfluid) jwat = 0;
+  // ---- up: Dunkin vectors from the half-space (sregn96.f90:401-489)
+  {
+    Layer H = load_layer(A, col, N - 1, true);
+    EMat M;
+    double g[5];
+    evalg0(H, allfluid, wvno, omega, om2, wvno2, M, g);
+    double cd[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      cd[j] = g[j];
+      scr[((long long)(N - 1) * 6 + j) * sstride] = cd[j];
+    }
+    scr[((long long)(N - 1) * 6 + 5) * sstride] = 0.0;
+    double exsum = 0.0;
+    for (int m = N - 2; m >= 0; --m) {
+      Layer L = load_layer(A, col, m, true);
+      double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
+      layer_trig(L, omega, wvno2, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
+      double ca[5][5];
+      dnka_r(ca, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat,
+             pex, pex + svex, wvno, wvno2, om2);
+      double ee[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        double cr = 0.0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) cr = cr + cd[j] * ca[j][i];
+        ee[i] = cr;
+      }
+      double t1 = 0.0;
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        if (fabs(ee[i]) > t1) t1 = fabs(ee[i]);
+      if (t1 < 1.0e-40) t1 = 1.0;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) cd[i] = ee[i] / t1;
+      double exn = log(t1);
+      exsum = exsum + pex + svex + exn;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) scr[((long long)m * 6 + j) * sstride] = cd[j];
+      scr[((long long)m * 6 + 5) * sstride] = exsum;
+    }
+  }
+  // ---- down + svfunc + energy + getdcdh, top to bottom
+  double cd0_1 = scr[(0 * 6 + 1) * sstride];
+  double cd0_2 = scr[(0 * 6 + 2) * sstride];
+  double exe0 = scr[(0 * 6 + 5) * sstride];
+  double f1213 = -cd0_1;
+  U4 uprev;
+  uprev.ur = cd0_2 / cd0_1;
+  uprev.uz = 1.0;
+  uprev.tz = 0.0;
+  uprev.tr = 0.0;
+  if (0 < jwat) { uprev.ur = 0.0; uprev.tr = 0.0; }
+  double vv[4] = {1.0, 0.0, 0.0, 0.0};
+  double exsum = 0.0;
+  double c = omega / wvno;
+  EnergySums S = {0.0, 0.0, 0.0, 0.0};
+  Layer Lprev = load_layer(A, col, 0, true);
+  double xmu_prev = Lprev.zrho * (Lprev.zb * Lprev.zb);
+  double xlam_prev = Lprev.zrho * (Lprev.za * Lprev.za) - 2 * xmu_prev;
+  // getdcdh for layer 0
+  {
+    double g = dcdh_terms(0, Lprev, Lprev, xmu_prev, xlam_prev, 0.0, 0.0, uprev, om2, wvno, wvno2);
+    D.p[3][D.base] = g;
+  }
+  Layer Lm1 = Lprev;  // layer m-1 props for getdcdh
+  double xmu_m1 = xmu_prev, xlam_m1 = xlam_prev;
+  for (int m = 0; m < N - 1; ++m) {
+    Layer L = Lprev;  // layer m
+    double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
+    layer_trig(L, omega, wvno2, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
+    double AA[4][4];
+    hska(AA, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat, pex,
+         svex, wvno, wvno2, om2);
+    double aa0[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double cc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cc = cc + AA[i][j] * vv[j];
+      aa0[i] = cc;
+    }
+    double t1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (fabs(aa0[i]) > t1) t1 = fabs(aa0[i]);
+    if (t1 < 1.0e-40) t1 = 1.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) vv[i] = aa0[i] / t1;
+    double ex2 = log(t1);
+    exsum = exsum + pex + ex2;
+    // eigenfunction at the top of layer m+1 (svfunc)
+    int i1 = m + 1;
+    long long so = (long long)i1 * 6;
+    double cd1 = scr[(so + 0) * sstride], cd2 = scr[(so + 1) * sstride],
+           cd3 = scr[(so + 2) * sstride], cd4 = -cd3, cd5 = scr[(so + 3) * sstride],
+           cd6 = scr[(so + 4) * sstride], exe_i = scr[(so + 5) * sstride];
+    double tz1 = -vv[3], tz2 = -vv[2], tz3 = vv[1], tz4 = vv[0];
+    double uu1 = tz2 * cd6 - tz3 * cd5 + tz4 * cd4;
+    double uu2 = -tz1 * cd6 + tz3 * cd3 - tz4 * cd2;
+    double uu3 = tz1 * cd5 - tz2 * cd3 + tz4 * cd1;
+    double uu4 = -tz1 * cd4 + tz2 * cd2 - tz3 * cd1;
+    double ext = exsum + exe_i - exe0;
+    U4 un;
+    if (ext > -80.0 && ext < 80.0) {
+      double fact = exp(ext);
+      un.ur = uu1 * fact / f1213;
+      un.uz = uu2 * fact / f1213;
+      un.tz = uu3 * fact / f1213;
+      un.tr = uu4 * fact / f1213;
+    } else {
+      un.ur = un.uz = un.tz = un.tr = 0.0;
+    }
+    if (i1 < jwat) { un.ur = 0.0; un.tr = 0.0; }
+    // energy of layer m (internal)
+    double ka, kb, kr, urf;
+    layer_energy(L, allfluid, 0, uprev, un, omega, om2, wvno, wvno2, c, S, ka, kb, kr, urf);
+    long long o = D.base + (long long)m * D.stride;
+    D.p[0][o] = ka;
+    D.p[1][o] = kb;
+    D.p[2][o] = kr;
+    if (eig) {
+      A.eig[0][ebase + m] = urf;
+      A.eig[1][ebase + m] = uprev.uz;
+      A.eig[2][ebase + m] = uprev.tr;
+      A.eig[3][ebase + m] = uprev.tz;
+    }
+    // getdcdh of layer m+1
+    Layer Ln = load_layer(A, col, i1, true);
+    double xmu_n = Ln.zrho * (Ln.zb * Ln.zb);
+    double xlam_n = Ln.zrho * (Ln.za * Ln.za) - 2 * xmu_n;
+    double g = dcdh_terms(i1, Ln, L, xmu_n, xlam_n, xmu_prev, xlam_prev, un, om2, wvno, wvno2);
+    D.p[3][D.base + (long long)i1 * D.stride] = g;
+    uprev = un;
+    Lprev = Ln;
+    xmu_prev = xmu_n;
+    xlam_prev = xlam_n;
+    (void)Lm1; (void)xmu_m1; (void)xlam_m1;
+  }
+  // half-space energy
+  {
+    double ka, kb, kr, urf;
+    layer_energy(Lprev, allfluid, 1, uprev, uprev, omega, om2, wvno, wvno2, c, S, ka, kb, kr, urf);
+    long long o = D.base + (long long)(N - 1) * D.stride;
+    D.p[0][o] = ka;
+    D.p[1][o] = kb;
+    D.p[2][o] = kr;
+    if (eig) {
+      A.eig[0][ebase + N - 1] = urf;
+      A.eig[1][ebase + N - 1] = uprev.uz;
+      A.eig[2][ebase + N - 1] = uprev.tr;
+      A.eig[3][ebase + N - 1] = uprev.tz;
+    }
+  }
+  ugr = (wvno * S.s1 + S.s2) / (omega * S.s0);
+  double are = wvno / (2.0 * omega * ugr * S.s0);
+  fac = are * c / wvno2;
+  sumi0 = S.s0;
+}
+
+// ------------------------------------------------------------ Love parts
+// slegn96.f90:248-328 varl
+struct VarL {
+  double cosq, sinq, yl, zl, mu, rb, xkb, eexl;
+};
+JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
+  VarL v;
+  v.xkb = omega / L.zb;
+  double wvnop = wvno + v.xkb;
+  double wvnom = fabs(wvno - v.xkb);
+  v.rb = sqrt(wvnop * wvnom);
+  double q = v.rb * dpth;
+  v.mu = L.zrho * L.zb * L.zb;
+  v.eexl = 0.0;
+  v.sinq = 0.0;
+  if (wvno < v.xkb) {
+    sincos(q, &v.sinq, &v.cosq);
+    v.yl = v.sinq / v.rb;
+    v.zl = -v.rb * v.sinq;
+  } else if (wvno == v.xkb) {
+    v.cosq = 1.0;
+    v.yl = dpth;
+    v.zl = 0.0;
+  } else {
+    v.eexl = q;
+    double fac = (q < 18.0) ? exp(-2.0 * q) : 0.0;
+    v.cosq = (1.0 + fac) * 0.5;
+    v.sinq = (1.0 - fac) * 0.5;
+    v.yl = v.sinq / v.rb;
+    v.zl = v.rb * v.sinq;
+  }
+  return v;
+}
+
+JDSG_DEV Layer load_layer_love(const EigenArgs &A, int col, int i, bool pu) {
+  Layer L = load_layer(A, col, i, false);
+  double zb = L.zb;
+  L.iwat = pu ? ((zb > 0.0) ? 0 : 1) : ((zb == 0.0) ? 1 : 0);
+  return L;
+}
+
+// One Love pass (shfunc + energy); scratch holds uu, tt, exl per layer.
+JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double wvno, bool pu,
+                        double *scr, long long sstride, const Dst &D, double &ugr,
+                        double &sumi1, double &fac, bool want_eig, long long ebase) {
+#define UU(i) scr[((long long)(i) * 3 + 0) * sstride]
+#define TT(i) scr[((long long)(i) * 3 + 1) * sstride]
+#define EXL(i) scr[((long long)(i) * 3 + 2) * sstride]
+  // up (slegn96.f90:372-445)
+  {
+    Layer H = load_layer_love(A, col, N - 1, pu);
+    double xmuH = H.zrho * (H.zb * H.zb);
+    if (H.zb > 0.01) {
+      VarL v = varl(H, omega, wvno, 0.0);
+      UU(N - 1) = 1.0;
+      TT(N - 1) = -xmuH * v.rb;
+    } else {
+      UU(N - 1) = 1.0;
+      TT(N - 1) = 0.0;
+    }
+    EXL(N - 1) = 0.0;
+    double u1 = UU(N - 1), t1v = TT(N - 1);
+    for (int k = N - 2; k >= 0; --k) {
+      Layer L = load_layer_love(A, col, k, pu);
+      if (L.iwat != 0) {
+        UU(k) = 0.0;
+        TT(k) = 0.0;
+        EXL(k) = 0.0;
+        continue;
+      }
+      VarL v = varl(L, omega, wvno, L.zd);
+      double a11 = v.cosq, a22 = v.cosq, a12 = -(v.yl / v.mu), a21 = -(v.zl * v.mu);
+      double amp0 = a11 * u1 + a12 * (t1v);
+      double str0 = a21 * u1 + a22 * (t1v);
+      double rr = fabs(amp0), ss = fabs(str0);
+      if (ss > rr) rr = ss;
+      if (rr < 1.0e-30) rr = 1.0;
+      EXL(k) = log(rr) + v.eexl;
+      u1 = amp0 / rr;
+      t1v = str0 / rr;
+      UU(k) = u1;
+      TT(k) = t1v;
+    }
+  }
+  // shfunc normalisation (slegn96.f90:179-246)
+  {
+    double ext = 0.0;
+    double umax = UU(0);
+    TT(0) = 0.0;
+    for (int k = 1; k < N; ++k) {
+      Layer L = load_layer_love(A, col, k, pu);
+      double uk, tk;
+      if (L.iwat == 0) {
+        ext = ext + EXL(k - 1);
+        double fact = (ext < 80.0) ? 1. / exp(ext) : 0.0;
+        uk = UU(k) * fact;
+        tk = TT(k) * fact;
+      } else {
+        uk = 0.0;
+        tk = 0.0;
+      }
+      UU(k) = uk;
+      TT(k) = tk;
+      if (fabs(uk) > fabs(umax)) umax = uk;
+    }
+    if (UU(0) != 0.0) umax = UU(0);
+    if (fabs(umax) > 0.0) {
+      for (int k = 0; k < N; ++k) {
+        Layer L = load_layer_love(A, col, k, pu);
+        if (L.iwat == 0) {
+          UU(k) = UU(k) / umax;
+          TT(k) = TT(k) / umax;
+        }
+      }
+    }
+  }
+  // energy (slegn96.f90:447-629)
+  double c = omega / wvno;
+  double omega2 = omega * omega, wvno2 = wvno * wvno;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int k = 0; k < N; ++k) {
+    Layer L = load_layer_love(A, col, k, pu);
+    long long o = D.base + (long long)k * D.stride;
+    if (L.iwat != 0) {
+      D.p[1][o] = 0.0;
+      D.p[2][o] = 0.0;
+      continue;
+    }
+    double TN = L.zrho * L.zb * L.zb, TL = TN;
+    double VSHH = L.zb, VSHV = L.zb;
+    double drho = L.zrho, dpth = L.zd;
+    VarL v = varl(L, omega, wvno, dpth);
+    double dmu = L.zrho * (L.zb * L.zb);
+    double rb = v.rb;
+    if (rb < 1.0e-10) rb = 1.0e-10;
+    double upup, dupdup;
+    double uk = UU(k);
+    if (k == N - 1) {
+      upup = (0.5 / rb) * uk * uk;
+      dupdup = (0.5 * rb) * uk * uk;
+    } else {
+      cplx nub = (wvno < v.xkb) ? cmk(0.0, rb) : cmk(rb, 0.0);
+      cplx xnub = dmu * nub;
+      cplx ei11 = cmk(0.5 / wvno, 0), ei12 = 0.5 / (wvno * xnub);
+      cplx ei21 = cmk(0.5 / wvno, 0), ei22 = -0.5 / (wvno * xnub);
+      cplx el11 = cmk(wvno, 0), el12 = cmk(wvno, 0);
+      cplx km1dn = ei21 * uk + ei22 * TT(k);
+      cplx kmup = ei11 * UU(k + 1) + ei12 * TT(k + 1);
+      cplx f3 = nub * dpth;
+      cplx exqq = (f3.re < 40.0) ? cexp_(-2.0 * f3) : cmk(0, 0);
+      cplx f = (1.0 - exqq) / (2.0 * nub);
+      exqq = (f3.re < 75.0) ? cexp_(-f3) : cmk(0, 0);
+      cplx g = dpth * exqq;
+      cplx f1 = f * (el11 * el11 * kmup * kmup + el12 * el12 * km1dn * km1dn);
+      cplx f2 = g * (el11 * el12 + el11 * el12) * kmup * km1dn;
+      upup = (f1 + f2).re;
+      dupdup = (nub * nub * (f1 - f2)).re;
+    }
+    s0 = s0 + drho * upup;
+    s1 = s1 + TN * upup;
+    s2 = s2 + TL * dupdup;
+    double DCDBH = c * drho * VSHH * upup;
+    double DCDBV = c * drho * VSHV * dupdup / wvno2;
+    D.p[1][o] = DCDBH + DCDBV;
+    D.p[2][o] = 0.5 * c * (-c * c * upup + VSHH * VSHH * upup + VSHV * VSHV * dupdup / wvno2);
+  }
+  ugr = s1 / (c * s0);
+  double ale = 0.5 / s1;
+  fac = ale * c / wvno2;
+  sumi1 = s1;
+  // dcdh (slegn96.f90:585-610), final value (fac known)
+  int llflag = 0;
+  double zr_prev = 0.0, xmu_prev = 0.0;
+  for (int k = 0; k < N; ++k) {
+    Layer L = load_layer_love(A, col, k, pu);
+    long long o = D.base + (long long)k * D.stride;
+    double xmu = L.zrho * (L.zb * L.zb);
+    if (L.iwat == 0) {
+      double drho, dmu, dvdz;
+      double tk = TT(k), uk = UU(k);
+      if (llflag == 0) {
+        drho = L.zrho;
+        dmu = xmu;
+        dvdz = 0.0;
+      } else {
+        drho = L.zrho - zr_prev;
+        dmu = xmu - xmu_prev;
+        dvdz = tk * tk * (1.0 / xmu - 1.0 / xmu_prev);
+      }
+      double dfac = fac * (uk * uk * (omega2 * drho - wvno2 * dmu) + dvdz);
+      D.p[3][o] = (fabs(dfac) < 1.0e-38) ? 0.0 : dfac;
+      llflag = llflag + 1;
+    } else {
+      D.p[3][o] = 0.0;
+    }
+    zr_prev = L.zrho;
+    xmu_prev = xmu;
+    if (want_eig) {
+      A.eig[0][ebase + k] = UU(k);
+      A.eig[1][ebase + k] = TT(k);
+    }
+  }
+#undef UU
+#undef TT
+#undef EXL
+}
+
+JDSG_DEV void zero_entry(const EigenArgs &A, long long e, long long kb, long long ls, int N,
+                         int st) {
+  A.c_out[e] = 0.0;
+  A.u_out[e] = 0.0;
+  A.status[e] = st;
+  for (int a = 0; a < 4; ++a) {
+    if (A.dc[a])
+      for (int i = 0; i < N; ++i) A.dc[a][kb + i * ls] = 0.0;
+    if (A.du[a])
+      for (int i = 0; i < N; ++i) A.du[a][kb + i * ls] = 0.0;
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(64)
+jdsg_eigen(EigenArgs A) {
+  const long long nlanes = (long long)gridDim.x * blockDim.x;
+  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double *scr = A.scr + lane;
+  const long long ss = nlanes;  // scratch slot stride
+  // scratch map (slots): [0, 6*nmax) sweep storage, [6*nmax, 10*nmax) raw kernels
+  long long it = lane;
+  const bool ray = (A.wave == 0);
+  while (it < A.nitem) {
+    int col, mode, k;
+    long long e;
+    decode_item(A, it, col, mode, k, e);
+    int N = A.cm.nlay[col];
+    int K = A.per_off[col + 1] - A.per_off[col];
+    long long el = e - A.ent_off[col];
+    long long kb, ls;
+    if (A.layout == 0) {
+      kb = col + el * (long long)N * A.cm.ncol;
+      ls = A.cm.ncol;
+    } else {
+      kb = A.kern_off[col] + el * N;
+      ls = 1;
+    }
+    (void)K;
+    double c0 = A.cflat[e];
+    int st = A.cstat[e];
+    double c1 = 0.0, c2 = 0.0;
+    if (A.group) {
+      c1 = A.cflat[A.nent + e];
+      c2 = A.cflat[2 * A.nent + e];
+      st |= A.cstat[A.nent + e] | A.cstat[2 * A.nent + e];
+    }
+    int status = 0;
+    if (st & JDSG_EXIT_PATH) status = JDSG_EXIT_PATH;
+    else if (c0 == 0.0) status = JDSG_NO_ROOT;
+    else if (A.group && (c1 == 0.0 || c2 == 0.0)) status = JDSG_NO_ROOT_PM;
+    if (status) {
+      zero_entry(A, e, kb, ls, N, status);
+      it = nlanes + (long long)atomicAdd(A.counter, 1u);
+      continue;
+    }
+    double T = A.periods[A.per_off[col] + k];
+    double twopi = 2.0 * kPiF;
+    // raw kernel sets v = 0,1,2: slots [(6 + 4v + a) * nmax, ...)
+    double *draw = scr + (long long)6 * A.nmax * ss;
+    Dst Dn;  // non-group destination = outputs
+    double u = 0.0, cout = 0.0;
+    const double ar = 6371.0;
+    long long ebase = (A.eig[0] && A.kern_off) ? A.kern_off[col] + el * N : 0;
+    if (!A.group) {
+      // ---------------- sregn96_/slegn96_ form
+      double cc = c0;
+      double omega = twopi / T;
+      double wvno = omega / cc;
+      for (int a = 0; a < 4; ++a) Dn.p[a] = draw + (long long)a * A.nmax * ss;
+      Dn.base = 0;
+      Dn.stride = ss;
+      double ugr, s0, fac;
+      if (ray) {
+        rayleigh_pass(A, col, N, omega, wvno, scr, ss, Dn, ugr, s0, fac,
+                      A.eig[0] ? A.eig[0] : nullptr, ebase);
+        if (fabs(cc) < 1.0e-36) cc = 0.0;
+        if (fabs(ugr) < 1.0e-36) ugr = 0.0;
+      } else {
+        love_pass(A, col, N, omega, wvno, false, scr, ss, Dn, ugr, s0, fac, A.eig[0] != nullptr,
+                  ebase);
+      }
+      double tm = 1.0, tm3 = 1.0;
+      if (A.iflsph > 0) {
+        double r = ray ? cc / (2. * ar * omega) : 3.0 * cc / (2. * ar * omega);
+        tm = sqrt(1. + r * r);
+        tm3 = tm * tm * tm;
+        cout = cc / tm;
+        u = ugr * tm;
+      } else {
+        cout = cc;
+        u = ugr;
+      }
+      // finalise kernels bottom-up with the thickness suffix sums
+      double hsum = 0.0;
+      for (int i = N - 1; i >= 0; --i) {
+        size_t q = (size_t)i * A.cm.ncol_pad + col;
+        double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
+        double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
+        long long o = (long long)i * ss;
+        double ka = 0.0, kbv, kr, kh;
+        if (ray) {
+          double us0 = ugr * s0;
+          ka = Dn.p[0][o] / us0;
+          kbv = Dn.p[1][o] / us0;
+          kr = Dn.p[2][o] / us0;
+          kh = fac * Dn.p[3][o];
+          if (fabs(kh) < 1.0e-38) kh = 0.0;
+        } else {
+          // Love: dcdb, dcdr divided by sumi1, dcdh final
+          Layer L = load_layer_love(A, col, i, false);
+          kbv = (L.iwat == 0) ? Dn.p[1][o] / s0 : 0.0;
+          kr = (L.iwat == 0) ? Dn.p[2][o] / s0 : 0.0;
+          kh = Dn.p[3][o];
+        }
+        if (A.iflsph > 0) {
+          ka = ka * vtp / tm3;
+          kbv = kbv * vtp / tm3;
+          kh = kh * dtp / tm3;
+          kr = kr * rtp / tm3;
+        }
+        long long oo = kb + (long long)i * ls;
+        if (A.dc[0]) A.dc[0][oo] = ka;
+        if (A.dc[1]) A.dc[1][oo] = kbv;
+        if (A.dc[2]) A.dc[2][oo] = kr;
+        if (A.dc[3]) A.dc[3][oo] = hsum;
+        hsum = hsum + kh;
+      }
+    } else {
+      // ---------------- sregnpu_/slegnpu_ form
+      double tt[3] = {T, T * (1.0 + 0.5 * 0.01), T * (1.0 - 0.5 * 0.01)};
+      if (A.tv) {
+        tt[0] = A.tv[3 * e];
+        tt[1] = A.tv[3 * e + 1];
+        tt[2] = A.tv[3 * e + 2];
+      }
+      double cv[3] = {c0, c1, c2};
+      double ugr0 = 0.0, s00 = 0.0;
+      double usc[3];  // per-pass kernel scale: us0 (Rayleigh) or sumi1 (Love)
+      double facv[3];
+      // passes T, T1, T2 leave raw kernels in scratch sets 0, 1, 2
+      for (int v = 0; v < 3; ++v) {
+        double omega = twopi / tt[v];
+        double wvno = omega / cv[v];
+        Dst D;
+        for (int a = 0; a < 4; ++a) D.p[a] = draw + (long long)(4 * v + a) * A.nmax * ss;
+        D.base = 0;
+        D.stride = ss;
+        double ugr, s0, fac;
+        if (ray)
+          rayleigh_pass(A, col, N, omega, wvno, scr, ss, D, ugr, s0, fac,
+                        (v == 0 && A.eig[0]) ? A.eig[0] : nullptr, ebase);
+        else
+          love_pass(A, col, N, omega, wvno, true, scr, ss, D, ugr, s0, fac,
+                    v == 0 && A.eig[0] != nullptr, ebase);
+        usc[v] = ray ? ugr * s0 : s0;
+        facv[v] = fac;
+        if (v == 0) { ugr0 = ugr; s00 = s0; }
+      }
+      (void)s00;
+      double cp = c0, cg = ugr0;
+      double uc1 = cg / cp;
+      double dtv = tt[2] - tt[1];
+      double tm = 1.0, tm1 = 0.0, tm3 = 1.0;
+      if (A.iflsph > 0) {
+        double omega = twopi / T;
+        double r = ray ? cp / (2. * ar * omega) : 3.0 * cp / (2. * ar * omega);
+        tm = sqrt(1. + r * r);
+        double q = (ray ? 0.5 : 1.5) / (ar * omega);
+        tm1 = q * q / tm;
+        tm3 = tm * tm * tm;
+      }
+      double hsc = 0.0, hsu = 0.0;
+      for (int i = N - 1; i >= 0; --i) {
+        size_t q = (size_t)i * A.cm.ncol_pad + col;
+        double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
+        double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
+        long long oo = kb + (long long)i * ls;
+        long long os = (long long)i * ss;
+        bool solid = true;
+        if (!ray) solid = (A.cm.zb[q] > 0.0);
+        double d0[4], d1[4], d2[4];
+        for (int a = 0; a < 4; ++a) {
+          double r0 = draw[(long long)(0 + a) * A.nmax * ss + os];
+          double r1 = draw[(long long)(4 + a) * A.nmax * ss + os];
+          double r2 = draw[(long long)(8 + a) * A.nmax * ss + os];
+          if (a == 3) {
+            if (ray) {
+              d0[a] = facv[0] * r0; if (fabs(d0[a]) < 1.0e-38) d0[a] = 0.0;
+              d1[a] = facv[1] * r1; if (fabs(d1[a]) < 1.0e-38) d1[a] = 0.0;
+              d2[a] = facv[2] * r2; if (fabs(d2[a]) < 1.0e-38) d2[a] = 0.0;
+            } else {
+              d0[a] = r0; d1[a] = r1; d2[a] = r2;
+            }
+          } else if (ray) {
+            d0[a] = r0 / usc[0];
+            d1[a] = r1 / usc[1];
+            d2[a] = r2 / usc[2];
+          } else {
+            d0[a] = (a != 0 && solid) ? r0 / usc[0] : 0.0;
+            d1[a] = (a != 0 && solid) ? r1 / usc[1] : 0.0;
+            d2[a] = (a != 0 && solid) ? r2 / usc[2] : 0.0;
+          }
+        }
+        double fct[4] = {vtp, vtp, rtp, dtp};
+        for (int a = 0; a < 4; ++a) {
+          double du = uc1 * (2.0 - uc1) * d2[a] - uc1 * uc1 * T * (d2[a] - d1[a]) / dtv;
+          double dc = d0[a];
+          if (A.iflsph > 0) {
+            du = (tm * du + cg * cp * dc * tm1) * fct[a];
+            dc = dc / tm3 * fct[a];
+          }
+          if (!ray && a == 0) { du = 0.0; dc = 0.0; }
+          if (a < 3) {
+            if (A.dc[a]) A.dc[a][oo] = dc;
+            if (A.du[a]) A.du[a][oo] = du;
+          } else {
+            if (A.dc[3]) A.dc[3][oo] = hsc;
+            if (A.du[3]) A.du[3][oo] = hsu;
+            hsc = hsc + dc;
+            hsu = hsu + du;
+          }
+        }
+      }
+      if (A.iflsph > 0) {
+        cout = cp / tm;
+        u = cg * tm;
+      } else {
+        cout = cp;
+        u = cg;
+      }
+    }
+    A.c_out[e] = cout;
+    A.u_out[e] = u;
+    A.status[e] = 0;
+    it = nlanes + (long long)atomicAdd(A.counter, 1u);
+  }
+}
+
+void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st) {
+  hipLaunchKernelGGL(jdsg_eigen, dim3(nwaves), dim3(64), 0, st, A);
+}

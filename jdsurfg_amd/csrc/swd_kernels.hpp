@@ -1,0 +1,68 @@
+// swd_kernels.hpp -- argument blocks and host launchers of the dispersion /
+// kernel path (prep, root chains, eigen stage).
+#pragma once
+#include "swd_common.hpp"
+
+struct PrepOut {
+  float4 *chainR, *chainL;
+  float *cc, *betmx;
+  int *llw, *nlay;
+  double *zd, *za, *zb, *rho0, *rtpR, *rtpL, *vtp, *dtp;
+};
+
+struct ChainArgs {
+  ColumnModel cm;
+  const long long *ent_off;   // [ncol+1]
+  const int *per_off;         // [ncol+1]
+  const double *periods;
+  const int *nmode;           // [2*ncol] (R, L)
+  int nvar;                   // 1 (phase only) or 3 (T, T*1.005, T*0.995)
+  int igr;                    // surfdisp96 FD group velocity (B1 ABI)
+  int retry;                  // surfdisp.cpp:95-109 retry of missing roots
+  long long nent;             // entries per variant
+  int kmax_pad;               // scratch stride for c/cb per chain
+  double *cflat;              // [nvar * nent] float-rounded flat c (or FD U)
+  int *cstat;                 // [nvar * nent] JDSG_EXIT_PATH bits
+  int *ierr_out;              // [nchain] or null
+  double *cscr;               // [nchain_slots * 2 * kmax_pad]
+  unsigned *counter;          // work counter
+  unsigned long long *nevals; // period-equation evaluations (flop model)
+  long long nchain;           // ncol * 2 * nvar
+};
+
+struct EigenArgs {
+  ColumnModel cm;
+  const long long *ent_off;  // [ncol+1]
+  const int *per_off;
+  const double *periods;
+  const int *nmode;          // [2*ncol]
+  int wave;                  // 0 Rayleigh, 1 Love (one launch per wave)
+  int group;                 // 1: sregnpu/slegnpu form
+  int iflsph;
+  long long nent;            // entries per variant in cflat/cstat
+  const double *cflat;       // [nvar * nent]
+  const int *cstat;
+  // item enumeration: items [0, nitem) map to entries of this wave
+  long long nitem;
+  const long long *item_ent;  // [nitem] entry index, or null: grid order (see decode_item)
+  int grid_K, grid_nR, grid_nL;  // grid order: item = ((mode*K)+k)*ncol + col
+  // outputs
+  double *c_out, *u_out;
+  int *status;
+  double *dc[4], *du[4];     // a, b, rho, h; null allowed
+  int layout;                // 0: grid (base = col + elocal*N*ncol, lstride ncol)
+                             // 1: csr  (base = kern_off[col] + elocal*N, lstride 1)
+  const long long *kern_off;
+  double *eig[4];            // optional eigenfunctions (ur,uz,tr,tz) / (uu,tt), csr
+  double *scr;               // lane scratch, [slot * nlanes + lane]
+  int nmax;
+  unsigned *counter;
+  const double *tv;          // optional [3*nent] explicit (T, T1, T2) per entry
+};
+
+void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float *thk,
+                      const float *vp, const float *vs, const float *rho, int iflsph,
+                      const PrepOut &o, hipStream_t st);
+void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st);
+void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st);
+void jdsg_launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t st);

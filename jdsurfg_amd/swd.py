@@ -1,0 +1,171 @@
+"""Host-side interface to libjdsg_swd.so (the HIP dispersion/kernel library).
+
+PyTorch provides device memory and the stream; every computation runs in the
+library's HIP kernels.  If the library is missing this module raises -- there
+is no CPU fallback.
+
+Main objects
+------------
+Plan     -- a layer-stack batch (the per-column call sequence of the reference's
+            kernel1D, SurfaceWave.cpp:266-412, for many columns at once);
+solve()  -- convenience: plan + execute for a Workload (jdsurfg_amd.workloads).
+Per-column functions mirroring src/SWD/swd.hpp (surfdisp96, sregn96, ...)
+are exported by the library itself under the reference's symbol names.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libjdsg_swd.so")
+
+_lib = None
+
+
+def lib():
+    """Load libjdsg_swd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                           "(python jdsurfg_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.jdsg_last_error.restype = C.c_char_p
+    L.jdsg_plan_create.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(vp)]
+    L.jdsg_plan_sizes.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.jdsg_plan_offsets.argtypes = [vp, vp, vp]
+    L.jdsg_plan_execute.argtypes = [vp] * 17 + [vp]
+    L.jdsg_plan_set_waves.argtypes = [vp, C.c_int, C.c_int]
+    L.jdsg_plan_destroy.argtypes = [vp]
+    L.jdsg_plan_timings.argtypes = [vp, vp]
+    L.jdsg_plan_eval_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.jdsg_surfdisp.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int,
+                                C.c_int, C.c_int]
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().jdsg_last_error().decode()
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+KERNEL_NAMES = ("dcda", "dcdb", "dcdr", "dcdh", "duda", "dudb", "dudr", "dudh")
+
+
+class Plan:
+    """A planned layer-stack batch.
+
+    layer_off, per_off: CSR offsets (ncol+1); nmode: (ncol, 2) Rayleigh/Love
+    mode counts; group: U kernels (three root chains per wave) or phase
+    kernels only; layout: "grid" (column-fastest, uniform batches) or "csr".
+    """
+
+    def __init__(self, layer_off, per_off, nmode, group=True, layout="grid", iflsph=1):
+        L = lib()
+        self.layer_off = np.ascontiguousarray(layer_off, np.int32)
+        self.per_off = np.ascontiguousarray(per_off, np.int32)
+        self.nmode = np.ascontiguousarray(np.asarray(nmode, np.int32).reshape(-1))
+        self.ncol = len(self.layer_off) - 1
+        self.group = bool(group)
+        self.layout = {"grid": 0, "csr": 1}[layout]
+        h = C.c_void_p()
+        rc = L.jdsg_plan_create(self.ncol, self.layer_off.ctypes.data, self.per_off.ctypes.data,
+                                self.nmode.ctypes.data, int(group), self.layout, int(iflsph),
+                                C.byref(h))
+        if rc:
+            raise RuntimeError("jdsg_plan_create: " + last_error())
+        self.h = h
+        ne, nk = C.c_longlong(), C.c_longlong()
+        nmax, kmax = C.c_int(), C.c_int()
+        L.jdsg_plan_sizes(h, C.byref(ne), C.byref(nk), C.byref(nmax), C.byref(kmax))
+        self.nent, self.nkern, self.nmax, self.kmax = ne.value, nk.value, nmax.value, kmax.value
+        self.ent_off = np.zeros(self.ncol + 1, np.int64)
+        self.kern_off = np.zeros(self.ncol + 1, np.int64)
+        L.jdsg_plan_offsets(h, self.ent_off.ctypes.data, self.kern_off.ctypes.data)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.jdsg_plan_destroy(self.h)
+            self.h = None
+
+    def set_waves(self, chain_waves=0, eigen_waves=0):
+        if lib().jdsg_plan_set_waves(self.h, int(chain_waves), int(eigen_waves)):
+            raise RuntimeError(last_error())
+
+    def alloc_outputs(self, kernels=KERNEL_NAMES, device="cuda"):
+        import torch
+        out = {"c": torch.empty(self.nent, dtype=torch.float64, device=device),
+               "u": torch.empty(self.nent, dtype=torch.float64, device=device),
+               "status": torch.empty(self.nent, dtype=torch.int32, device=device)}
+        for n in kernels:
+            out[n] = torch.empty(self.nkern, dtype=torch.float64, device=device)
+        return out
+
+    def execute(self, thk, vp, vs, rho, periods, out, stream=None):
+        """Launch the solve.  Inputs/outputs are CUDA tensors (float32 model,
+        float64 periods); out comes from alloc_outputs().  Asynchronous."""
+        import torch
+        for t, dt in ((thk, torch.float32), (vp, torch.float32), (vs, torch.float32),
+                      (rho, torch.float32), (periods, torch.float64)):
+            if not t.is_cuda or t.dtype != dt or not t.is_contiguous():
+                raise ValueError("inputs must be contiguous CUDA tensors of the documented dtype")
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        ks = [_ptr(out.get(n)) for n in KERNEL_NAMES]
+        rc = lib().jdsg_plan_execute(self.h, _ptr(thk), _ptr(vp), _ptr(vs), _ptr(rho),
+                                     _ptr(periods), _ptr(out["c"]), _ptr(out["u"]),
+                                     _ptr(out["status"]), *ks, C.c_void_p(stream.cuda_stream))
+        if rc:
+            raise RuntimeError("jdsg_plan_execute: " + last_error())
+        return out
+
+    def eval_count(self):
+        v = C.c_longlong()
+        if lib().jdsg_plan_eval_count(self.h, C.byref(v)):
+            raise RuntimeError(last_error())
+        return v.value
+
+    def timings(self):
+        """Per-stage milliseconds of the last execute (prep, chains, eigen R,
+        eigen L) when JDSG_TIMING=1 was set before the plan was created."""
+        ms = np.zeros(4, np.float32)
+        if lib().jdsg_plan_timings(self.h, ms.ctypes.data):
+            return None
+        return ms
+
+
+def upload(w, device="cuda"):
+    import torch
+    f = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dt)
+    return (f(w.thk, torch.float32), f(w.vp, torch.float32), f(w.vs, torch.float32),
+            f(w.rho, torch.float32), f(w.periods, torch.float64))
+
+
+def solve(w, layout=None, kernels=KERNEL_NAMES, iflsph=1):
+    """Plan + execute a Workload; returns (plan, dict of CUDA tensors)."""
+    import torch
+    uniform = w.grid is not None
+    if layout is None:
+        layout = "grid" if uniform else "csr"
+    p = Plan(w.layer_off, w.per_off, w.nmode, w.group, layout, iflsph)
+    inp = upload(w)
+    out = p.alloc_outputs(kernels)
+    p.execute(*inp, out)
+    torch.cuda.synchronize()
+    return p, out
+
+
+def grid_to_csr(plan, arr, N):
+    """Reorder a grid-layout kernel array (uniform batch) into csr order."""
+    ncol = plan.ncol
+    a = np.asarray(arr).reshape(-1, N, ncol)     # (entry_local, layer, col)
+    return np.ascontiguousarray(a.transpose(2, 0, 1)).reshape(-1)

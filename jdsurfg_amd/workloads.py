@@ -13,8 +13,9 @@ Named workloads (sizes as the benchmark configurations state them):
   grid128   128x128 columns x 48 x 1.5 km, same field plus a random
             sediment top (0-4 km thick, Vs 0.8-2.0 km/s), 40 periods 1-60 s,
             Rayleigh modes 0-2, Love modes 0-1.
-  uneven    50,000 columns, 16-128 layers of 0.5-4 km, per-column sorted
-            period sets of 8-64 periods in 1-80 s, 1-3 modes.
+  uneven    50,000 columns, 16-128 layers of 0.5-4 km (Vs trend capped at
+            4.6 km/s, the range of the Brocher fit), per-column sorted period
+            sets of 8-64 periods in 1-80 s, 1-3 modes.
   grid256   256x256 columns x 64 x 1 km, depth trend with a +-8 % Vs
             checkerboard of 40 km cells, 60 periods 1-80 s, Rayleigh modes
             0-3, Love modes 0-1.
@@ -189,7 +190,7 @@ def uneven(seed=50000, ncol=50000, group=True):
         h = rng.uniform(0.5, 4.0, n)
         ztop = np.concatenate([[0.0], np.cumsum(h)])
         zmid = np.concatenate([ztop[:-1] + 0.5 * h, [ztop[-1]]])
-        v = _trend(zmid) + rng.normal(0.0, 0.15, n + 1) * 0.5
+        v = np.clip(np.minimum(_trend(zmid), 4.6) + rng.normal(0.0, 0.075, n + 1), 1.0, 4.8)
         s = slice(layer_off[c], layer_off[c + 1])
         thk[s] = np.concatenate([h, [0.0]])
         vs[s] = v

@@ -1,0 +1,61 @@
+"""Shared comparison helpers for parity tests (GPU path vs the CPU oracle)."""
+import numpy as np
+
+KERNELS = ("dcda", "dcdb", "dcdr", "dcdh", "duda", "dudb", "dudr", "dudh")
+
+
+def oracle_solve(oracle, w, nthreads=4):
+    return oracle.batch_solve(w.layer_off, w.thk, w.vp, w.vs, w.rho, w.per_off, w.periods,
+                              w.nmode.ravel(), w.group, nthreads=nthreads)
+
+
+def gpu_solve(w, layout=None):
+    """Run the HIP path; returns numpy dict in csr order (like the oracle)."""
+    import torch
+    from jdsurfg_amd import swd
+    plan, out = swd.solve(w, layout=layout)
+    res = {k: v.cpu().numpy() for k, v in out.items()}
+    if plan.layout == 0:
+        N = int(w.layer_off[1] - w.layer_off[0])
+        for k in KERNELS:
+            res[k] = swd.grid_to_csr(plan, res[k], N)
+    res["ent_off"] = plan.ent_off
+    res["kern_off"] = plan.kern_off
+    return res
+
+
+def kernel_errors(a, b, w, names=KERNELS):
+    """max over entries of max|a-b| / max|b| per entry (entries with status 0)."""
+    ok = (a["status"] == 0) & (b["status"] == 0)
+    N = np.diff(w.layer_off)
+    out = {}
+    for n in names:
+        worst = 0.0
+        x, y = a[n], b[n]
+        for c in range(w.ncol):
+            e0, e1 = b["ent_off"][c], b["ent_off"][c + 1]
+            nn = N[c]
+            for e in range(e0, e1):
+                if not ok[e]:
+                    continue
+                q = b["kern_off"][c] + (e - e0) * nn
+                ys = y[q:q + nn]
+                m = np.abs(ys).max()
+                if m > 0:
+                    worst = max(worst, float(np.abs(x[q:q + nn] - ys).max() / m))
+        out[n] = worst
+    return out
+
+
+def scalar_errors(a, b):
+    ok = (a["status"] == 0) & (b["status"] == 0)
+    res = {}
+    for n in ("c", "u"):
+        if ok.any():
+            r = np.abs(a[n][ok] - b[n][ok]) / np.abs(b[n][ok])
+            res[n] = float(r.max())
+            res[n + "_exact"] = float(np.mean(a[n][ok] == b[n][ok]))
+        else:
+            res[n] = 0.0
+            res[n + "_exact"] = 1.0
+    return res
