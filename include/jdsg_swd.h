@@ -75,7 +75,9 @@ int jdsg_plan_sizes(const jdsg_plan *plan, long long *nent, long long *nkern, in
 int jdsg_plan_offsets(const jdsg_plan *plan, long long *ent_off, long long *kern_off);
 /* Launch the solve on `stream` (a hipStream_t, may be null).  Every pointer
  * is device memory; kernel outputs may be null to skip them.  status per
- * entry: 0 ok, 1 no root, 2 no root at T(1+-0.005), 4 reference exit(1) path.
+ * entry: 0 ok; 1 no root at T (outputs zero); 2 no root at T(1+-0.005)
+ * (c, U, dc/dm stand, dU/dm zero); 4 the reference's exit(1) path in the T
+ * chain (outputs zero); 8 the exit path in a T(1+-0.005) chain (as 2).
  * Allocates nothing and does not synchronise, so it may be graph-captured. */
 int jdsg_plan_execute(jdsg_plan *plan, const float *thk, const float *vp, const float *vs,
                       const float *rho, const double *periods, double *c, double *u,
@@ -85,12 +87,49 @@ int jdsg_plan_execute(jdsg_plan *plan, const float *thk, const float *vp, const 
  * the default). */
 int jdsg_plan_set_waves(jdsg_plan *plan, int chain_waves, int eigen_waves);
 void jdsg_plan_destroy(jdsg_plan *plan);
+/* Stop rule of the Neville polish, |c1-c2| <= tol*c1.  The default 1e-6 is
+ * the reference's (surfdisp96.f:634) and returns float-rounded roots like it;
+ * tol < 1e-6 is the exact-root mode (roots kept in double). */
+int jdsg_plan_set_tolerance(jdsg_plan *plan, double tol);
 /* Number of period-equation evaluations the last execute performed
  * (synchronises; for the flop model). */
 int jdsg_plan_eval_count(const jdsg_plan *plan, long long *nevals);
 /* Per-stage milliseconds (prep, chains, eigen Rayleigh, eigen Love) of the
  * last execute, when JDSG_TIMING=1 was set at plan creation. */
 int jdsg_plan_timings(const jdsg_plan *plan, float *ms4);
+
+/* SurfTime grid form (replaces SurfTime::get_1d_kernel, SurfaceWave.hpp:103,
+ * SurfaceWave.cpp:458-502; depth grid and period tables as set_model /
+ * read_swd_data give them, SurfaceWave.hpp:114-115).
+ *   dep[nz]      node depths (km);
+ *   water        1 if vs(:,:,0) == 0 for every column (water top), else 0;
+ *   t{Rc,Rg,Lc,Lg}  column-major (nt x nmode) period tables padded with -1
+ *                (SurfaceWave.cpp:157-169); nmode 0 or a null table = absent.
+ * Output index order is SurfaceWave.cpp:14-48 get_period_index: Rc modes,
+ * Rg modes, Lc modes, Lg modes, each mode's periods contiguous. */
+typedef struct jdsg_surftime jdsg_surftime;
+int jdsg_surftime_create(int nx, int ny, int nz, const float *dep, int water,
+                         const double *tRc, int ntRc, int nmRc, const double *tRg, int ntRg,
+                         int nmRg, const double *tLc, int ntLc, int nmLc, const double *tLg,
+                         int ntLg, int nmLg, jdsg_surftime **st);
+int jdsg_surftime_kmax(const jdsg_surftime *st, int *kmax);
+/* Device pointers: vs (nx,ny,nz) column-major float; vc, vout, status
+ * (nx*ny, kmax) and kernel (nx*ny, nz, kmax) column-major (kernel may be
+ * null: the get_2d_map-style outputs only).  Rc/Lc entries hold the
+ * spherical phase velocity in vc and vout and dc/dVs in kernel; Rg/Lg
+ * entries hold c in vc, U in vout and dU/dVs in kernel (SurfaceWave.cpp
+ * :296-405, chain rule :492-500).  status as for plans; entries with a
+ * nonzero status hold zeros where the reference would compute from c = 0.
+ * Synchronises `stream` at the end (water-flag check). */
+int jdsg_surftime_kernel(jdsg_surftime *st, const float *vs, double *vc, double *vout,
+                         double *kernel, int *status, void *stream);
+void jdsg_surftime_destroy(jdsg_surftime *st);
+/* Host-array convenience form of the above (allocates, copies, solves). */
+int jdsg_get_1d_kernel(int nx, int ny, int nz, const float *vs, const float *dep,
+                       const double *tRc, int ntRc, int nmRc, const double *tRg, int ntRg,
+                       int nmRg, const double *tLc, int ntLc, int nmLc, const double *tLg,
+                       int ntLg, int nmLg, double *vc, double *vout, double *kernel,
+                       int *status);
 
 const char *jdsg_last_error(void);
 

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -56,6 +58,7 @@ struct jdsg_plan {
   unsigned *counters = nullptr;
   unsigned long long *nevals = nullptr;
   int chain_waves = 0, eigen_waves = 0;
+  double tol = 1.0e-6;
   // explicit per-entry periods for the eigen stage (per-column ABI only)
   double *tv = nullptr;
   double *eig[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -254,6 +257,12 @@ extern "C" int jdsg_plan_set_waves(jdsg_plan *P, int cw, int ew) {
 
 extern "C" void jdsg_plan_destroy(jdsg_plan *P) { delete P; }
 
+extern "C" int jdsg_plan_set_tolerance(jdsg_plan *P, double tol) {
+  if (!(tol > 0.0) || tol > 1.0e-2) return fail("tolerance must be in (0, 1e-2]");
+  P->tol = tol;
+  return 0;
+}
+
 extern "C" int jdsg_plan_eval_count(const jdsg_plan *P, long long *out) {
   unsigned long long v = 0;
   HIPCHK(hipMemcpy(&v, P->nevals, sizeof(v), hipMemcpyDeviceToHost));
@@ -296,6 +305,7 @@ int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
   A.counter = P->counters + 0;
   A.nchain = P->nchain;
   A.nevals = P->nevals;
+  A.tol = P->tol;
   jdsg_launch_chains(A, P->chain_waves, st);
   HIPCHK(hipGetLastError());
   if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
@@ -572,4 +582,284 @@ extern "C" void slegnpu_(float *thk, float *vs, float *rhom, int nlayer, const d
   double tv[3] = {*t, *t1, *t2}, cv[3] = {*cp, *cp1, *cp2};
   if (column_eigen(1, thk, nullptr, vs, rhom, nlayer, iflsph, tv, cv, 3, cp, cg, eig, dc, du))
     report("slegnpu_");
+}
+
+// ============================================================================
+// SurfTime grid entry point (SurfaceWave.cpp:458-502 get_1d_kernel).
+struct jdsg_surftime {
+  int nx = 0, ny = 0, nz = 0, nxy = 0, water = 0, N = 0, kmax = 0, chunk = 0;
+  struct Job {
+    int wave, group, M;
+    std::vector<double> list;
+  };
+  std::vector<Job> jobs;
+  std::vector<StEntry> ents;
+  struct Set {
+    int ncol = 0;
+    std::vector<jdsg_plan *> plans;
+    std::vector<void *> allocs;
+    float *thk = nullptr, *vp = nullptr, *vs = nullptr, *rho = nullptr;
+    std::vector<double *> periods;
+    std::vector<StJob> hjobs;
+    std::vector<double *> out_c, out_u;
+    std::vector<int *> out_st;
+    std::vector<std::array<double *, 6>> out_k;
+    StJob *djobs = nullptr;
+    ~Set() {
+      for (auto *p : plans) jdsg_plan_destroy(p);
+      for (void *q : allocs) (void)hipFree(q);
+    }
+    template <class T>
+    int dalloc(T **p, size_t n) {
+      void *q = nullptr;
+      if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+        return fail("hipMalloc (surftime)");
+      allocs.push_back(q);
+      *p = (T *)q;
+      return 0;
+    }
+  };
+  Set sets[2];
+  std::vector<void *> allocs;
+  float *dep_d = nullptr, *node_vp = nullptr, *node_vs = nullptr;
+  int *err_d = nullptr;
+  StEntry *ents_d = nullptr;
+  ~jdsg_surftime() {
+    for (void *q : allocs) (void)hipFree(q);
+  }
+};
+
+namespace {
+
+int build_set(jdsg_surftime *S, jdsg_surftime::Set &T, int ncol) {
+  T.ncol = ncol;
+  size_t nl = (size_t)ncol * S->N;
+  if (T.dalloc(&T.thk, nl) || T.dalloc(&T.vp, nl) || T.dalloc(&T.vs, nl) || T.dalloc(&T.rho, nl))
+    return 1;
+  std::vector<int> lo(ncol + 1);
+  for (int c = 0; c <= ncol; ++c) lo[c] = c * S->N;
+  for (auto &J : S->jobs) {
+    int K = (int)J.list.size();
+    std::vector<int> po(ncol + 1), nm(2 * (size_t)ncol);
+    for (int c = 0; c <= ncol; ++c) po[c] = c * K;
+    for (int c = 0; c < ncol; ++c) {
+      nm[2 * c] = J.wave == 0 ? J.M : 0;
+      nm[2 * c + 1] = J.wave == 1 ? J.M : 0;
+    }
+    jdsg_plan *P = nullptr;
+    if (jdsg_plan_create(ncol, lo.data(), po.data(), nm.data(), J.group, 0, 1, &P)) return 1;
+    T.plans.push_back(P);
+    std::vector<double> per((size_t)ncol * K);
+    for (int c = 0; c < ncol; ++c)
+      std::copy(J.list.begin(), J.list.end(), per.begin() + (size_t)c * K);
+    double *pd = nullptr;
+    if (T.dalloc(&pd, per.size())) return 1;
+    if (hipMemcpy(pd, per.data(), sizeof(double) * per.size(), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return fail("surftime period upload");
+    T.periods.push_back(pd);
+    double *c = nullptr, *u = nullptr;
+    int *st = nullptr;
+    if (T.dalloc(&c, P->nent) || T.dalloc(&u, P->nent) || T.dalloc(&st, P->nent)) return 1;
+    std::array<double *, 6> k{};
+    int nk = J.group ? 6 : 3;
+    for (int a = 0; a < nk; ++a)
+      if (T.dalloc(&k[a], P->nkern)) return 1;
+    T.out_c.push_back(c);
+    T.out_u.push_back(u);
+    T.out_st.push_back(st);
+    T.out_k.push_back(k);
+    StJob h{};
+    h.c = c;
+    h.u = u;
+    h.status = st;
+    for (int a = 0; a < 3; ++a) {
+      h.dc[a] = k[a];
+      h.du[a] = J.group ? k[3 + a] : nullptr;
+    }
+    h.K = K;
+    h.nR = J.wave == 0 ? J.M : 0;
+    h.nL = J.wave == 1 ? J.M : 0;
+    T.hjobs.push_back(h);
+  }
+  if (T.dalloc(&T.djobs, T.hjobs.size())) return 1;
+  if (hipMemcpy(T.djobs, T.hjobs.data(), sizeof(StJob) * T.hjobs.size(),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return fail("surftime job upload");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int jdsg_surftime_create(int nx, int ny, int nz, const float *dep, int water,
+                                    const double *tRc, int ntRc, int nmRc, const double *tRg,
+                                    int ntRg, int nmRg, const double *tLc, int ntLc, int nmLc,
+                                    const double *tLg, int ntLg, int nmLg, jdsg_surftime **out) {
+  *out = nullptr;
+  if (nx <= 0 || ny <= 0 || nz < 2) return fail("bad grid dimensions");
+  if (water && nz < 3) return fail("a water top needs nz >= 3");
+  auto *S = new jdsg_surftime();
+  S->nx = nx;
+  S->ny = ny;
+  S->nz = nz;
+  S->nxy = nx * ny;
+  S->water = water ? 1 : 0;
+  S->N = water ? (nz - 1 + (nz - 2) * 3 + 1) : (nz + (nz - 1) * 3);
+  const double *tab[4] = {tRc, tRg, tLc, tLg};
+  int ntm[4] = {ntRc, ntRg, ntLc, ntLg}, nmd[4] = {nmRc, nmRg, nmLc, nmLg};
+  auto ntrow = [&](int w, int m) {
+    const double *col = tab[w] + (size_t)m * ntm[w];
+    int n = 0;
+    while (n < ntm[w] && col[n] != -1) ++n;  // SurfaceWave.cpp:157-168
+    return n;
+  };
+  auto find_job = [&](int wave, int group_needed, const std::vector<double> &L) -> int {
+    for (size_t j = 0; j < S->jobs.size(); ++j) {
+      auto &J = S->jobs[j];
+      if (J.wave == wave && J.list == L && (J.group || !group_needed)) return (int)j;
+    }
+    return -1;
+  };
+  // group tables first so a phase table with the same periods reuses them
+  std::vector<std::vector<int>> jobof(4);
+  for (int w : {1, 3, 0, 2}) {
+    int wave = w >= 2 ? 1 : 0, group = (w == 1 || w == 3) ? 1 : 0;
+    if (!tab[w] || nmd[w] <= 0) continue;
+    jobof[w].assign(nmd[w], -1);
+    for (int m = 0; m < nmd[w]; ++m) {
+      int nt = ntrow(w, m);
+      if (nt <= 0) continue;
+      std::vector<double> L(tab[w] + (size_t)m * ntm[w], tab[w] + (size_t)m * ntm[w] + nt);
+      int j = find_job(wave, group, L);
+      if (j < 0) {
+        S->jobs.push_back({wave, group, 0, L});
+        j = (int)S->jobs.size() - 1;
+      }
+      S->jobs[j].M = std::max(S->jobs[j].M, m + 1);
+      if (S->jobs[j].M > JDSG_MAXMODE) {
+        delete S;
+        return fail("at most 8 modes per table");
+      }
+      jobof[w][m] = j;
+    }
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (!tab[w] || nmd[w] <= 0) continue;
+    for (int m = 0; m < nmd[w]; ++m) {
+      int nt = ntrow(w, m);
+      for (int k = 0; k < nt; ++k)
+        S->ents.push_back({jobof[w][m], m, k, (w == 1 || w == 3) ? 1 : 0, w >= 2 ? 1 : 0});
+    }
+  }
+  S->kmax = (int)S->ents.size();
+  // memory per column of the intermediate layer kernels
+  double per_col = 0;
+  for (auto &J : S->jobs)
+    per_col += (double)J.M * J.list.size() * S->N * 8.0 * (J.group ? 6 : 3) + 64.0 * J.list.size();
+  double budget = 12e9;
+  if (const char *b = getenv("JDSG_SURFTIME_BUDGET_GB")) budget = atof(b) * 1e9;
+  S->chunk = (int)std::max(1.0, std::min<double>(S->nxy, per_col > 0 ? budget / per_col : S->nxy));
+  auto alloc = [&](auto **p, size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(**p)) != hipSuccess) return 1;
+    S->allocs.push_back(q);
+    *p = (std::remove_reference_t<decltype(*p)>)q;
+    return 0;
+  };
+  if (alloc(&S->dep_d, nz) || alloc(&S->node_vp, (size_t)nz * S->nxy) ||
+      alloc(&S->node_vs, (size_t)nz * S->nxy) || alloc(&S->err_d, 1) ||
+      alloc(&S->ents_d, S->ents.size())) {
+    delete S;
+    return fail("hipMalloc (surftime)");
+  }
+  if (hipMemcpy(S->dep_d, dep, sizeof(float) * nz, hipMemcpyHostToDevice) != hipSuccess ||
+      (!S->ents.empty() && hipMemcpy(S->ents_d, S->ents.data(), sizeof(StEntry) * S->ents.size(),
+                                     hipMemcpyHostToDevice) != hipSuccess)) {
+    delete S;
+    return fail("surftime upload");
+  }
+  if (!S->jobs.empty()) {
+    if (build_set(S, S->sets[0], S->chunk)) {
+      delete S;
+      return 1;
+    }
+    int tail = S->nxy % S->chunk;
+    if (tail && build_set(S, S->sets[1], tail)) {
+      delete S;
+      return 1;
+    }
+  }
+  *out = S;
+  return 0;
+}
+
+extern "C" int jdsg_surftime_kmax(const jdsg_surftime *S, int *kmax) {
+  *kmax = S->kmax;
+  return 0;
+}
+
+extern "C" void jdsg_surftime_destroy(jdsg_surftime *S) { delete S; }
+
+extern "C" int jdsg_surftime_kernel(jdsg_surftime *S, const float *vs, double *vc, double *vout,
+                                    double *kernel, int *status, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (S->kmax == 0) return 0;
+  jdsg_launch_fill_u32((unsigned *)S->err_d, 1, 0u, st);
+  for (int col0 = 0; col0 < S->nxy; col0 += S->chunk) {
+    int nch = std::min(S->chunk, S->nxy - col0);
+    jdsg_surftime::Set &T = (nch == S->sets[0].ncol) ? S->sets[0] : S->sets[1];
+    jdsg_launch_grid_to_layers(col0, nch, S->nxy, S->nz, S->N, S->water, vs, S->dep_d, T.thk,
+                               T.vp, T.vs, T.rho, S->node_vp, S->node_vs, S->err_d, st);
+    for (size_t j = 0; j < S->jobs.size(); ++j) {
+      auto &k = T.out_k[j];
+      bool g = S->jobs[j].group;
+      if (jdsg_plan_execute(T.plans[j], T.thk, T.vp, T.vs, T.rho, T.periods[j], T.out_c[j],
+                            T.out_u[j], T.out_st[j], k[1], k[0], k[2], nullptr,
+                            g ? k[4] : nullptr, g ? k[3] : nullptr, g ? k[5] : nullptr,
+                            nullptr, st))
+        return 1;
+    }
+    jdsg_launch_surftime_epilogue(col0, nch, S->nxy, S->nz, S->N, S->water, S->kmax, S->ents_d,
+                                  T.djobs, S->node_vp, S->node_vs, vc, vout, kernel, status,
+                                  kernel != nullptr, st);
+    HIPCHK(hipGetLastError());
+  }
+  int err = 0;
+  HIPCHK(hipMemcpyAsync(&err, S->err_d, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (err) return fail("vs(0)==0 (water top) must hold for all columns or none, as declared");
+  return 0;
+}
+
+extern "C" int jdsg_get_1d_kernel(int nx, int ny, int nz, const float *vs, const float *dep,
+                                  const double *tRc, int ntRc, int nmRc, const double *tRg,
+                                  int ntRg, int nmRg, const double *tLc, int ntLc, int nmLc,
+                                  const double *tLg, int ntLg, int nmLg, double *vc,
+                                  double *vout, double *kernel, int *status) {
+  int nxy = nx * ny;
+  int water = (nxy > 0 && vs[0] == 0.0f) ? 1 : 0;
+  jdsg_surftime *S = nullptr;
+  if (jdsg_surftime_create(nx, ny, nz, dep, water, tRc, ntRc, nmRc, tRg, ntRg, nmRg, tLc, ntLc,
+                           nmLc, tLg, ntLg, nmLg, &S))
+    return 1;
+  DevBuf B;
+  size_t nk = (size_t)nxy * S->kmax;
+  float *dvs = B.up(vs, (size_t)nxy * nz);
+  double *dvc = B.up<double>(nullptr, nk), *dvo = B.up<double>(nullptr, nk);
+  double *dker = kernel ? B.up<double>(nullptr, nk * nz) : nullptr;
+  int *dst = B.up<int>(nullptr, nk);
+  int rc = (!dvs || !dvc || !dvo || !dst || (kernel && !dker)) ? fail("device allocation") : 0;
+  if (!rc && kernel) (void)hipMemset(dker, 0, sizeof(double) * nk * nz);
+  if (!rc) rc = jdsg_surftime_kernel(S, dvs, dvc, dvo, dker, dst, nullptr);
+  if (!rc) {
+    hipError_t e = hipMemcpy(vc, dvc, sizeof(double) * nk, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(vout, dvo, sizeof(double) * nk, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && kernel)
+      e = hipMemcpy(kernel, dker, sizeof(double) * nk * nz, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && status)
+      e = hipMemcpy(status, dst, sizeof(int) * nk, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail("copy back");
+  }
+  jdsg_surftime_destroy(S);
+  return rc;
 }

@@ -554,7 +554,7 @@ JDSG_DEV bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
           S.c1 = S.c3;
           S.del1 = S.del3;
         }
-        if (fabs(S.c1 - S.c2) <= 1.0e-6 * S.c1) goto nv_done;
+        if (fabs(S.c1 - S.c2) <= A.tol * S.c1) goto nv_done;
         if (sign1(s13) != sign1(s32)) S.nev = 0;
         double ss1 = fabs(S.del1);
         double s1 = (double)0.01f * ss1;
@@ -672,7 +672,7 @@ JDSG_DEV bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
     float ccg = (float)S.c1;
     double v;
     if (A.igr == 0) {
-      v = (double)cc0;
+      v = (A.tol < 1.0e-6) ? cs[S.k] : (double)cc0;
     } else {
       float gvel = (1 / S.t1a - 1 / S.t1b) / (1 / (S.t1a * cc0) - 1 / (S.t1b * ccg));
       v = (double)gvel;

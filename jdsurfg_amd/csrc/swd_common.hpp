@@ -72,8 +72,9 @@ struct ColumnModel {
 enum : int {
   JDSG_OK = 0,
   JDSG_NO_ROOT = 1,        // no root at T (mode beyond cutoff, or chain failure)
-  JDSG_NO_ROOT_PM = 2,     // root at T but none at T*(1 +- 0.005): no U kernel
-  JDSG_EXIT_PATH = 4,      // the reference wrapper would exit(1) here
+  JDSG_NO_ROOT_PM = 2,     // root at T but none at T*(1 +- 0.005): dU/dm zero
+  JDSG_EXIT_PATH = 4,      // the reference wrapper would exit(1) in the T chain
+  JDSG_EXIT_PATH_PM = 8,   // ... in a T*(1 +- 0.005) chain: dU/dm zero
 };
 
 #define JDSG_MAXMODE 8

@@ -989,19 +989,25 @@ jdsg_eigen(EigenArgs A) {
     double c0 = A.cflat[e];
     int st = A.cstat[e];
     double c1 = 0.0, c2 = 0.0;
+    int stpm = 0;
     if (A.group) {
       c1 = A.cflat[A.nent + e];
       c2 = A.cflat[2 * A.nent + e];
-      st |= A.cstat[A.nent + e] | A.cstat[2 * A.nent + e];
+      stpm = A.cstat[A.nent + e] | A.cstat[2 * A.nent + e];
     }
     int status = 0;
     if (st & JDSG_EXIT_PATH) status = JDSG_EXIT_PATH;
     else if (c0 == 0.0) status = JDSG_NO_ROOT;
-    else if (A.group && (c1 == 0.0 || c2 == 0.0)) status = JDSG_NO_ROOT_PM;
     if (status) {
       zero_entry(A, e, kb, ls, N, status);
       it = nlanes + (long long)atomicAdd(A.counter, 1u);
       continue;
+    }
+    // T+-0.5 % roots: without them the T outputs stand and dU/dm is zero
+    bool pm_ok = true;
+    if (A.group) {
+      if (stpm & JDSG_EXIT_PATH) { pm_ok = false; status = JDSG_EXIT_PATH_PM; }
+      else if (c1 == 0.0 || c2 == 0.0) { pm_ok = false; status = JDSG_NO_ROOT_PM; }
     }
     double T = A.periods[A.per_off[col] + k];
     double twopi = 2.0 * kPiF;
@@ -1088,7 +1094,8 @@ jdsg_eigen(EigenArgs A) {
       double usc[3];  // per-pass kernel scale: us0 (Rayleigh) or sumi1 (Love)
       double facv[3];
       // passes T, T1, T2 leave raw kernels in scratch sets 0, 1, 2
-      for (int v = 0; v < 3; ++v) {
+      const int nv = pm_ok ? 3 : 1;
+      for (int v = 0; v < nv; ++v) {
         double omega = twopi / tt[v];
         double wvno = omega / cv[v];
         Dst D;
@@ -1131,8 +1138,8 @@ jdsg_eigen(EigenArgs A) {
         double d0[4], d1[4], d2[4];
         for (int a = 0; a < 4; ++a) {
           double r0 = draw[(long long)(0 + a) * A.nmax * ss + os];
-          double r1 = draw[(long long)(4 + a) * A.nmax * ss + os];
-          double r2 = draw[(long long)(8 + a) * A.nmax * ss + os];
+          double r1 = pm_ok ? draw[(long long)(4 + a) * A.nmax * ss + os] : 0.0;
+          double r2 = pm_ok ? draw[(long long)(8 + a) * A.nmax * ss + os] : 0.0;
           if (a == 3) {
             if (ray) {
               d0[a] = facv[0] * r0; if (fabs(d0[a]) < 1.0e-38) d0[a] = 0.0;
@@ -1160,6 +1167,7 @@ jdsg_eigen(EigenArgs A) {
             dc = dc / tm3 * fct[a];
           }
           if (!ray && a == 0) { du = 0.0; dc = 0.0; }
+          if (!pm_ok) du = 0.0;
           if (a < 3) {
             if (A.dc[a]) A.dc[a][oo] = dc;
             if (A.du[a]) A.du[a][oo] = du;
@@ -1181,7 +1189,7 @@ jdsg_eigen(EigenArgs A) {
     }
     A.c_out[e] = cout;
     A.u_out[e] = u;
-    A.status[e] = 0;
+    A.status[e] = status;
     it = nlanes + (long long)atomicAdd(A.counter, 1u);
   }
 }

@@ -27,6 +27,7 @@ struct ChainArgs {
   double *cscr;               // [nchain_slots * 2 * kmax_pad]
   unsigned *counter;          // work counter
   unsigned long long *nevals; // period-equation evaluations (flop model)
+  double tol;                 // Neville stop rule (1e-6 reference; smaller = exact roots, unrounded)
   long long nchain;           // ncol * 2 * nvar
 };
 
@@ -60,9 +61,36 @@ struct EigenArgs {
   const double *tv;          // optional [3*nent] explicit (T, T1, T2) per entry
 };
 
+// SurfTime output entry: which job / mode / period feeds SurfTime index idx
+struct StEntry {
+  int job;   // index into jobs
+  int mode;  // mode within the job's plan (0-based)
+  int k;     // period index within the job's list
+  int kind;  // 0: Rc/Lc (vout = c, dc kernels), 1: Rg/Lg (vout = U, dU kernels)
+  int love;  // Love: no Vp kernel (SurfaceWave.cpp:367,403)
+};
+// Outputs of one layer plan over a chunk (grid layout: value of column nl,
+// entry el, layer i at (el*N + i)*nchunk + nl).
+struct StJob {
+  const double *c, *u;
+  const int *status;
+  const double *dc[3], *du[3];  // b, a, rho
+  int K, nR, nL;
+};
+
+
 void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float *thk,
                       const float *vp, const float *vs, const float *rho, int iflsph,
                       const PrepOut &o, hipStream_t st);
 void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st);
 void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st);
 void jdsg_launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t st);
+void jdsg_launch_grid_to_layers(int col0, int nchunk, int nxy, int nz, int N, int water,
+                                const float *vs, const float *dep, float *thk, float *vp,
+                                float *vsl, float *rho, float *node_vp, float *node_vs, int *err,
+                                hipStream_t st);
+void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N, int water,
+                                   int kmax, const void *ents, const void *jobs,
+                                   const float *node_vp, const float *node_vs, double *vc,
+                                   double *vout, double *kernel, int *status, int want_kernel,
+                                   hipStream_t st);

@@ -44,6 +44,7 @@ def lib():
     L.jdsg_plan_destroy.argtypes = [vp]
     L.jdsg_plan_timings.argtypes = [vp, vp]
     L.jdsg_plan_eval_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.jdsg_plan_set_tolerance.argtypes = [vp, C.c_double]
     L.jdsg_surfdisp.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int,
                                 C.c_int, C.c_int]
     _lib = L
@@ -128,6 +129,10 @@ class Plan:
             raise RuntimeError("jdsg_plan_execute: " + last_error())
         return out
 
+    def set_tolerance(self, tol):
+        if lib().jdsg_plan_set_tolerance(self.h, float(tol)):
+            raise RuntimeError(last_error())
+
     def eval_count(self):
         v = C.c_longlong()
         if lib().jdsg_plan_eval_count(self.h, C.byref(v)):
@@ -169,3 +174,57 @@ def grid_to_csr(plan, arr, N):
     ncol = plan.ncol
     a = np.asarray(arr).reshape(-1, N, ncol)     # (entry_local, layer, col)
     return np.ascontiguousarray(a.transpose(2, 0, 1)).reshape(-1)
+
+
+def get_1d_kernel(vs, dep, tables, water=None):
+    """SurfTime::get_1d_kernel on the GPU (host arrays in, host arrays out).
+
+    vs: (nx, ny, nz) float array; dep: (nz,) node depths; tables: dict with
+    keys 'Rc','Rg','Lc','Lg' -> (nt, nmode) period arrays padded with -1.
+    Returns vc, vout (nx*ny, kmax), kernel (nx*ny, nz, kmax) in the reference's
+    column-major order, and status (nx*ny, kmax)."""
+    import torch
+    L = lib()
+    nx, ny, nz = vs.shape
+    vsf = torch.from_numpy(np.ascontiguousarray(np.asarray(vs, np.float32).ravel(order="F"))).cuda()
+    args = []
+    keep = []
+    for w in ("Rc", "Rg", "Lc", "Lg"):
+        a = tables.get(w)
+        if a is None or np.size(a) == 0:
+            args += [None, 0, 0]
+        else:
+            a = np.asfortranarray(np.asarray(a, np.float64))
+            keep.append(a)
+            args += [C.c_void_p(a.ctypes.data), a.shape[0], a.shape[1]]
+    if water is None:
+        water = bool(np.asarray(vs)[0, 0, 0] == 0.0)
+    h = C.c_void_p()
+    depa = np.ascontiguousarray(dep, np.float32)
+    L.jdsg_surftime_create.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_int] + \
+        [C.c_void_p, C.c_int, C.c_int] * 4 + [C.POINTER(C.c_void_p)]
+    rc = L.jdsg_surftime_create(nx, ny, nz, depa.ctypes.data, int(water), *args, C.byref(h))
+    if rc:
+        raise RuntimeError("jdsg_surftime_create: " + last_error())
+    try:
+        km = C.c_int()
+        L.jdsg_surftime_kmax.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.jdsg_surftime_kmax(h, C.byref(km))
+        kmax = km.value
+        nxy = nx * ny
+        vc = torch.zeros(nxy * kmax, dtype=torch.float64, device="cuda")
+        vout = torch.zeros_like(vc)
+        ker = torch.zeros(nxy * nz * kmax, dtype=torch.float64, device="cuda")
+        st = torch.zeros(nxy * kmax, dtype=torch.int32, device="cuda")
+        L.jdsg_surftime_kernel.argtypes = [C.c_void_p] * 7
+        rc = L.jdsg_surftime_kernel(h, _ptr(vsf), _ptr(vc), _ptr(vout), _ptr(ker), _ptr(st),
+                                    C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        if rc:
+            raise RuntimeError("jdsg_surftime_kernel: " + last_error())
+        torch.cuda.synchronize()
+        f = lambda t, shp: t.cpu().numpy().reshape(shp, order="F")
+        return (f(vc, (nxy, kmax)), f(vout, (nxy, kmax)), f(ker, (nxy, nz, kmax)),
+                f(st, (nxy, kmax)))
+    finally:
+        L.jdsg_surftime_destroy.argtypes = [C.c_void_p]
+        L.jdsg_surftime_destroy(h)

@@ -88,6 +88,7 @@ class Oracle:
         L.or_batch_solve.argtypes = [C.c_void_p, C.POINTER(_Batch), C.POINTER(_BatchOut),
                                      C.c_int]
         L.or_restatement_backend.restype = C.c_void_p
+        L.or_set_tolerance.argtypes = [C.c_double]
         if kind == "ref":
             L.ref_backend.restype = C.c_void_p
             self.backend = L.ref_backend()
@@ -96,6 +97,10 @@ class Oracle:
                                          C.POINTER(C.c_int)]
         else:
             self.backend = L.or_restatement_backend()
+
+    def set_tolerance(self, tol=1e-6):
+        """Neville stop rule; below 1e-6 the exact-root mode (restatement only)."""
+        self.lib.or_set_tolerance(float(tol))
 
     # ---------------- per-column ABI ----------------
     def surfdisp96(self, thk, vp, vs, rho, t, iflsph=1, iwave=2, mode=1, igr=0):

@@ -43,6 +43,10 @@ void or_slegnpu(const float *thk, const float *vs, const float *rhom,
                 double *dc2dr, double *du2db, double *du2dh, double *du2dr,
                 int iflsph);
 
+/* Stop rule of the Neville polish (default 1e-6 as surfdisp96.f:634); a
+ * tolerance below 1e-6 gives the exact-root oracle with unrounded roots. */
+void or_set_tolerance(double tol);
+
 /* surfdisp.cpp:64-116 with swdtp coded 0=Rc 1=Rg 2=Lc 3=Lg; returns 0, or 1
  * where the reference would print the model and exit(1). */
 int or_surfdisp(const float *thk, const float *vp, const float *vs,
@@ -118,8 +122,9 @@ int or_get_2d_map(const or_backend *be, const float *vs, int nx, int ny,
  * (one chain, sregn96_/slegn96_).  Entry (c,w,m,k) is written at
  * ent_off[c] + (w*nmode[2c] + m)*K_c + k (for w=1 the R modes come first);
  * kernels at that entry index times N_c plus layer, offsets from kern_off.
- * status: 0 ok, 1 no root at T, 2 no root at T*(1+-0.005), 4 the
- * reference's exit(1) path. */
+ * status: 0 ok, 1 no root at T, 2 no root at T*(1+-0.005) (T outputs in
+ * the phase form, dU/dm zero), 4 the reference's exit(1) path in the T
+ * chain (all zero), 8 the exit path in a T*(1+-0.005) chain (as 2). */
 typedef struct {
   int ncol;
   const int *layer_off;

@@ -12,6 +12,11 @@
 #include <string.h>
 #include "swd_oracle.h"
 
+/* Stop rule of nevill: |c1-c2| <= tol*c1 (surfdisp96.f:634 uses 1e-6).
+ * tol < 1e-6 selects the exact-root mode: roots are returned unrounded. */
+static double g_tol = 1.0e-6;
+void or_set_tolerance(double tol) { g_tol = tol; }
+
 typedef struct {
   int mmax, llw;
   float *d, *a, *b, *rho, *rtp, *dtp, *btp;
@@ -363,7 +368,7 @@ static double nevill(chain_ctx *x, double t, double c1, double c2,
       c1 = c3;
       del1 = del3;
     }
-    if (fabs(c1 - c2) <= 1.0e-6 * c1) break;
+    if (fabs(c1 - c2) <= g_tol * c1) break;
     if (fsign1(s13) != fsign1(s32)) nev = 0;
     double ss1 = fabs(del1);
     double s1 = (double)0.01f * ss1;
@@ -590,7 +595,7 @@ void or_surfdisp96(const float *thkm, const float *vpm, const float *vsm,
         float cc0 = (float)c[k - 1];
         float ccg = (float)c1;
         if (igr == 0) {
-          cg[k - 1] = cc0;
+          cg[k - 1] = (g_tol < 1.0e-6) ? c[k - 1] : (double)cc0;
         } else {
           float gvel = (1 / t1a - 1 / t1b) / (1 / (t1a * cc0) - 1 / (t1b * ccg));
           cg[k - 1] = gvel;

@@ -404,39 +404,40 @@ static void solve_column(const or_backend *be, const or_batch *b,
     for (int m = 0; m < M; ++m) {
       long long ebase = o->ent_off[col] + (long long)((w == 0 ? 0 : nR) + m) * K;
       int err = be->surfdisp(thk, vp, vs, rho, N, T, cp, K, code, m, 1, 1);
+      int errpm = 0;
       if (b->group) {
-        err |= be->surfdisp(thk, vp, vs, rho, N, t1, c1, K, code, m, 1, 1);
-        err |= be->surfdisp(thk, vp, vs, rho, N, t2, c2, K, code, m, 1, 1);
+        errpm |= be->surfdisp(thk, vp, vs, rho, N, t1, c1, K, code, m, 1, 1);
+        errpm |= be->surfdisp(thk, vp, vs, rho, N, t2, c2, K, code, m, 1, 1);
       }
       for (int k = 0; k < K; ++k) {
         long long e = ebase + k;
         long long q = o->kern_off[col] + (e - o->ent_off[col]) * N;
         if (err) { zero_entry(o, e, q, N, 4); continue; }
         if (cp[k] == 0.0) { zero_entry(o, e, q, N, 1); continue; }
-        if (b->group && (c1[k] == 0.0 || c2[k] == 0.0)) {
-          zero_entry(o, e, q, N, 2);
-          continue;
-        }
+        /* without the T*(1+-0.005) roots the T outputs stand (phase form)
+         * and dU/dm is zero: status 2, or 8 on the exit path */
+        int pm_bad = b->group && (errpm || c1[k] == 0.0 || c2[k] == 0.0);
+        int st = pm_bad ? (errpm ? 8 : 2) : 0;
         double c = cp[k], u = 0.0;
         double *da = k8, *db = k8 + N, *dr = k8 + 2 * N, *dh = k8 + 3 * N;
         double *ua = k8 + 4 * N, *ub = k8 + 5 * N, *ur_ = k8 + 6 * N,
                *uh = k8 + 7 * N;
         memset(k8, 0, sizeof(double) * 8 * (size_t)N);
-        if (w == 0 && b->group)
+        if (w == 0 && b->group && !pm_bad)
           be->sregnpu(thk, vp, vs, rho, N, T + k, &c, &u, ur, uz, tr, tz,
                       t1 + k, c1 + k, t2 + k, c2 + k, da, db, dh, dr, ua, ub,
                       uh, ur_, 1);
         else if (w == 0)
           be->sregn96(thk, vp, vs, rho, N, T + k, &c, &u, ur, uz, tr, tz, da,
                       db, dh, dr, 1);
-        else if (b->group)
+        else if (b->group && !pm_bad)
           be->slegnpu(thk, vs, rho, N, T + k, &c, &u, uz, tz, t1 + k, c1 + k,
                       t2 + k, c2 + k, db, dh, dr, ub, uh, ur_, 1);
         else
           be->slegn96(thk, vs, rho, N, T + k, &c, &u, uz, tz, db, dh, dr, 1);
         o->c[e] = c;
         o->u[e] = u;
-        o->status[e] = 0;
+        o->status[e] = st;
         double *src[8] = {da, db, dr, dh, ua, ub, ur_, uh};
         double *dst[8] = {o->dcda, o->dcdb, o->dcdr, o->dcdh,
                           o->duda, o->dudb, o->dudr, o->dudh};
