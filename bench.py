@@ -112,17 +112,17 @@ def main():
     stages = {"chains": (sm[1], chain_flops), "eigen_rayleigh": (sm[2], eigR),
               "eigen_love": (sm[3], eigL)}
     dom = max(stages, key=lambda k: stages[k][0])
-    dms, dfl = stages[dom]
+    dms, dfl = float(stages[dom][0]), float(stages[dom][1])
     achieved = dfl / (dms * 1e-3) / 1e12
     peak = fm["peak_fp64_tflops"]
     line = {
         "metric": "column-periods/s (c, U, dc/dVs, dU/dVs), plain FP64 method",
-        "value": value,
+        "value": float(value),
         "unit": "column-periods/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": float(ms_step),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -143,7 +143,7 @@ def main():
     if not args.no_cpu and args.cpu_sample > 0:
         nthr = min(16, os.cpu_count() or 1)
         v, kind, ncols, dt = cpu_baseline(w, args.cpu_sample, nthr)
-        line["cpu_baseline"] = {"value": v, "unit": "column-periods/s", "cores": nthr,
+        line["cpu_baseline"] = {"value": float(v), "unit": "column-periods/s", "cores": nthr,
                                 "kind": kind,
                                 "sample": f"{ncols} strided columns of the same workload "
                                           f"({ncols * int(np.diff(w.per_off)[0])} column-periods, "

@@ -94,6 +94,9 @@ int jdsg_plan_set_tolerance(jdsg_plan *plan, double tol);
 /* Number of period-equation evaluations the last execute performed
  * (synchronises; for the flop model). */
 int jdsg_plan_eval_count(const jdsg_plan *plan, long long *nevals);
+/* Flat roots of the last execute into a host array of nvar*nent doubles
+ * (nvar = 3 with group kernels: T, T(1+.005), T(1-.005); else 1). */
+int jdsg_plan_flat_roots(const jdsg_plan *plan, double *host);
 /* Per-stage milliseconds (prep, chains, eigen Rayleigh, eigen Love) of the
  * last execute, when JDSG_TIMING=1 was set at plan creation. */
 int jdsg_plan_timings(const jdsg_plan *plan, float *ms4);

@@ -59,6 +59,10 @@ struct jdsg_plan {
   unsigned long long *nevals = nullptr;
   int chain_waves = 0, eigen_waves = 0;
   double tol = 1.0e-6;
+  // surrogate path buffers (allocated by jdsg_plan_enable_surrogate)
+  double *known = nullptr, *cen = nullptr;
+  unsigned *sfail = nullptr;
+  const double *known_in = nullptr;
   // explicit per-entry periods for the eigen stage (per-column ABI only)
   double *tv = nullptr;
   double *eig[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -93,7 +97,7 @@ struct jdsg_plan {
       allocs.erase(std::find(allocs.begin(), allocs.end(), (void *)escr));
       escr = nullptr;
     }
-    if (dalloc(&cscr, (size_t)chain_waves * 64 * 2 * kpad)) return 1;
+    if (dalloc(&cscr, (size_t)chain_waves * 64 * (2 * kpad + 22))) return 1;
     if (dalloc(&escr, (size_t)eigen_waves * 64 * 18 * nmax)) return 1;
     return 0;
   }
@@ -149,7 +153,7 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
   P->kpad = std::max(1, P->kmax);
   P->nchain = (long long)ncol * 2 * P->nvar;
   int dw = default_waves();
-  P->chain_waves = (int)std::min<long long>(dw, (P->nchain + 63) / 64);
+  P->chain_waves = (int)std::min<long long>(dw, ((long long)ncol * P->nvar + 63) / 64);
   P->eigen_waves = (int)std::min<long long>(dw, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
   P->chain_waves = std::max(1, P->chain_waves);
   P->eigen_waves = std::max(1, P->eigen_waves);
@@ -166,7 +170,7 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
       P->dalloc(&P->prep.rtpL, cp) || P->dalloc(&P->prep.vtp, cp) ||
       P->dalloc(&P->prep.dtp, cp) || P->dalloc(&P->cflat, (size_t)P->nvar * P->nent) ||
       P->dalloc(&P->cstat, (size_t)P->nvar * P->nent) || P->dalloc(&P->ierr, P->nchain) ||
-      P->dalloc(&P->counters, 4) || P->dalloc(&P->nevals, 2) || P->size_scratch()) {
+      P->dalloc(&P->counters, 4) || P->dalloc(&P->nevals, 4) || P->size_scratch()) {
     delete P;
     return 1;
   }
@@ -270,6 +274,29 @@ extern "C" int jdsg_plan_eval_count(const jdsg_plan *P, long long *out) {
   return 0;
 }
 
+// Flat roots of the last execute ([nvar*nent]: T, then T(1+.005), T(1-.005);
+// float-rounded in the reference mode), device to host.  Synchronises.
+extern "C" int jdsg_plan_flat_roots(const jdsg_plan *P, double *host) {
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(host, P->cflat, sizeof(double) * P->nvar * P->nent, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Prepared root-search model of the plan's columns (after an execute):
+// float4 {d, a, b, rho} per layer for Rayleigh and Love ([layer*ncol_pad+col]),
+// starting velocity and betmx per column.  For tests.
+extern "C" int jdsg_plan_chain_model(const jdsg_plan *P, float *R, float *L, float *cc,
+                                     float *betmx, int *ncol_pad) {
+  size_t n = (size_t)P->ncol_pad * P->nmax;
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(R, P->prep.chainR, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(L, P->prep.chainL, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(cc, P->prep.cc, sizeof(float) * P->ncol, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(betmx, P->prep.betmx, sizeof(float) * P->ncol, hipMemcpyDeviceToHost));
+  *ncol_pad = P->ncol_pad;
+  return 0;
+}
+
 extern "C" int jdsg_plan_timings(const jdsg_plan *P, float *ms4) {
   for (int i = 0; i < 4; ++i) ms4[i] = P->last_ms[i];
   return P->timing ? 0 : fail("timing disabled (set JDSG_TIMING=1 before plan creation)");
@@ -284,7 +311,7 @@ int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
   jdsg_launch_prep(P->ncol, P->ncol_pad, P->layer_off_d, thk, vp, vs, rho, P->iflsph, P->prep,
                    st);
   jdsg_launch_fill_u32(P->counters, 4, 0u, st);
-  jdsg_launch_fill_u32((unsigned *)P->nevals, 4, 0u, st);
+  jdsg_launch_fill_u32((unsigned *)P->nevals, 2, 0u, st);
   jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, st);
   if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
   ChainArgs A{};
@@ -306,6 +333,9 @@ int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
   A.nchain = P->nchain;
   A.nevals = P->nevals;
   A.tol = P->tol;
+  A.known = P->known_in;
+  A.any_wave[0] = P->nitem[0] > 0;
+  A.any_wave[1] = P->nitem[1] > 0;
   jdsg_launch_chains(A, P->chain_waves, st);
   HIPCHK(hipGetLastError());
   if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
@@ -348,7 +378,7 @@ int run_eigen(jdsg_plan *P, const double *periods, double *c, double *u, int *st
     E.kern_off = P->kern_off_d;
     E.scr = P->escr;
     E.nmax = P->nmax;
-    E.counter = P->counters + 1 + w;
+    E.counter = P->counters + 2 + w;
     E.tv = P->tv;
     int waves = (int)std::min<long long>(P->eigen_waves, (P->nitem[w] + 63) / 64);
     jdsg_launch_eigen(E, waves, st);
@@ -582,6 +612,148 @@ extern "C" void slegnpu_(float *thk, float *vs, float *rhom, int nlayer, const d
   double tv[3] = {*t, *t1, *t2}, cv[3] = {*cp, *cp1, *cp2};
   if (column_eigen(1, thk, nullptr, vs, rhom, nlayer, iflsph, tv, cv, 3, cp, cg, eig, dc, du))
     report("slegnpu_");
+}
+
+// ============================================================================
+// Surrogate (learned bracket) path.
+extern "C" int jdsg_plan_roots(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
+                               const float *rho, const double *periods, double *c_out,
+                               int *stat_out, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (run_front(P, thk, vp, vs, rho, periods, 0, 1, st)) return 1;
+  size_t n = (size_t)P->nvar * P->nent;
+  HIPCHK(hipMemcpyAsync(c_out, P->cflat, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  if (stat_out)
+    HIPCHK(hipMemcpyAsync(stat_out, P->cstat, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+extern "C" int jdsg_plan_features(jdsg_plan *P, const float *thk, const float *vp,
+                                  const float *vs, const float *rho, const double *periods,
+                                  long long e0, long long nrow, void *X, float *colf,
+                                  int column_pass, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (e0 < 0 || e0 + nrow > P->nent) return fail("feature rows out of range");
+  if (column_pass)
+    jdsg_launch_column_features(P->ncol, P->layer_off_d, thk, vp, vs, rho, colf, st);
+  jdsg_launch_entry_features(e0, nrow, P->ncol, P->ent_off_d, P->per_off_d, periods, P->nmode_d,
+                             colf, (uint16_t *)X, st);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int jdsg_plan_enable_surrogate(jdsg_plan *P) {
+  if (P->known) return 0;
+  if (P->dalloc(&P->known, (size_t)P->nvar * P->nent) || P->dalloc(&P->cen, 2 * (size_t)P->nent) ||
+      P->dalloc(&P->sfail, 2))
+    return 1;
+  return 0;
+}
+
+extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const float *vp,
+                                           const float *vs, const float *rho,
+                                           const double *periods, const double *pred,
+                                           double half_width, double *c, double *u, int *status,
+                                           double *dcda, double *dcdb, double *dcdr, double *dcdh,
+                                           double *duda, double *dudb, double *dudr, double *dudh,
+                                           void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!P->known) return fail("call jdsg_plan_enable_surrogate first");
+  if (P->tol >= 1.0e-6) return fail("the surrogate path polishes to exact roots: set tol < 1e-6");
+  if (P->timing) HIPCHK(hipEventRecord(P->ev[0], st));
+  jdsg_launch_prep(P->ncol, P->ncol_pad, P->layer_off_d, thk, vp, vs, rho, P->iflsph, P->prep,
+                   st);
+  jdsg_launch_fill_u32(P->sfail, 2, 0u, st);
+  jdsg_launch_fill_u32((unsigned *)(P->nevals + 1), 2, 0u, st);
+  PolishArgs Q{};
+  Q.cm = P->cm;
+  Q.ent_off = P->ent_off_d;
+  Q.per_off = P->per_off_d;
+  Q.periods = periods;
+  Q.nmode = P->nmode_d;
+  Q.nent = P->nent;
+  Q.tol = P->tol;
+  Q.known = P->known;
+  Q.nfail = P->sfail;
+  Q.nevals = P->nevals + 1;
+  Q.nitem = P->nent;
+  Q.variant = 0;
+  Q.center = pred;
+  Q.half_width = half_width;
+  jdsg_launch_polish(Q, st);
+  jdsg_launch_mode_order(P->cm, P->ent_off_d, P->per_off_d, P->nmode_d, P->nent, P->kmax, 0,
+                         P->known, P->sfail, st);
+  if (P->group) {
+    jdsg_launch_pm_centers(P->ent_off_d, P->per_off_d, periods, P->ncol, P->nent, P->known, pred,
+                           P->cen, P->cen + P->nent, st);
+    for (int v = 1; v <= 2; ++v) {
+      Q.variant = v;
+      Q.center = P->cen + (size_t)(v - 1) * P->nent;
+      Q.half_width = 0.5 * half_width;
+      jdsg_launch_polish(Q, st);
+      jdsg_launch_mode_order(P->cm, P->ent_off_d, P->per_off_d, P->nmode_d, P->nent, P->kmax, v,
+                             P->known, P->sfail, st);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
+  // the chains walk in reference order and solve only the roots left open
+  P->known_in = P->known;
+  int rc = 0;
+  {
+    jdsg_launch_fill_u32(P->counters, 4, 0u, st);
+    jdsg_launch_fill_u32((unsigned *)P->nevals, 2, 0u, st);
+    jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, st);
+    ChainArgs A{};
+    A.cm = P->cm;
+    A.ent_off = P->ent_off_d;
+    A.per_off = P->per_off_d;
+    A.periods = periods;
+    A.nmode = P->nmode_d;
+    A.nvar = P->nvar;
+    A.igr = 0;
+    A.retry = 1;
+    A.nent = P->nent;
+    A.kmax_pad = P->kpad;
+    A.cflat = P->cflat;
+    A.cstat = P->cstat;
+    A.ierr_out = P->ierr;
+    A.cscr = P->cscr;
+    A.counter = P->counters + 0;
+    A.nchain = P->nchain;
+    A.nevals = P->nevals;
+    A.tol = P->tol;
+    A.known = P->known;
+    A.any_wave[0] = P->nitem[0] > 0;
+    A.any_wave[1] = P->nitem[1] > 0;
+    jdsg_launch_chains(A, P->chain_waves, st);
+    rc = hipGetLastError() != hipSuccess;
+  }
+  P->known_in = nullptr;
+  if (rc) return fail("chain launch failed");
+  if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
+  double *dc[4] = {dcda, dcdb, dcdr, dcdh};
+  double *du[4] = {duda, dudb, dudr, dudh};
+  if (run_eigen(P, periods, c, u, status, dc, du, st)) return 1;
+  if (P->timing) {
+    HIPCHK(hipEventSynchronize(P->ev[4]));
+    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
+  }
+  return 0;
+}
+
+// Roots left to the plain chain by the last surrogate execute, and the
+// period-equation evaluations of the bracket/polish and chain stages.
+extern "C" int jdsg_plan_surrogate_stats(const jdsg_plan *P, long long *nfail,
+                                         long long *polish_evals, long long *chain_evals) {
+  unsigned f[2] = {0, 0};
+  unsigned long long ev[2] = {0, 0};
+  if (P->sfail) HIPCHK(hipMemcpy(f, P->sfail, sizeof(f), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ev, P->nevals, sizeof(ev), hipMemcpyDeviceToHost));
+  if (nfail) *nfail = f[0];
+  if (chain_evals) *chain_evals = (long long)ev[0];
+  if (polish_evals) *polish_evals = (long long)ev[1];
+  return 0;
 }
 
 // ============================================================================

@@ -14,6 +14,10 @@
 // double recursion, sngl() of the returned root.  FP contraction is off so
 // the recursion rounds like the reference build.
 #include "swd_kernels.hpp"
+#include "swd_dltar.hpp"
+
+using jdsg_pe::dltar1;
+using jdsg_pe::dltar4;
 
 #pragma clang fp contract(off)
 
@@ -94,7 +98,9 @@ jdsg_prep_columns(int ncol, int ncol_pad, const int *__restrict__ layer_off,
         float btp = (float)tmp;
         r0 = r1;
         rL = rh * powm5f(btp);
-        rR = rh * powf(btp, -2.275f);
+        // REAL**REAL: the float power rounded from a double evaluation, as
+        // the host libm's powf delivers it (correctly rounded in practice)
+        rR = rh * (float)pow((double)btp, (double)(-2.275f));
       }
       if (i == N - 1) d = 0.0f;
       size_t q = (size_t)i * ncol_pad + col;
@@ -151,208 +157,7 @@ jdsg_prep_columns(int ncol, int ncol_pad, const int *__restrict__ layer_off,
   }
 }
 
-// ----------------------------------------------------------------------------
-// Period equations (surfdisp96.f:734-898), model read from the float4 stack.
-
 namespace {
-
-// dltar1: Love (Haskell SH) from the half-space to the surface
-JDSG_DEV double dltar1(const float4 *__restrict__ st, int stride, int mmax, int llw,
-                       double wvno, double omega) {
-  float4 h = st[(size_t)(mmax - 1) * stride];
-  double beta1 = (double)h.z;
-  double rho1 = (double)h.w;
-  double xkb = omega / beta1;
-  double wvnop = wvno + xkb;
-  double wvnom = fabs(wvno - xkb);
-  double rb = sqrt(wvnop * wvnom);
-  double e1 = rho1 * rb;
-  double e2 = 1.0 / (beta1 * beta1);
-  for (int m = mmax - 2; m >= llw - 1; --m) {
-    float4 L = st[(size_t)m * stride];
-    beta1 = (double)L.z;
-    rho1 = (double)L.w;
-    double xmu = rho1 * beta1 * beta1;
-    xkb = omega / beta1;
-    wvnop = wvno + xkb;
-    wvnom = fabs(wvno - xkb);
-    rb = sqrt(wvnop * wvnom);
-    double dm = (double)L.x;
-    double q = dm * rb;
-    double sinq, cosq, y, z;
-    if (wvno < xkb) {
-      sincos(q, &sinq, &cosq);
-      y = sinq / rb;
-      z = -rb * sinq;
-    } else if (wvno == xkb) {
-      cosq = 1.0;
-      y = dm;
-      z = 0.0;
-    } else {
-      double fac = (q < 16) ? exp(-2.0 * q) : 0.0;
-      cosq = (1.0 + fac) * 0.5;
-      sinq = (1.0 - fac) * 0.5;
-      y = sinq / rb;
-      z = rb * sinq;
-    }
-    double e10 = e1 * cosq + e2 * xmu * z;
-    double e20 = e1 * y / xmu + e2 * cosq;
-    double xnor = fabs(e10);
-    double ynor = fabs(e20);
-    if (ynor > xnor) xnor = ynor;
-    if (xnor < 1.0e-40) xnor = 1.0;
-    e1 = e10 / xnor;
-    e2 = e20 / xnor;
-  }
-  return e1;
-}
-
-// var (surfdisp96.f:901-1018): one of sin/cos, (cosh,sinh)*exp(-p), or the
-// coincident case for each of P and S
-JDSG_DEV void var_ps(double p, double ra, double wvno, double xka, double dpth,
-                     double &cosp, double &w, double &x, double &pex) {
-  pex = 0.0;
-  if (wvno < xka) {
-    double sinp;
-    sincos(p, &sinp, &cosp);
-    w = sinp / ra;
-    x = -ra * sinp;
-  } else if (wvno == xka) {
-    cosp = 1.0;
-    w = dpth;
-    x = 0.0;
-  } else {
-    pex = p;
-    double fac = (p < 16) ? exp(-2.0 * p) : 0.0;
-    cosp = (1.0 + fac) * 0.5;
-    double sinp = (1.0 - fac) * 0.5;
-    w = sinp / ra;
-    x = ra * sinp;
-  }
-}
-
-// dltar4: Rayleigh, Dunkin compound matrices from the half-space upward
-JDSG_DEV double dltar4(const float4 *__restrict__ st, int stride, int mmax, int llw,
-                       double wvno, double omga) {
-  double omega = omga;
-  if (omega < 1.0e-4) omega = 1.0e-4;
-  double wvno2 = wvno * wvno;
-  float4 h = st[(size_t)(mmax - 1) * stride];
-  double xka = omega / (double)h.y;
-  double xkb = omega / (double)h.z;
-  double wvnop = wvno + xka;
-  double wvnom = fabs(wvno - xka);
-  double ra = sqrt(wvnop * wvnom);
-  wvnop = wvno + xkb;
-  wvnom = fabs(wvno - xkb);
-  double rb = sqrt(wvnop * wvnom);
-  double t = (double)h.z / omega;
-  double gammk = 2.0 * t * t;
-  double gam = gammk * wvno2;
-  double gamm1 = gam - 1.0;
-  double rho1 = (double)h.w;
-  double e0 = rho1 * rho1 * (gamm1 * gamm1 - gam * gammk * ra * rb);
-  double e1 = -rho1 * ra;
-  double e2 = rho1 * (gamm1 - gammk * ra * rb);
-  double e3 = rho1 * rb;
-  double e4 = wvno2 - ra * rb;
-  for (int m = mmax - 2; m >= llw - 1; --m) {
-    float4 L = st[(size_t)m * stride];
-    xka = omega / (double)L.y;
-    xkb = omega / (double)L.z;
-    t = (double)L.z / omega;
-    gammk = 2.0 * t * t;
-    gam = gammk * wvno2;
-    wvnop = wvno + xka;
-    wvnom = fabs(wvno - xka);
-    ra = sqrt(wvnop * wvnom);
-    wvnop = wvno + xkb;
-    wvnom = fabs(wvno - xkb);
-    rb = sqrt(wvnop * wvnom);
-    double dpth = (double)L.x;
-    rho1 = (double)L.w;
-    double p = ra * dpth;
-    double q = rb * dpth;
-    double cosp, w, x, pex, cosq, y, z, sex;
-    var_ps(p, ra, wvno, xka, dpth, cosp, w, x, pex);
-    var_ps(q, rb, wvno, xkb, dpth, cosq, y, z, sex);
-    double exa = pex + sex;
-    double a0 = (exa < 60.0) ? exp(-exa) : 0.0;
-    double cpcq = cosp * cosq, cpy = cosp * y, cpz = cosp * z, cqw = cosq * w,
-           cqx = cosq * x, xy = x * y, xz = x * z, wy = w * y, wz = w * z;
-    // dnka (surfdisp96.f:1051-1094)
-    const double two = 2.0;
-    double gamm1_ = gam - 1.0;
-    double twgm1 = gam + gamm1_;
-    double gmgmk = gam * gammk;
-    double gmgm1 = gam * gamm1_;
-    double gm1sq = gamm1_ * gamm1_;
-    double rho2 = rho1 * rho1;
-    double a0pq = a0 - cpcq;
-    double c00 = cpcq - two * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
-    double c01 = (wvno2 * cpy - cqx) / rho1;
-    double c02 = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1_ * wy) / rho1;
-    double c03 = (cpz - wvno2 * cqw) / rho1;
-    double c04 = -(two * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) / rho2;
-    double c10 = (gmgmk * cpz - gm1sq * cqw) * rho1;
-    double c11 = cpcq;
-    double c12 = gammk * cpz - gamm1_ * cqw;
-    double c13 = -wz;
-    double c14 = c03;
-    double c30 = (gm1sq * cpy - gmgmk * cqx) * rho1;
-    double c31 = -xy;
-    double c32 = gamm1_ * cpy - gammk * cqx;
-    double c33 = c11;
-    double c34 = c01;
-    double c40 = -(two * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2;
-    double c41 = c30;
-    double c42 = -(gammk * gamm1_ * twgm1 * a0pq + gam * gammk * gammk * xz +
-                   gamm1_ * gm1sq * wy) * rho1;
-    double c43 = c10;
-    double c44 = c00;
-    double tt = -two * wvno2;
-    double c20 = tt * c42;
-    double c21 = tt * c32;
-    double c22 = a0 + two * (cpcq - c00);
-    double c23 = tt * c12;
-    double c24 = tt * c02;
-    // ee(i) = sum_j e(j) ca(j,i), in j order
-    double ee0 = 0.0, ee1 = 0.0, ee2 = 0.0, ee3 = 0.0, ee4 = 0.0;
-    ee0 = ee0 + e0 * c00; ee0 = ee0 + e1 * c10; ee0 = ee0 + e2 * c20; ee0 = ee0 + e3 * c30; ee0 = ee0 + e4 * c40;
-    ee1 = ee1 + e0 * c01; ee1 = ee1 + e1 * c11; ee1 = ee1 + e2 * c21; ee1 = ee1 + e3 * c31; ee1 = ee1 + e4 * c41;
-    ee2 = ee2 + e0 * c02; ee2 = ee2 + e1 * c12; ee2 = ee2 + e2 * c22; ee2 = ee2 + e3 * c32; ee2 = ee2 + e4 * c42;
-    ee3 = ee3 + e0 * c03; ee3 = ee3 + e1 * c13; ee3 = ee3 + e2 * c23; ee3 = ee3 + e3 * c33; ee3 = ee3 + e4 * c43;
-    ee4 = ee4 + e0 * c04; ee4 = ee4 + e1 * c14; ee4 = ee4 + e2 * c24; ee4 = ee4 + e3 * c34; ee4 = ee4 + e4 * c44;
-    // normc (surfdisp96.f:1022-1047)
-    double t1 = 0.0;
-    if (fabs(ee0) > t1) t1 = fabs(ee0);
-    if (fabs(ee1) > t1) t1 = fabs(ee1);
-    if (fabs(ee2) > t1) t1 = fabs(ee2);
-    if (fabs(ee3) > t1) t1 = fabs(ee3);
-    if (fabs(ee4) > t1) t1 = fabs(ee4);
-    if (t1 < 1.0e-40) t1 = 1.0;
-    e0 = ee0 / t1;
-    e1 = ee1 / t1;
-    e2 = ee2 / t1;
-    e3 = ee3 / t1;
-    e4 = ee4 / t1;
-  }
-  if (llw != 1) {
-    float4 L = st[0];
-    xka = omega / (double)L.y;
-    wvnop = wvno + xka;
-    wvnom = fabs(wvno - xka);
-    ra = sqrt(wvnop * wvnom);
-    double dpth = (double)L.x;
-    rho1 = (double)L.w;
-    double p = ra * dpth;
-    double cosp, w, x, pex;
-    var_ps(p, ra, wvno, xka, dpth, cosp, w, x, pex);
-    double w0 = -rho1 * w;
-    return cosp * e0 + w0 * e1;
-  }
-  return e0;
-}
 
 // ----------------------------------------------------------------------------
 // Chain state machine.
@@ -367,6 +172,7 @@ enum Phase : int {
   PH_NV_BODY,
   PH_NV_END,
   PH_RETRY_NEXT,
+  PH_DEL1ST,
   PH_DONE
 };
 
@@ -389,9 +195,11 @@ struct ChainState {
   // nevill
   double c3, del3;
   int nev, nctrl, m;
-  double x[11], y[11];
   int phase;
   double ec;  // requested evaluation point
+  // known-root walk: start point of the mode's first period (for del1st)
+  double tfirst, cfirst;
+  int del1st_ok;
 };
 
 
@@ -426,7 +234,7 @@ JDSG_DEV void begin_pass(ChainState &S, double *cs, int kpad) {
 
 // Returns true with S.ec set when a period-equation value is needed at
 // phase velocity S.ec (period S.t1); false when the chain is complete.
-JDSG_DEV bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
+__device__ __attribute__((noinline)) bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
   const int kpad = A.kmax_pad;
   for (;;) {
     switch (S.phase) {
@@ -479,15 +287,46 @@ JDSG_DEV bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
             S.clow = S.cm;
           }
           S.sub = 1;
+          if (k0 == 0) {
+            S.tfirst = S.t1;
+            S.cfirst = S.c1;
+            S.del1st_ok = 0;
+          }
+          if (A.known && !S.retry && A.igr == 0) {
+            long long e = S.e0 + (long long)(S.iq - 1) * S.K + k0;
+            double kr = A.known[(long long)S.variant * A.nent + e];
+            if (kr > 0.0) {  // root already bracketed and polished
+              S.c3 = kr;
+              goto nv_done;
+            }
+            if (S.ifirst == 0 && !S.del1st_ok) {
+              // the polarity reference of this mode's first period was skipped
+              S.omega = kTwoPi / S.tfirst;
+              S.ec = S.cfirst;
+              S.phase = PH_DEL1ST;
+              return true;
+            }
+          }
           S.omega = kTwoPi / S.t1;
           S.ec = S.c1;
           S.phase = PH_GS_FIRST;
           return true;
         }
       }
+      case PH_DEL1ST: {
+        S.del1st = F;
+        S.del1st_ok = 1;
+        S.omega = kTwoPi / S.t1;
+        S.ec = S.c1;
+        S.phase = PH_GS_FIRST;
+        return true;
+      }
       case PH_GS_FIRST: {
         S.del1 = F;
-        if (S.ifirst == 1) S.del1st = S.del1;
+        if (S.ifirst == 1) {
+          S.del1st = S.del1;
+          S.del1st_ok = 1;
+        }
         double plmn = sign1(S.del1st) * sign1(S.del1);
         S.idir = (S.ifirst == 1 || plmn >= 0.0) ? 1 : -1;
         S.phase = PH_SCAN_NEXT;
@@ -568,38 +407,33 @@ JDSG_DEV bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
           S.phase = PH_NV_END;
           return true;
         }
+        // Neville tableau x(y) in the lane's scratch slot
+        double *xx = cs + 2 * kpad, *yy = xx + 11;
         if (S.nev == 2) {
-#pragma unroll
-          for (int j = 1; j <= 10; ++j)
-            if (j == S.m) { S.x[j] = S.c3; S.y[j] = S.del3; }
+          xx[S.m] = S.c3;
+          yy[S.m] = S.del3;
         } else {
-          S.x[0] = S.c1; S.y[0] = S.del1;
-          S.x[1] = S.c2; S.y[1] = S.del2;
+          xx[0] = S.c1; yy[0] = S.del1;
+          xx[1] = S.c2; yy[1] = S.del2;
           S.m = 1;
         }
         // Neville on x(y), newest point at index m (surfdisp96.f:660-668)
-        double ym = 0.0;
-#pragma unroll
-        for (int j = 1; j <= 10; ++j)
-          if (j == S.m) ym = S.y[j];
+        double ym = yy[S.m];
         bool bad = false;
-#pragma unroll
-        for (int j = 9; j >= 0; --j) {
-          if (j < S.m && !bad) {
-            double denom = ym - S.y[j];
-            if (fabs(denom) < 1.0e-10 * fabs(ym)) {
-              bad = true;
-            } else {
-              S.x[j] = (-S.y[j] * S.x[j + 1] + ym * S.x[j]) / denom;
-            }
+        for (int j = S.m - 1; j >= 0; --j) {
+          double denom = ym - yy[j];
+          if (fabs(denom) < 1.0e-10 * fabs(ym)) {
+            bad = true;
+            break;
           }
+          xx[j] = (-yy[j] * xx[j + 1] + ym * xx[j]) / denom;
         }
         if (bad) {
           S.c3 = 0.5 * (S.c1 + S.c2);
           S.nev = 1;
           S.m = 1;
         } else {
-          S.c3 = S.x[0];
+          S.c3 = xx[0];
           S.nev = 2;
           S.m = S.m + 1;
           if (S.m > 10) S.m = 10;
@@ -711,12 +545,10 @@ JDSG_DEV bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
 }
 
 // Decode chain id -> job; returns false when the chain has no work.
-JDSG_DEV bool setup_chain(ChainState &S, const ChainArgs &A, long long g, double *cs) {
+JDSG_DEV bool setup_chain(ChainState &S, const ChainArgs &A, long long g, double *cs, int w) {
   int ncol = A.cm.ncol;
   int col = (int)(g % ncol);
-  long long r = g / ncol;
-  int variant = (int)(r % A.nvar);
-  int w = (int)(r / A.nvar);  // 0 Rayleigh, 1 Love
+  int variant = (int)(g / ncol);
   int nR = A.nmode[2 * col], nL = A.nmode[2 * col + 1];
   int M = (w == 0) ? nR : nL;
   int K = A.per_off[col + 1] - A.per_off[col];
@@ -736,6 +568,9 @@ JDSG_DEV bool setup_chain(ChainState &S, const ChainArgs &A, long long g, double
   S.betmx = A.cm.betmx[col];
   S.del1st = 0.0;
   S.retry = 0;
+  S.del1st_ok = 0;
+  S.tfirst = 0.0;
+  S.cfirst = 0.0;
   S.exitmask = 0;
   S.need = 0;
   S.Mrun = M;
@@ -745,7 +580,7 @@ JDSG_DEV bool setup_chain(ChainState &S, const ChainArgs &A, long long g, double
   return true;
 }
 
-JDSG_DEV void finish_chain(ChainState &S, const ChainArgs &A, long long g) {
+JDSG_DEV void finish_chain(ChainState &S, const ChainArgs &A, long long g, int w) {
   if (S.exitmask) {
     for (int md = 0; md < S.M; ++md) {
       if (!(S.exitmask & (1u << md))) continue;
@@ -755,54 +590,64 @@ JDSG_DEV void finish_chain(ChainState &S, const ChainArgs &A, long long g) {
       }
     }
   }
-  if (A.ierr_out) A.ierr_out[g] = S.ierr;
+  if (A.ierr_out) A.ierr_out[(long long)w * A.cm.ncol * A.nvar + g] = S.ierr;
 }
 
 }  // namespace
 
-// Persistent chain kernel: lane-level work fetching.  Every trip of the main
-// loop evaluates one period equation per live lane.
-extern "C" __global__ void __launch_bounds__(64)
+// Persistent chain kernel, one launch per wave type (W = 0 Rayleigh, 1 Love):
+// lane-level work fetching.  Every trip of the main loop evaluates one period
+// equation per live lane.  Chain g of the launch is column g % ncol, period
+// scale g / ncol.
+template <int W>
+__global__ void __launch_bounds__(64)
 jdsg_root_chains(ChainArgs A) {
   const long long nlanes = (long long)gridDim.x * blockDim.x;
   const long long lane_id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  double *cs = A.cscr + lane_id * 2 * A.kmax_pad;
+  double *cs = A.cscr + lane_id * (2 * A.kmax_pad + 22);
+  unsigned *counter = A.counter + W;
+  const long long nch = (long long)A.cm.ncol * A.nvar;
   unsigned long long nev = 0;
   ChainState S;
   long long g = lane_id;
   bool live = false;
-  // find first chain with work
-  while (g < A.nchain) {
-    if (setup_chain(S, A, g, cs)) { live = true; break; }
-    if (A.ierr_out) A.ierr_out[g] = 0;
-    g = nlanes + (long long)atomicAdd(A.counter, 1u);
+  while (g < nch) {
+    if (setup_chain(S, A, g, cs, W)) { live = true; break; }
+    if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
+    g = nlanes + (long long)atomicAdd(counter, 1u);
   }
   double F = 0.0;
   bool need = live ? advance(S, A, cs, F) : false;
+  const float4 *base = (W == 0) ? A.cm.chainR : A.cm.chainL;
   while (live) {
     if (need) {
-      const float4 *st = (S.wave == 2 ? A.cm.chainR : A.cm.chainL) + S.col;
+      const float4 *st = base + S.col;
       int mmax = A.cm.nlay[S.col];
       int llw = A.cm.llw[S.col];
       double wvno = S.omega / S.ec;
-      F = (S.wave == 2) ? dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega)
-                        : dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
+      if (W == 0)
+        F = dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
+      else
+        F = dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       ++nev;
       need = advance(S, A, cs, F);
       continue;
     }
-    finish_chain(S, A, g);
+    finish_chain(S, A, g, W);
     live = false;
     for (;;) {
-      g = nlanes + (long long)atomicAdd(A.counter, 1u);
-      if (g >= A.nchain) break;
-      if (setup_chain(S, A, g, cs)) { live = true; break; }
-      if (A.ierr_out) A.ierr_out[g] = 0;
+      g = nlanes + (long long)atomicAdd(counter, 1u);
+      if (g >= nch) break;
+      if (setup_chain(S, A, g, cs, W)) { live = true; break; }
+      if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
     }
     if (live) need = advance(S, A, cs, 0.0);
   }
   if (A.nevals && nev) atomicAdd(A.nevals, nev);
 }
+
+template __global__ void jdsg_root_chains<0>(ChainArgs);
+template __global__ void jdsg_root_chains<1>(ChainArgs);
 
 // ----------------------------------------------------------------------------
 // host launchers
@@ -815,7 +660,12 @@ void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float 
 }
 
 void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st) {
-  hipLaunchKernelGGL(jdsg_root_chains, dim3(nwaves), dim3(64), 0, st, A);
+  // Rayleigh chains, then Love chains (the Love launch is skipped when no
+  // column asks for Love modes, and likewise for Rayleigh)
+  if (A.any_wave[0])
+    hipLaunchKernelGGL(jdsg_root_chains<0>, dim3(nwaves), dim3(64), 0, st, A);
+  if (A.any_wave[1])
+    hipLaunchKernelGGL(jdsg_root_chains<1>, dim3(nwaves), dim3(64), 0, st, A);
 }
 
 __global__ void jdsg_fill_u32(unsigned *p, long long n, unsigned v) {

@@ -25,9 +25,11 @@ struct ChainArgs {
   int *cstat;                 // [nvar * nent] JDSG_EXIT_PATH bits
   int *ierr_out;              // [nchain] or null
   double *cscr;               // [nchain_slots * 2 * kmax_pad]
-  unsigned *counter;          // work counter
+  unsigned *counter;          // work counters [2] (Rayleigh, Love launches)
   unsigned long long *nevals; // period-equation evaluations (flop model)
   double tol;                 // Neville stop rule (1e-6 reference; smaller = exact roots, unrounded)
+  const double *known;        // optional [nvar*nent] roots already found (>0), else solved here
+  int any_wave[2];            // some column asks for Rayleigh / Love modes
   long long nchain;           // ncol * 2 * nvar
 };
 
@@ -79,6 +81,23 @@ struct StJob {
 };
 
 
+struct PolishArgs {
+  ColumnModel cm;
+  const long long *ent_off;
+  const int *per_off;
+  const double *periods;
+  const int *nmode;
+  long long nent;
+  int variant;             // 0: T, 1: T(1+.005), 2: T(1-.005)
+  const double *center;    // [nent] bracket centres (flat c)
+  double half_width;       // relative half width of the bracket
+  double tol;              // polish stop rule
+  double *known;           // [nvar*nent] out: root or 0
+  unsigned *nfail;         // counter of roots left to the plain chain
+  unsigned long long *nevals;
+  long long nitem;         // entries of both waves: items = entries
+};
+
 void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float *thk,
                       const float *vp, const float *vs, const float *rho, int iflsph,
                       const PrepOut &o, hipStream_t st);
@@ -94,3 +113,16 @@ void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N,
                                    const float *node_vp, const float *node_vs, double *vc,
                                    double *vout, double *kernel, int *status, int want_kernel,
                                    hipStream_t st);
+void jdsg_launch_polish(const PolishArgs &A, hipStream_t st);
+void jdsg_launch_mode_order(const ColumnModel &cm, const long long *ent_off, const int *per_off,
+                            const int *nmode, long long nent, int kmax, int variant,
+                            double *known, unsigned *nfail, hipStream_t st);
+void jdsg_launch_pm_centers(const long long *ent_off, const int *per_off, const double *periods,
+                            int ncol, long long nent, const double *known0, const double *pred,
+                            double *cen1, double *cen2, hipStream_t st);
+void jdsg_launch_column_features(int ncol, const int *layer_off, const float *thk,
+                                 const float *vp, const float *vs, const float *rho, float *colf,
+                                 hipStream_t st);
+void jdsg_launch_entry_features(long long e0, long long nrow, int ncol, const long long *ent_off,
+                                const int *per_off, const double *periods, const int *nmode,
+                                const float *colf, uint16_t *X, hipStream_t st);

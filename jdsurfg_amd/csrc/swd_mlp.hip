@@ -1,0 +1,258 @@
+// swd_mlp.hip -- BF16 MLP kernels for the learned root-bracket surrogate.
+//
+// GEMMs run on CDNA matrix cores (v_mfma_f32_16x16x16_bf16: bf16 operands,
+// fp32 accumulation) from 64x64x32 LDS tiles, one workgroup of four waves per
+// output tile, each wave a 32x32 block of 2x2 MFMA tiles.  Operands may be
+// read transposed so one kernel serves the forward X W^T, the input gradient
+// dZ W and the weight gradient dZ^T X.  Epilogues fuse the bias and the GELU.
+// Dimensions are padded by the caller: M, N multiples of 64, K of 32.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float gelu(float z) {
+  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+  float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+  return cdf + z * pdf;
+}
+
+constexpr int BM = 64, BN = 64, BK = 32, LDSK = BK + 4;
+
+}  // namespace
+
+// C(i,j) = sum_k A(i,k) B(k,j);  A(i,k) = TA ? A[k*lda+i] : A[i*lda+k];
+// B(k,j) = TB ? B[j*ldb+k] : B[k*ldb+j].
+// epi 0: C fp32;  1: Z = C + bias (fp32) and Y = gelu(Z) (bf16);
+// epi 2: C + bias fp32;  3: C fp32 accumulated into C (+=);
+// epi 4: Y = gelu(C + bias) bf16 only (inference).
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256)
+jdsg_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
+               const uint16_t *__restrict__ B, int ldb, float *C, int ldc, int epi,
+               const float *__restrict__ bias, uint16_t *Y, int ldy) {
+  __shared__ uint16_t As[BM][LDSK];
+  __shared__ uint16_t Bs[BN][LDSK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int bm = blockIdx.y * BM, bn = blockIdx.x * BN;
+  v4f acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    // stage 64x32 tiles of A and B^T (k contiguous) through LDS
+#pragma unroll
+    for (int r = 0; r < (BM * BK) / 256; ++r) {
+      int e = r * 256 + tid;
+      int i, k;
+      if (!TA) { i = e / BK; k = e % BK; } else { k = e / BM; i = e % BM; }
+      size_t ga = TA ? (size_t)(k0 + k) * lda + (bm + i) : (size_t)(bm + i) * lda + (k0 + k);
+      As[i][k] = A[ga];
+      int j, kb;
+      if (TB) { j = e / BK; kb = e % BK; } else { kb = e / BN; j = e % BN; }
+      size_t gb = TB ? (size_t)(bn + j) * ldb + (k0 + kb) : (size_t)(k0 + kb) * ldb + (bn + j);
+      Bs[j][kb] = B[gb];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      v4s af[2], bfr[2];
+      const int kq = kk + 4 * (lane >> 4);
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi) {
+        const uint16_t *p = &As[wm * 32 + mi * 16 + (lane & 15)][kq];
+        af[mi] = v4s{(short)p[0], (short)p[1], (short)p[2], (short)p[3]};
+      }
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni) {
+        const uint16_t *p = &Bs[wn * 32 + ni * 16 + (lane & 15)][kq];
+        bfr[ni] = v4s{(short)p[0], (short)p[1], (short)p[2], (short)p[3]};
+      }
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[mi], bfr[ni], acc[mi][ni],
+                                                                 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // D(i, j): lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int i = bm + wm * 32 + mi * 16 + 4 * (lane >> 4) + r;
+        int j = bn + wn * 32 + ni * 16 + (lane & 15);
+        if (i >= M || j >= N) continue;
+        float v = acc[mi][ni][r];
+        size_t o = (size_t)i * ldc + j;
+        if (epi == 0) {
+          C[o] = v;
+        } else if (epi == 1) {
+          float z = v + bias[j];
+          C[o] = z;
+          Y[(size_t)i * ldy + j] = f2bf(gelu(z));
+        } else if (epi == 2) {
+          C[o] = v + bias[j];
+        } else if (epi == 4) {
+          Y[(size_t)i * ldy + j] = f2bf(gelu(v + bias[j]));
+        } else {
+          C[o] += v;
+        }
+      }
+}
+
+// dZ = dY * gelu'(Z) -> bf16 (M x N row-major, ld = N)
+__global__ void jdsg_gelu_bwd(long long n, const float *__restrict__ dY,
+                              const float *__restrict__ Z, uint16_t *dZ, float *dZf) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float g = dY[i] * gelu_grad(Z[i]);
+  dZ[i] = f2bf(g);
+  if (dZf) dZf[i] = g;
+}
+
+// column sums of an M x N fp32 matrix (bias gradient): one block per column
+// strip of 64, 256 threads reduce over rows
+__global__ void __launch_bounds__(256)
+jdsg_colsum(int M, int N, const float *__restrict__ X, float *out) {
+  __shared__ float part[4][64];
+  int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  int r0 = threadIdx.x >> 6;
+  float s = 0.f;
+  if (j < N)
+    for (int i = r0; i < M; i += 4) s += X[(size_t)i * N + j];
+  part[r0][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (r0 == 0 && j < N) out[j] = part[0][threadIdx.x] + part[1][threadIdx.x] +
+                                 part[2][threadIdx.x] + part[3][threadIdx.x];
+}
+
+// MSE loss on column 0 of the output (ld = ldo); writes dY (fp32, ld ldo,
+// zero elsewhere) and per-block partial losses
+__global__ void jdsg_mse(int M, int ldo, const float *__restrict__ out,
+                         const float *__restrict__ target, const float *__restrict__ wt,
+                         float *dY, float *loss_part) {
+  __shared__ float red[256];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < M) {
+    float w = wt ? wt[i] : 1.0f;
+    float d = out[(size_t)i * ldo] - target[i];
+    l = w * d * d;
+    dY[(size_t)i * ldo] = 2.0f * w * d / (float)M;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// Adam with fp32 master weights; refreshes the bf16 copy
+__global__ void jdsg_adam(long long n, float *w, float *m, float *v, const float *__restrict__ g,
+                          float lr, float b1, float b2, float eps, float c1, float c2, float wd,
+                          uint16_t *wbf) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i] + wd * w[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wbf) wbf[i] = f2bf(wi);
+}
+
+__global__ void jdsg_f32_to_bf16(long long n, const float *__restrict__ x, uint16_t *y) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// ----------------------------------------------------------------------------
+extern "C" {
+
+const char *jdsg_last_error(void);
+
+// C = op(A) op(B) with the epilogue codes above; hipStream_t as void*.
+int jdsg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
+                   int ldb, float *C, int ldc, int epi, const float *bias, void *Y, int ldy,
+                   void *stream) {
+  if (M % BM || N % BN || K % BK) return 1;
+  dim3 grid(N / BN, M / BM);
+  hipStream_t st = (hipStream_t)stream;
+  const uint16_t *a = (const uint16_t *)A, *b = (const uint16_t *)B;
+  uint16_t *y = (uint16_t *)Y;
+  if (!ta && tb)
+    hipLaunchKernelGGL((jdsg_gemm_bf16<false, true>), grid, dim3(256), 0, st, M, N, K, a, lda, b,
+                       ldb, C, ldc, epi, bias, y, ldy);
+  else if (!ta && !tb)
+    hipLaunchKernelGGL((jdsg_gemm_bf16<false, false>), grid, dim3(256), 0, st, M, N, K, a, lda,
+                       b, ldb, C, ldc, epi, bias, y, ldy);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((jdsg_gemm_bf16<true, false>), grid, dim3(256), 0, st, M, N, K, a, lda, b,
+                       ldb, C, ldc, epi, bias, y, ldy);
+  else
+    hipLaunchKernelGGL((jdsg_gemm_bf16<true, true>), grid, dim3(256), 0, st, M, N, K, a, lda, b,
+                       ldb, C, ldc, epi, bias, y, ldy);
+  return hipGetLastError() != hipSuccess;
+}
+
+int jdsg_gelu_bwd(long long n, const float *dY, const float *Z, void *dZ, float *dZf,
+                  void *stream) {
+  hipLaunchKernelGGL(jdsg_gelu_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, dY, Z, (uint16_t *)dZ, dZf);
+  return hipGetLastError() != hipSuccess;
+}
+
+int jdsg_colsum(int M, int N, const float *X, float *out, void *stream) {
+  hipLaunchKernelGGL(jdsg_colsum, dim3((N + 63) / 64), dim3(256), 0, (hipStream_t)stream, M, N, X,
+                     out);
+  return hipGetLastError() != hipSuccess;
+}
+
+int jdsg_mse(int M, int ldo, const float *out, const float *target, const float *wt, float *dY,
+             float *loss_part, void *stream) {
+  hipLaunchKernelGGL(jdsg_mse, dim3((M + 255) / 256), dim3(256), 0, (hipStream_t)stream, M, ldo,
+                     out, target, wt, dY, loss_part);
+  return hipGetLastError() != hipSuccess;
+}
+
+int jdsg_adam(long long n, float *w, float *m, float *v, const float *g, float lr, float b1,
+              float b2, float eps, int step, float wd, void *wbf, void *stream) {
+  float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(jdsg_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, w, m, v, g, lr, b1, b2, eps, c1, c2, wd,
+                     (uint16_t *)wbf);
+  return hipGetLastError() != hipSuccess;
+}
+
+int jdsg_f32_to_bf16(long long n, const float *x, void *y, void *stream) {
+  hipLaunchKernelGGL(jdsg_f32_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, x, (uint16_t *)y);
+  return hipGetLastError() != hipSuccess;
+}
+
+}  // extern "C"

@@ -139,6 +139,16 @@ class Plan:
             raise RuntimeError(last_error())
         return v.value
 
+    def flat_roots(self):
+        """Flat roots of the last execute: (nvar, nent) array (T, T+, T-)."""
+        nvar = 3 if self.group else 1
+        h = np.zeros(nvar * self.nent, np.float64)
+        L = lib()
+        L.jdsg_plan_flat_roots.argtypes = [C.c_void_p, C.c_void_p]
+        if L.jdsg_plan_flat_roots(self.h, h.ctypes.data):
+            raise RuntimeError(last_error())
+        return h.reshape(nvar, self.nent)
+
     def timings(self):
         """Per-stage milliseconds of the last execute (prep, chains, eigen R,
         eigen L) when JDSG_TIMING=1 was set before the plan was created."""

@@ -43,7 +43,7 @@ class _BatchOut(C.Structure):
                 ("u", C.c_void_p)] + [(n, C.c_void_p) for n in
                                        ("dcda", "dcdb", "dcdr", "dcdh",
                                         "duda", "dudb", "dudr", "dudh")] + \
-               [("status", C.c_void_p)]
+               [("status", C.c_void_p), ("cflat", C.c_void_p)]
 
 
 def available(kind="rest"):
@@ -229,14 +229,14 @@ class Oracle:
         self.lib.or_batch_offsets(C.byref(b), ent_off.ctypes.data, kern_off.ctypes.data)
         ne, nk = int(ent_off[-1]), int(kern_off[-1])
         res = {"c": np.zeros(ne, _d), "u": np.zeros(ne, _d), "status": np.zeros(ne, np.int32),
-               "ent_off": ent_off, "kern_off": kern_off}
+               "ent_off": ent_off, "kern_off": kern_off, "cflat": np.zeros(3 * ne, _d)}
         names = ("dcda", "dcdb", "dcdr", "dcdh", "duda", "dudb", "dudr", "dudh")
         for n in names:
             res[n] = np.zeros(nk if kernels else 0, _d)
         o = _BatchOut(ent_off.ctypes.data, kern_off.ctypes.data, res["c"].ctypes.data,
                       res["u"].ctypes.data,
                       *[(res[n].ctypes.data if kernels else None) for n in names],
-                      res["status"].ctypes.data)
+                      res["status"].ctypes.data, res["cflat"].ctypes.data)
         self.lib.or_batch_solve(self._be(), C.byref(b), C.byref(o), nthreads)
         return res
 

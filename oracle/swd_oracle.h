@@ -53,6 +53,11 @@ int or_surfdisp(const float *thk, const float *vp, const float *vs,
                 const float *rho, int nlayer, const double *t, double *cg,
                 int kmax, int swdtp, int mode, int sphere, int keep_flat);
 
+/* surfdisp96.f:148-221,499-571: flattened model and start velocity */
+void or_chain_model(const float *thk, const float *vp, const float *vs, const float *rho, int n,
+                    int iflsph, float *d, float *a, float *b, float *rhoR, float *rhoL,
+                    float *cc, float *betmx);
+
 /* empirical.cpp:9-34 */
 void or_empirical_relation(float vs, float *vp, float *rho);
 void or_empirical_deriv(float vp, float vs, float *drda, float *dadb);
@@ -142,6 +147,7 @@ typedef struct {
   double *dcda, *dcdb, *dcdr, *dcdh;
   double *duda, *dudb, *dudr, *dudh;
   int *status;
+  double *cflat; /* optional [3*nent]: flat roots at T, T(1+.005), T(1-.005) */
 } or_batch_out;
 
 void or_batch_offsets(const or_batch *b, long long *ent_off,

@@ -673,3 +673,38 @@ void or_empirical_deriv(float vp, float vs, float *drda, float *dadb) {
   *dadb = (float)(2.0947 - 0.8206 * 2 * vs + 0.2683 * 3 * pow(vs, 2) -
                   0.0251 * 4 * pow(vs, 3));
 }
+
+/* Flattened root-search model as surfdisp96 builds it (sphere, then the
+ * betmn/betmx scan and the starting velocity), for tests of the GPU prep. */
+void or_chain_model(const float *thk, const float *vp, const float *vs, const float *rho, int n,
+                    int iflsph, float *d, float *a, float *b, float *rhoR, float *rhoL,
+                    float *cc, float *betmx_out) {
+  chain_ctx X;
+  float *buf = (float *)malloc(sizeof(float) * 7 * (size_t)n);
+  X.mmax = n;
+  X.d = buf; X.a = buf + n; X.b = buf + 2 * n; X.rho = buf + 3 * n;
+  X.rtp = buf + 4 * n; X.dtp = buf + 5 * n; X.btp = buf + 6 * n;
+  for (int i = 0; i < n; ++i) {
+    X.d[i] = thk[i]; X.a[i] = vp[i]; X.b[i] = vs[i]; X.rho[i] = rho[i];
+  }
+  if (iflsph == 1) sphere(&X, 0, 0);
+  int jmn = 0, jsol = 1;
+  float bx = -1.e20f, bn = 1.e20f;
+  for (int i = 0; i < n; ++i) {
+    if (X.b[i] > 0.01f && X.b[i] < bn) { bn = X.b[i]; jmn = i; jsol = 1; }
+    else if (X.b[i] <= 0.01f && X.a[i] < bn) { bn = X.a[i]; jmn = i; jsol = 0; }
+    if (X.b[i] > bx) bx = X.b[i];
+  }
+  float c1 = (jsol == 0) ? bn : gtsolh(X.a[jmn], X.b[jmn]);
+  c1 = 0.95f * c1;
+  c1 = 0.90f * c1;
+  *cc = c1;
+  *betmx_out = bx;
+  for (int w = 1; w <= 2; ++w) {
+    if (iflsph == 1) sphere(&X, w, 1);
+    for (int i = 0; i < n; ++i) (w == 1 ? rhoL : rhoR)[i] = X.rho[i];
+    for (int i = 0; i < n; ++i) X.rho[i] = rho[i];
+  }
+  for (int i = 0; i < n; ++i) { d[i] = X.d[i]; a[i] = X.a[i]; b[i] = X.b[i]; }
+  free(buf);
+}

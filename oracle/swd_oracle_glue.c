@@ -412,6 +412,12 @@ static void solve_column(const or_backend *be, const or_batch *b,
       for (int k = 0; k < K; ++k) {
         long long e = ebase + k;
         long long q = o->kern_off[col] + (e - o->ent_off[col]) * N;
+        if (o->cflat) {
+          long long ne = o->ent_off[b->ncol];
+          o->cflat[e] = cp[k];
+          o->cflat[ne + e] = b->group ? c1[k] : 0.0;
+          o->cflat[2 * ne + e] = b->group ? c2[k] : 0.0;
+        }
         if (err) { zero_entry(o, e, q, N, 4); continue; }
         if (cp[k] == 0.0) { zero_entry(o, e, q, N, 1); continue; }
         /* without the T*(1+-0.005) roots the T outputs stand (phase form)

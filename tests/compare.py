@@ -15,6 +15,7 @@ def gpu_solve(w, layout=None):
     from jdsurfg_amd import swd
     plan, out = swd.solve(w, layout=layout)
     res = {k: v.cpu().numpy() for k, v in out.items()}
+    res["cflat"] = plan.flat_roots().ravel()
     if plan.layout == 0:
         N = int(w.layer_off[1] - w.layer_off[0])
         for k in KERNELS:
@@ -24,9 +25,22 @@ def gpu_solve(w, layout=None):
     return res
 
 
-def kernel_errors(a, b, w, names=KERNELS):
+def same_roots(a, b, w):
+    """Entries whose flat roots (T and, with group kernels, T(1+-.005)) are
+    bit-identical in both solutions."""
+    ne = len(b["c"])
+    nv = 3 if w.group else 1
+    same = np.ones(ne, bool)
+    for v in range(nv):
+        same &= a["cflat"][v * ne:(v + 1) * ne] == b["cflat"][v * ne:(v + 1) * ne]
+    return same
+
+
+def kernel_errors(a, b, w, names=KERNELS, mask=None):
     """max over entries of max|a-b| / max|b| per entry (entries with status 0)."""
     ok = (a["status"] == 0) & (b["status"] == 0)
+    if mask is not None:
+        ok &= mask
     N = np.diff(w.layer_off)
     out = {}
     for n in names:
@@ -47,8 +61,10 @@ def kernel_errors(a, b, w, names=KERNELS):
     return out
 
 
-def scalar_errors(a, b):
+def scalar_errors(a, b, mask=None):
     ok = (a["status"] == 0) & (b["status"] == 0)
+    if mask is not None:
+        ok &= mask
     res = {}
     for n in ("c", "u"):
         if ok.any():
