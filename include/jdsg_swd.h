@@ -134,6 +134,66 @@ int jdsg_get_1d_kernel(int nx, int ny, int nz, const float *vs, const float *dep
                        int ntLg, int nmLg, double *vc, double *vout, double *kernel,
                        int *status);
 
+/* Learned-bracket root finding (no counterpart in the reference: an optional
+ * accelerated form of the root search of surfdisp96.f:213-339 that returns
+ * the same roots in exact-root mode).  An MLP predicts every flat phase
+ * velocity; each prediction is bracketed, sign-checked and polished in FP64,
+ * and the roots that fail (no sign change, mode order, an odd number of
+ * roots skipped below the bracket) are solved by the plain chain.
+ *
+ * Flat roots of the plain solver (nvar*nent doubles, 0 where none; status
+ * as for plans, may be null) -- the training targets. */
+int jdsg_plan_roots(jdsg_plan *plan, const float *thk, const float *vp, const float *vs,
+                    const float *rho, const double *periods, double *c_out, int *stat_out,
+                    void *stream);
+/* MLP input rows, bf16 [nrow][192], for entries e0..e0+nrow-1 or, with rows
+ * non-null, for the entry indices rows[0..nrow); colf is a device scratch
+ * of ncol*144 floats rebuilt when column_pass is 1. */
+int jdsg_plan_features(jdsg_plan *plan, const float *thk, const float *vp, const float *vs,
+                       const float *rho, const double *periods, long long e0, long long nrow,
+                       const long long *rows, void *X, float *colf, int column_pass,
+                       void *stream);
+/* pred[e0 + r] = out[r*ldo]*scale + shift, r < n (network output to c). */
+int jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,
+                double *pred, long long e0, void *stream);
+/* Allocates the surrogate buffers of a plan (once). */
+int jdsg_plan_enable_surrogate(jdsg_plan *plan);
+/* The solve with bracket centres pred[nent] (device); the plan must be in
+ * exact-root mode (tolerance < 1e-6).  Outputs as jdsg_plan_execute. */
+int jdsg_plan_execute_surrogate(jdsg_plan *plan, const float *thk, const float *vp,
+                                const float *vs, const float *rho, const double *periods,
+                                const double *pred, double half_width, double *c, double *u,
+                                int *status, double *dcda, double *dcdb, double *dcdr,
+                                double *dcdh, double *duda, double *dudb, double *dudr,
+                                double *dudh, void *stream);
+/* Roots the last surrogate execute left to the chain, and the period-
+ * equation evaluations of its bracket/polish and chain stages. */
+int jdsg_plan_surrogate_stats(const jdsg_plan *plan, long long *nfail, long long *polish_evals,
+                              long long *chain_evals);
+
+/* BF16 MFMA kernels of the surrogate network (training and inference).
+ * C = op(A) op(B), A(i,k) = ta ? A[k*lda+i] : A[i*lda+k], B(k,j) = tb ?
+ * B[j*ldb+k] : B[k*ldb+j]; M, N multiples of 64, K of 32; bf16 operands,
+ * fp32 accumulation.  epi 0: C; 1: C = Z = acc+bias, Y = bf16(gelu(Z));
+ * 2: C = acc+bias; 3: C += acc; 4: Y = bf16(gelu(acc+bias)) only. */
+int jdsg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, const void *B,
+                   int ldb, float *C, int ldc, int epi, const float *bias, void *Y, int ldy,
+                   void *stream);
+/* dZ = dY * gelu'(Z): bf16 dZ and (if non-null) fp32 dZf. */
+int jdsg_gelu_bwd(long long n, const float *dY, const float *Z, void *dZ, float *dZf,
+                  void *stream);
+/* out[j] = sum_i X[i*N + j] (bias gradients). */
+int jdsg_colsum(int M, int N, const float *X, float *out, void *stream);
+/* Weighted MSE on column 0 of out (ld ldo): dY = 2 w (out - target) / M,
+ * per-256-row partial losses. */
+int jdsg_mse(int M, int ldo, const float *out, const float *target, const float *wt, float *dY,
+             float *loss_part, void *stream);
+/* Adam (bias-corrected; wd adds wd*w to the gradient) on fp32 master
+ * weights; wbf, if non-null, receives the bf16 copy. */
+int jdsg_adam(long long n, float *w, float *m, float *v, const float *g, float lr, float b1,
+              float b2, float eps, int step, float wd, void *wbf, void *stream);
+int jdsg_f32_to_bf16(long long n, const float *x, void *y, void *stream);
+
 const char *jdsg_last_error(void);
 
 #ifdef __cplusplus

@@ -61,6 +61,7 @@ struct jdsg_plan {
   double tol = 1.0e-6;
   // surrogate path buffers (allocated by jdsg_plan_enable_surrogate)
   double *known = nullptr, *cen = nullptr;
+  signed char *sg = nullptr;
   unsigned *sfail = nullptr;
   const double *known_in = nullptr;
   // explicit per-entry periods for the eigen stage (per-column ABI only)
@@ -628,16 +629,27 @@ extern "C" int jdsg_plan_roots(jdsg_plan *P, const float *thk, const float *vp, 
   return 0;
 }
 
+// MLP input rows (bf16, 192 wide) for entries e0..e0+nrow-1, or for the
+// entry indices rows[0..nrow) when rows is not null; column_pass=1 first
+// rebuilds the per-column profile features colf[ncol*144].
 extern "C" int jdsg_plan_features(jdsg_plan *P, const float *thk, const float *vp,
                                   const float *vs, const float *rho, const double *periods,
-                                  long long e0, long long nrow, void *X, float *colf,
-                                  int column_pass, void *stream) {
+                                  long long e0, long long nrow, const long long *rows, void *X,
+                                  float *colf, int column_pass, void *stream) {
   hipStream_t st = (hipStream_t)stream;
-  if (e0 < 0 || e0 + nrow > P->nent) return fail("feature rows out of range");
+  if (!rows && (e0 < 0 || e0 + nrow > P->nent)) return fail("feature rows out of range");
   if (column_pass)
     jdsg_launch_column_features(P->ncol, P->layer_off_d, thk, vp, vs, rho, colf, st);
   jdsg_launch_entry_features(e0, nrow, P->ncol, P->ent_off_d, P->per_off_d, periods, P->nmode_d,
-                             colf, (uint16_t *)X, st);
+                             colf, rows, (uint16_t *)X, st);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// pred[e0 + r] = out[r*ldo]*scale + shift for r < n (MLP output to flat c)
+extern "C" int jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,
+                           double *pred, long long e0, void *stream) {
+  jdsg_launch_denorm(n, out, ldo, scale, shift, pred, e0, (hipStream_t)stream);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -645,7 +657,7 @@ extern "C" int jdsg_plan_features(jdsg_plan *P, const float *thk, const float *v
 extern "C" int jdsg_plan_enable_surrogate(jdsg_plan *P) {
   if (P->known) return 0;
   if (P->dalloc(&P->known, (size_t)P->nvar * P->nent) || P->dalloc(&P->cen, 2 * (size_t)P->nent) ||
-      P->dalloc(&P->sfail, 2))
+      P->dalloc(&P->sg, 3 * (size_t)P->nvar * P->nent) || P->dalloc(&P->sfail, 2))
     return 1;
   return 0;
 }
@@ -674,6 +686,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   Q.nent = P->nent;
   Q.tol = P->tol;
   Q.known = P->known;
+  Q.sg = P->sg;
   Q.nfail = P->sfail;
   Q.nevals = P->nevals + 1;
   Q.nitem = P->nent;
@@ -681,8 +694,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   Q.center = pred;
   Q.half_width = half_width;
   jdsg_launch_polish(Q, st);
-  jdsg_launch_mode_order(P->cm, P->ent_off_d, P->per_off_d, P->nmode_d, P->nent, P->kmax, 0,
-                         P->known, P->sfail, st);
+  jdsg_launch_mode_order(Q, P->kmax, st);
   if (P->group) {
     jdsg_launch_pm_centers(P->ent_off_d, P->per_off_d, periods, P->ncol, P->nent, P->known, pred,
                            P->cen, P->cen + P->nent, st);
@@ -691,8 +703,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
       Q.center = P->cen + (size_t)(v - 1) * P->nent;
       Q.half_width = 0.5 * half_width;
       jdsg_launch_polish(Q, st);
-      jdsg_launch_mode_order(P->cm, P->ent_off_d, P->per_off_d, P->nmode_d, P->nent, P->kmax, v,
-                             P->known, P->sfail, st);
+      jdsg_launch_mode_order(Q, P->kmax, st);
     }
   }
   HIPCHK(hipGetLastError());

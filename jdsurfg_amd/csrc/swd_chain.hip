@@ -598,9 +598,10 @@ JDSG_DEV void finish_chain(ChainState &S, const ChainArgs &A, long long g, int w
 // Persistent chain kernel, one launch per wave type (W = 0 Rayleigh, 1 Love):
 // lane-level work fetching.  Every trip of the main loop evaluates one period
 // equation per live lane.  Chain g of the launch is column g % ncol, period
-// scale g / ncol.
+// scale g / ncol.  Three waves per SIMD (at most 168 VGPRs, a small spill)
+// measured faster than two (202 VGPRs, no spill) or four.
 template <int W>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
 jdsg_root_chains(ChainArgs A) {
   const long long nlanes = (long long)gridDim.x * blockDim.x;
   const long long lane_id = (long long)blockIdx.x * blockDim.x + threadIdx.x;

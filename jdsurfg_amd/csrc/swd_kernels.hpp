@@ -96,6 +96,7 @@ struct PolishArgs {
   unsigned *nfail;         // counter of roots left to the plain chain
   unsigned long long *nevals;
   long long nitem;         // entries of both waves: items = entries
+  signed char *sg;         // [3*nvar*nent] out: sign of F at bracket low, high, floor
 };
 
 void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float *thk,
@@ -114,9 +115,7 @@ void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N,
                                    double *vout, double *kernel, int *status, int want_kernel,
                                    hipStream_t st);
 void jdsg_launch_polish(const PolishArgs &A, hipStream_t st);
-void jdsg_launch_mode_order(const ColumnModel &cm, const long long *ent_off, const int *per_off,
-                            const int *nmode, long long nent, int kmax, int variant,
-                            double *known, unsigned *nfail, hipStream_t st);
+void jdsg_launch_mode_order(const PolishArgs &A, int kmax, hipStream_t st);
 void jdsg_launch_pm_centers(const long long *ent_off, const int *per_off, const double *periods,
                             int ncol, long long nent, const double *known0, const double *pred,
                             double *cen1, double *cen2, hipStream_t st);
@@ -125,4 +124,7 @@ void jdsg_launch_column_features(int ncol, const int *layer_off, const float *th
                                  hipStream_t st);
 void jdsg_launch_entry_features(long long e0, long long nrow, int ncol, const long long *ent_off,
                                 const int *per_off, const double *periods, const int *nmode,
-                                const float *colf, uint16_t *X, hipStream_t st);
+                                const float *colf, const long long *rows, uint16_t *X,
+                                hipStream_t st);
+void jdsg_launch_denorm(long long n, const float *out, int ldo, double scale, double shift,
+                        double *pred, long long e0, hipStream_t st);

@@ -4,10 +4,14 @@
 // period).  Each root is then bracketed at +-half_width around the prediction
 // (one period-equation evaluation per end), accepted only if the period
 // equation changes sign, and polished in FP64 by Illinois false position to
-// |c1-c2| <= tol*c.  Roots that fail (no sign change, beyond betmx, out of
-// mode order) are left to the plain chain, which walks each chain in the
-// reference order and solves only those (see the `known` input of the chain
-// kernel).  Every lane works on an independent root: no sequential chain.
+// |c1-c2| <= tol*c.  A polished root is kept only if it is the root the
+// reference's chain would reach: the mode-order pass checks, with the signs
+// already evaluated, that no odd number of roots lies between the previous
+// mode's bracket (or, for the lowest mode, the chain's floor cm) and this
+// bracket.  Roots that fail any check are left to the plain chain, which walks
+// each chain in the reference order and solves only those (the `known` input
+// of the chain kernel).  Every lane works on an independent root: no
+// sequential chain.
 #include "swd_kernels.hpp"
 #include "swd_dltar.hpp"
 
@@ -15,9 +19,14 @@
 
 namespace {
 constexpr double kTwoPi = 2.0 * 3.141592653589793;
+
+__device__ __forceinline__ void bracket_ends(double cen, double hw, double cm, double *a,
+                                             double *b) {
+  *a = cen * (1.0 - hw);
+  *b = cen * (1.0 + hw);
+  if (*a < cm) *a = cm;
 }
-
-
+}  // namespace
 
 // One lane per entry e (all waves): decode column by binary search.
 extern "C" __global__ void __launch_bounds__(64)
@@ -52,11 +61,20 @@ jdsg_polish(PolishArgs A) {
   double cen = A.center[e];
   double root = 0.0;
   double betmx = (double)A.cm.betmx[col], cm = (double)A.cm.cc[col];
+  signed char *sg = A.sg + ((long long)A.variant * A.nent + e) * 3;
+  signed char s_lo = 0, s_hi = 0, s_floor = 0;
+  int mode = ray ? wm : wm - nR;
+  if (mode == 0) {
+    s_floor = (signed char)sign1(F(cm));
+    ++nev;
+  }
   if (cen > 0.0 && cen < betmx + 0.5) {
-    double a = cen * (1.0 - A.half_width), b = cen * (1.0 + A.half_width);
-    if (a < cm) a = cm;
+    double a, b;
+    bracket_ends(cen, A.half_width, cm, &a, &b);
     double fa = F(a), fb = F(b);
     nev += 2;
+    s_lo = (signed char)sign1(fa);
+    s_hi = (signed char)sign1(fb);
     if (sign1(fa) != sign1(fb)) {
       // Illinois false position
       for (int it = 0; it < 80; ++it) {
@@ -80,6 +98,9 @@ jdsg_polish(PolishArgs A) {
     }
   }
   A.known[(long long)A.variant * A.nent + e] = root;
+  sg[0] = s_lo;
+  sg[1] = s_hi;
+  sg[2] = s_floor;
   if (root == 0.0) nf = 1;
   }
   // one atomic per wavefront for the counters
@@ -93,43 +114,73 @@ jdsg_polish(PolishArgs A) {
   }
 }
 
-// Mode order: c_m must exceed c_{m-1} + 0.01 dc (the chain's floor,
-// surfdisp96.f:270-276); a violation sends that mode and the ones above it
-// back to the plain chain.  One thread per (column, wave, period).
-extern "C" __global__ void jdsg_mode_order(ColumnModel cm, const long long *ent_off,
-                                           const int *per_off, const int *nmode, long long nent,
-                                           int kmax, int variant, double *known,
-                                           unsigned *nfail) {
+// Mode order and root identity, one thread per (column, wave, period).  The
+// chain reaches mode m as the first sign change above mode m-1 (the lowest
+// mode: above its floor cm), with c_m > c_{m-1} + 0.01 dc
+// (surfdisp96.f:270-276).  A polished root is kept only if
+//   - the mode below it is kept (otherwise its chain start is not known),
+//   - the period equation has the same sign at cm and at the bottom of the
+//     lowest mode's bracket, and at the top of bracket m-1 and the bottom of
+//     bracket m, so no odd number of roots is skipped in between,
+//   - brackets m-1 and m do not overlap and c_m clears the floor gap.
+// A failed pairwise check between m-1 and m leaves it open which of the two
+// is off, so both go back to the chain; the first failure sends every mode
+// above it to the chain as well.  What these checks cannot see is an even
+// number of skipped roots with consistent brackets above it.
+extern "C" __global__ void jdsg_mode_order(PolishArgs A, int kmax) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int ncol = cm.ncol;
+  int ncol = A.cm.ncol;
   if (t >= 2LL * ncol * kmax) return;
   int col = (int)(t % ncol);
   long long r = t / ncol;
   int w = (int)(r & 1);
   int k = (int)(r >> 1);
-  int K = per_off[col + 1] - per_off[col];
+  int K = A.per_off[col + 1] - A.per_off[col];
   if (k >= K) return;
-  int nR = nmode[2 * col], nL = nmode[2 * col + 1];
+  int nR = A.nmode[2 * col], nL = A.nmode[2 * col + 1];
   int M = w == 0 ? nR : nL, off = w == 0 ? 0 : nR;
   const double floor_gap = 1.0e-2 * (double)0.005f;
-  double prev = 0.0;
+  const double cm = (double)A.cm.cc[col];
+  double prev = 0.0, prev_hi = cm;
+  double *prev_p = nullptr;
+  int prev_sign = 0;
   bool bad = false;
+  unsigned nf = 0;
   for (int m = 0; m < M; ++m) {
-    long long e = ent_off[col] + (long long)(off + m) * K + k;
-    double *p = known + (long long)variant * nent + e;
+    long long e = A.ent_off[col] + (long long)(off + m) * K + k;
+    long long q = (long long)A.variant * A.nent + e;
+    double *p = A.known + q;
     double v = *p;
-    if (bad) {
-      if (v > 0.0) { *p = 0.0; atomicAdd(nfail, 1u); }
-      continue;
-    }
-    if (v > 0.0 && m > 0 && prev > 0.0 && !(v > prev + floor_gap)) {
+    const signed char *sg = A.sg + q * 3;
+    if (m == 0) prev_sign = sg[2];
+    if (!bad) {
+      if (!(v > 0.0)) {  // not bracketed: this mode and the ones above go to the chain
+        bad = true;
+        continue;
+      }
+      double a, b;
+      bracket_ends(A.center[e], A.half_width, cm, &a, &b);
+      bool ok = prev_sign != 0 && sg[0] == prev_sign &&
+                (m == 0 || (v > prev + floor_gap && a > prev_hi));
+      if (ok) {
+        prev = v;
+        prev_hi = b;
+        prev_sign = sg[1];
+        prev_p = p;
+        continue;
+      }
       bad = true;
-      *p = 0.0;
-      atomicAdd(nfail, 1u);
-      continue;
+      if (prev_p) {
+        *prev_p = 0.0;
+        ++nf;
+      }
     }
-    prev = v;
+    if (v > 0.0) {
+      *p = 0.0;
+      ++nf;
+    }
   }
+  if (nf) atomicAdd(A.nfail, nf);
 }
 
 // Bracket centres for T(1 +- .005) from the polished T roots and the slope
@@ -199,10 +250,11 @@ __device__ __forceinline__ uint16_t f2bf_(float f) {
 extern "C" __global__ void jdsg_entry_features(long long e0, long long nrow, int ncol,
                                                const long long *ent_off, const int *per_off,
                                                const double *periods, const int *nmode,
-                                               const float *colf, uint16_t *X) {
+                                               const float *colf, const long long *rows,
+                                               uint16_t *X) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nrow * 192) return;
-  long long e = e0 + t / 192;
+  long long e = rows ? rows[t / 192] : e0 + t / 192;
   int f = (int)(t % 192);
   int lo = 0, hi = ncol - 1;
   while (lo < hi) {
@@ -228,12 +280,10 @@ extern "C" __global__ void jdsg_entry_features(long long e0, long long nrow, int
 void jdsg_launch_polish(const PolishArgs &A, hipStream_t st) {
   hipLaunchKernelGGL(jdsg_polish, dim3((unsigned)((A.nitem + 63) / 64)), dim3(64), 0, st, A);
 }
-void jdsg_launch_mode_order(const ColumnModel &cm, const long long *ent_off, const int *per_off,
-                            const int *nmode, long long nent, int kmax, int variant,
-                            double *known, unsigned *nfail, hipStream_t st) {
-  long long n = 2LL * cm.ncol * kmax;
-  hipLaunchKernelGGL(jdsg_mode_order, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, cm,
-                     ent_off, per_off, nmode, nent, kmax, variant, known, nfail);
+void jdsg_launch_mode_order(const PolishArgs &A, int kmax, hipStream_t st) {
+  long long n = 2LL * A.cm.ncol * kmax;
+  hipLaunchKernelGGL(jdsg_mode_order, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A,
+                     kmax);
 }
 void jdsg_launch_pm_centers(const long long *ent_off, const int *per_off, const double *periods,
                             int ncol, long long nent, const double *known0, const double *pred,
@@ -249,9 +299,23 @@ void jdsg_launch_column_features(int ncol, const int *layer_off, const float *th
 }
 void jdsg_launch_entry_features(long long e0, long long nrow, int ncol, const long long *ent_off,
                                 const int *per_off, const double *periods, const int *nmode,
-                                const float *colf, uint16_t *X, hipStream_t st) {
+                                const float *colf, const long long *rows, uint16_t *X,
+                                hipStream_t st) {
   long long n = nrow * 192;
   if (n <= 0) return;
   hipLaunchKernelGGL(jdsg_entry_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e0,
-                     nrow, ncol, ent_off, per_off, periods, nmode, colf, X);
+                     nrow, ncol, ent_off, per_off, periods, nmode, colf, rows, X);
+}
+
+// pred[r] = out[r*ldo] * scale + shift  (MLP output column 0 to flat c)
+__global__ void jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,
+                            double *pred, long long e0) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pred[e0 + i] = (double)out[i * ldo] * scale + shift;
+}
+void jdsg_launch_denorm(long long n, const float *out, int ldo, double scale, double shift,
+                        double *pred, long long e0, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(jdsg_denorm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, out,
+                     ldo, scale, shift, pred, e0);
 }
