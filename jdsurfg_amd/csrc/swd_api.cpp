@@ -703,7 +703,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
       Q.center = P->cen + (size_t)(v - 1) * P->nent;
       Q.half_width = 0.5 * half_width;
       jdsg_launch_polish(Q, st);
-      jdsg_launch_mode_order(Q, P->kmax, st);
+          jdsg_launch_mode_order(Q, P->kmax, st);
     }
   }
   HIPCHK(hipGetLastError());
