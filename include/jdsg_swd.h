@@ -148,12 +148,13 @@ int jdsg_plan_roots(jdsg_plan *plan, const float *thk, const float *vp, const fl
                     void *stream);
 /* MLP input rows, bf16 [nrow][192], for entries e0..e0+nrow-1 or, with rows
  * non-null, for the entry indices rows[0..nrow); colf is a device scratch
- * of ncol*144 floats rebuilt when column_pass is 1. */
+ * of ncol*148 floats rebuilt when column_pass is 1. */
 int jdsg_plan_features(jdsg_plan *plan, const float *thk, const float *vp, const float *vs,
                        const float *rho, const double *periods, long long e0, long long nrow,
                        const long long *rows, void *X, float *colf, int column_pass,
                        void *stream);
-/* pred[e0 + r] = out[r*ldo]*scale + shift, r < n (network output to c). */
+/* pred[e0 + r] = exp(out[r*ldo]*scale + shift), r < n (the network predicts
+ * a standardised ln c). */
 int jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,
                 double *pred, long long e0, void *stream);
 /* Allocates the surrogate buffers of a plan (once). */
@@ -184,10 +185,10 @@ int jdsg_gelu_bwd(long long n, const float *dY, const float *Z, void *dZ, float 
                   void *stream);
 /* out[j] = sum_i X[i*N + j] (bias gradients). */
 int jdsg_colsum(int M, int N, const float *X, float *out, void *stream);
-/* Weighted MSE on column 0 of out (ld ldo): dY = 2 w (out - target) / M,
- * per-256-row partial losses. */
-int jdsg_mse(int M, int ldo, const float *out, const float *target, const float *wt, float *dY,
-             float *loss_part, void *stream);
+/* Weighted loss on column 0 of out (ld ldo): squared error, or Huber with
+ * delta > 0; dY = dloss/dout / M, per-256-row partial losses. */
+int jdsg_mse(int M, int ldo, const float *out, const float *target, const float *wt, float delta,
+             float *dY, float *loss_part, void *stream);
 /* Adam (bias-corrected; wd adds wd*w to the gradient) on fp32 master
  * weights; wbf, if non-null, receives the bf16 copy. */
 int jdsg_adam(long long n, float *w, float *m, float *v, const float *g, float lr, float b1,

@@ -631,7 +631,7 @@ extern "C" int jdsg_plan_roots(jdsg_plan *P, const float *thk, const float *vp, 
 
 // MLP input rows (bf16, 192 wide) for entries e0..e0+nrow-1, or for the
 // entry indices rows[0..nrow) when rows is not null; column_pass=1 first
-// rebuilds the per-column profile features colf[ncol*144].
+// rebuilds the per-column profile features colf[ncol*148].
 extern "C" int jdsg_plan_features(jdsg_plan *P, const float *thk, const float *vp,
                                   const float *vs, const float *rho, const double *periods,
                                   long long e0, long long nrow, const long long *rows, void *X,
@@ -646,7 +646,8 @@ extern "C" int jdsg_plan_features(jdsg_plan *P, const float *thk, const float *v
   return 0;
 }
 
-// pred[e0 + r] = out[r*ldo]*scale + shift for r < n (MLP output to flat c)
+// pred[e0 + r] = exp(out[r*ldo]*scale + shift) for r < n (MLP output, a
+// standardised ln c, to flat c)
 extern "C" int jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,
                            double *pred, long long e0, void *stream) {
   jdsg_launch_denorm(n, out, ldo, scale, shift, pred, e0, (hipStream_t)stream);
@@ -689,22 +690,38 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   Q.sg = P->sg;
   Q.nfail = P->sfail;
   Q.nevals = P->nevals + 1;
-  Q.nitem = P->nent;
-  Q.variant = 0;
-  Q.center = pred;
-  Q.half_width = half_width;
-  jdsg_launch_polish(Q, st);
-  jdsg_launch_mode_order(Q, P->kmax, st);
+  Q.grid_K = P->uK;
+  Q.grid_nR = P->unR;
+  Q.centers[0] = pred;
+  Q.hw[0] = half_width;
+  Q.centers[1] = P->cen;
+  Q.centers[2] = P->cen + P->nent;
+  Q.hw[1] = Q.hw[2] = 0.5 * half_width;
+  const int pw = default_waves();
+  auto polish = [&](int v0, int nv) {
+    Q.v0 = v0;
+    Q.nv = nv;
+    for (int w = 0; w < 2; ++w) {
+      if (P->nitem[w] == 0) continue;
+      Q.nitem = P->nitem[w];
+      Q.item_ent = P->uniform ? nullptr : P->items_d[w];
+      jdsg_launch_polish(Q, w, pw, st);
+    }
+  };
+  auto order = [&](int v) {
+    Q.variant = v;
+    Q.center = Q.centers[v];
+    Q.half_width = Q.hw[v];
+    jdsg_launch_mode_order(Q, P->kmax, st);
+  };
+  polish(0, 1);
+  order(0);
   if (P->group) {
     jdsg_launch_pm_centers(P->ent_off_d, P->per_off_d, periods, P->ncol, P->nent, P->known, pred,
                            P->cen, P->cen + P->nent, st);
-    for (int v = 1; v <= 2; ++v) {
-      Q.variant = v;
-      Q.center = P->cen + (size_t)(v - 1) * P->nent;
-      Q.half_width = 0.5 * half_width;
-      jdsg_launch_polish(Q, st);
-          jdsg_launch_mode_order(Q, P->kmax, st);
-    }
+    polish(1, 2);
+    order(1);
+    order(2);
   }
   HIPCHK(hipGetLastError());
   if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));

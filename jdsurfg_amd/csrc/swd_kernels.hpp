@@ -88,15 +88,22 @@ struct PolishArgs {
   const double *periods;
   const int *nmode;
   long long nent;
-  int variant;             // 0: T, 1: T(1+.005), 2: T(1-.005)
-  const double *center;    // [nent] bracket centres (flat c)
-  double half_width;       // relative half width of the bracket
-  double tol;              // polish stop rule
-  double *known;           // [nvar*nent] out: root or 0
-  unsigned *nfail;         // counter of roots left to the plain chain
+  // polish launch: items [0, nitem*nv) of one wave type, variant v0 + item/nitem
+  long long nitem;            // entries of this wave type
+  const long long *item_ent;  // [nitem] entry index, or null: grid order
+  int grid_K, grid_nR;        // grid order: item = ((mode*K)+k)*ncol + col
+  int v0, nv;
+  const double *centers[3];   // [nent] bracket centres (flat c) per variant
+  double hw[3];               // relative half width per variant
+  // mode-order pass: one variant
+  int variant;                // 0: T, 1: T(1+.005), 2: T(1-.005)
+  const double *center;
+  double half_width;
+  double tol;                 // polish stop rule
+  double *known;              // [nvar*nent] out: root or 0
+  unsigned *nfail;            // counter of roots left to the plain chain
   unsigned long long *nevals;
-  long long nitem;         // entries of both waves: items = entries
-  signed char *sg;         // [3*nvar*nent] out: sign of F at bracket low, high, floor
+  signed char *sg;            // [3*nvar*nent] out: sign of F at bracket low, high, floor
 };
 
 void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float *thk,
@@ -114,7 +121,7 @@ void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N,
                                    const float *node_vp, const float *node_vs, double *vc,
                                    double *vout, double *kernel, int *status, int want_kernel,
                                    hipStream_t st);
-void jdsg_launch_polish(const PolishArgs &A, hipStream_t st);
+void jdsg_launch_polish(const PolishArgs &A, int wave, int nwaves, hipStream_t st);
 void jdsg_launch_mode_order(const PolishArgs &A, int kmax, hipStream_t st);
 void jdsg_launch_pm_centers(const long long *ent_off, const int *per_off, const double *periods,
                             int ncol, long long nent, const double *known0, const double *pred,

@@ -121,6 +121,127 @@ jdsg_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
       }
 }
 
+// Large-tile form for M, N multiples of 128 (every hidden layer): 128x128
+// output tiles, four waves each owning a 64x64 block of 4x4 MFMA tiles on the
+// gfx950 K=32 instruction (v_mfma_f32_16x16x32_bf16), 16-byte global loads
+// with the next K step held in registers while the current one computes.  A
+// lane's A fragment is 8 consecutive k of one row and its B fragment the same
+// 8 k of one column, so the product is the same whatever order the hardware
+// sums k in.  Workgroups are numbered so that consecutive tiles of one M row
+// band run on one XCD and share its L2.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+constexpr int BM2 = 128, BN2 = 128, BK2 = 32, LDK2 = BK2 + 8;
+
+// 16-byte chunk c (0..511) of a 128x32 operand tile.  Row-major along k
+// (KMAJOR false: element (r, k) at P[r*ld + k]) or stored k-major (element
+// (r, k) at P[k*ld + r]), the latter scattered into the k-contiguous LDS tile.
+template <bool KMAJOR>
+__device__ __forceinline__ uint4 gload_tile(const uint16_t *__restrict__ P, int ld, int r0,
+                                            int k0, int c) {
+  if (!KMAJOR) {
+    int r = c >> 2, kc = (c & 3) * 8;
+    return *reinterpret_cast<const uint4 *>(P + (size_t)(r0 + r) * ld + k0 + kc);
+  }
+  int k = c >> 4, rc = (c & 15) * 8;
+  return *reinterpret_cast<const uint4 *>(P + (size_t)(k0 + k) * ld + r0 + rc);
+}
+template <bool KMAJOR>
+__device__ __forceinline__ void lstore_tile(uint16_t (*S)[BK2 + 8], uint4 v, int c) {
+  if (!KMAJOR) {
+    int r = c >> 2, kc = (c & 3) * 8;
+    *reinterpret_cast<uint4 *>(&S[r][kc]) = v;
+  } else {
+    int k = c >> 4, rc = (c & 15) * 8;
+    S[rc + 0][k] = (uint16_t)(v.x & 0xFFFFu);
+    S[rc + 1][k] = (uint16_t)(v.x >> 16);
+    S[rc + 2][k] = (uint16_t)(v.y & 0xFFFFu);
+    S[rc + 3][k] = (uint16_t)(v.y >> 16);
+    S[rc + 4][k] = (uint16_t)(v.z & 0xFFFFu);
+    S[rc + 5][k] = (uint16_t)(v.z >> 16);
+    S[rc + 6][k] = (uint16_t)(v.w & 0xFFFFu);
+    S[rc + 7][k] = (uint16_t)(v.w >> 16);
+  }
+}
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256)
+jdsg_gemm_bf16_128(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
+                   const uint16_t *__restrict__ B, int ldb, float *C, int ldc, int epi,
+                   const float *__restrict__ bias, uint16_t *Y, int ldy) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM2][LDK2];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN2][LDK2];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int nN = N / BN2, ntile = nN * (M / BM2);
+  int t = blockIdx.x;
+  if ((ntile & 7) == 0) t = (t & 7) * (ntile >> 3) + (t >> 3);
+  const int bm = (t / nN) * BM2, bn = (t % nN) * BN2;
+  v4f acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  uint4 ra0, ra1, rb0, rb1;
+  const int c0 = tid, c1 = 256 + tid;
+#define JDSG_GLOAD(k0)                                                                      \
+  do {                                                                                      \
+    ra0 = gload_tile<TA>(A, lda, bm, (k0), c0);                                             \
+    ra1 = gload_tile<TA>(A, lda, bm, (k0), c1);                                             \
+    rb0 = gload_tile<!TB>(B, ldb, bn, (k0), c0);                                            \
+    rb1 = gload_tile<!TB>(B, ldb, bn, (k0), c1);                                            \
+  } while (0)
+  JDSG_GLOAD(0);
+  for (int k0 = 0; k0 < K; k0 += BK2) {
+    lstore_tile<TA>(As, ra0, c0);
+    lstore_tile<TA>(As, ra1, c1);
+    lstore_tile<!TB>(Bs, rb0, c0);
+    lstore_tile<!TB>(Bs, rb1, c1);
+    __syncthreads();
+    if (k0 + BK2 < K) JDSG_GLOAD(k0 + BK2);
+#undef JDSG_GLOAD
+    bf8v af[4], bfv[4];
+    const int kq = 8 * (lane >> 4);
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+      af[mi] = *reinterpret_cast<const bf8v *>(&As[wm * 64 + mi * 16 + (lane & 15)][kq]);
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+      bfv[ni] = *reinterpret_cast<const bf8v *>(&Bs[wn * 64 + ni * 16 + (lane & 15)][kq]);
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfv[ni], acc[mi][ni], 0, 0,
+                                                              0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int i = bm + wm * 64 + mi * 16 + 4 * (lane >> 4) + r;
+        int j = bn + wn * 64 + ni * 16 + (lane & 15);
+        float v = acc[mi][ni][r];
+        size_t o = (size_t)i * ldc + j;
+        if (epi == 0) {
+          C[o] = v;
+        } else if (epi == 1) {
+          float z = v + bias[j];
+          C[o] = z;
+          Y[(size_t)i * ldy + j] = f2bf(gelu(z));
+        } else if (epi == 2) {
+          C[o] = v + bias[j];
+        } else if (epi == 4) {
+          Y[(size_t)i * ldy + j] = f2bf(gelu(v + bias[j]));
+        } else {
+          C[o] += v;
+        }
+      }
+}
+
 // dZ = dY * gelu'(Z) -> bf16 (M x N row-major, ld = N)
 __global__ void jdsg_gelu_bwd(long long n, const float *__restrict__ dY,
                               const float *__restrict__ Z, uint16_t *dZ, float *dZf) {
@@ -147,19 +268,27 @@ jdsg_colsum(int M, int N, const float *__restrict__ X, float *out) {
                                  part[2][threadIdx.x] + part[3][threadIdx.x];
 }
 
-// MSE loss on column 0 of the output (ld = ldo); writes dY (fp32, ld ldo,
-// zero elsewhere) and per-block partial losses
+// Loss on column 0 of the output (ld = ldo): squared error, or with
+// delta > 0 the Huber loss (quadratic inside |d| <= delta, linear outside),
+// which keeps rare far-off targets from dominating the fit.  Writes dY (fp32,
+// ld ldo, zero elsewhere) and per-block partial losses.
 __global__ void jdsg_mse(int M, int ldo, const float *__restrict__ out,
                          const float *__restrict__ target, const float *__restrict__ wt,
-                         float *dY, float *loss_part) {
+                         float delta, float *dY, float *loss_part) {
   __shared__ float red[256];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (i < M) {
     float w = wt ? wt[i] : 1.0f;
     float d = out[(size_t)i * ldo] - target[i];
-    l = w * d * d;
-    dY[(size_t)i * ldo] = 2.0f * w * d / (float)M;
+    float ad = fabsf(d);
+    if (delta > 0.f && ad > delta) {
+      l = w * delta * (2.0f * ad - delta);
+      dY[(size_t)i * ldo] = 2.0f * w * copysignf(delta, d) / (float)M;
+    } else {
+      l = w * d * d;
+      dY[(size_t)i * ldo] = 2.0f * w * d / (float)M;
+    }
   }
   red[threadIdx.x] = l;
   __syncthreads();
@@ -201,10 +330,29 @@ int jdsg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, 
                    int ldb, float *C, int ldc, int epi, const float *bias, void *Y, int ldy,
                    void *stream) {
   if (M % BM || N % BN || K % BK) return 1;
-  dim3 grid(N / BN, M / BM);
   hipStream_t st = (hipStream_t)stream;
   const uint16_t *a = (const uint16_t *)A, *b = (const uint16_t *)B;
   uint16_t *y = (uint16_t *)Y;
+  // 16-byte loads need 8-element aligned leading dimensions
+  bool big = M % BM2 == 0 && N % BN2 == 0 && K % BK2 == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
+             ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+  if (big) {
+    dim3 g((unsigned)((M / BM2) * (N / BN2)));
+    if (!ta && tb)
+      hipLaunchKernelGGL((jdsg_gemm_bf16_128<false, true>), g, dim3(256), 0, st, M, N, K, a, lda,
+                         b, ldb, C, ldc, epi, bias, y, ldy);
+    else if (!ta && !tb)
+      hipLaunchKernelGGL((jdsg_gemm_bf16_128<false, false>), g, dim3(256), 0, st, M, N, K, a,
+                         lda, b, ldb, C, ldc, epi, bias, y, ldy);
+    else if (ta && !tb)
+      hipLaunchKernelGGL((jdsg_gemm_bf16_128<true, false>), g, dim3(256), 0, st, M, N, K, a, lda,
+                         b, ldb, C, ldc, epi, bias, y, ldy);
+    else
+      hipLaunchKernelGGL((jdsg_gemm_bf16_128<true, true>), g, dim3(256), 0, st, M, N, K, a, lda,
+                         b, ldb, C, ldc, epi, bias, y, ldy);
+    return hipGetLastError() != hipSuccess;
+  }
+  dim3 grid(N / BN, M / BM);
   if (!ta && tb)
     hipLaunchKernelGGL((jdsg_gemm_bf16<false, true>), grid, dim3(256), 0, st, M, N, K, a, lda, b,
                        ldb, C, ldc, epi, bias, y, ldy);
@@ -233,10 +381,10 @@ int jdsg_colsum(int M, int N, const float *X, float *out, void *stream) {
   return hipGetLastError() != hipSuccess;
 }
 
-int jdsg_mse(int M, int ldo, const float *out, const float *target, const float *wt, float *dY,
-             float *loss_part, void *stream) {
+int jdsg_mse(int M, int ldo, const float *out, const float *target, const float *wt, float delta,
+             float *dY, float *loss_part, void *stream) {
   hipLaunchKernelGGL(jdsg_mse, dim3((M + 255) / 256), dim3(256), 0, (hipStream_t)stream, M, ldo,
-                     out, target, wt, dY, loss_part);
+                     out, target, wt, delta, dY, loss_part);
   return hipGetLastError() != hipSuccess;
 }
 

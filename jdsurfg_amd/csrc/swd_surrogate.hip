@@ -13,7 +13,7 @@
 // of the chain kernel).  Every lane works on an independent root: no
 // sequential chain.
 #include "swd_kernels.hpp"
-#include "swd_dltar.hpp"
+#include "swd_dltar_fast.hpp"
 
 #pragma clang fp contract(off)
 
@@ -28,82 +28,186 @@ __device__ __forceinline__ void bracket_ends(double cen, double hw, double cm, d
 }
 }  // namespace
 
-// One lane per entry e (all waves): decode column by binary search.
-extern "C" __global__ void __launch_bounds__(64)
-jdsg_polish(PolishArgs A) {
-  long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long nev = 0;
-  unsigned nf = 0;
-  if (e < A.nitem) {
-  int ncol = A.cm.ncol;
-  int lo = 0, hi = ncol - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (A.ent_off[mid] <= e) lo = mid; else hi = mid - 1;
-  }
-  int col = lo;
-  int K = A.per_off[col + 1] - A.per_off[col];
-  long long el = e - A.ent_off[col];
-  int wm = (int)(el / K), k = (int)(el % K);
-  int nR = A.nmode[2 * col];
-  bool ray = wm < nR;
-  double T = A.periods[A.per_off[col] + k];
-  if (A.variant == 1) T = T * (1.0 + 0.5 * 0.01);
-  if (A.variant == 2) T = T * (1.0 - 0.5 * 0.01);
-  double omega = kTwoPi / T;
-  const float4 *st = (ray ? A.cm.chainR : A.cm.chainL) + col;
-  int stride = A.cm.ncol_pad, mmax = A.cm.nlay[col], llw = A.cm.llw[col];
-  auto F = [&](double c) {
-    double wv = omega / c;
-    return ray ? jdsg_pe::dltar4(st, stride, mmax, llw, wv, omega)
-               : jdsg_pe::dltar1(st, stride, mmax, llw, wv, omega);
-  };
-  double cen = A.center[e];
-  double root = 0.0;
-  double betmx = (double)A.cm.betmx[col], cm = (double)A.cm.cc[col];
-  signed char *sg = A.sg + ((long long)A.variant * A.nent + e) * 3;
-  signed char s_lo = 0, s_hi = 0, s_floor = 0;
-  int mode = ray ? wm : wm - nR;
-  if (mode == 0) {
-    s_floor = (signed char)sign1(F(cm));
-    ++nev;
-  }
-  if (cen > 0.0 && cen < betmx + 0.5) {
-    double a, b;
-    bracket_ends(cen, A.half_width, cm, &a, &b);
-    double fa = F(a), fb = F(b);
-    nev += 2;
-    s_lo = (signed char)sign1(fa);
-    s_hi = (signed char)sign1(fb);
-    if (sign1(fa) != sign1(fb)) {
-      // Illinois false position
-      for (int it = 0; it < 80; ++it) {
-        double c = (a * fb - b * fa) / (fb - fa);
-        if (!(c > fmin(a, b) && c < fmax(a, b))) c = 0.5 * (a + b);
-        double fc = F(c);
-        ++nev;
-        if (fc == 0.0) { a = b = c; break; }
-        if (sign1(fc) != sign1(fb)) {
-          a = b;
-          fa = fb;
-        } else {
-          fa = 0.5 * fa;
-        }
-        b = c;
-        fb = fc;
-        if (fabs(b - a) <= A.tol * fabs(b)) break;
+// Persistent polish, one launch per wave type (W = 0 Rayleigh, 1 Love).  Each
+// lane walks its items (item = variant-major, then the wave's entries in
+// grid or list order) with a small state machine -- floor sign (lowest mode
+// only), bracket low, bracket high, Illinois steps -- so every trip of the
+// main loop evaluates exactly one period equation per live lane, and a lane
+// that finishes a root takes its next item at once instead of idling until
+// the slowest lane of its wave is done.
+namespace {
+enum { PP_FLOOR = 0, PP_LO, PP_HI, PP_ILL };
+
+struct PLane {
+  long long e;          // entry
+  int col, variant, phase, nit;
+  double omega, cm, betmx, a, b, fa, fb, c;
+  signed char s_floor, s_lo;
+};
+
+// Decode item `it` into the lane; false when `it` is past the last item.
+// Items whose centre is unusable are finished here without an evaluation.
+template <int W>
+JDSG_DEV bool polish_begin(const PolishArgs &A, PLane &S, long long &it, long long stride_items,
+                           unsigned &nf) {
+  const long long total = A.nitem * A.nv;
+  const int ncol = A.cm.ncol;
+  for (; it < total; it += stride_items) {
+    int v = A.v0 + (int)(it / A.nitem);
+    long long i = it % A.nitem;
+    int col, mode, k, K;
+    long long e;
+    if (!A.item_ent) {
+      K = A.grid_K;
+      col = (int)(i % ncol);
+      long long r = i / ncol;
+      k = (int)(r % K);
+      mode = (int)(r / K);
+      e = A.ent_off[col] + (long long)((W == 0 ? 0 : A.grid_nR) + mode) * K + k;
+    } else {
+      e = A.item_ent[i];
+      int lo = 0, hi = ncol - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (A.ent_off[mid] <= e) lo = mid; else hi = mid - 1;
       }
-      root = b;
-      if (!(root > cm) || root > betmx) root = 0.0;
+      col = lo;
+      K = A.per_off[col + 1] - A.per_off[col];
+      long long el = e - A.ent_off[col];
+      int wm = (int)(el / K);
+      k = (int)(el % K);
+      mode = (W == 0) ? wm : wm - A.nmode[2 * col];
+    }
+    double T = A.periods[A.per_off[col] + k];
+    if (v == 1) T = T * (1.0 + 0.5 * 0.01);
+    if (v == 2) T = T * (1.0 - 0.5 * 0.01);
+    S.e = e;
+    S.col = col;
+    S.variant = v;
+    S.omega = kTwoPi / T;
+    S.cm = (double)A.cm.cc[col];
+    S.betmx = (double)A.cm.betmx[col];
+    S.s_floor = 0;
+    S.s_lo = 0;
+    S.nit = 0;
+    double cen = A.centers[v][e];
+    bool usable = cen > 0.0 && cen < S.betmx + 0.5;
+    if (usable) bracket_ends(cen, A.hw[v], S.cm, &S.a, &S.b);
+    if (mode == 0) {
+      S.phase = PP_FLOOR;
+      S.c = S.cm;
+      if (!usable) S.a = S.b = 0.0;  // floor sign still wanted by mode order
+      return true;
+    }
+    if (usable) {
+      S.phase = PP_LO;
+      S.c = S.a;
+      return true;
+    }
+    long long q = (long long)v * A.nent + e;
+    A.known[q] = 0.0;
+    A.sg[q * 3 + 0] = 0;
+    A.sg[q * 3 + 1] = 0;
+    A.sg[q * 3 + 2] = 0;
+    ++nf;
+  }
+  return false;
+}
+
+JDSG_DEV void polish_finish(const PolishArgs &A, const PLane &S, double root, signed char s_hi,
+                            unsigned &nf) {
+  long long q = (long long)S.variant * A.nent + S.e;
+  A.known[q] = root;
+  A.sg[q * 3 + 0] = S.s_lo;
+  A.sg[q * 3 + 1] = s_hi;
+  A.sg[q * 3 + 2] = S.s_floor;
+  if (root == 0.0) ++nf;
+}
+
+// Feed F(S.c); true when the lane's root is finished (results written).
+JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf) {
+  switch (S.phase) {
+    case PP_FLOOR:
+      S.s_floor = (signed char)sign1(F);
+      if (S.a > 0.0) {
+        S.phase = PP_LO;
+        S.c = S.a;
+        return false;
+      }
+      polish_finish(A, S, 0.0, 0, nf);
+      return true;
+    case PP_LO:
+      S.fa = F;
+      S.s_lo = (signed char)sign1(F);
+      S.phase = PP_HI;
+      S.c = S.b;
+      return false;
+    case PP_HI: {
+      S.fb = F;
+      signed char s_hi = (signed char)sign1(F);
+      if (sign1(S.fa) == sign1(S.fb)) {
+        polish_finish(A, S, 0.0, s_hi, nf);
+        return true;
+      }
+      S.phase = PP_ILL;
+      break;
+    }
+    default: {  // Illinois false position step with F = F(S.c)
+      double fc = F;
+      if (fc == 0.0) {
+        S.a = S.b = S.c;
+      } else {
+        if (sign1(fc) != sign1(S.fb)) {
+          S.a = S.b;
+          S.fa = S.fb;
+        } else {
+          S.fa = 0.5 * S.fa;
+        }
+        S.b = S.c;
+        S.fb = fc;
+      }
+      ++S.nit;
+      if (fc == 0.0 || fabs(S.b - S.a) <= A.tol * fabs(S.b) || S.nit >= 80) {
+        double root = S.b;
+        if (!(root > S.cm) || root > S.betmx) root = 0.0;
+        // the bracket changed sign, so F at its top had the sign opposite to
+        // its bottom: that is the sign the mode-order pass compares
+        polish_finish(A, S, root, (signed char)(-S.s_lo), nf);
+        return true;
+      }
+      break;
     }
   }
-  A.known[(long long)A.variant * A.nent + e] = root;
-  sg[0] = s_lo;
-  sg[1] = s_hi;
-  sg[2] = s_floor;
-  if (root == 0.0) nf = 1;
+  double c = (S.a * S.fb - S.b * S.fa) / (S.fb - S.fa);
+  if (!(c > fmin(S.a, S.b) && c < fmax(S.a, S.b))) c = 0.5 * (S.a + S.b);
+  S.c = c;
+  return false;
+}
+}  // namespace
+
+template <int W>
+__global__ void __launch_bounds__(64) jdsg_polish_w(PolishArgs A) {
+  const long long nl = (long long)gridDim.x * blockDim.x;
+  long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long nev = 0;
+  unsigned nf = 0;
+  PLane S;
+  bool live = polish_begin<W>(A, S, it, nl, nf);
+  const int stride = A.cm.ncol_pad;
+  while (__any(live)) {
+    if (live) {
+      const float4 *st = (W == 0 ? A.cm.chainR : A.cm.chainL) + S.col;
+      int mmax = A.cm.nlay[S.col], llw = A.cm.llw[S.col];
+      double wv = S.omega / S.c;
+      double F = (W == 0) ? jdsg_pf::dltar4(st, stride, mmax, llw, wv, S.omega)
+                          : jdsg_pf::dltar1(st, stride, mmax, llw, wv, S.omega);
+      ++nev;
+      if (polish_step(A, S, F, nf)) {
+        it += nl;
+        live = polish_begin<W>(A, S, it, nl, nf);
+      }
+    }
   }
-  // one atomic per wavefront for the counters
   for (int o = 32; o > 0; o >>= 1) {
     nev += __shfl_xor(nev, o);
     nf += __shfl_xor(nf, o);
@@ -213,30 +317,61 @@ extern "C" __global__ void jdsg_pm_centers(const long long *ent_off, const int *
   cen2[e] = c0 * exp(slope * l2);
 }
 
-// Per-column MLP features: Vs, Vp, rho sampled at 48 depths (2 km apart,
-// layer mid-points z = 1, 3, ..., 95 km; below the stack the half-space),
-// normalised.  Entry features append ln T, the wave and the mode one-hot.
+// Per-column MLP features: Vs, Vp and rho averaged (thickness-weighted) over
+// 48 depth bins with graded edges -- 8 of 0.5 km to 4 km, 16 of 1 km to 20 km,
+// 24 of 3 km to 92 km; below the stack the half-space -- so thin soft layers
+// near the surface still show, then normalised; plus the half-space Vs, Vp,
+// rho and the depth of its top.  colf row: [0,144) bins, [144,148) extras.
+// Entry features append ln T, the wave and the mode one-hot.
 // X row layout (ld = 192 bf16): [0,144) profile, 144 ln T, 145-146 wave,
-// 147-150 mode, rest zero.
+// 147-150 mode, 151-154 half-space, rest zero.
+constexpr int kColF = 148;
+
+__device__ __forceinline__ float bin_top(int j) {
+  return j < 8 ? 0.5f * j : (j < 24 ? 4.0f + (j - 8) : 20.0f + 3.0f * (j - 24));
+}
+
 extern "C" __global__ void jdsg_column_features(int ncol, const int *layer_off, const float *thk,
                                                 const float *vp, const float *vs,
                                                 const float *rho, float *colf) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
   int l0 = layer_off[col], N = layer_off[col + 1] - l0;
-  float ztop = 0.f;
+  float *out = colf + (size_t)col * kColF;
   int i = 0;
+  float ltop = 0.f;  // top of layer i
   for (int j = 0; j < 48; ++j) {
-    float z = 1.0f + 2.0f * j;
-    while (i < N - 1 && ztop + thk[l0 + i] <= z) {
-      ztop += thk[l0 + i];
-      ++i;
+    float z0 = bin_top(j), z1 = bin_top(j + 1);
+    float as = 0.f, ap = 0.f, ar = 0.f;
+    float z = z0;
+    while (z < z1) {
+      float lbot = (i < N - 1) ? ltop + thk[l0 + i] : 3.0e38f;
+      float seg = fminf(lbot, z1) - z;
+      if (seg > 0.f) {
+        as += seg * vs[l0 + i];
+        ap += seg * vp[l0 + i];
+        ar += seg * rho[l0 + i];
+        z += seg;
+      }
+      if (lbot <= z1 && i < N - 1) {
+        ltop = lbot;
+        ++i;
+      } else {
+        break;
+      }
     }
-    float s = vs[l0 + i], p = vp[l0 + i], r = rho[l0 + i];
-    colf[(size_t)col * 144 + j] = (s - 3.5f) / 0.8f;
-    colf[(size_t)col * 144 + 48 + j] = (p - 6.2f) / 1.2f;
-    colf[(size_t)col * 144 + 96 + j] = (r - 2.8f) / 0.4f;
+    float inv = 1.0f / (z1 - z0);
+    out[j] = (as * inv - 3.5f) / 0.8f;
+    out[48 + j] = (ap * inv - 6.2f) / 1.2f;
+    out[96 + j] = (ar * inv - 2.8f) / 0.4f;
   }
+  float zh = 0.f;
+  for (int m = 0; m < N - 1; ++m) zh += thk[l0 + m];
+  int h = l0 + N - 1;
+  out[144] = (vs[h] - 4.5f) / 0.5f;
+  out[145] = (vp[h] - 8.0f) / 1.0f;
+  out[146] = (rho[h] - 3.3f) / 0.3f;
+  out[147] = zh / 100.0f - 1.0f;
 }
 
 __device__ __forceinline__ uint16_t f2bf_(float f) {
@@ -245,17 +380,16 @@ __device__ __forceinline__ uint16_t f2bf_(float f) {
   return (uint16_t)(u >> 16);
 }
 
-// One thread per (entry, feature): X[e][f] (bf16, ld 192) for the entries
-// of variant-0 periods.
+// One thread per entry row: decodes (column, wave, mode, period) once and
+// writes the 192 bf16 features of the row as 24 16-byte stores.
 extern "C" __global__ void jdsg_entry_features(long long e0, long long nrow, int ncol,
                                                const long long *ent_off, const int *per_off,
                                                const double *periods, const int *nmode,
                                                const float *colf, const long long *rows,
                                                uint16_t *X) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrow * 192) return;
-  long long e = rows ? rows[t / 192] : e0 + t / 192;
-  int f = (int)(t % 192);
+  if (t >= nrow) return;
+  long long e = rows ? rows[t] : e0 + t;
   int lo = 0, hi = ncol - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -268,17 +402,39 @@ extern "C" __global__ void jdsg_entry_features(long long e0, long long nrow, int
   int nR = nmode[2 * col];
   int wave = wm < nR ? 0 : 1;
   int mode = wave == 0 ? wm : wm - nR;
-  float v = 0.f;
-  if (f < 144) v = colf[(size_t)col * 144 + f];
-  else if (f == 144) v = (float)((log(periods[per_off[col] + k]) - 2.0) / 1.5);
-  else if (f == 145) v = wave == 0 ? 1.f : 0.f;
-  else if (f == 146) v = wave == 1 ? 1.f : 0.f;
-  else if (f >= 147 && f < 151) v = (mode == f - 147) ? 1.f : 0.f;
-  X[t] = f2bf_(v);
+  const float *cf = colf + (size_t)col * kColF;
+  float lnT = (float)((log(periods[per_off[col] + k]) - 2.0) / 1.5);
+  uint4 *dst = reinterpret_cast<uint4 *>(X + (size_t)t * 192);
+  for (int g = 0; g < 24; ++g) {
+    uint32_t w[4];
+    for (int h = 0; h < 4; ++h) {
+      uint32_t pr[2];
+      for (int q = 0; q < 2; ++q) {
+        int f = g * 8 + h * 2 + q;
+        float v = 0.f;
+        if (f < 144) v = cf[f];
+        else if (f == 144) v = lnT;
+        else if (f == 145) v = wave == 0 ? 1.f : 0.f;
+        else if (f == 146) v = wave == 1 ? 1.f : 0.f;
+        else if (f < 151) v = (mode == f - 147) ? 1.f : 0.f;
+        else if (f < 155) v = cf[144 + (f - 151)];
+        pr[q] = f2bf_(v);
+      }
+      w[h] = pr[0] | (pr[1] << 16);
+    }
+    dst[g] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
-void jdsg_launch_polish(const PolishArgs &A, hipStream_t st) {
-  hipLaunchKernelGGL(jdsg_polish, dim3((unsigned)((A.nitem + 63) / 64)), dim3(64), 0, st, A);
+void jdsg_launch_polish(const PolishArgs &A, int wave, int nwaves, hipStream_t st) {
+  long long items = A.nitem * A.nv;
+  if (items <= 0) return;
+  long long need = (items + 63) / 64;
+  unsigned g = (unsigned)(need < nwaves ? need : nwaves);
+  if (wave == 0)
+    hipLaunchKernelGGL(jdsg_polish_w<0>, dim3(g), dim3(64), 0, st, A);
+  else
+    hipLaunchKernelGGL(jdsg_polish_w<1>, dim3(g), dim3(64), 0, st, A);
 }
 void jdsg_launch_mode_order(const PolishArgs &A, int kmax, hipStream_t st) {
   long long n = 2LL * A.cm.ncol * kmax;
@@ -301,7 +457,7 @@ void jdsg_launch_entry_features(long long e0, long long nrow, int ncol, const lo
                                 const int *per_off, const double *periods, const int *nmode,
                                 const float *colf, const long long *rows, uint16_t *X,
                                 hipStream_t st) {
-  long long n = nrow * 192;
+  long long n = nrow;
   if (n <= 0) return;
   hipLaunchKernelGGL(jdsg_entry_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e0,
                      nrow, ncol, ent_off, per_off, periods, nmode, colf, rows, X);
@@ -311,7 +467,7 @@ void jdsg_launch_entry_features(long long e0, long long nrow, int ncol, const lo
 __global__ void jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,
                             double *pred, long long e0) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) pred[e0 + i] = (double)out[i * ldo] * scale + shift;
+  if (i < n) pred[e0 + i] = exp((double)out[i * ldo] * scale + shift);
 }
 void jdsg_launch_denorm(long long n, const float *out, int ldo, double scale, double shift,
                         double *pred, long long e0, hipStream_t st) {

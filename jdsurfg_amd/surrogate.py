@@ -24,7 +24,7 @@ from . import workloads as W
 
 FEAT, HID, OUTP = 192, 512, 64
 DIMS = [(HID, FEAT), (HID, HID), (HID, HID), (HID, HID), (OUTP, HID)]
-C_SHIFT, C_SCALE = 3.5, 1.0
+C_SHIFT, C_SCALE = float(np.log(3.5)), 0.3   # target: (ln c - C_SHIFT) / C_SCALE
 
 
 def _L():
@@ -34,7 +34,7 @@ def _L():
                                                  vp, vp, C.c_int, vp]
     L.jdsg_gelu_bwd.argtypes = [C.c_longlong, vp, vp, vp, vp, vp]
     L.jdsg_colsum.argtypes = [C.c_int, C.c_int, vp, vp, vp]
-    L.jdsg_mse.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+    L.jdsg_mse.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_float, vp, vp, vp]
     L.jdsg_adam.argtypes = [C.c_longlong, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float,
                             C.c_float, C.c_int, C.c_float, vp, vp]
     L.jdsg_f32_to_bf16.argtypes = [C.c_longlong, vp, vp, vp]
@@ -124,14 +124,15 @@ class MLP:
                               _p(d["out"]), OUTP, 2, _p(self.b[4]), None, 0, stream), "gemm")
         return d["out"]
 
-    def loss_backward(self, target, weight=None, stream=None):
-        """MSE on column 0 against target (M,) fp32; fills the gradients and
-        returns the loss tensor (partial sums)."""
+    def loss_backward(self, target, weight=None, delta=0.0, stream=None):
+        """Squared-error (or, delta > 0, Huber) loss on column 0 against
+        target (M,) fp32; fills the gradients and returns the loss tensor
+        (partial sums)."""
         L = _L()
         M = target.shape[0]
         d = self._bufs(M, True)
-        _chk(L.jdsg_mse(M, OUTP, _p(d["out"]), _p(target), _p(weight), _p(d["dOut"]),
-                        _p(d["loss"]), stream), "mse")
+        _chk(L.jdsg_mse(M, OUTP, _p(d["out"]), _p(target), _p(weight), float(delta),
+                        _p(d["dOut"]), _p(d["loss"]), stream), "loss")
         _chk(L.jdsg_f32_to_bf16(M * OUTP, _p(d["dOut"]), _p(d["dOutbf"]), stream), "cvt")
         Y = d["Y"]
         # output layer
@@ -165,7 +166,10 @@ class MLP:
                              None, stream), "adam")
 
     def save(self, path):
-        np.savez(path, **{f"W{i}": w.cpu().numpy() for i, w in enumerate(self.W)},
+        """Weights as bf16 bit patterns (what inference uses) plus fp32 biases."""
+        import torch
+        np.savez(path, **{f"W{i}": w.view(torch.int16).cpu().numpy()
+                          for i, w in enumerate(self.Wbf)},
                  **{f"b{i}": b.cpu().numpy() for i, b in enumerate(self.b)})
 
     @classmethod
@@ -174,46 +178,80 @@ class MLP:
         m = cls()
         z = np.load(path)
         for i in range(len(DIMS)):
-            m.W[i].copy_(torch.from_numpy(z[f"W{i}"]))
+            wb = torch.from_numpy(z[f"W{i}"]).view(torch.bfloat16).cuda()
+            m.Wbf[i] = wb.contiguous()
+            m.W[i].copy_(wb.float())
             m.b[i].copy_(torch.from_numpy(z[f"b{i}"]))
-            m.Wbf[i] = m.W[i].to(torch.bfloat16)
         return m
 
 
-def training_columns(ncol, seed):
-    """Seeded random layered columns spanning the benchmark families:
-    40-70 layers of 0.8-1.6 km, the depth trend with a smooth random field,
-    a +-8 % checkerboard-like step or none, optional soft sediments; 40
-    log-uniform periods in 1-80 s; Rayleigh modes 0-3, Love modes 0-1."""
+def training_columns(ncol, seed, K=40):
+    """Seeded random layered columns spanning the workload families, built
+    vectorised over all columns.  Each column draws
+      - 16-128 finite layers: uniform thickness 0.8-2 km (grid models) or
+        per-layer thickness 0.5-4 km (uneven stacks);
+      - Vs = 2.6 + 0.035 z (capped at 4.6 for half of the uneven stacks) plus
+        one of: a correlated random profile (AR(1) along depth, sd up to
+        0.2 km/s, correlation 5-40 km), four sinusoids (20-120 km, rms up to
+        0.15 km/s), or nothing; a +-(0-10) % step at 20-60 km (checkerboards)
+        with probability 1/2; per-layer noise (sd 0.075) for uneven stacks;
+        and a soft sediment top (0-4 km, 0.8-2.0 km/s) with probability 1/2;
+        Vs kept in 0.6-5.6 km/s;
+      - K log-uniform periods in 1-80 s; Rayleigh modes 0-3, Love modes 0-1.
+    Vp and rho follow Brocher's relations as in the workloads."""
     rng = np.random.default_rng(seed)
-    nl = rng.integers(40, 71, ncol)
-    dz = rng.uniform(0.8, 1.6, ncol)
-    layer_off = np.concatenate([[0], np.cumsum(nl + 1)]).astype(np.int64)
+    uneven = rng.random(ncol) < 0.3
+    nl = rng.integers(16, 129, ncol)
+    nl = np.where(uneven, nl, rng.integers(30, 81, ncol))
+    n1 = nl + 1
+    layer_off = np.concatenate([[0], np.cumsum(n1)]).astype(np.int64)
     tot = int(layer_off[-1])
-    thk = np.empty(tot, np.float32)
-    vs = np.empty(tot, np.float32)
-    K = 40
-    periods = np.sort(np.exp(rng.uniform(0.0, np.log(80.0), (ncol, K))), axis=1)
-    amp = rng.uniform(0.0, 0.12, ncol)
-    wl = rng.uniform(10.0, 60.0, ncol)
-    ph = rng.uniform(0, 2 * np.pi, ncol)
-    step = rng.uniform(-0.08, 0.08, ncol) * (rng.random(ncol) < 0.5)
-    zstep = rng.uniform(5, 60, ncol)
-    sed = rng.random(ncol) < 0.3
+    col = np.repeat(np.arange(ncol), n1)
+    li = np.arange(tot) - layer_off[col]
+    last = li == nl[col]
+    dz = rng.uniform(0.8, 2.0, ncol)
+    h = np.where(uneven[col], rng.uniform(0.5, 4.0, tot), dz[col])
+    h[last] = 0.0
+    ctop = np.cumsum(h) - h                      # running top depth (global)
+    ztop = ctop - ctop[layer_off[:-1]][col]      # per column
+    z = np.where(last, ztop, ztop + 0.5 * h)     # mid-points; half-space at the base
+    trend = 2.6 + 0.035 * z
+    cap = uneven & (rng.random(ncol) < 0.5)
+    trend = np.where(cap[col], np.minimum(trend, 4.6), trend)
+    kind = rng.integers(0, 3, ncol)              # 0 AR(1), 1 sinusoids, 2 none
+    # AR(1) along depth: x_i = r x_{i-1} + sqrt(1 - r^2) e_i, r = exp(-dz/l)
+    sd = rng.uniform(0.0, 0.2, ncol)
+    ell = rng.uniform(5.0, 40.0, ncol)
+    eps = rng.normal(0.0, 1.0, tot)
+    ar = np.empty(tot)
+    first = layer_off[:-1]
+    ar[first] = eps[first]
+    dzl = np.diff(np.concatenate([z, [0.0]]))    # spacing to the next mid-point
+    for i in range(1, int(n1.max())):
+        idx = first[n1 > i] + i
+        r = np.exp(-np.abs(dzl[idx - 1]) / ell[col[idx]])
+        ar[idx] = r * ar[idx - 1] + np.sqrt(1.0 - r * r) * eps[idx]
+    rms = rng.uniform(0.0, 0.15, ncol)
+    fld = np.zeros(tot)
+    for _ in range(4):
+        lam = rng.uniform(20.0, 120.0, ncol)
+        ph = rng.uniform(0.0, 2 * np.pi, ncol)
+        fld += np.sin(2 * np.pi * z / lam[col] + ph[col])
+    pert = np.where(kind[col] == 0, sd[col] * ar,
+                    np.where(kind[col] == 1, rms[col] * np.sqrt(0.5) * fld, 0.0))
+    v = trend + pert
+    stp = rng.random(ncol) < 0.5
+    sgn = rng.choice([-1.0, 1.0], ncol) * rng.uniform(0.0, 0.10, ncol)
+    zst = rng.uniform(20.0, 60.0, ncol)
+    v = v * (1.0 + np.where(stp[col], sgn[col] * np.where(z < zst[col], 1.0, -1.0), 0.0))
+    v = v + np.where(uneven[col], rng.normal(0.0, 0.075, tot), 0.0)
+    sed = rng.random(ncol) < 0.5
     hsed = rng.uniform(0.0, 4.0, ncol)
     vsed = rng.uniform(0.8, 2.0, ncol)
-    for c in range(ncol):
-        n = nl[c]
-        z = np.concatenate([(np.arange(n) + 0.5) * dz[c], [n * dz[c]]])
-        v = (2.6 + 0.035 * z) * (1 + amp[c] * np.sin(2 * np.pi * z / wl[c] + ph[c]))
-        v = v * (1 + step[c] * np.where(z < zstep[c], 1.0, -1.0))
-        if sed[c]:
-            v = np.where(z < hsed[c], vsed[c], v)
-        v = np.clip(v, 0.6, 4.8)
-        s = slice(layer_off[c], layer_off[c + 1])
-        thk[s] = dz[c]
-        thk[layer_off[c + 1] - 1] = 0.0
-        vs[s] = v
+    v = np.where(sed[col] & (z < hsed[col]), vsed[col], v)
+    vs = np.clip(v, 0.6, 5.6).astype(np.float32)
+    thk = h.astype(np.float32)
+    periods = np.sort(np.exp(rng.uniform(0.0, np.log(80.0), (ncol, K))), axis=1)
     vp, rho = W.brocher(vs)
     return W.Workload("train", layer_off, thk, vp, vs, rho,
                       np.arange(ncol + 1) * K, periods.ravel(),
@@ -237,7 +275,7 @@ class Features:
     def __init__(self, plan, inp):
         import torch
         self.plan, self.inp = plan, inp
-        self.colf = torch.empty(plan.ncol * 144, device="cuda")
+        self.colf = torch.empty(plan.ncol * 148, device="cuda")
         self.first = True
 
     def rows(self, rows=None, e0=0, n=None, X=None, stream=None):
@@ -252,7 +290,8 @@ class Features:
         return X
 
 
-def train(ncol=200000, steps=3000, batch=8192, lr=1e-3, seed=7, log=print, valid_cols=4000):
+def train(ncol=200000, steps=3000, batch=8192, lr=1e-3, seed=7, log=print, valid_cols=4000,
+          huber=0.02):
     """Generate data with the plain GPU solver and train the MLP on the GPU.
     Returns (mlp, report)."""
     import torch
@@ -273,9 +312,9 @@ def train(ncol=200000, steps=3000, batch=8192, lr=1e-3, seed=7, log=print, valid
     for it in range(steps):
         idx = tr[torch.randint(0, tr.numel(), (batch,), device="cuda", generator=g)]
         X = feats.rows(rows=idx.contiguous())
-        tgt = ((roots[idx] - C_SHIFT) / C_SCALE).float().contiguous()
+        tgt = ((torch.log(roots[idx]) - C_SHIFT) / C_SCALE).float().contiguous()
         mlp.forward(X, train=True)
-        loss = mlp.loss_backward(tgt)
+        loss = mlp.loss_backward(tgt, delta=huber)
         frac = it / max(1, steps - 1)
         lr_t = lr * (0.02 + 0.98 * 0.5 * (1 + np.cos(np.pi * frac)))
         mlp.adam(lr_t)
@@ -370,3 +409,105 @@ def surrogate_stats(plan):
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
                                "surrogate_mlp.npz")
+
+
+def _timed(fn, reps=1):
+    import torch
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        r = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps, r
+
+
+def compare_solves(w, mlp, half_width=0.005, kernels=("dcdb", "dudb")):
+    """Plain solve (reference tolerance), plain exact-root solve and the
+    surrogate solve of one workload: times (ms, after one warm-up each), the
+    surrogate's fallback statistics, its identity with the plain exact-root
+    roots, and its relative differences to the plain (reference-mode)
+    outputs."""
+    import torch
+    os.environ["JDSG_TIMING"] = "1"
+    layout = "grid" if w.grid is not None else "csr"
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
+    inp = swd.upload(w)
+    ref = plan.alloc_outputs(kernels)
+    out = plan.alloc_outputs(kernels)
+    rep = {"workload": w.name, "columns": w.ncol, "column_periods": w.column_periods()}
+    plan.execute(*inp, ref)
+    rep["plain_ms"], _ = _timed(lambda: plan.execute(*inp, ref))
+    rep["plain_stage_ms"] = [float(x) for x in plan.timings()]
+    plan.set_tolerance(1e-10)
+    plan.execute(*inp, out)
+    rep["plain_exact_ms"], _ = _timed(lambda: plan.execute(*inp, out))
+    exact_roots = torch.from_numpy(plan.flat_roots()[0]).cuda()
+    exact_status = out["status"].clone()
+    surrogate_solve(plan, inp, out, mlp=mlp, half_width=half_width)
+    plan._feats.first = True
+    rep["predict_ms"], pred = _timed(lambda: predict(mlp, plan._feats, e0=0, n=plan.nent))
+    rep["surrogate_ms"], _ = _timed(lambda: surrogate_solve(plan, inp, out, mlp=mlp,
+                                                            half_width=half_width))
+    rep["surrogate_stage_ms"] = [float(x) for x in plan.timings()]
+    st = surrogate_stats(plan)
+    nvar = 3 if w.group else 1
+    rep.update(st)
+    rep["roots"] = nvar * plan.nent
+    has = exact_roots > 0
+    rel = ((pred - exact_roots).abs() / exact_roots)[has].cpu().numpy()
+    rep["pred_rel_err_median"] = float(np.median(rel))
+    rep["pred_rel_err_p99"] = float(np.percentile(rel, 99))
+    rep["pred_within_bracket"] = float(np.mean(rel < half_width))
+    sroots = torch.from_numpy(plan.flat_roots()[0]).cuda()
+    okm = (exact_status == 0) & (out["status"] == 0)
+    same = ((sroots - exact_roots).abs() <= 1e-9 * exact_roots.abs()) | ~has
+    rep["status_identical"] = bool(torch.equal(exact_status, out["status"]))
+    rep["entries_other_root"] = int((~same & okm).sum().item())
+    ok = (ref["status"] == 0) & (out["status"] == 0)
+    for n in ("c", "u"):
+        d = ((out[n] - ref[n]).abs() / ref[n].abs())[ok]
+        rep[f"max_rel_{n}_vs_plain"] = float(d.max().item()) if d.numel() else 0.0
+    rep["speedup_vs_plain"] = rep["plain_ms"] / rep["surrogate_ms"]
+    return rep
+
+
+def main(argv=None):
+    """python -m jdsurfg_amd.surrogate train|eval ...: train the network on
+    the GPU and write its weights, or compare the surrogate solve with the
+    plain one on a benchmark workload."""
+    import argparse
+    import json
+    import torch
+    ap = argparse.ArgumentParser(prog="python -m jdsurfg_amd.surrogate")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--ncol", type=int, default=2_000_000)
+    t.add_argument("--steps", type=int, default=30000)
+    t.add_argument("--batch", type=int, default=8192)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--seed", type=int, default=7)
+    t.add_argument("--out", default=DEFAULT_WEIGHTS)
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    e.add_argument("--workload", default="grid128")
+    a = ap.parse_args(argv)
+    if not torch.cuda.is_available():
+        raise SystemExit("the surrogate runs on the GPU")
+    if a.cmd == "train":
+        mlp, rep = train(ncol=a.ncol, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                         log=lambda m: print(m, flush=True))
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        mlp.save(a.out)
+        rep["weights"] = a.out
+        print(json.dumps(rep), flush=True)
+    else:
+        mlp = MLP.load(a.weights)
+        w = W.BUILDERS[a.workload]()
+        print(json.dumps(compare_solves(w, mlp)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
