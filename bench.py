@@ -7,6 +7,16 @@ Rayleigh modes 0-3 and Love modes 0-1; phase + group velocity + dc/dVs and
 dU/dVs kernels (the Rg/Lg form: three root chains per wave, spherical Earth).
 Synthetic, seeded inputs resident in HBM before the timed region.
 
+Two legs on the same inputs:
+  plain      the reference method on the GPU (dc-grid scan + Neville chains,
+             float-rounded roots), then the eigen/kernel stage;
+  surrogate  the learned-bracket solve: BF16 MFMA network prediction of every
+             root, +-0.5 % bracket, FP64 polish to 1e-10, the plain chain for
+             roots the checks reject, then the same eigen/kernel stage.
+`value` is the surrogate leg (the quoted metric is surrogate-bracket against
+plain); the plain leg, the ratio and the accuracy of the surrogate against
+the plain method in exact-root mode ride along.
+
 Prints one JSON line: throughput in column-periods/s (one column-period = c,
 U, dc/dVs, dU/dVs for every wave type and mode at one period of one column),
 the roofline of the dominant kernel (algorithmic FP64 flops from the frozen
@@ -33,21 +43,20 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=384,
                     help="columns in the CPU baseline sample (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     return ap.parse_args()
 
 
-def flop_model(w, plan, nevals):
-    """Algorithmic flops of the chain launch and the two eigen launches."""
+def flop_model(w):
+    """Algorithmic flops of the two eigen launches; the frozen model."""
     import numpy as np
     with open(os.path.join(ROOT, "jdsurfg_amd", "flops_model.json")) as f:
         fm = json.load(f)
     N = np.diff(w.layer_off).astype(np.float64)
     K = np.diff(w.per_off).astype(np.float64)
     neig = 3 if w.group else 1
-    itemsR = (w.nmode[:, 0] * K * N).sum()
-    itemsL = (w.nmode[:, 1] * K * N).sum()
-    eigR = itemsR * neig * fm["f_eig"]["rayleigh"]
-    eigL = itemsL * neig * fm["f_eig"]["love"]
+    eigR = (w.nmode[:, 0] * K * N).sum() * neig * fm["f_eig"]["rayleigh"]
+    eigL = (w.nmode[:, 1] * K * N).sum() * neig * fm["f_eig"]["love"]
     return fm, eigR, eigL
 
 
@@ -69,6 +78,20 @@ def cpu_baseline(w, ncols, nthreads):
     return sub.column_periods() / dt, kind, len(cols), dt
 
 
+def timed(fn, steps, warmup):
+    """Run fn warmup times, then time `steps` calls between device syncs."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = []
+    for _ in range(steps):
+        per.append(fn())
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, per
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -83,71 +106,123 @@ def main():
 
     os.environ["JDSG_TIMING"] = "1"
     w = W.BUILDERS[args.workload]()
-    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group,
-                    "grid" if w.grid is not None else "csr")
+    layout = "grid" if w.grid is not None else "csr"
+    kern = ("dcdb", "dudb")
     inp = swd.upload(w)
-    out = plan.alloc_outputs(kernels=("dcdb", "dudb"))
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        plan.execute(*inp, out)
-    torch.cuda.synchronize()
-    stage_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.execute(*inp, out)   # each execute synchronises on its last event
-        stage_ms.append(plan.timings())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    st = out["status"].cpu().numpy()
-    nevals = plan.eval_count()
     cps = w.column_periods()
-    value = cps / (ms_step * 1e-3)
-    sm = np.mean(np.array(stage_ms), axis=0)  # prep, chains, eigR, eigL (ms)
-    fm, eigR, eigL = flop_model(w, plan, nevals)
     N = float(np.mean(np.diff(w.layer_off)))
-    # chain flops: evaluations are counted by the kernel; split R/L by the
-    # per-layer cost is not observable, so the Rayleigh figure bounds it.
-    chain_flops = nevals * N * fm["f_disp"]["rayleigh"]
-    stages = {"chains": (sm[1], chain_flops), "eigen_rayleigh": (sm[2], eigR),
-              "eigen_love": (sm[3], eigL)}
-    dom = max(stages, key=lambda k: stages[k][0])
-    dms, dfl = float(stages[dom][0]), float(stages[dom][1])
-    achieved = dfl / (dms * 1e-3) / 1e12
+    fm, eigR, eigL = flop_model(w)
     peak = fm["peak_fp64_tflops"]
+
+    # ---- plain leg (the reference method)
+    plain = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
+    pout = plain.alloc_outputs(kernels=kern)
+
+    def run_plain():
+        plain.execute(*inp, pout)   # synchronises on its last event (timing on)
+        return plain.timings()
+    ms_plain, st_plain = timed(run_plain, args.steps, args.warmup)
+    sm_plain = np.mean(np.array(st_plain), axis=0)  # prep, chains, eigR, eigL
+    nevals = plain.eval_count()
+    status = pout["status"].cpu().numpy()
     line = {
-        "metric": "column-periods/s (c, U, dc/dVs, dU/dVs), plain FP64 method",
-        "value": float(value),
-        "unit": "column-periods/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": float(ms_step),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
+        "metric": "column-periods/s (c, U, dc/dVs, dU/dVs), surrogate-bracketed FP64 solve",
+        "value": None, "unit": "column-periods/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.ncol} columns x {int(N)} layer entries, "
                                f"{int(np.diff(w.per_off)[0])} periods 1-80 s, "
                                "Rayleigh modes 0-3 + Love modes 0-1, phase+group+kernels",
-                   "columns": w.ncol, "column_periods": cps,
-                   "parallelism": "single GPU"},
-        "stage_ms": {"prep": float(sm[0]), "chains": float(sm[1]),
-                     "eigen_rayleigh": float(sm[2]), "eigen_love": float(sm[3])},
-        "period_equation_evals": int(nevals),
-        "status_counts": {str(i): int(c) for i, c in enumerate(np.bincount(st)) if c},
-        "roofline": {"bound": "fp64-valu", "kernel": dom, "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None},
+                   "columns": w.ncol, "column_periods": cps, "parallelism": "single GPU"},
+        "plain": {"value": float(cps / (ms_plain * 1e-3)), "ms_per_step": float(ms_plain),
+                  "stage_ms": {"prep": float(sm_plain[0]), "chains": float(sm_plain[1]),
+                               "eigen_rayleigh": float(sm_plain[2]),
+                               "eigen_love": float(sm_plain[3])},
+                  "period_equation_evals": int(nevals),
+                  "status_counts": {str(i): int(c) for i, c in enumerate(np.bincount(status))
+                                    if c}},
     }
+    # roofline of the plain leg's chain kernel (the plain method's dominant launch)
+    chain_flops = nevals * N * fm["f_disp"]["rayleigh"]
+    line["plain"]["chain_roofline"] = {
+        "achieved": float(chain_flops / (sm_plain[1] * 1e-3) / 1e12), "peak": peak,
+        "unit": "TFLOP/s"}
+
+    dom_ms, dom_name, dom_flops = float(sm_plain[2]), "jdsg_eigen (Rayleigh)", eigR
+    if not args.no_surrogate:
+        from jdsurfg_amd import surrogate as S
+        mlp = S.MLP.load(S.DEFAULT_WEIGHTS)
+        sp = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
+        sp.set_tolerance(1e-10)
+        sout = sp.alloc_outputs(kernels=kern)
+        feats = S.Features(sp, inp)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+        def run_sur():
+            ev[0].record()
+            feats.first = True
+            pred = S.predict(mlp, feats, e0=0, n=sp.nent)
+            ev[1].record()
+            S.surrogate_solve(sp, inp, sout, pred=pred)
+            t = sp.timings()
+            return [ev[0].elapsed_time(ev[1])] + [float(x) for x in t]
+        ms_sur, st_sur = timed(run_sur, args.steps, args.warmup)
+        sm = np.mean(np.array(st_sur), axis=0)  # predict, prep+polish, chains, eigR, eigL
+        stats = S.surrogate_stats(sp)
+        nvar = 3 if w.group else 1
+        # accuracy against the plain method in exact-root mode, same inputs
+        ex = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
+        ex.set_tolerance(1e-10)
+        eout = ex.alloc_outputs(kernels=kern)
+        ex.execute(*inp, eout)
+        torch.cuda.synchronize()
+        ok = (eout["status"] == 0) & (sout["status"] == 0)
+        relc = ((sout["c"] - eout["c"]).abs() / eout["c"].abs())[ok]
+        relu = ((sout["u"] - eout["u"]).abs() / eout["u"].abs())[ok]
+        okp = (pout["status"] == 0) & (sout["status"] == 0)
+        relcp = ((sout["c"] - pout["c"]).abs() / pout["c"].abs())[okp]
+        relup = ((sout["u"] - pout["u"]).abs() / pout["u"].abs())[okp]
+        # MLP inference: 2 * rows * sum(in*out) flops on the MFMA units
+        mlp_flops = 2.0 * sp.nent * sum(o * i for o, i in S.DIMS)
+        line.update({
+            "value": float(cps / (ms_sur * 1e-3)), "ms_per_step": float(ms_sur),
+            "speedup_vs_plain": float(ms_plain / ms_sur),
+            "stage_ms": {"mlp_predict": float(sm[0]), "prep_polish": float(sm[1]),
+                         "fallback_chains": float(sm[2]), "eigen_rayleigh": float(sm[3]),
+                         "eigen_love": float(sm[4])},
+            "surrogate": {"roots": nvar * sp.nent,
+                          "roots_left_to_chain": stats["roots_left_to_chain"],
+                          "fallback_rate": stats["roots_left_to_chain"] / (nvar * sp.nent),
+                          "polish_evals": stats["polish_evals"],
+                          "chain_evals": stats["chain_evals"],
+                          "status_identical_to_plain_exact":
+                              bool(torch.equal(eout["status"], sout["status"])),
+                          "c_max_rel_vs_plain_exact": float(relc.max().item()),
+                          "u_max_rel_vs_plain_exact": float(relu.max().item()),
+                          "c_max_rel_vs_plain": float(relcp.max().item()),
+                          "u_max_rel_vs_plain": float(relup.max().item()),
+                          "u_frac_above_1e-6_vs_plain": float((relup > 1e-6).double().mean()
+                                                              .item()),
+                          "mlp_tflops": float(mlp_flops / (sm[0] * 1e-3) / 1e12),
+                          "mlp_mfma_frac_bf16": float(mlp_flops / (sm[0] * 1e-3) / 2.5e15),
+                          "weights": os.path.relpath(S.DEFAULT_WEIGHTS, ROOT)},
+        })
+        dom_ms = float(sm[3])
+    else:
+        line.update({"value": line["plain"]["value"], "ms_per_step": float(ms_plain),
+                     "metric": "column-periods/s (c, U, dc/dVs, dU/dVs), plain FP64 method"})
+    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    line["roofline"] = {"bound": "fp64-valu", "kernel": dom_name, "achieved": achieved,
+                        "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                        "traffic": None}
     if not args.no_cpu and args.cpu_sample > 0:
         nthr = min(16, os.cpu_count() or 1)
         v, kind, ncols, dt = cpu_baseline(w, args.cpu_sample, nthr)
         line["cpu_baseline"] = {"value": float(v), "unit": "column-periods/s", "cores": nthr,
                                 "kind": kind,
                                 "sample": f"{ncols} strided columns of the same workload "
-                                          f"({ncols * int(np.diff(w.per_off)[0])} column-periods, "
-                                          f"{dt:.1f} s, OpenMP over columns)"}
+                                          f"({ncols * int(np.diff(w.per_off)[0])} "
+                                          f"column-periods, {dt:.1f} s, OpenMP over columns)"}
     print(json.dumps(line), flush=True)
 
 
