@@ -101,6 +101,31 @@ int jdsg_plan_flat_roots(const jdsg_plan *plan, double *host);
  * last execute, when JDSG_TIMING=1 was set at plan creation. */
 int jdsg_plan_timings(const jdsg_plan *plan, float *ms4);
 
+/* Iterated batches (no counterpart in the reference, whose inversion loop
+ * re-runs get_1d_kernel each iteration, surftomo.cpp:27-138).
+ * jdsg_plan_autotune times the chain and eigen launches over candidate
+ * widths on the given inputs, keeps the fastest (reported through
+ * chain_waves / eigen_waves, may be null) and leaves a solve in the outputs.
+ * jdsg_plan_capture records one iteration -- with perturb_amp > 0 first the
+ * model update Vs *= 1 + perturb_amp*u (u uniform in [-1, 1), seeded by seed
+ * and the replay count; Vp, rho by Brocher's relations; buffers updated in
+ * place), then the whole solve -- on a non-null stream into a graph that
+ * jdsg_graph_launch replays with one launch.  Plan and buffers must outlive
+ * the graph. */
+typedef struct jdsg_graph jdsg_graph;
+int jdsg_plan_autotune(jdsg_plan *plan, const float *thk, const float *vp, const float *vs,
+                       const float *rho, const double *periods, double *c, double *u,
+                       int *status, double *dcda, double *dcdb, double *dcdr, double *dcdh,
+                       double *duda, double *dudb, double *dudr, double *dudh, void *stream,
+                       int *chain_waves, int *eigen_waves);
+int jdsg_plan_capture(jdsg_plan *plan, float *thk, float *vp, float *vs, float *rho,
+                      const double *periods, double *c, double *u, int *status, double *dcda,
+                      double *dcdb, double *dcdr, double *dcdh, double *duda, double *dudb,
+                      double *dudr, double *dudh, float perturb_amp, unsigned seed,
+                      void *stream, jdsg_graph **graph);
+int jdsg_graph_launch(jdsg_graph *graph, void *stream);
+void jdsg_graph_destroy(jdsg_graph *graph);
+
 /* SurfTime grid form (replaces SurfTime::get_1d_kernel, SurfaceWave.hpp:103,
  * SurfaceWave.cpp:458-502; depth grid and period tables as set_model /
  * read_swd_data give them, SurfaceWave.hpp:114-115).

@@ -50,6 +50,7 @@ struct jdsg_plan {
   std::vector<void *> allocs;
   // device
   int *layer_off_d = nullptr, *per_off_d = nullptr, *nmode_d = nullptr;
+  long long nlay_total = 0;  // layer entries of all columns
   long long *ent_off_d = nullptr, *kern_off_d = nullptr, *items_d[2] = {nullptr, nullptr};
   PrepOut prep{};
   ColumnModel cm{};
@@ -67,8 +68,9 @@ struct jdsg_plan {
   // explicit per-entry periods for the eigen stage (per-column ABI only)
   double *tv = nullptr;
   double *eig[4] = {nullptr, nullptr, nullptr, nullptr};
-  // timing
+  // timing (event records are skipped while a graph is being captured)
   bool timing = false;
+  bool capturing = false;
   hipEvent_t ev[5] = {};
   float last_ms[4] = {0, 0, 0, 0};
 
@@ -175,6 +177,7 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
     delete P;
     return 1;
   }
+  P->nlay_total = layer_off[ncol];
   std::vector<int> lo(layer_off, layer_off + ncol + 1), po(per_off, per_off + ncol + 1),
       nm(nmode, nmode + 2 * ncol);
   hipError_t err = hipSuccess;
@@ -308,13 +311,13 @@ namespace {
 // Prepare + chains.  igr/retry as in surfdisp96 / surfdisp.
 int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
               const float *rho, const double *periods, int igr, int retry, hipStream_t st) {
-  if (P->timing) HIPCHK(hipEventRecord(P->ev[0], st));
+  if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[0], st));
   jdsg_launch_prep(P->ncol, P->ncol_pad, P->layer_off_d, thk, vp, vs, rho, P->iflsph, P->prep,
                    st);
   jdsg_launch_fill_u32(P->counters, 4, 0u, st);
   jdsg_launch_fill_u32((unsigned *)P->nevals, 2, 0u, st);
   jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, st);
-  if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
+  if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[1], st));
   ChainArgs A{};
   A.cm = P->cm;
   A.ent_off = P->ent_off_d;
@@ -339,7 +342,7 @@ int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
   A.any_wave[1] = P->nitem[1] > 0;
   jdsg_launch_chains(A, P->chain_waves, st);
   HIPCHK(hipGetLastError());
-  if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
+  if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[2], st));
   return 0;
 }
 
@@ -347,7 +350,7 @@ int run_eigen(jdsg_plan *P, const double *periods, double *c, double *u, int *st
               double *const dc[4], double *const du[4], hipStream_t st) {
   for (int w = 0; w < 2; ++w) {
     if (P->nitem[w] == 0) {
-      if (P->timing) HIPCHK(hipEventRecord(P->ev[3 + w], st));
+      if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[3 + w], st));
       continue;
     }
     EigenArgs E{};
@@ -384,7 +387,7 @@ int run_eigen(jdsg_plan *P, const double *periods, double *c, double *u, int *st
     int waves = (int)std::min<long long>(P->eigen_waves, (P->nitem[w] + 63) / 64);
     jdsg_launch_eigen(E, waves, st);
     HIPCHK(hipGetLastError());
-    if (P->timing) HIPCHK(hipEventRecord(P->ev[3 + w], st));
+    if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[3 + w], st));
   }
   return 0;
 }
@@ -403,7 +406,7 @@ extern "C" int jdsg_plan_execute(jdsg_plan *P, const float *thk, const float *vp
   double *dc[4] = {dcda, dcdb, dcdr, dcdh};
   double *du[4] = {duda, dudb, dudr, dudh};
   if (run_eigen(P, periods, c, u, status, dc, du, st)) return 1;
-  if (P->timing) {
+  if (P->timing && !P->capturing) {
     HIPCHK(hipEventSynchronize(P->ev[4]));
     for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
   }
@@ -673,7 +676,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   hipStream_t st = (hipStream_t)stream;
   if (!P->known) return fail("call jdsg_plan_enable_surrogate first");
   if (P->tol >= 1.0e-6) return fail("the surrogate path polishes to exact roots: set tol < 1e-6");
-  if (P->timing) HIPCHK(hipEventRecord(P->ev[0], st));
+  if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[0], st));
   jdsg_launch_prep(P->ncol, P->ncol_pad, P->layer_off_d, thk, vp, vs, rho, P->iflsph, P->prep,
                    st);
   jdsg_launch_fill_u32(P->sfail, 2, 0u, st);
@@ -724,7 +727,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     order(2);
   }
   HIPCHK(hipGetLastError());
-  if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
+  if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[1], st));
   // the chains walk in reference order and solve only the roots left open
   P->known_in = P->known;
   int rc = 0;
@@ -759,11 +762,11 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   }
   P->known_in = nullptr;
   if (rc) return fail("chain launch failed");
-  if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
+  if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[2], st));
   double *dc[4] = {dcda, dcdb, dcdr, dcdh};
   double *du[4] = {duda, dudb, dudr, dudh};
   if (run_eigen(P, periods, c, u, status, dc, du, st)) return 1;
-  if (P->timing) {
+  if (P->timing && !P->capturing) {
     HIPCHK(hipEventSynchronize(P->ev[4]));
     for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
   }
@@ -783,6 +786,161 @@ extern "C" int jdsg_plan_surrogate_stats(const jdsg_plan *P, long long *nfail,
   if (polish_evals) *polish_evals = (long long)ev[1];
   return 0;
 }
+
+// ============================================================================
+// Iterated batch: launch widths picked by measurement, and the per-iteration
+// model update + solve captured once in a hipGraph and replayed.
+void jdsg_launch_perturb(long long n, float *vs, float *vp, float *rho, const int *iter,
+                         float amp, uint32_t seed, hipStream_t st);
+void jdsg_launch_bump(int *iter, hipStream_t st);
+
+namespace {
+// milliseconds of fn() on stream st between two events
+template <class Fn>
+int time_ms(hipStream_t st, Fn fn, float *ms) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, st));
+  int rc = fn();
+  HIPCHK(hipEventRecord(b, st));
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventElapsedTime(ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+}  // namespace
+
+// Autotuner: times the chain stage over candidate launch widths (workgroups
+// of one wave; multiples of the CU count up to the chains available), keeps
+// the fastest, then does the same for the eigen stage on the roots just
+// found, and sizes the scratch for the winners.  The outputs are written as
+// by jdsg_plan_execute (the last candidate's solve).  The chosen widths are
+// returned through chain_waves / eigen_waves (may be null).
+extern "C" int jdsg_plan_autotune(jdsg_plan *P, const float *thk, const float *vp,
+                                  const float *vs, const float *rho, const double *periods,
+                                  double *c, double *u, int *status, double *dcda, double *dcdb,
+                                  double *dcdr, double *dcdh, double *duda, double *dudb,
+                                  double *dudr, double *dudh, void *stream, int *chain_waves,
+                                  int *eigen_waves) {
+  hipStream_t st = (hipStream_t)stream;
+  double *dc[4] = {dcda, dcdb, dcdr, dcdh};
+  double *du[4] = {duda, dudb, dudr, dudh};
+  const int ncu = std::max(1, default_waves() / 16);
+  const long long max_cw = std::max(1LL, ((long long)P->ncol * P->nvar + 63) / 64);
+  const long long max_ew =
+      std::max(1LL, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
+  bool was_timing = P->timing;
+  P->timing = false;
+  auto run = [&]() {
+    if (run_front(P, thk, vp, vs, rho, periods, 0, 1, st)) return 1;
+    return run_eigen(P, periods, c, u, status, dc, du, st);
+  };
+  // warm-up at the current widths
+  float ms = 0.f;
+  int rc = time_ms(st, run, &ms);
+  int best_cw = P->chain_waves, best_ew = P->eigen_waves;
+  float best = 3.0e38f;
+  const int cmul[] = {4, 8, 12, 16, 24};
+  for (int m : cmul) {
+    if (rc) break;
+    int cw = (int)std::min<long long>((long long)m * ncu, max_cw);
+    P->chain_waves = cw;
+    if (P->size_scratch()) { rc = 1; break; }
+    rc = time_ms(st, [&]() { return run_front(P, thk, vp, vs, rho, periods, 0, 1, st); }, &ms);
+    if (!rc && ms < best) { best = ms; best_cw = cw; }
+    if (cw == max_cw) break;
+  }
+  best = 3.0e38f;
+  const int emul[] = {2, 4, 8, 16, 32};
+  for (int m : emul) {
+    if (rc) break;
+    int ew = (int)std::min<long long>((long long)m * ncu, max_ew);
+    P->eigen_waves = ew;
+    if (P->size_scratch()) { rc = 1; break; }
+    rc = time_ms(st, [&]() { return run_eigen(P, periods, c, u, status, dc, du, st); }, &ms);
+    if (!rc && ms < best) { best = ms; best_ew = ew; }
+    if (ew == max_ew) break;
+  }
+  P->chain_waves = best_cw;
+  P->eigen_waves = best_ew;
+  P->timing = was_timing;
+  if (P->size_scratch()) return 1;
+  if (chain_waves) *chain_waves = best_cw;
+  if (eigen_waves) *eigen_waves = best_ew;
+  return rc;
+}
+
+struct jdsg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int *iter = nullptr;
+  ~jdsg_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (iter) (void)hipFree(iter);
+  }
+};
+
+// Captures one iteration -- optionally the model update (Vs *= 1 + amp*u,
+// u uniform in [-1, 1) seeded by `seed` and the replay count; Vp and rho by
+// Brocher's relations) followed by the whole solve of jdsg_plan_execute --
+// on `stream` (which must not be the null stream) into a graph.  Each
+// jdsg_graph_launch then runs one iteration with a single launch; the model
+// buffers are updated in place.  The plan and the buffers must outlive the
+// graph.
+extern "C" int jdsg_plan_capture(jdsg_plan *P, float *thk, float *vp, float *vs, float *rho,
+                                 const double *periods, double *c, double *u, int *status,
+                                 double *dcda, double *dcdb, double *dcdr, double *dcdh,
+                                 double *duda, double *dudb, double *dudr, double *dudh,
+                                 float perturb_amp, unsigned seed, void *stream,
+                                 jdsg_graph **out) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!st) return fail("graph capture needs a non-null stream");
+  if (!thk || !vp || !vs || !rho || !periods || !c || !u || !status)
+    return fail("null input or scalar output pointer");
+  auto *G = new jdsg_graph();
+  if (hipMalloc(&G->iter, sizeof(int)) != hipSuccess ||
+      hipMemset(G->iter, 0, sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    delete G;
+    return fail("graph counter allocation failed");
+  }
+  double *dc[4] = {dcda, dcdb, dcdr, dcdh};
+  double *du[4] = {duda, dudb, dudr, dudh};
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    delete G;
+    return fail(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  }
+  P->capturing = true;
+  int rc = 0;
+  long long nlay = P->nlay_total;
+  if (perturb_amp > 0.f) jdsg_launch_perturb(nlay, vs, vp, rho, G->iter, perturb_amp, seed, st);
+  rc = run_front(P, thk, vp, vs, rho, periods, 0, 1, st);
+  if (!rc) rc = run_eigen(P, periods, c, u, status, dc, du, st);
+  if (perturb_amp > 0.f) jdsg_launch_bump(G->iter, st);
+  P->capturing = false;
+  e = hipStreamEndCapture(st, &G->graph);
+  if (rc || e != hipSuccess) {
+    delete G;
+    return rc ? 1 : fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  e = hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    delete G;
+    return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  *out = G;
+  return 0;
+}
+
+extern "C" int jdsg_graph_launch(jdsg_graph *G, void *stream) {
+  HIPCHK(hipGraphLaunch(G->exec, (hipStream_t)stream));
+  return 0;
+}
+
+extern "C" void jdsg_graph_destroy(jdsg_graph *G) { delete G; }
 
 // ============================================================================
 // SurfTime grid entry point (SurfaceWave.cpp:458-502 get_1d_kernel).

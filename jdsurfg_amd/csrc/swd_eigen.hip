@@ -318,41 +318,102 @@ struct U4 {
   double ur, uz, tz, tr;
 };
 
+// The layer matrices E and E^-1 of evalg (sregn96.f90:649-825, jbdry = 0)
+// have entries that are real multiples of 1, ra, rb, 1/ra, 1/rb, so the
+// energy integrals take them from six scalars as needed instead of holding
+// two complex 4x4 matrices live (which would cost 128 registers per lane).
+struct EScal {
+  cplx ra, rb, ira, irb;  // ra, rb and their reciprocals
+  double k, z, g, g1;     // wavenumber, rho*om^2, gamma, gamma-1
+  double gk, izh;         // z*gamma/k and 0.5/z
+};
+
+JDSG_DEV EScal escal(const Layer &L, double wvno, double om, double om2, double wvno2) {
+  EScal s;
+  double xka = om / L.za;
+  double xkb = (L.zb > 0.0) ? om / L.zb : 0.0;
+  s.ra = csqrt_real(wvno2 - xka * xka);
+  s.rb = csqrt_real(wvno2 - xkb * xkb);
+  s.ira = 1.0 / s.ra;
+  s.irb = (L.iwat == 0) ? 1.0 / s.rb : cmk(0, 0);
+  double gam = L.zb * wvno / om;
+  s.g = 2.0 * (gam * gam);
+  s.g1 = s.g - 1.0;
+  s.k = wvno;
+  s.z = L.zrho * om2;
+  s.gk = s.z * s.g / wvno;
+  s.izh = 0.5 / s.z;
+  return s;
+}
+
+// solid layer entries E(I,J), E^-1(I,J)
+template <int I, int J>
+JDSG_DEV cplx Es(const EScal &s) {
+  if (I == 0) return (J == 0 || J == 2) ? cmk(s.k, 0) : (J == 1 ? s.rb : -s.rb);
+  if (I == 1) return (J == 1 || J == 3) ? cmk(s.k, 0) : (J == 0 ? s.ra : -s.ra);
+  if (I == 2)
+    return (J == 0 || J == 2) ? cmk(s.z * s.g1, 0) : (J == 1 ? s.gk * s.rb : -(s.gk * s.rb));
+  return (J == 1 || J == 3) ? cmk(s.z * s.g1, 0) : (J == 0 ? s.gk * s.ra : -(s.gk * s.ra));
+}
+template <int I, int J>
+JDSG_DEV cplx Eis(const EScal &s) {
+  const double hg = 0.5 * s.g / s.k, hk = s.k * s.izh;
+  if (I == 0)
+    return J == 0 ? cmk(hg, 0) : J == 1 ? (-0.5 * s.g1) * s.ira
+           : J == 2 ? cmk(-s.izh, 0) : hk * s.ira;
+  if (I == 1)
+    return J == 0 ? (-0.5 * s.g1) * s.irb : J == 1 ? cmk(hg, 0)
+           : J == 2 ? hk * s.irb : cmk(-s.izh, 0);
+  if (I == 2)
+    return J == 0 ? cmk(hg, 0) : J == 1 ? (0.5 * s.g1) * s.ira
+           : J == 2 ? cmk(-s.izh, 0) : -(hk * s.ira);
+  return J == 0 ? (0.5 * s.g1) * s.irb : J == 1 ? cmk(hg, 0)
+         : J == 2 ? -(hk * s.irb) : cmk(-s.izh, 0);
+}
+// fluid layer entries (rows and columns 0-1 only)
+template <int I, int J>
+JDSG_DEV cplx Ef(const EScal &s) {
+  if (I == 0) return J == 0 ? s.ra : -s.ra;
+  return cmk(-s.z, 0);
+}
+template <int I, int J>
+JDSG_DEV cplx Eif(const EScal &s) {
+  if (J == 1) return cmk(-s.izh, 0);
+  return I == 0 ? 0.5 * s.ira : -0.5 * s.ira;
+}
+
 // sregn96.f90:1198-1317 intijr for a solid internal layer (typelyr 0)
 template <int I, int J>
-JDSG_DEV double int_solid(const EMat &M, cplx kmpu, cplx kmsu, cplx km1pd, cplx km1sd,
+JDSG_DEV double int_solid(const EScal &s, cplx kmpu, cplx kmsu, cplx km1pd, cplx km1sd,
                           cplx FA, cplx GA, cplx FB, cplx GB, cplx H1, cplx H2) {
-  const cplx(*e)[4] = M.e;
-  cplx c = e[I][0] * e[J][0] * kmpu * kmpu * FA + e[I][2] * e[J][2] * km1pd * km1pd * FA +
-           e[I][1] * e[J][1] * kmsu * kmsu * FB + e[I][3] * e[J][3] * km1sd * km1sd * FB +
-           H1 * ((e[I][0] * e[J][1] + e[I][1] * e[J][0]) * kmpu * kmsu +
-                 (e[I][2] * e[J][3] + e[I][3] * e[J][2]) * km1pd * km1sd) +
-           H2 * ((e[I][0] * e[J][3] + e[I][3] * e[J][0]) * kmpu * km1sd +
-                 (e[I][1] * e[J][2] + e[I][2] * e[J][1]) * km1pd * kmsu) +
-           GA * (e[I][0] * e[J][2] + e[I][2] * e[J][0]) * kmpu * km1pd +
-           GB * (e[I][1] * e[J][3] + e[I][3] * e[J][1]) * kmsu * km1sd;
+  cplx i0 = Es<I, 0>(s), i1 = Es<I, 1>(s), i2 = Es<I, 2>(s), i3 = Es<I, 3>(s);
+  cplx j0 = Es<J, 0>(s), j1 = Es<J, 1>(s), j2 = Es<J, 2>(s), j3 = Es<J, 3>(s);
+  cplx c = i0 * j0 * kmpu * kmpu * FA + i2 * j2 * km1pd * km1pd * FA +
+           i1 * j1 * kmsu * kmsu * FB + i3 * j3 * km1sd * km1sd * FB +
+           H1 * ((i0 * j1 + i1 * j0) * kmpu * kmsu + (i2 * j3 + i3 * j2) * km1pd * km1sd) +
+           H2 * ((i0 * j3 + i3 * j0) * kmpu * km1sd + (i1 * j2 + i2 * j1) * km1pd * kmsu) +
+           GA * (i0 * j2 + i2 * j0) * kmpu * km1pd + GB * (i1 * j3 + i3 * j1) * kmsu * km1sd;
   return c.re;
 }
 // lower half-space (typelyr 1)
 template <int I, int J>
-JDSG_DEV double int_half(const EMat &M, cplx km1pd, cplx km1sd) {
-  const cplx(*e)[4] = M.e;
-  cplx c = e[I][2] * e[J][2] * km1pd * km1pd / (2.0 * M.ra) +
-           (e[I][2] * e[J][3] + e[I][3] * e[J][2]) * km1pd * km1sd / (M.ra + M.rb) +
-           e[I][3] * e[J][3] * km1sd * km1sd / (2.0 * M.rb);
+JDSG_DEV double int_half(const EScal &s, cplx km1pd, cplx km1sd) {
+  cplx i2 = Es<I, 2>(s), i3 = Es<I, 3>(s), j2 = Es<J, 2>(s), j3 = Es<J, 3>(s);
+  cplx c = i2 * j2 * km1pd * km1pd / (2.0 * s.ra) +
+           (i2 * j3 + i3 * j2) * km1pd * km1sd / (s.ra + s.rb) +
+           i3 * j3 * km1sd * km1sd / (2.0 * s.rb);
   return c.re;
 }
 template <int I, int J>
-JDSG_DEV double int_fluid(const EMat &M, int typelyr, cplx kmpu, cplx km1pd, cplx FA,
+JDSG_DEV double int_fluid(const EScal &s, int typelyr, cplx kmpu, cplx km1pd, cplx FA,
                           cplx GA) {
-  const cplx(*e)[4] = M.e;
+  cplx i0 = Ef<I, 0>(s), i1 = Ef<I, 1>(s), j0 = Ef<J, 0>(s), j1 = Ef<J, 1>(s);
   cplx c;
   if (typelyr == 0)
-    c = e[I][0] * e[J][0] * kmpu * kmpu * FA +
-        (e[I][0] * e[J][1] + e[I][1] * e[J][0]) * kmpu * km1pd * GA +
-        e[I][1] * e[J][1] * km1pd * km1pd * FA;
+    c = i0 * j0 * kmpu * kmpu * FA + (i0 * j1 + i1 * j0) * kmpu * km1pd * GA +
+        i1 * j1 * km1pd * km1pd * FA;
   else
-    c = e[I][1] * e[J][1] * km1pd * km1pd / (2.0 * M.ra);
+    c = i1 * j1 * km1pd * km1pd / (2.0 * s.ra);
   return c.re;
 }
 
@@ -366,23 +427,23 @@ JDSG_DEV void layer_energy(const Layer &L, bool allfluid, int typelyr, const U4 
                            const U4 &u1, double om, double om2, double wvno, double wvno2,
                            double c, EnergySums &S, double &ka, double &kb, double &kr,
                            double &ur_fluid) {
-  EMat M;
-  evalg0(L, allfluid, wvno, om, om2, wvno2, M, nullptr);
+  (void)allfluid;
+  const EScal s = escal(L, wvno, om, om2, wvno2);
   double ah = L.za, av = L.za, bv = L.zb, rho = L.zrho, eta = 1.0;
   double TA = L.zrho * L.za * L.za, TC = TA;
   if (L.iwat == 1) {
     double TN = 0.0;
     double TF = TA - 2. * TN;
     double a12 = -(wvno * wvno - om * om / (ah * ah)) / (rho * om * om);
-    cplx km1pd = M.ei[1][0] * u0.uz + M.ei[1][1] * u0.tz;
+    cplx km1pd = Eif<1, 0>(s) * u0.uz + Eif<1, 1>(s) * u0.tz;
     cplx kmpu = cmk(0, 0), FA = cmk(0, 0), GA = cmk(0, 0);
     if (typelyr == 0) {
-      kmpu = M.ei[0][0] * u1.uz + M.ei[0][1] * u1.tz;
-      FA = ffunc(M.ra, L.zd);
-      GA = gfunc(M.ra, L.zd);
+      kmpu = Eif<0, 0>(s) * u1.uz + Eif<0, 1>(s) * u1.tz;
+      FA = ffunc(s.ra, L.zd);
+      GA = gfunc(s.ra, L.zd);
     }
-    double INT11 = int_fluid<0, 0>(M, typelyr, kmpu, km1pd, FA, GA);
-    double INT22 = int_fluid<1, 1>(M, typelyr, kmpu, km1pd, FA, GA);
+    double INT11 = int_fluid<0, 0>(s, typelyr, kmpu, km1pd, FA, GA);
+    double INT22 = int_fluid<1, 1>(s, typelyr, kmpu, km1pd, FA, GA);
     double wr = wvno / (rho * om2);
     double URUR = INT22 * (wr * wr);
     double UZUZ = INT11;
@@ -404,32 +465,35 @@ JDSG_DEV void layer_energy(const Layer &L, bool allfluid, int typelyr, const U4 
   double TL = L.zrho * L.zb * L.zb, TN = TL;
   double TF = TA - 2. * TN;
   double a12 = -wvno, a14 = 1.0 / TL, a21 = wvno * TF / TC, a23 = 1.0 / TC;
-  const cplx(*ei)[4] = M.ei;
-  cplx km1pd = ei[2][0] * u0.ur + ei[2][1] * u0.uz + ei[2][2] * u0.tz + ei[2][3] * u0.tr;
-  cplx km1sd = ei[3][0] * u0.ur + ei[3][1] * u0.uz + ei[3][2] * u0.tz + ei[3][3] * u0.tr;
+  cplx km1pd = Eis<2, 0>(s) * u0.ur + Eis<2, 1>(s) * u0.uz + Eis<2, 2>(s) * u0.tz +
+               Eis<2, 3>(s) * u0.tr;
+  cplx km1sd = Eis<3, 0>(s) * u0.ur + Eis<3, 1>(s) * u0.uz + Eis<3, 2>(s) * u0.tz +
+               Eis<3, 3>(s) * u0.tr;
   double INT11, INT13, INT22, INT24, INT33, INT44;
   if (typelyr == 0) {
-    cplx kmpu = ei[0][0] * u1.ur + ei[0][1] * u1.uz + ei[0][2] * u1.tz + ei[0][3] * u1.tr;
-    cplx kmsu = ei[1][0] * u1.ur + ei[1][1] * u1.uz + ei[1][2] * u1.tz + ei[1][3] * u1.tr;
-    cplx FA = ffunc(M.ra, L.zd);
-    cplx GA = gfunc(M.ra, L.zd);
-    cplx FB = ffunc(M.rb, L.zd);
-    cplx GB = gfunc(M.rb, L.zd);
-    cplx H1 = h1func(M.ra, M.rb, L.zd);
-    cplx H2 = h2func(M.ra, M.rb, L.zd);
-    INT11 = int_solid<0, 0>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT13 = int_solid<0, 2>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT22 = int_solid<1, 1>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT24 = int_solid<1, 3>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT33 = int_solid<2, 2>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT44 = int_solid<3, 3>(M, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    cplx kmpu = Eis<0, 0>(s) * u1.ur + Eis<0, 1>(s) * u1.uz + Eis<0, 2>(s) * u1.tz +
+                Eis<0, 3>(s) * u1.tr;
+    cplx kmsu = Eis<1, 0>(s) * u1.ur + Eis<1, 1>(s) * u1.uz + Eis<1, 2>(s) * u1.tz +
+                Eis<1, 3>(s) * u1.tr;
+    cplx FA = ffunc(s.ra, L.zd);
+    cplx GA = gfunc(s.ra, L.zd);
+    cplx FB = ffunc(s.rb, L.zd);
+    cplx GB = gfunc(s.rb, L.zd);
+    cplx H1 = h1func(s.ra, s.rb, L.zd);
+    cplx H2 = h2func(s.ra, s.rb, L.zd);
+    INT11 = int_solid<0, 0>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT13 = int_solid<0, 2>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT22 = int_solid<1, 1>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT24 = int_solid<1, 3>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT33 = int_solid<2, 2>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT44 = int_solid<3, 3>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
   } else {
-    INT11 = int_half<0, 0>(M, km1pd, km1sd);
-    INT13 = int_half<0, 2>(M, km1pd, km1sd);
-    INT22 = int_half<1, 1>(M, km1pd, km1sd);
-    INT24 = int_half<1, 3>(M, km1pd, km1sd);
-    INT33 = int_half<2, 2>(M, km1pd, km1sd);
-    INT44 = int_half<3, 3>(M, km1pd, km1sd);
+    INT11 = int_half<0, 0>(s, km1pd, km1sd);
+    INT13 = int_half<0, 2>(s, km1pd, km1sd);
+    INT22 = int_half<1, 1>(s, km1pd, km1sd);
+    INT24 = int_half<1, 3>(s, km1pd, km1sd);
+    INT33 = int_half<2, 2>(s, km1pd, km1sd);
+    INT44 = int_half<3, 3>(s, km1pd, km1sd);
   }
   double URUR = INT11;
   double UZUZ = INT22;

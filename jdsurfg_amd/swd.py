@@ -129,6 +129,41 @@ class Plan:
             raise RuntimeError("jdsg_plan_execute: " + last_error())
         return out
 
+    def autotune(self, thk, vp, vs, rho, periods, out, stream=None):
+        """Pick the chain and eigen launch widths by timing candidates on
+        these inputs; returns (chain_waves, eigen_waves).  Leaves a solve in
+        out."""
+        import torch
+        L = lib()
+        L.jdsg_plan_autotune.argtypes = [C.c_void_p] * 18 + [C.POINTER(C.c_int)] * 2
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        cw, ew = C.c_int(), C.c_int()
+        ks = [_ptr(out.get(n)) for n in KERNEL_NAMES]
+        rc = L.jdsg_plan_autotune(self.h, _ptr(thk), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(periods),
+                                  _ptr(out["c"]), _ptr(out["u"]), _ptr(out["status"]), *ks,
+                                  C.c_void_p(stream.cuda_stream), C.byref(cw), C.byref(ew))
+        if rc:
+            raise RuntimeError("jdsg_plan_autotune: " + last_error())
+        return cw.value, ew.value
+
+    def capture(self, thk, vp, vs, rho, periods, out, stream, perturb_amp=0.0, seed=0):
+        """Record one iteration (optional in-place model update, then the
+        solve) on `stream` (a torch.cuda.Stream, not the default stream) into
+        a graph; returns a Graph whose launch() replays it."""
+        L = lib()
+        L.jdsg_plan_capture.argtypes = [C.c_void_p] * 17 + [C.c_float, C.c_uint, C.c_void_p,
+                                                            C.POINTER(C.c_void_p)]
+        g = C.c_void_p()
+        ks = [_ptr(out.get(n)) for n in KERNEL_NAMES]
+        rc = L.jdsg_plan_capture(self.h, _ptr(thk), _ptr(vp), _ptr(vs), _ptr(rho), _ptr(periods),
+                                 _ptr(out["c"]), _ptr(out["u"]), _ptr(out["status"]), *ks,
+                                 float(perturb_amp), int(seed), C.c_void_p(stream.cuda_stream),
+                                 C.byref(g))
+        if rc:
+            raise RuntimeError("jdsg_plan_capture: " + last_error())
+        return Graph(g, self)
+
     def set_tolerance(self, tol):
         if lib().jdsg_plan_set_tolerance(self.h, float(tol)):
             raise RuntimeError(last_error())
@@ -156,6 +191,25 @@ class Plan:
         if lib().jdsg_plan_timings(self.h, ms.ctypes.data):
             return None
         return ms
+
+
+class Graph:
+    """A captured iteration (see Plan.capture)."""
+
+    def __init__(self, h, plan):
+        self.h, self.plan = h, plan
+
+    def launch(self, stream):
+        L = lib()
+        L.jdsg_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+        if L.jdsg_graph_launch(self.h, C.c_void_p(stream.cuda_stream)):
+            raise RuntimeError("jdsg_graph_launch: " + last_error())
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.jdsg_graph_destroy.argtypes = [C.c_void_p]
+            _lib.jdsg_graph_destroy(self.h)
+            self.h = None
 
 
 def upload(w, device="cuda"):

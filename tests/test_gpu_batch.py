@@ -1,0 +1,103 @@
+"""Iterated batched solve: autotuned launch widths and hipGraph replay give
+the same results as eager launches, replay after replay, with eager work in
+between; the captured model update changes Vs by at most the amplitude and
+keeps Vp and rho on Brocher's relations."""
+import numpy as np
+import pytest
+
+from jdsurfg_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+KERN = ("dcdb", "dudb")
+
+
+def _same(a, b):
+    for n in ("c", "u", "status") + KERN:
+        assert np.array_equal(a[n], b[n]), n
+
+
+def _host(out):
+    return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def test_autotune_keeps_results():
+    import torch
+    from jdsurfg_amd import swd
+    w = W.uneven(ncol=300, seed=21)
+    print("case autotune uneven ncol=300", flush=True)
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    inp = swd.upload(w)
+    ref = plan.alloc_outputs(KERN)
+    plan.execute(*inp, ref)
+    torch.cuda.synchronize()
+    ref = _host(ref)
+    out = plan.alloc_outputs(KERN)
+    cw, ew = plan.autotune(*inp, out)
+    torch.cuda.synchronize()
+    print("tuned widths", cw, ew, flush=True)
+    assert cw >= 1 and ew >= 1
+    _same(_host(out), ref)
+    out2 = plan.alloc_outputs(KERN)
+    plan.execute(*inp, out2)
+    torch.cuda.synchronize()
+    _same(_host(out2), ref)
+
+
+def test_graph_replay_matches_eager():
+    import torch
+    from jdsurfg_amd import swd
+    w = W.uneven(ncol=200, seed=22)
+    print("case graph replay uneven ncol=200", flush=True)
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    inp = swd.upload(w)
+    ref = plan.alloc_outputs(KERN)
+    plan.execute(*inp, ref)
+    torch.cuda.synchronize()
+    ref = _host(ref)
+    other = swd.Plan(w.layer_off, w.per_off, w.nmode, False, "csr")
+    oout = other.alloc_outputs(("dcdb",))
+    st = torch.cuda.Stream()
+    out = plan.alloc_outputs(KERN)
+    g = plan.capture(*inp, out, st, perturb_amp=0.0)
+    for rep in range(3):
+        for n in out:
+            out[n].fill_(-1)
+        torch.cuda.synchronize()
+        g.launch(st)
+        st.synchronize()
+        _same(_host(out), ref)
+        other.execute(*inp, oout)       # eager work between replays
+        torch.cuda.synchronize()
+        print("replay", rep, "identical", flush=True)
+
+
+def test_graph_model_update_matches_eager_solve():
+    import torch
+    from jdsurfg_amd import swd
+    w = W.uneven(ncol=150, seed=23)
+    print("case graph with 1 % Vs update, uneven ncol=150", flush=True)
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    inp = list(swd.upload(w))
+    vs0 = inp[2].clone()
+    out = plan.alloc_outputs(KERN)
+    st = torch.cuda.Stream()
+    g = plan.capture(*inp, out, st, perturb_amp=0.01, seed=5)
+    eager = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    eout = eager.alloc_outputs(KERN)
+    prev = vs0.clone()
+    for it in range(3):
+        g.launch(st)
+        st.synchronize()
+        vs = inp[2]
+        r = (vs / prev - 1.0).abs().max().item()
+        assert 0.0 < r <= 0.01 + 1e-6, r
+        vsd = vs.double().cpu().numpy()
+        vp_b = (0.9409 + 2.0947 * vsd - 0.8206 * vsd ** 2 + 0.2683 * vsd ** 3
+                - 0.0251 * vsd ** 4)
+        assert np.allclose(inp[1].double().cpu().numpy(), vp_b, rtol=1e-6)
+        prev = vs.clone()
+        eager.execute(*inp, eout)
+        torch.cuda.synchronize()
+        _same(_host(out), _host(eout))
+        print("iteration", it, "max relative Vs step", r, flush=True)
