@@ -41,9 +41,12 @@ JDSG_DEV cplx cconj(cplx a) { return cplx{a.re, -a.im}; }
 JDSG_DEV cplx csqrt_real(double x) {
   return (x >= 0.0) ? cplx{sqrt(x), 0.0} : cplx{0.0, sqrt(-x)};
 }
+// exp of a complex number; a real or imaginary argument takes only the one
+// function it needs (exp(0) = 1, sincos(0) = (0, 1) exactly, so the value is
+// the same)
 JDSG_DEV cplx cexp_(cplx z) {
-  double e = exp(z.re), s, c;
-  sincos(z.im, &s, &c);
+  double e = (z.re == 0.0) ? 1.0 : exp(z.re), s = 0.0, c = 1.0;
+  if (z.im != 0.0) sincos(z.im, &s, &c);
   return cplx{e * c, e * s};
 }
 JDSG_DEV double cabs_(cplx z) { return hypot(z.re, z.im); }

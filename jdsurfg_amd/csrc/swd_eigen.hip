@@ -43,11 +43,13 @@ JDSG_DEV void varsv(double pr, double pi, double qr, double qi, cplx rp, cplx rs
   const double tiny = (double)1.0e-5f;
   pex = pr;
   svex = 0.0;
-  double s, c;
-  sincos(pi, &s, &c);
+  // rp is real (pi = 0) or imaginary (pr = 0): evaluate only the function
+  // the argument needs (sincos(0) and exp(-0) are exact)
+  double s = 0.0, c = 1.0;
+  if (pi != 0.0) sincos(pi, &s, &c);
   cplx epp = cmk(c, s) / 2.0;
   cplx epm = cconj(epp);
-  double pfac = (pr < 30.) ? exp(-2. * pr) : 0.0;
+  double pfac = (pr == 0.0) ? 1.0 : (pr < 30.) ? exp(-2. * pr) : 0.0;
   cosp = (epp + pfac * epm).re;
   cplx sinp = epp - pfac * epm;
   rsinp = (rp * sinp).re;
@@ -62,10 +64,12 @@ JDSG_DEV void varsv(double pr, double pi, double qr, double qi, cplx rp, cplx rs
     return;
   }
   svex = qr;
-  sincos(qi, &s, &c);
+  s = 0.0;
+  c = 1.0;
+  if (qi != 0.0) sincos(qi, &s, &c);
   cplx eqp = cmk(c, s) / 2.0;
   cplx eqm = cconj(eqp);
-  double svfac = (qr < 30.) ? exp(-2. * qr) : 0.0;
+  double svfac = (qr == 0.0) ? 1.0 : (qr < 30.) ? exp(-2. * qr) : 0.0;
   cosq = (eqp + svfac * eqm).re;
   cplx sinq = eqp - svfac * eqm;
   rsinq = (rsv * sinq).re;
@@ -382,17 +386,36 @@ JDSG_DEV cplx Eif(const EScal &s) {
   return I == 0 ? 0.5 * s.ira : -0.5 * s.ira;
 }
 
-// sregn96.f90:1198-1317 intijr for a solid internal layer (typelyr 0)
+// sregn96.f90:1198-1317 intijr for a solid internal layer (typelyr 0).  The
+// ten products of layer functions and wave amplitudes are the same for all
+// six integrals, so they are formed once (SolidBasis) and each integral is a
+// combination of them with coefficients from E.
+struct SolidBasis {
+  cplx t[10];
+};
+JDSG_DEV SolidBasis solid_basis(cplx kmpu, cplx kmsu, cplx km1pd, cplx km1sd, cplx FA, cplx GA,
+                                cplx FB, cplx GB, cplx H1, cplx H2) {
+  SolidBasis b;
+  b.t[0] = kmpu * kmpu * FA;
+  b.t[1] = km1pd * km1pd * FA;
+  b.t[2] = kmsu * kmsu * FB;
+  b.t[3] = km1sd * km1sd * FB;
+  b.t[4] = H1 * kmpu * kmsu;
+  b.t[5] = H1 * km1pd * km1sd;
+  b.t[6] = H2 * kmpu * km1sd;
+  b.t[7] = H2 * km1pd * kmsu;
+  b.t[8] = GA * kmpu * km1pd;
+  b.t[9] = GB * kmsu * km1sd;
+  return b;
+}
 template <int I, int J>
-JDSG_DEV double int_solid(const EScal &s, cplx kmpu, cplx kmsu, cplx km1pd, cplx km1sd,
-                          cplx FA, cplx GA, cplx FB, cplx GB, cplx H1, cplx H2) {
+JDSG_DEV double int_solid(const EScal &s, const SolidBasis &b) {
   cplx i0 = Es<I, 0>(s), i1 = Es<I, 1>(s), i2 = Es<I, 2>(s), i3 = Es<I, 3>(s);
   cplx j0 = Es<J, 0>(s), j1 = Es<J, 1>(s), j2 = Es<J, 2>(s), j3 = Es<J, 3>(s);
-  cplx c = i0 * j0 * kmpu * kmpu * FA + i2 * j2 * km1pd * km1pd * FA +
-           i1 * j1 * kmsu * kmsu * FB + i3 * j3 * km1sd * km1sd * FB +
-           H1 * ((i0 * j1 + i1 * j0) * kmpu * kmsu + (i2 * j3 + i3 * j2) * km1pd * km1sd) +
-           H2 * ((i0 * j3 + i3 * j0) * kmpu * km1sd + (i1 * j2 + i2 * j1) * km1pd * kmsu) +
-           GA * (i0 * j2 + i2 * j0) * kmpu * km1pd + GB * (i1 * j3 + i3 * j1) * kmsu * km1sd;
+  cplx c = (i0 * j0) * b.t[0] + (i2 * j2) * b.t[1] + (i1 * j1) * b.t[2] + (i3 * j3) * b.t[3] +
+           (i0 * j1 + i1 * j0) * b.t[4] + (i2 * j3 + i3 * j2) * b.t[5] +
+           (i0 * j3 + i3 * j0) * b.t[6] + (i1 * j2 + i2 * j1) * b.t[7] +
+           (i0 * j2 + i2 * j0) * b.t[8] + (i1 * j3 + i3 * j1) * b.t[9];
   return c.re;
 }
 // lower half-space (typelyr 1)
@@ -481,12 +504,13 @@ JDSG_DEV void layer_energy(const Layer &L, bool allfluid, int typelyr, const U4 
     cplx GB = gfunc(s.rb, L.zd);
     cplx H1 = h1func(s.ra, s.rb, L.zd);
     cplx H2 = h2func(s.ra, s.rb, L.zd);
-    INT11 = int_solid<0, 0>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT13 = int_solid<0, 2>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT22 = int_solid<1, 1>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT24 = int_solid<1, 3>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT33 = int_solid<2, 2>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT44 = int_solid<3, 3>(s, kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    const SolidBasis B = solid_basis(kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
+    INT11 = int_solid<0, 0>(s, B);
+    INT13 = int_solid<0, 2>(s, B);
+    INT22 = int_solid<1, 1>(s, B);
+    INT24 = int_solid<1, 3>(s, B);
+    INT33 = int_solid<2, 2>(s, B);
+    INT44 = int_solid<3, 3>(s, B);
   } else {
     INT11 = int_half<0, 0>(s, km1pd, km1sd);
     INT13 = int_half<0, 2>(s, km1pd, km1sd);
