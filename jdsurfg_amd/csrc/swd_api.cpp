@@ -101,7 +101,7 @@ struct jdsg_plan {
       escr = nullptr;
     }
     if (dalloc(&cscr, (size_t)chain_waves * 64 * (2 * kpad + 22))) return 1;
-    if (dalloc(&escr, (size_t)eigen_waves * 64 * 18 * nmax)) return 1;
+    if (dalloc(&escr, (size_t)eigen_waves * 64 * (18 * (size_t)nmax + 9))) return 1;
     return 0;
   }
 };

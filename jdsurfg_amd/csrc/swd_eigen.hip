@@ -1049,239 +1049,282 @@ JDSG_DEV void zero_entry(const EigenArgs &A, long long e, long long kb, long lon
 }  // namespace
 
 // ----------------------------------------------------------------------------
-extern "C" __global__ void __launch_bounds__(64)
-jdsg_eigen(EigenArgs A) {
-  const long long nlanes = (long long)gridDim.x * blockDim.x;
-  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  double *scr = A.scr + lane;
-  const long long ss = nlanes;  // scratch slot stride
-  // scratch map (slots): [0, 6*nmax) sweep storage, [6*nmax, 10*nmax) raw kernels
-  long long it = lane;
-  const bool ray = (A.wave == 0);
-  while (it < A.nitem) {
-    int col, mode, k;
-    long long e;
-    decode_item(A, it, col, mode, k, e);
-    int N = A.cm.nlay[col];
-    int K = A.per_off[col + 1] - A.per_off[col];
-    long long el = e - A.ent_off[col];
-    long long kb, ls;
-    if (A.layout == 0) {
-      kb = col + el * (long long)N * A.cm.ncol;
-      ls = A.cm.ncol;
-    } else {
-      kb = A.kern_off[col] + el * N;
-      ls = 1;
-    }
-    (void)K;
-    double c0 = A.cflat[e];
-    int st = A.cstat[e];
-    double c1 = 0.0, c2 = 0.0;
-    int stpm = 0;
-    if (A.group) {
-      c1 = A.cflat[A.nent + e];
-      c2 = A.cflat[2 * A.nent + e];
-      stpm = A.cstat[A.nent + e] | A.cstat[2 * A.nent + e];
-    }
-    int status = 0;
-    if (st & JDSG_EXIT_PATH) status = JDSG_EXIT_PATH;
-    else if (c0 == 0.0) status = JDSG_NO_ROOT;
-    if (status) {
-      zero_entry(A, e, kb, ls, N, status);
-      it = nlanes + (long long)atomicAdd(A.counter, 1u);
-      continue;
-    }
-    // T+-0.5 % roots: without them the T outputs stand and dU/dm is zero
-    bool pm_ok = true;
-    if (A.group) {
-      if (stpm & JDSG_EXIT_PATH) { pm_ok = false; status = JDSG_EXIT_PATH_PM; }
-      else if (c1 == 0.0 || c2 == 0.0) { pm_ok = false; status = JDSG_NO_ROOT_PM; }
-    }
-    double T = A.periods[A.per_off[col] + k];
-    double twopi = 2.0 * kPiF;
-    // raw kernel sets v = 0,1,2: slots [(6 + 4v + a) * nmax, ...)
-    double *draw = scr + (long long)6 * A.nmax * ss;
-    Dst Dn;  // non-group destination = outputs
-    double u = 0.0, cout = 0.0;
-    const double ar = 6371.0;
-    long long ebase = (A.eig[0] && A.kern_off) ? A.kern_off[col] + el * N : 0;
-    if (!A.group) {
-      // ---------------- sregn96_/slegn96_ form
-      double cc = c0;
-      double omega = twopi / T;
-      double wvno = omega / cc;
-      for (int a = 0; a < 4; ++a) Dn.p[a] = draw + (long long)a * A.nmax * ss;
-      Dn.base = 0;
-      Dn.stride = ss;
-      double ugr, s0, fac;
-      if (ray) {
-        rayleigh_pass(A, col, N, omega, wvno, scr, ss, Dn, ugr, s0, fac,
-                      A.eig[0] ? A.eig[0] : nullptr, ebase);
-        if (fabs(cc) < 1.0e-36) cc = 0.0;
-        if (fabs(ugr) < 1.0e-36) ugr = 0.0;
-      } else {
-        love_pass(A, col, N, omega, wvno, false, scr, ss, Dn, ugr, s0, fac, A.eig[0] != nullptr,
-                  ebase);
-      }
-      double tm = 1.0, tm3 = 1.0;
-      if (A.iflsph > 0) {
-        double r = ray ? cc / (2. * ar * omega) : 3.0 * cc / (2. * ar * omega);
-        tm = sqrt(1. + r * r);
-        tm3 = tm * tm * tm;
-        cout = cc / tm;
-        u = ugr * tm;
-      } else {
-        cout = cc;
-        u = ugr;
-      }
-      // finalise kernels bottom-up with the thickness suffix sums
-      double hsum = 0.0;
-      for (int i = N - 1; i >= 0; --i) {
-        size_t q = (size_t)i * A.cm.ncol_pad + col;
-        double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
-        double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
-        long long o = (long long)i * ss;
-        double ka = 0.0, kbv, kr, kh;
-        if (ray) {
-          double us0 = ugr * s0;
-          ka = Dn.p[0][o] / us0;
-          kbv = Dn.p[1][o] / us0;
-          kr = Dn.p[2][o] / us0;
-          kh = fac * Dn.p[3][o];
-          if (fabs(kh) < 1.0e-38) kh = 0.0;
-        } else {
-          // Love: dcdb, dcdr divided by sumi1, dcdh final
-          Layer L = load_layer_love(A, col, i, false);
-          kbv = (L.iwat == 0) ? Dn.p[1][o] / s0 : 0.0;
-          kr = (L.iwat == 0) ? Dn.p[2][o] / s0 : 0.0;
-          kh = Dn.p[3][o];
-        }
-        if (A.iflsph > 0) {
-          ka = ka * vtp / tm3;
-          kbv = kbv * vtp / tm3;
-          kh = kh * dtp / tm3;
-          kr = kr * rtp / tm3;
-        }
-        long long oo = kb + (long long)i * ls;
-        if (A.dc[0]) A.dc[0][oo] = ka;
-        if (A.dc[1]) A.dc[1][oo] = kbv;
-        if (A.dc[2]) A.dc[2][oo] = kr;
-        if (A.dc[3]) A.dc[3][oo] = hsum;
-        hsum = hsum + kh;
-      }
-    } else {
-      // ---------------- sregnpu_/slegnpu_ form
-      double tt[3] = {T, T * (1.0 + 0.5 * 0.01), T * (1.0 - 0.5 * 0.01)};
-      if (A.tv) {
-        tt[0] = A.tv[3 * e];
-        tt[1] = A.tv[3 * e + 1];
-        tt[2] = A.tv[3 * e + 2];
-      }
-      double cv[3] = {c0, c1, c2};
-      double ugr0 = 0.0, s00 = 0.0;
-      double usc[3];  // per-pass kernel scale: us0 (Rayleigh) or sumi1 (Love)
-      double facv[3];
-      // passes T, T1, T2 leave raw kernels in scratch sets 0, 1, 2
-      const int nv = pm_ok ? 3 : 1;
-      for (int v = 0; v < nv; ++v) {
-        double omega = twopi / tt[v];
-        double wvno = omega / cv[v];
-        Dst D;
-        for (int a = 0; a < 4; ++a) D.p[a] = draw + (long long)(4 * v + a) * A.nmax * ss;
-        D.base = 0;
-        D.stride = ss;
-        double ugr, s0, fac;
-        if (ray)
-          rayleigh_pass(A, col, N, omega, wvno, scr, ss, D, ugr, s0, fac,
-                        (v == 0 && A.eig[0]) ? A.eig[0] : nullptr, ebase);
-        else
-          love_pass(A, col, N, omega, wvno, true, scr, ss, D, ugr, s0, fac,
-                    v == 0 && A.eig[0] != nullptr, ebase);
-        usc[v] = ray ? ugr * s0 : s0;
-        facv[v] = fac;
-        if (v == 0) { ugr0 = ugr; s00 = s0; }
-      }
-      (void)s00;
-      double cp = c0, cg = ugr0;
-      double uc1 = cg / cp;
-      double dtv = tt[2] - tt[1];
-      double tm = 1.0, tm1 = 0.0, tm3 = 1.0;
-      if (A.iflsph > 0) {
-        double omega = twopi / T;
-        double r = ray ? cp / (2. * ar * omega) : 3.0 * cp / (2. * ar * omega);
-        tm = sqrt(1. + r * r);
-        double q = (ray ? 0.5 : 1.5) / (ar * omega);
-        tm1 = q * q / tm;
-        tm3 = tm * tm * tm;
-      }
-      double hsc = 0.0, hsu = 0.0;
-      for (int i = N - 1; i >= 0; --i) {
-        size_t q = (size_t)i * A.cm.ncol_pad + col;
-        double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
-        double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
-        long long oo = kb + (long long)i * ls;
-        long long os = (long long)i * ss;
-        bool solid = true;
-        if (!ray) solid = (A.cm.zb[q] > 0.0);
-        double d0[4], d1[4], d2[4];
-        for (int a = 0; a < 4; ++a) {
-          double r0 = draw[(long long)(0 + a) * A.nmax * ss + os];
-          double r1 = pm_ok ? draw[(long long)(4 + a) * A.nmax * ss + os] : 0.0;
-          double r2 = pm_ok ? draw[(long long)(8 + a) * A.nmax * ss + os] : 0.0;
-          if (a == 3) {
-            if (ray) {
-              d0[a] = facv[0] * r0; if (fabs(d0[a]) < 1.0e-38) d0[a] = 0.0;
-              d1[a] = facv[1] * r1; if (fabs(d1[a]) < 1.0e-38) d1[a] = 0.0;
-              d2[a] = facv[2] * r2; if (fabs(d2[a]) < 1.0e-38) d2[a] = 0.0;
-            } else {
-              d0[a] = r0; d1[a] = r1; d2[a] = r2;
-            }
-          } else if (ray) {
-            d0[a] = r0 / usc[0];
-            d1[a] = r1 / usc[1];
-            d2[a] = r2 / usc[2];
-          } else {
-            d0[a] = (a != 0 && solid) ? r0 / usc[0] : 0.0;
-            d1[a] = (a != 0 && solid) ? r1 / usc[1] : 0.0;
-            d2[a] = (a != 0 && solid) ? r2 / usc[2] : 0.0;
-          }
-        }
-        double fct[4] = {vtp, vtp, rtp, dtp};
-        for (int a = 0; a < 4; ++a) {
-          double du = uc1 * (2.0 - uc1) * d2[a] - uc1 * uc1 * T * (d2[a] - d1[a]) / dtv;
-          double dc = d0[a];
-          if (A.iflsph > 0) {
-            du = (tm * du + cg * cp * dc * tm1) * fct[a];
-            dc = dc / tm3 * fct[a];
-          }
-          if (!ray && a == 0) { du = 0.0; dc = 0.0; }
-          if (!pm_ok) du = 0.0;
-          if (a < 3) {
-            if (A.dc[a]) A.dc[a][oo] = dc;
-            if (A.du[a]) A.du[a][oo] = du;
-          } else {
-            if (A.dc[3]) A.dc[3][oo] = hsc;
-            if (A.du[3]) A.du[3][oo] = hsu;
-            hsc = hsc + dc;
-            hsu = hsu + du;
-          }
-        }
-      }
-      if (A.iflsph > 0) {
-        cout = cp / tm;
-        u = cg * tm;
-      } else {
-        cout = cp;
-        u = cg;
-      }
-    }
-    A.c_out[e] = cout;
-    A.u_out[e] = u;
-    A.status[e] = status;
-    it = nlanes + (long long)atomicAdd(A.counter, 1u);
+// The stage runs in chunks of one entry per lane.  For each chunk a pass
+// kernel per period (T, and with group kernels T(1+.005), T(1-.005)) runs
+// the sweep + energy pass of one lane's entry and leaves the raw kernels and
+// (U, energy, fac) in the lane's scratch; a combine kernel then forms the
+// outputs.  Splitting the passes from the assembly keeps each kernel's
+// register use to one pass (the fused form needed ~450 registers and ran one
+// wave per SIMD).
+//
+// Lane scratch (slots of `lanes` doubles): [0, 6 nmax) sweep storage;
+// [(6 + 4v + a) nmax, +nmax) raw kernel a of pass v; 18 nmax + 3v + {0,1,2}
+// the pass's U, energy sum and dc/dh factor.
+namespace {
+
+struct EntryRef {
+  int col, mode, k, N;
+  long long e, el, kb, ls;
+};
+
+JDSG_DEV EntryRef entry_ref(const EigenArgs &A, long long it) {
+  EntryRef r;
+  decode_item(A, it, r.col, r.mode, r.k, r.e);
+  r.N = A.cm.nlay[r.col];
+  r.el = r.e - A.ent_off[r.col];
+  if (A.layout == 0) {
+    r.kb = r.col + r.el * (long long)r.N * A.cm.ncol;
+    r.ls = A.cm.ncol;
+  } else {
+    r.kb = A.kern_off[r.col] + r.el * r.N;
+    r.ls = 1;
   }
+  return r;
 }
 
+// 0: solve; JDSG_NO_ROOT / JDSG_EXIT_PATH: entry without outputs; *pm_ok
+// false when the T(1+-0.005) roots are missing (group kernels)
+JDSG_DEV int entry_status(const EigenArgs &A, long long e, bool *pm_ok) {
+  int st = A.cstat[e];
+  *pm_ok = true;
+  if (st & JDSG_EXIT_PATH) return JDSG_EXIT_PATH;
+  if (A.cflat[e] == 0.0) return JDSG_NO_ROOT;
+  if (A.group) {
+    int stpm = A.cstat[A.nent + e] | A.cstat[2 * A.nent + e];
+    if ((stpm & JDSG_EXIT_PATH) || A.cflat[A.nent + e] == 0.0 || A.cflat[2 * A.nent + e] == 0.0)
+      *pm_ok = false;
+  }
+  return 0;
+}
+
+JDSG_DEV double entry_period(const EigenArgs &A, const EntryRef &r, int v) {
+  if (A.tv) return A.tv[3 * r.e + v];
+  double T = A.periods[A.per_off[r.col] + r.k];
+  return v == 0 ? T : (v == 1 ? T * (1.0 + 0.5 * 0.01) : T * (1.0 - 0.5 * 0.01));
+}
+
+}  // namespace
+
+template <int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+jdsg_eigen_pass(EigenArgs A, long long it0, long long lanes, int v) {
+  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long it = it0 + lane;
+  if (lane >= lanes || it >= A.nitem) return;
+  const EntryRef r = entry_ref(A, it);
+  bool pm_ok;
+  if (entry_status(A, r.e, &pm_ok)) return;
+  if (v > 0 && !pm_ok) return;
+  double *scr = A.scr + lane;
+  const long long ss = lanes;
+  const double twopi = 2.0 * kPiF;
+  double omega = twopi / entry_period(A, r, v);
+  double wvno = omega / A.cflat[(long long)v * A.nent + r.e];
+  double *draw = scr + (long long)6 * A.nmax * ss;
+  Dst D;
+  for (int a = 0; a < 4; ++a) D.p[a] = draw + (long long)(4 * v + a) * A.nmax * ss;
+  D.base = 0;
+  D.stride = ss;
+  long long ebase = (A.eig[0] && A.kern_off) ? A.kern_off[r.col] + r.el * r.N : 0;
+  double ugr, s0, fac;
+  if (W == 0)
+    rayleigh_pass(A, r.col, r.N, omega, wvno, scr, ss, D, ugr, s0, fac,
+                  (v == 0 && A.eig[0]) ? A.eig[0] : nullptr, ebase);
+  else
+    love_pass(A, r.col, r.N, omega, wvno, A.group != 0, scr, ss, D, ugr, s0, fac,
+              v == 0 && A.eig[0] != nullptr, ebase);
+  double *sc = scr + ((long long)18 * A.nmax + 3 * v) * ss;
+  sc[0] = ugr;
+  sc[ss] = s0;
+  sc[2 * ss] = fac;
+}
+
+template <int W>
+__global__ void __launch_bounds__(64)
+jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
+  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long it = it0 + lane;
+  if (lane >= lanes || it >= A.nitem) return;
+  const bool ray = (W == 0);
+  const EntryRef r = entry_ref(A, it);
+  const int col = r.col, N = r.N;
+  const long long e = r.e, kb = r.kb, ls = r.ls;
+  bool pm_ok;
+  int status = entry_status(A, e, &pm_ok);
+  if (status) {
+    zero_entry(A, e, kb, ls, N, status);
+    return;
+  }
+  if (A.group && !pm_ok) {
+    int stpm = A.cstat[A.nent + e] | A.cstat[2 * A.nent + e];
+    status = (stpm & JDSG_EXIT_PATH) ? JDSG_EXIT_PATH_PM : JDSG_NO_ROOT_PM;
+  }
+  const double *scr = A.scr + lane;
+  const long long ss = lanes;
+  const double *draw = scr + (long long)6 * A.nmax * ss;
+  const double *scal = scr + (long long)18 * A.nmax * ss;
+  const double twopi = 2.0 * kPiF;
+  const double ar = 6371.0;
+  const double T = entry_period(A, r, 0);
+  double u = 0.0, cout = 0.0;
+  if (!A.group) {
+    // ---------------- sregn96_/slegn96_ form
+    double cc = A.cflat[e];
+    double omega = twopi / T;
+    double ugr = scal[0], s0 = scal[ss], fac = scal[2 * ss];
+    if (ray) {
+      if (fabs(cc) < 1.0e-36) cc = 0.0;
+      if (fabs(ugr) < 1.0e-36) ugr = 0.0;
+    }
+    double tm = 1.0, tm3 = 1.0;
+    if (A.iflsph > 0) {
+      double rr = ray ? cc / (2. * ar * omega) : 3.0 * cc / (2. * ar * omega);
+      tm = sqrt(1. + rr * rr);
+      tm3 = tm * tm * tm;
+      cout = cc / tm;
+      u = ugr * tm;
+    } else {
+      cout = cc;
+      u = ugr;
+    }
+    // finalise kernels bottom-up with the thickness suffix sums
+    double hsum = 0.0;
+    for (int i = N - 1; i >= 0; --i) {
+      size_t q = (size_t)i * A.cm.ncol_pad + col;
+      double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
+      double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
+      long long o = (long long)i * ss;
+      const double *p0 = draw, *p1 = draw + (long long)A.nmax * ss,
+                   *p2 = draw + (long long)2 * A.nmax * ss, *p3 = draw + (long long)3 * A.nmax * ss;
+      double ka = 0.0, kbv, kr, kh;
+      if (ray) {
+        double us0 = ugr * s0;
+        ka = p0[o] / us0;
+        kbv = p1[o] / us0;
+        kr = p2[o] / us0;
+        kh = fac * p3[o];
+        if (fabs(kh) < 1.0e-38) kh = 0.0;
+      } else {
+        // Love: dcdb, dcdr divided by sumi1, dcdh final
+        Layer L = load_layer_love(A, col, i, false);
+        kbv = (L.iwat == 0) ? p1[o] / s0 : 0.0;
+        kr = (L.iwat == 0) ? p2[o] / s0 : 0.0;
+        kh = p3[o];
+      }
+      if (A.iflsph > 0) {
+        ka = ka * vtp / tm3;
+        kbv = kbv * vtp / tm3;
+        kh = kh * dtp / tm3;
+        kr = kr * rtp / tm3;
+      }
+      long long oo = kb + (long long)i * ls;
+      if (A.dc[0]) A.dc[0][oo] = ka;
+      if (A.dc[1]) A.dc[1][oo] = kbv;
+      if (A.dc[2]) A.dc[2][oo] = kr;
+      if (A.dc[3]) A.dc[3][oo] = hsum;
+      hsum = hsum + kh;
+    }
+  } else {
+    // ---------------- sregnpu_/slegnpu_ form
+    double tt1 = entry_period(A, r, 1), tt2 = entry_period(A, r, 2);
+    double usc[3], facv[3];
+    for (int v = 0; v < 3; ++v) {
+      double ugr = scal[(long long)(3 * v) * ss], s0 = scal[(long long)(3 * v + 1) * ss];
+      usc[v] = ray ? ugr * s0 : s0;
+      facv[v] = scal[(long long)(3 * v + 2) * ss];
+    }
+    double cp = A.cflat[e], cg = scal[0];
+    double uc1 = cg / cp;
+    double dtv = tt2 - tt1;
+    double tm = 1.0, tm1 = 0.0, tm3 = 1.0;
+    if (A.iflsph > 0) {
+      double omega = twopi / T;
+      double rr = ray ? cp / (2. * ar * omega) : 3.0 * cp / (2. * ar * omega);
+      tm = sqrt(1. + rr * rr);
+      double q = (ray ? 0.5 : 1.5) / (ar * omega);
+      tm1 = q * q / tm;
+      tm3 = tm * tm * tm;
+    }
+    double hsc = 0.0, hsu = 0.0;
+    for (int i = N - 1; i >= 0; --i) {
+      size_t q = (size_t)i * A.cm.ncol_pad + col;
+      double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
+      double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
+      long long oo = kb + (long long)i * ls;
+      long long os = (long long)i * ss;
+      bool solid = true;
+      if (!ray) solid = (A.cm.zb[q] > 0.0);
+      double fct[4] = {vtp, vtp, rtp, dtp};
+      for (int a = 0; a < 4; ++a) {
+        double r0 = draw[(long long)(0 + a) * A.nmax * ss + os];
+        double r1 = pm_ok ? draw[(long long)(4 + a) * A.nmax * ss + os] : 0.0;
+        double r2 = pm_ok ? draw[(long long)(8 + a) * A.nmax * ss + os] : 0.0;
+        double d0, d1, d2;
+        if (a == 3) {
+          if (ray) {
+            d0 = facv[0] * r0; if (fabs(d0) < 1.0e-38) d0 = 0.0;
+            d1 = facv[1] * r1; if (fabs(d1) < 1.0e-38) d1 = 0.0;
+            d2 = facv[2] * r2; if (fabs(d2) < 1.0e-38) d2 = 0.0;
+          } else {
+            d0 = r0; d1 = r1; d2 = r2;
+          }
+        } else if (ray) {
+          d0 = r0 / usc[0];
+          d1 = pm_ok ? r1 / usc[1] : 0.0;
+          d2 = pm_ok ? r2 / usc[2] : 0.0;
+        } else {
+          d0 = (a != 0 && solid) ? r0 / usc[0] : 0.0;
+          d1 = (a != 0 && solid && pm_ok) ? r1 / usc[1] : 0.0;
+          d2 = (a != 0 && solid && pm_ok) ? r2 / usc[2] : 0.0;
+        }
+        double du = uc1 * (2.0 - uc1) * d2 - uc1 * uc1 * T * (d2 - d1) / dtv;
+        double dc = d0;
+        if (A.iflsph > 0) {
+          du = (tm * du + cg * cp * dc * tm1) * fct[a];
+          dc = dc / tm3 * fct[a];
+        }
+        if (!ray && a == 0) { du = 0.0; dc = 0.0; }
+        if (!pm_ok) du = 0.0;
+        if (a < 3) {
+          if (A.dc[a]) A.dc[a][oo] = dc;
+          if (A.du[a]) A.du[a][oo] = du;
+        } else {
+          if (A.dc[3]) A.dc[3][oo] = hsc;
+          if (A.du[3]) A.du[3][oo] = hsu;
+          hsc = hsc + dc;
+          hsu = hsu + du;
+        }
+      }
+    }
+    if (A.iflsph > 0) {
+      cout = cp / tm;
+      u = cg * tm;
+    } else {
+      cout = cp;
+      u = cg;
+    }
+  }
+  A.c_out[e] = cout;
+  A.u_out[e] = u;
+  A.status[e] = status;
+}
+
+// Chunks of `lanes` entries: the pass(es), then the assembly.
 void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st) {
-  hipLaunchKernelGGL(jdsg_eigen, dim3(nwaves), dim3(64), 0, st, A);
+  const long long lanes = (long long)nwaves * 64;
+  const int nv = A.group ? 3 : 1;
+  for (long long it0 = 0; it0 < A.nitem; it0 += lanes) {
+    long long n = A.nitem - it0 < lanes ? A.nitem - it0 : lanes;
+    dim3 g((unsigned)((n + 63) / 64));
+    for (int v = 0; v < nv; ++v) {
+      if (A.wave == 0)
+        hipLaunchKernelGGL(jdsg_eigen_pass<0>, g, dim3(64), 0, st, A, it0, lanes, v);
+      else
+        hipLaunchKernelGGL(jdsg_eigen_pass<1>, g, dim3(64), 0, st, A, it0, lanes, v);
+    }
+    if (A.wave == 0)
+      hipLaunchKernelGGL(jdsg_eigen_combine<0>, g, dim3(64), 0, st, A, it0, lanes);
+    else
+      hipLaunchKernelGGL(jdsg_eigen_combine<1>, g, dim3(64), 0, st, A, it0, lanes);
+  }
 }
