@@ -32,12 +32,24 @@ def _bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+_G0, _G1 = np.sqrt(2.0 / np.pi), 0.044715
+
+
 def _gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    """GELU, tanh form (the kernels' activation)."""
+    return 0.5 * z * (1.0 + np.tanh(_G0 * (z + _G1 * z ** 3)))
 
 
 def _gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2 * np.pi)
+    th = np.tanh(_G0 * (z + _G1 * z ** 3))
+    return 0.5 * (1.0 + th) + 0.5 * z * (1.0 - th * th) * _G0 * (1.0 + 3.0 * _G1 * z * z)
+
+
+def test_gelu_forms_agree():
+    """The tanh form stays within 5e-4 of the erf form (host check)."""
+    z = np.linspace(-8, 8, 20001)
+    exact = 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    assert np.abs(_gelu(z) - exact).max() < 5e-4
 
 
 def test_mlp_forward_backward_matches_fp64():
