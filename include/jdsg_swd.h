@@ -97,6 +97,12 @@ int jdsg_plan_eval_count(const jdsg_plan *plan, long long *nevals);
 /* Flat roots of the last execute into a host array of nvar*nent doubles
  * (nvar = 3 with group kernels: T, T(1+.005), T(1-.005); else 1). */
 int jdsg_plan_flat_roots(const jdsg_plan *plan, double *host);
+/* Diagnostics of the root chains: the first call enables per-chain
+ * evaluation counting (from the next solve on); later calls copy the counts
+ * of the last solve and each chain's ierr into host arrays of 2*ncol*nvar
+ * (chain g = variant*ncol + column of wave w at w*ncol*nvar + g; either
+ * pointer may be null). */
+int jdsg_plan_chain_stats(jdsg_plan *plan, unsigned *evals, int *ierr);
 /* Per-stage milliseconds (prep, chains, eigen Rayleigh, eigen Love) of the
  * last execute, when JDSG_TIMING=1 was set at plan creation. */
 int jdsg_plan_timings(const jdsg_plan *plan, float *ms4);

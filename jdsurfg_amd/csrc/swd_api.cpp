@@ -63,6 +63,7 @@ struct jdsg_plan {
   // surrogate path buffers (allocated by jdsg_plan_enable_surrogate)
   double *known = nullptr, *cen = nullptr;
   signed char *sg = nullptr;
+  unsigned *chain_ev = nullptr;  // per-chain evaluation counts (diagnostics)
   unsigned *sfail = nullptr;
   const double *known_in = nullptr;
   // explicit per-entry periods for the eigen stage (per-column ABI only)
@@ -280,6 +281,23 @@ extern "C" int jdsg_plan_eval_count(const jdsg_plan *P, long long *out) {
 
 // Flat roots of the last execute ([nvar*nent]: T, then T(1+.005), T(1-.005);
 // float-rounded in the reference mode), device to host.  Synchronises.
+// Diagnostics: per-chain evaluation counts and the reference's ierr of
+// each root chain (chain g of wave w at [w*ncol*nvar + g], g = variant*ncol
+// + column) of the last solve, into host arrays.  Counting starts with the
+// first call and costs one store per chain.
+extern "C" int jdsg_plan_chain_stats(jdsg_plan *P, unsigned *evals, int *ierr) {
+  size_t n = (size_t)2 * P->ncol * P->nvar;
+  if (!P->chain_ev) {
+    if (P->dalloc(&P->chain_ev, n)) return 1;
+    HIPCHK(hipMemset(P->chain_ev, 0, sizeof(unsigned) * n));
+    return 0;
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (evals) HIPCHK(hipMemcpy(evals, P->chain_ev, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+  if (ierr) HIPCHK(hipMemcpy(ierr, P->ierr, sizeof(int) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 extern "C" int jdsg_plan_flat_roots(const jdsg_plan *P, double *host) {
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(host, P->cflat, sizeof(double) * P->nvar * P->nent, hipMemcpyDeviceToHost));
@@ -338,6 +356,7 @@ int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
   A.nevals = P->nevals;
   A.tol = P->tol;
   A.known = P->known_in;
+  A.chain_evals = P->chain_ev;
   A.any_wave[0] = P->nitem[0] > 0;
   A.any_wave[1] = P->nitem[1] > 0;
   jdsg_launch_chains(A, P->chain_waves, st);
@@ -755,6 +774,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     A.nevals = P->nevals;
     A.tol = P->tol;
     A.known = P->known;
+    A.chain_evals = P->chain_ev;
     A.any_wave[0] = P->nitem[0] > 0;
     A.any_wave[1] = P->nitem[1] > 0;
     jdsg_launch_chains(A, P->chain_waves, st);

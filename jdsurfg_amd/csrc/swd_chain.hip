@@ -609,6 +609,7 @@ jdsg_root_chains(ChainArgs A) {
   unsigned *counter = A.counter + W;
   const long long nch = (long long)A.cm.ncol * A.nvar;
   unsigned long long nev = 0;
+  unsigned cev = 0;  // evaluations of the current chain
   ChainState S;
   long long g = lane_id;
   bool live = false;
@@ -631,10 +632,13 @@ jdsg_root_chains(ChainArgs A) {
       else
         F = dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       ++nev;
+      ++cev;
       need = advance(S, A, cs, F);
       continue;
     }
     finish_chain(S, A, g, W);
+    if (A.chain_evals) A.chain_evals[(long long)W * nch + g] = cev;
+    cev = 0;
     live = false;
     for (;;) {
       g = nlanes + (long long)atomicAdd(counter, 1u);

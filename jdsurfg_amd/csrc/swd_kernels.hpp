@@ -31,6 +31,7 @@ struct ChainArgs {
   const double *known;        // optional [nvar*nent] roots already found (>0), else solved here
   int any_wave[2];            // some column asks for Rayleigh / Love modes
   long long nchain;           // ncol * 2 * nvar
+  unsigned *chain_evals;  // optional [2*ncol*nvar] evaluations per chain
 };
 
 struct EigenArgs {
