@@ -15,6 +15,7 @@
 // the recursion rounds like the reference build.
 #include "swd_kernels.hpp"
 #include "swd_dltar.hpp"
+#include "swd_dltar_fast.hpp"
 
 using jdsg_pe::dltar1;
 using jdsg_pe::dltar4;
@@ -599,8 +600,11 @@ JDSG_DEV void finish_chain(ChainState &S, const ChainArgs &A, long long g, int w
 // lane-level work fetching.  Every trip of the main loop evaluates one period
 // equation per live lane.  Chain g of the launch is column g % ncol, period
 // scale g / ncol.  Three waves per SIMD (at most 168 VGPRs, a small spill)
-// measured faster than two (202 VGPRs, no spill) or four.
-template <int W>
+// measured faster than two (202 VGPRs, no spill) or four.  FAST selects the
+// throughput form of the period equations for exact-root mode (tolerance
+// below 1e-6), where the roots are polished past the last-bit differences
+// between the two forms; the reference-tolerance mode keeps the faithful one.
+template <int W, bool FAST>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
 jdsg_root_chains(ChainArgs A) {
   const long long nlanes = (long long)gridDim.x * blockDim.x;
@@ -627,10 +631,12 @@ jdsg_root_chains(ChainArgs A) {
       int mmax = A.cm.nlay[S.col];
       int llw = A.cm.llw[S.col];
       double wvno = S.omega / S.ec;
-      if (W == 0)
-        F = dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
+      if (FAST)
+        F = (W == 0) ? jdsg_pf::dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega)
+                     : jdsg_pf::dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       else
-        F = dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
+        F = (W == 0) ? dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega)
+                     : dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       ++nev;
       ++cev;
       need = advance(S, A, cs, F);
@@ -651,8 +657,10 @@ jdsg_root_chains(ChainArgs A) {
   if (A.nevals && nev) atomicAdd(A.nevals, nev);
 }
 
-template __global__ void jdsg_root_chains<0>(ChainArgs);
-template __global__ void jdsg_root_chains<1>(ChainArgs);
+template __global__ void jdsg_root_chains<0, false>(ChainArgs);
+template __global__ void jdsg_root_chains<1, false>(ChainArgs);
+template __global__ void jdsg_root_chains<0, true>(ChainArgs);
+template __global__ void jdsg_root_chains<1, true>(ChainArgs);
 
 // ----------------------------------------------------------------------------
 // host launchers
@@ -667,10 +675,19 @@ void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float 
 void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st) {
   // Rayleigh chains, then Love chains (the Love launch is skipped when no
   // column asks for Love modes, and likewise for Rayleigh)
-  if (A.any_wave[0])
-    hipLaunchKernelGGL(jdsg_root_chains<0>, dim3(nwaves), dim3(64), 0, st, A);
-  if (A.any_wave[1])
-    hipLaunchKernelGGL(jdsg_root_chains<1>, dim3(nwaves), dim3(64), 0, st, A);
+  const bool fast = A.tol < 1.0e-6;
+  if (A.any_wave[0]) {
+    if (fast)
+      hipLaunchKernelGGL((jdsg_root_chains<0, true>), dim3(nwaves), dim3(64), 0, st, A);
+    else
+      hipLaunchKernelGGL((jdsg_root_chains<0, false>), dim3(nwaves), dim3(64), 0, st, A);
+  }
+  if (A.any_wave[1]) {
+    if (fast)
+      hipLaunchKernelGGL((jdsg_root_chains<1, true>), dim3(nwaves), dim3(64), 0, st, A);
+    else
+      hipLaunchKernelGGL((jdsg_root_chains<1, false>), dim3(nwaves), dim3(64), 0, st, A);
+  }
 }
 
 __global__ void jdsg_fill_u32(unsigned *p, long long n, unsigned v) {
