@@ -198,6 +198,11 @@ int jdsg_plan_execute_surrogate(jdsg_plan *plan, const float *thk, const float *
                                 int *status, double *dcda, double *dcdb, double *dcdr,
                                 double *dcdh, double *duda, double *dudb, double *dudr,
                                 double *dudh, void *stream);
+/* The polished roots the identity checks kept after the last surrogate
+ * execute (nvar*nent doubles, 0 where the chain took over) and the signs of
+ * the period equation at bracket low, high and the chain floor (3*nvar*nent
+ * bytes); either pointer may be null. */
+int jdsg_plan_surrogate_roots(const jdsg_plan *plan, double *known, signed char *signs);
 /* Roots the last surrogate execute left to the chain, and the period-
  * equation evaluations of its bracket/polish and chain stages. */
 int jdsg_plan_surrogate_stats(const jdsg_plan *plan, long long *nfail, long long *polish_evals,

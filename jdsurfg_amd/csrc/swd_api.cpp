@@ -63,6 +63,7 @@ struct jdsg_plan {
   // surrogate path buffers (allocated by jdsg_plan_enable_surrogate)
   double *known = nullptr, *cen = nullptr;
   signed char *sg = nullptr;
+  double *brk = nullptr;
   unsigned *chain_ev = nullptr;  // per-chain evaluation counts (diagnostics)
   unsigned *sfail = nullptr;
   const double *known_in = nullptr;
@@ -680,7 +681,8 @@ extern "C" int jdsg_denorm(long long n, const float *out, int ldo, double scale,
 extern "C" int jdsg_plan_enable_surrogate(jdsg_plan *P) {
   if (P->known) return 0;
   if (P->dalloc(&P->known, (size_t)P->nvar * P->nent) || P->dalloc(&P->cen, 2 * (size_t)P->nent) ||
-      P->dalloc(&P->sg, 3 * (size_t)P->nvar * P->nent) || P->dalloc(&P->sfail, 2))
+      P->dalloc(&P->sg, 3 * (size_t)P->nvar * P->nent) ||
+      P->dalloc(&P->brk, 2 * (size_t)P->nvar * P->nent) || P->dalloc(&P->sfail, 2))
     return 1;
   return 0;
 }
@@ -710,6 +712,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   Q.tol = P->tol;
   Q.known = P->known;
   Q.sg = P->sg;
+  Q.brk = P->brk;
   Q.nfail = P->sfail;
   Q.nevals = P->nevals + 1;
   Q.grid_K = P->uK;
@@ -790,6 +793,19 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     HIPCHK(hipEventSynchronize(P->ev[4]));
     for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
   }
+  return 0;
+}
+
+// Diagnostics of the last surrogate execute: the polished roots that the
+// checks kept (0 where the chain took over), nvar*nent doubles, and the signs
+// of the period equation at bracket low, high and the chain floor, 3*nvar*nent
+// bytes (either pointer may be null).
+extern "C" int jdsg_plan_surrogate_roots(const jdsg_plan *P, double *known, signed char *signs) {
+  if (!P->known) return fail("surrogate path not enabled");
+  HIPCHK(hipDeviceSynchronize());
+  size_t n = (size_t)P->nvar * P->nent;
+  if (known) HIPCHK(hipMemcpy(known, P->known, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (signs) HIPCHK(hipMemcpy(signs, P->sg, 3 * n, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -105,6 +105,7 @@ struct PolishArgs {
   unsigned *nfail;            // counter of roots left to the plain chain
   unsigned long long *nevals;
   signed char *sg;            // [3*nvar*nent] out: sign of F at bracket low, high, floor
+  double *brk;                // [2*nvar*nent] out: the bracket each root was taken from
 };
 
 void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float *thk,

@@ -2,8 +2,10 @@
 //
 // The MLP predicts the flat phase velocity of every (column, wave, mode,
 // period).  Each root is then bracketed at +-half_width around the prediction
-// (one period-equation evaluation per end), accepted only if the period
-// equation changes sign, and polished in FP64 by Illinois false position to
+// (one period-equation evaluation per end; without a sign change the bracket
+// is widened to three half widths, below and then above), accepted only if
+// the period equation changes sign, and polished in FP64 by Illinois false
+// position to
 // |c1-c2| <= tol*c.  A polished root is kept only if it is the root the
 // reference's chain would reach: the mode-order pass checks, with the signs
 // already evaluated, that no odd number of roots lies between the previous
@@ -36,12 +38,14 @@ __device__ __forceinline__ void bracket_ends(double cen, double hw, double cm, d
 // that finishes a root takes its next item at once instead of idling until
 // the slowest lane of its wave is done.
 namespace {
-enum { PP_FLOOR = 0, PP_LO, PP_HI, PP_ILL };
+enum { PP_FLOOR = 0, PP_LO, PP_HI, PP_LO2, PP_HI2, PP_ILL };
 
 struct PLane {
   long long e;          // entry
   int col, variant, phase, nit;
   double omega, cm, betmx, a, b, fa, fb, c;
+  double a2, b2;        // the widened bracket ends (three half widths)
+  double lo, hi;        // the bracket the root was taken from
   signed char s_floor, s_lo;
 };
 
@@ -92,7 +96,14 @@ JDSG_DEV bool polish_begin(const PolishArgs &A, PLane &S, long long &it, long lo
     S.nit = 0;
     double cen = A.centers[v][e];
     bool usable = cen > 0.0 && cen < S.betmx + 0.5;
-    if (usable) bracket_ends(cen, A.hw[v], S.cm, &S.a, &S.b);
+    if (usable) {
+      bracket_ends(cen, A.hw[v], S.cm, &S.a, &S.b);
+      bracket_ends(cen, 3.0 * A.hw[v], S.cm, &S.a2, &S.b2);
+      S.lo = S.a;
+      S.hi = S.b;
+    } else {
+      S.lo = S.hi = 0.0;
+    }
     if (mode == 0) {
       S.phase = PP_FLOOR;
       S.c = S.cm;
@@ -109,6 +120,8 @@ JDSG_DEV bool polish_begin(const PolishArgs &A, PLane &S, long long &it, long lo
     A.sg[q * 3 + 0] = 0;
     A.sg[q * 3 + 1] = 0;
     A.sg[q * 3 + 2] = 0;
+    A.brk[2 * q] = 0.0;
+    A.brk[2 * q + 1] = 0.0;
     ++nf;
   }
   return false;
@@ -121,6 +134,8 @@ JDSG_DEV void polish_finish(const PolishArgs &A, const PLane &S, double root, si
   A.sg[q * 3 + 0] = S.s_lo;
   A.sg[q * 3 + 1] = s_hi;
   A.sg[q * 3 + 2] = S.s_floor;
+  A.brk[2 * q] = S.lo;
+  A.brk[2 * q + 1] = S.hi;
   if (root == 0.0) ++nf;
 }
 
@@ -144,14 +159,59 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
       return false;
     case PP_HI: {
       S.fb = F;
-      signed char s_hi = (signed char)sign1(F);
       if (sign1(S.fa) == sign1(S.fb)) {
-        polish_finish(A, S, 0.0, s_hi, nf);
+        // no sign change: widen to three half widths below, then above
+        // (not past betmx, where a root would be rejected anyway)
+        if (S.a2 < S.a) {
+          S.phase = PP_LO2;
+          S.c = S.a2;
+          return false;
+        }
+        if (S.b < S.betmx) {
+          S.phase = PP_HI2;
+          S.c = S.b2;
+          return false;
+        }
+        polish_finish(A, S, 0.0, (signed char)sign1(F), nf);
         return true;
       }
       S.phase = PP_ILL;
       break;
     }
+    case PP_LO2:
+      if (sign1(F) != sign1(S.fa)) {  // root in [a2, a]
+        S.b = S.a;
+        S.fb = S.fa;
+        S.a = S.a2;
+        S.fa = F;
+        S.lo = S.a2;
+        S.hi = S.b;
+        S.s_lo = (signed char)sign1(F);
+        S.phase = PP_ILL;
+        break;
+      }
+      if (S.b < S.betmx) {
+        S.phase = PP_HI2;
+        S.c = S.b2;
+        return false;
+      }
+      S.lo = S.a2;
+      polish_finish(A, S, 0.0, (signed char)sign1(S.fb), nf);
+      return true;
+    case PP_HI2:
+      if (sign1(F) != sign1(S.fb)) {  // root in [b, b2]
+        S.a = S.b;
+        S.fa = S.fb;
+        S.b = S.b2;
+        S.fb = F;
+        S.lo = S.a;
+        S.hi = S.b2;
+        S.s_lo = (signed char)sign1(S.fa);
+        S.phase = PP_ILL;
+        break;
+      }
+      polish_finish(A, S, 0.0, (signed char)sign1(F), nf);
+      return true;
     default: {  // Illinois false position step with F = F(S.c)
       double fc = F;
       if (fc == 0.0) {
@@ -262,8 +322,7 @@ extern "C" __global__ void jdsg_mode_order(PolishArgs A, int kmax) {
         bad = true;
         continue;
       }
-      double a, b;
-      bracket_ends(A.center[e], A.half_width, cm, &a, &b);
+      double a = A.brk[2 * q], b = A.brk[2 * q + 1];
       bool ok = prev_sign != 0 && sg[0] == prev_sign &&
                 (m == 0 || (v > prev + floor_gap && a > prev_hi));
       if (ok) {

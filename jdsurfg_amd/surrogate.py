@@ -195,7 +195,7 @@ def training_columns(ncol, seed, K=40):
         0.2 km/s, correlation 5-40 km), four sinusoids (20-120 km, rms up to
         0.15 km/s), or nothing; a +-(0-10) % step at 20-60 km (checkerboards)
         with probability 1/2; per-layer noise (sd 0.075) for uneven stacks;
-        and a soft sediment top (0-4 km, 0.8-2.0 km/s) with probability 1/2;
+        and a soft sediment top (0-6 km, 0.7-2.0 km/s) with probability 1/2;
         Vs kept in 0.6-5.6 km/s;
       - K log-uniform periods in 1-80 s; Rayleigh modes 0-3, Love modes 0-1.
     Vp and rho follow Brocher's relations as in the workloads."""
@@ -246,8 +246,8 @@ def training_columns(ncol, seed, K=40):
     v = v * (1.0 + np.where(stp[col], sgn[col] * np.where(z < zst[col], 1.0, -1.0), 0.0))
     v = v + np.where(uneven[col], rng.normal(0.0, 0.075, tot), 0.0)
     sed = rng.random(ncol) < 0.5
-    hsed = rng.uniform(0.0, 4.0, ncol)
-    vsed = rng.uniform(0.8, 2.0, ncol)
+    hsed = rng.uniform(0.0, 6.0, ncol)
+    vsed = rng.uniform(0.7, 2.0, ncol)
     v = np.where(sed[col] & (z < hsed[col]), vsed[col], v)
     vs = np.clip(v, 0.6, 5.6).astype(np.float32)
     thk = h.astype(np.float32)

@@ -5,10 +5,11 @@ The surrogate changes how roots are found, not which root is returned: with
 bracket centres within the bracket half-width of the roots (exact or noisy
 centres) every entry must come out as the plain chain's exact root (tolerance
 1e-10) with identical status, and the eigen/kernel stage then sees the same
-roots.  The exception is a cluster of three or more roots inside one bracket
-(modes crowding near the half-space velocity): the polish may settle on
-another member, as the reference's own dc-grid scan may; such entries are
-counted and must stay below 1 in 1000.  Centres moved onto the neighbouring
+roots.  The exception is a cluster of roots around one bracket (modes
+crowding near the half-space velocity): a bracket holding three roots, or two
+roots that cancel its sign change so that the widened bracket takes a
+neighbour, may settle on another member, as the reference's own dc-grid scan
+may; such entries are counted and must stay below 2 in 1000.  Centres moved onto the neighbouring
 mode's root exercise the identity checks, which reject them.  What the checks
 cannot see is a centre two modes off with a consistent bracket above it: the
 surrogate's identity with the chain rests on the network erring by less than
@@ -248,7 +249,7 @@ def test_surrogate_matches_plain_exact(name, mode):
     same = np.all(np.abs(fa - fb) <= 1e-9 * np.abs(fa), axis=0)
     nclu = int((ok & ~same).sum())
     print(f"entries with another in-bracket root: {nclu} of {int(ok.sum())}", flush=True)
-    assert nclu <= 1e-3 * ok.sum()
+    assert nclu <= 2e-3 * ok.sum()
     s = scalar_errors(b, a, mask=same)
     k = kernel_errors(b, a, w, names=_names(w), mask=same)
     print(s, k, flush=True)
