@@ -148,7 +148,7 @@ def main():
         "achieved": float(chain_flops / (sm_plain[1] * 1e-3) / 1e12), "peak": peak,
         "unit": "TFLOP/s"}
 
-    dom_ms, dom_name, dom_flops = float(sm_plain[2]), "jdsg_eigen (Rayleigh)", eigR
+    dom_ms, dom_name, dom_flops = float(sm_plain[2]), "jdsg_eigen_pass<0> (Rayleigh eigen stage)", eigR
     if not args.no_surrogate:
         from jdsurfg_amd import surrogate as S
         mlp = S.MLP.load(S.DEFAULT_WEIGHTS)
@@ -212,9 +212,18 @@ def main():
         line.update({"value": line["plain"]["value"], "ms_per_step": float(ms_plain),
                      "metric": "column-periods/s (c, U, dc/dVs, dU/dVs), plain FP64 method"})
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a device copy)
+    traffic = None
+    tm_path = os.path.join(ROOT, "jdsurfg_amd", "traffic_model.json")
+    if os.path.exists(tm_path):
+        with open(tm_path) as f:
+            k = json.load(f)["kernels"].get("jdsg_eigen_pass<0>")
+        if k:
+            traffic = k["hbm_read_bytes_per_dispatch"] + k["hbm_write_bytes_per_dispatch"]
     line["roofline"] = {"bound": "fp64-valu", "kernel": dom_name, "achieved": achieved,
                         "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                        "traffic": None}
+                        "traffic": traffic}
     if not args.no_cpu and args.cpu_sample > 0:
         nthr = min(16, os.cpu_count() or 1)
         v, kind, ncols, dt = cpu_baseline(w, args.cpu_sample, nthr)
