@@ -860,11 +860,20 @@ JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
   return v;
 }
 
+// Love layers need no Vp: four loads per layer instead of five
 JDSG_DEV Layer load_layer_love(const EigenArgs &A, int col, int i, bool pu) {
-  Layer L = load_layer(A, col, i, false);
-  double zb = L.zb;
-  L.iwat = pu ? ((zb > 0.0) ? 0 : 1) : ((zb == 0.0) ? 1 : 0);
+  size_t q = (size_t)i * A.cm.ncol_pad + col;
+  Layer L;
+  L.za = 0.0;
+  L.zb = A.cm.zb[q];
+  L.zd = A.cm.zd[q];
+  L.zrho = (A.iflsph > 0) ? A.cm.rho0[q] * A.cm.rtpL[q] : A.cm.rho0[q];
+  L.iwat = pu ? ((L.zb > 0.0) ? 0 : 1) : ((L.zb == 0.0) ? 1 : 0);
   return L;
+}
+JDSG_DEV bool love_solid(const EigenArgs &A, int col, int i, bool pu) {
+  double zb = A.cm.zb[(size_t)i * A.cm.ncol_pad + col];
+  return pu ? (zb > 0.0) : (zb != 0.0);
 }
 
 // One Love pass (shfunc + energy); scratch holds uu, tt, exl per layer.
@@ -910,124 +919,123 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
       TT(k) = t1v;
     }
   }
-  // shfunc normalisation (slegn96.f90:179-246)
+  // shfunc normalisation (slegn96.f90:179-246): the running exponent and
+  // the largest normalised amplitude; the normalised amplitudes are formed
+  // where they are used, so the sweep storage is read, not rewritten
+  double umax = UU(0);
   {
     double ext = 0.0;
-    double umax = UU(0);
-    TT(0) = 0.0;
     for (int k = 1; k < N; ++k) {
-      Layer L = load_layer_love(A, col, k, pu);
-      double uk, tk;
-      if (L.iwat == 0) {
+      double uk = 0.0;
+      if (love_solid(A, col, k, pu)) {
         ext = ext + EXL(k - 1);
         double fact = (ext < 80.0) ? 1. / exp(ext) : 0.0;
         uk = UU(k) * fact;
-        tk = TT(k) * fact;
-      } else {
-        uk = 0.0;
-        tk = 0.0;
       }
-      UU(k) = uk;
-      TT(k) = tk;
       if (fabs(uk) > fabs(umax)) umax = uk;
     }
     if (UU(0) != 0.0) umax = UU(0);
-    if (fabs(umax) > 0.0) {
-      for (int k = 0; k < N; ++k) {
-        Layer L = load_layer_love(A, col, k, pu);
-        if (L.iwat == 0) {
-          UU(k) = UU(k) / umax;
-          TT(k) = TT(k) / umax;
-        }
-      }
-    }
   }
-  // energy (slegn96.f90:447-629)
+  const bool norm = fabs(umax) > 0.0;
+  // energy (slegn96.f90:447-629) fused with the dc/dh terms (:585-610,
+  // stored before the factor fac, which needs the finished sums) and the
+  // eigenfunctions
   double c = omega / wvno;
   double omega2 = omega * omega, wvno2 = wvno * wvno;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int k = 0; k < N; ++k) {
-    Layer L = load_layer_love(A, col, k, pu);
-    long long o = D.base + (long long)k * D.stride;
-    if (L.iwat != 0) {
-      D.p[1][o] = 0.0;
-      D.p[2][o] = 0.0;
-      continue;
-    }
-    double TN = L.zrho * L.zb * L.zb, TL = TN;
-    double VSHH = L.zb, VSHV = L.zb;
-    double drho = L.zrho, dpth = L.zd;
-    VarL v = varl(L, omega, wvno, dpth);
-    double dmu = L.zrho * (L.zb * L.zb);
-    double rb = v.rb;
-    if (rb < 1.0e-10) rb = 1.0e-10;
-    double upup, dupdup;
-    double uk = UU(k);
-    if (k == N - 1) {
-      upup = (0.5 / rb) * uk * uk;
-      dupdup = (0.5 * rb) * uk * uk;
-    } else {
-      cplx nub = (wvno < v.xkb) ? cmk(0.0, rb) : cmk(rb, 0.0);
-      cplx xnub = dmu * nub;
-      cplx ei11 = cmk(0.5 / wvno, 0), ei12 = 0.5 / (wvno * xnub);
-      cplx ei21 = cmk(0.5 / wvno, 0), ei22 = -0.5 / (wvno * xnub);
-      cplx el11 = cmk(wvno, 0), el12 = cmk(wvno, 0);
-      cplx km1dn = ei21 * uk + ei22 * TT(k);
-      cplx kmup = ei11 * UU(k + 1) + ei12 * TT(k + 1);
-      cplx f3 = nub * dpth;
-      cplx exqq = (f3.re < 40.0) ? cexp_(-2.0 * f3) : cmk(0, 0);
-      cplx f = (1.0 - exqq) / (2.0 * nub);
-      exqq = (f3.re < 75.0) ? cexp_(-f3) : cmk(0, 0);
-      cplx g = dpth * exqq;
-      cplx f1 = f * (el11 * el11 * kmup * kmup + el12 * el12 * km1dn * km1dn);
-      cplx f2 = g * (el11 * el12 + el11 * el12) * kmup * km1dn;
-      upup = (f1 + f2).re;
-      dupdup = (nub * nub * (f1 - f2)).re;
-    }
-    s0 = s0 + drho * upup;
-    s1 = s1 + TN * upup;
-    s2 = s2 + TL * dupdup;
-    double DCDBH = c * drho * VSHH * upup;
-    double DCDBV = c * drho * VSHV * dupdup / wvno2;
-    D.p[1][o] = DCDBH + DCDBV;
-    D.p[2][o] = 0.5 * c * (-c * c * upup + VSHH * VSHH * upup + VSHV * VSHV * dupdup / wvno2);
-  }
-  ugr = s1 / (c * s0);
-  double ale = 0.5 / s1;
-  fac = ale * c / wvno2;
-  sumi1 = s1;
-  // dcdh (slegn96.f90:585-610), final value (fac known)
+  bool solid0 = love_solid(A, col, 0, pu);
+  double u_k = (norm && solid0) ? UU(0) / umax : UU(0);
+  double t_k = (norm && solid0) ? 0.0 / umax : 0.0;
+  double ext = 0.0;
   int llflag = 0;
   double zr_prev = 0.0, xmu_prev = 0.0;
   for (int k = 0; k < N; ++k) {
     Layer L = load_layer_love(A, col, k, pu);
     long long o = D.base + (long long)k * D.stride;
+    // normalised amplitudes at the top of layer k+1
+    double u_n = 0.0, t_n = 0.0;
+    if (k + 1 < N) {
+      if (love_solid(A, col, k + 1, pu)) {
+        ext = ext + EXL(k);
+        double fact = (ext < 80.0) ? 1. / exp(ext) : 0.0;
+        u_n = UU(k + 1) * fact;
+        t_n = TT(k + 1) * fact;
+        if (norm) {
+          u_n = u_n / umax;
+          t_n = t_n / umax;
+        }
+      }
+    }
     double xmu = L.zrho * (L.zb * L.zb);
-    if (L.iwat == 0) {
-      double drho, dmu, dvdz;
-      double tk = TT(k), uk = UU(k);
+    if (L.iwat != 0) {
+      D.p[1][o] = 0.0;
+      D.p[2][o] = 0.0;
+      D.p[3][o] = 0.0;
+    } else {
+      double TN = L.zrho * L.zb * L.zb, TL = TN;
+      double VSHH = L.zb, VSHV = L.zb;
+      double drho = L.zrho, dpth = L.zd;
+      VarL v = varl(L, omega, wvno, dpth);
+      double dmu = L.zrho * (L.zb * L.zb);
+      double rb = v.rb;
+      if (rb < 1.0e-10) rb = 1.0e-10;
+      double upup, dupdup;
+      double uk = u_k;
+      if (k == N - 1) {
+        upup = (0.5 / rb) * uk * uk;
+        dupdup = (0.5 * rb) * uk * uk;
+      } else {
+        cplx nub = (wvno < v.xkb) ? cmk(0.0, rb) : cmk(rb, 0.0);
+        cplx xnub = dmu * nub;
+        cplx ei11 = cmk(0.5 / wvno, 0), ei12 = 0.5 / (wvno * xnub);
+        cplx ei21 = cmk(0.5 / wvno, 0), ei22 = -0.5 / (wvno * xnub);
+        cplx el11 = cmk(wvno, 0), el12 = cmk(wvno, 0);
+        cplx km1dn = ei21 * uk + ei22 * t_k;
+        cplx kmup = ei11 * u_n + ei12 * t_n;
+        cplx f3 = nub * dpth;
+        cplx exqq = (f3.re < 40.0) ? cexp_(-2.0 * f3) : cmk(0, 0);
+        cplx f = (1.0 - exqq) / (2.0 * nub);
+        exqq = (f3.re < 75.0) ? cexp_(-f3) : cmk(0, 0);
+        cplx g = dpth * exqq;
+        cplx f1 = f * (el11 * el11 * kmup * kmup + el12 * el12 * km1dn * km1dn);
+        cplx f2 = g * (el11 * el12 + el11 * el12) * kmup * km1dn;
+        upup = (f1 + f2).re;
+        dupdup = (nub * nub * (f1 - f2)).re;
+      }
+      s0 = s0 + drho * upup;
+      s1 = s1 + TN * upup;
+      s2 = s2 + TL * dupdup;
+      double DCDBH = c * drho * VSHH * upup;
+      double DCDBV = c * drho * VSHV * dupdup / wvno2;
+      D.p[1][o] = DCDBH + DCDBV;
+      D.p[2][o] = 0.5 * c * (-c * c * upup + VSHH * VSHH * upup + VSHV * VSHV * dupdup / wvno2);
+      // dc/dh terms before fac
+      double drh, dm, dvdz;
       if (llflag == 0) {
-        drho = L.zrho;
-        dmu = xmu;
+        drh = L.zrho;
+        dm = xmu;
         dvdz = 0.0;
       } else {
-        drho = L.zrho - zr_prev;
-        dmu = xmu - xmu_prev;
-        dvdz = tk * tk * (1.0 / xmu - 1.0 / xmu_prev);
+        drh = L.zrho - zr_prev;
+        dm = xmu - xmu_prev;
+        dvdz = t_k * t_k * (1.0 / xmu - 1.0 / xmu_prev);
       }
-      double dfac = fac * (uk * uk * (omega2 * drho - wvno2 * dmu) + dvdz);
-      D.p[3][o] = (fabs(dfac) < 1.0e-38) ? 0.0 : dfac;
+      D.p[3][o] = uk * uk * (omega2 * drh - wvno2 * dm) + dvdz;
       llflag = llflag + 1;
-    } else {
-      D.p[3][o] = 0.0;
     }
     zr_prev = L.zrho;
     xmu_prev = xmu;
     if (want_eig) {
-      A.eig[0][ebase + k] = UU(k);
-      A.eig[1][ebase + k] = TT(k);
+      A.eig[0][ebase + k] = u_k;
+      A.eig[1][ebase + k] = t_k;
     }
+    u_k = u_n;
+    t_k = t_n;
   }
+  ugr = s1 / (c * s0);
+  double ale = 0.5 / s1;
+  fac = ale * c / wvno2;
+  sumi1 = s1;
 #undef UU
 #undef TT
 #undef EXL
@@ -1209,7 +1217,8 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
         Layer L = load_layer_love(A, col, i, false);
         kbv = (L.iwat == 0) ? p1[o] / s0 : 0.0;
         kr = (L.iwat == 0) ? p2[o] / s0 : 0.0;
-        kh = p3[o];
+        kh = fac * p3[o];
+        if (fabs(kh) < 1.0e-38) kh = 0.0;
       }
       if (A.iflsph > 0) {
         ka = ka * vtp / tm3;
@@ -1260,14 +1269,10 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
         double r1 = pm_ok ? draw[(long long)(4 + a) * A.nmax * ss + os] : 0.0;
         double r2 = pm_ok ? draw[(long long)(8 + a) * A.nmax * ss + os] : 0.0;
         double d0, d1, d2;
-        if (a == 3) {
-          if (ray) {
-            d0 = facv[0] * r0; if (fabs(d0) < 1.0e-38) d0 = 0.0;
-            d1 = facv[1] * r1; if (fabs(d1) < 1.0e-38) d1 = 0.0;
-            d2 = facv[2] * r2; if (fabs(d2) < 1.0e-38) d2 = 0.0;
-          } else {
-            d0 = r0; d1 = r1; d2 = r2;
-          }
+        if (a == 3) {  // dc/dh: the passes leave it before the factor fac
+          d0 = facv[0] * r0; if (fabs(d0) < 1.0e-38) d0 = 0.0;
+          d1 = facv[1] * r1; if (fabs(d1) < 1.0e-38) d1 = 0.0;
+          d2 = facv[2] * r2; if (fabs(d2) < 1.0e-38) d2 = 0.0;
         } else if (ray) {
           d0 = r0 / usc[0];
           d1 = pm_ok ? r1 / usc[1] : 0.0;
