@@ -132,25 +132,26 @@ jdsg_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
       }
 }
 
-// Large-tile form for M, N multiples of 128 (every hidden layer): 128x128
-// output tiles, four waves each owning a 64x64 block of 4x4 MFMA tiles on the
+// Large-tile form for M, N multiples of 128 (every hidden layer) and K of
+// 64: 128x128 output tiles, four waves each owning a 64x64 block of 4x4 MFMA tiles on the
 // gfx950 K=32 instruction (v_mfma_f32_16x16x32_bf16), 16-byte global loads
-// with the next K step held in registers while the current one computes.  A
+// with the next 64-wide K step held in registers while the current one
+// computes.  A
 // lane's A fragment is 8 consecutive k of one row and its B fragment the same
 // 8 k of one column, so the product is the same whatever order the hardware
 // sums k in.  Workgroups are numbered so that consecutive tiles of one M row
 // band run on one XCD and share its L2.
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
-constexpr int BM2 = 128, BN2 = 128, BK2 = 32, LDK2 = BK2 + 8;
+constexpr int BM2 = 128, BN2 = 128, BK2 = 64, LDK2 = BK2 + 8;
 
-// 16-byte chunk c (0..511) of a 128x32 operand tile.  Row-major along k
+// 16-byte chunk c (0..1023) of a 128x64 operand tile.  Row-major along k
 // (KMAJOR false: element (r, k) at P[r*ld + k]) or stored k-major (element
 // (r, k) at P[k*ld + r]), the latter scattered into the k-contiguous LDS tile.
 template <bool KMAJOR>
 __device__ __forceinline__ uint4 gload_tile(const uint16_t *__restrict__ P, int ld, int r0,
                                             int k0, int c) {
   if (!KMAJOR) {
-    int r = c >> 2, kc = (c & 3) * 8;
+    int r = c >> 3, kc = (c & 7) * 8;
     return *reinterpret_cast<const uint4 *>(P + (size_t)(r0 + r) * ld + k0 + kc);
   }
   int k = c >> 4, rc = (c & 15) * 8;
@@ -159,7 +160,7 @@ __device__ __forceinline__ uint4 gload_tile(const uint16_t *__restrict__ P, int 
 template <bool KMAJOR>
 __device__ __forceinline__ void lstore_tile(uint16_t (*S)[BK2 + 8], uint4 v, int c) {
   if (!KMAJOR) {
-    int r = c >> 2, kc = (c & 3) * 8;
+    int r = c >> 3, kc = (c & 7) * 8;
     *reinterpret_cast<uint4 *>(&S[r][kc]) = v;
   } else {
     int k = c >> 4, rc = (c & 15) * 8;
@@ -175,7 +176,7 @@ __device__ __forceinline__ void lstore_tile(uint16_t (*S)[BK2 + 8], uint4 v, int
 }
 
 template <bool TA, bool TB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 jdsg_gemm_bf16_128(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
                    const uint16_t *__restrict__ B, int ldb, float *C, int ldc, int epi,
                    const float *__restrict__ bias, uint16_t *Y, int ldy) {
@@ -193,38 +194,41 @@ jdsg_gemm_bf16_128(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  uint4 ra0, ra1, rb0, rb1;
-  const int c0 = tid, c1 = 256 + tid;
+  uint4 ra[4], rb[4];
 #define JDSG_GLOAD(k0)                                                                      \
   do {                                                                                      \
-    ra0 = gload_tile<TA>(A, lda, bm, (k0), c0);                                             \
-    ra1 = gload_tile<TA>(A, lda, bm, (k0), c1);                                             \
-    rb0 = gload_tile<!TB>(B, ldb, bn, (k0), c0);                                            \
-    rb1 = gload_tile<!TB>(B, ldb, bn, (k0), c1);                                            \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) {                                         \
+      ra[r] = gload_tile<TA>(A, lda, bm, (k0), r * 256 + tid);                              \
+      rb[r] = gload_tile<!TB>(B, ldb, bn, (k0), r * 256 + tid);                             \
+    }                                                                                       \
   } while (0)
   JDSG_GLOAD(0);
   for (int k0 = 0; k0 < K; k0 += BK2) {
-    lstore_tile<TA>(As, ra0, c0);
-    lstore_tile<TA>(As, ra1, c1);
-    lstore_tile<!TB>(Bs, rb0, c0);
-    lstore_tile<!TB>(Bs, rb1, c1);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      lstore_tile<TA>(As, ra[r], r * 256 + tid);
+      lstore_tile<!TB>(Bs, rb[r], r * 256 + tid);
+    }
     __syncthreads();
     if (k0 + BK2 < K) JDSG_GLOAD(k0 + BK2);
 #undef JDSG_GLOAD
-    bf8v af[4], bfv[4];
-    const int kq = 8 * (lane >> 4);
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
-      af[mi] = *reinterpret_cast<const bf8v *>(&As[wm * 64 + mi * 16 + (lane & 15)][kq]);
+    for (int kk = 0; kk < BK2; kk += 32) {
+      bf8v af[4], bfv[4];
+      const int kq = kk + 8 * (lane >> 4);
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
-      bfv[ni] = *reinterpret_cast<const bf8v *>(&Bs[wn * 64 + ni * 16 + (lane & 15)][kq]);
-#pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+      for (int mi = 0; mi < 4; ++mi)
+        af[mi] = *reinterpret_cast<const bf8v *>(&As[wm * 64 + mi * 16 + (lane & 15)][kq]);
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni)
-        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfv[ni], acc[mi][ni], 0, 0,
-                                                              0);
+        bfv[ni] = *reinterpret_cast<const bf8v *>(&Bs[wn * 64 + ni * 16 + (lane & 15)][kq]);
+#pragma unroll
+      for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfv[ni], acc[mi][ni], 0,
+                                                                0, 0);
+    }
     __syncthreads();
   }
 #pragma unroll
