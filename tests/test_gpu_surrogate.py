@@ -329,3 +329,37 @@ def test_training_learns_roots():
     l1 = rep["losses"][-1][1]
     assert l1 < 0.1 * l0
     assert rep["rel_err_median"] < 0.02
+
+
+def test_trained_network_gain():
+    """What the shipped network buys on a C2/C4-like grid (2,048 columns of
+    the 128x128x48 sediment model, 40 periods, Rayleigh 0-2 + Love 0-1,
+    group kernels): the surrogate solve evaluates the period equation far
+    less often than the plain exact-root solve and returns the same roots."""
+    import ctypes as C
+    import torch
+    from jdsurfg_amd import swd, surrogate as S
+    w = W.grid128().subset(np.arange(0, 16384, 8))
+    print(f"case trained-network gain, grid128 subset ncol={w.ncol}", flush=True)
+    mlp = S.MLP.load(S.DEFAULT_WEIGHTS)
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    plan.set_tolerance(EXACT_TOL)
+    inp = swd.upload(w)
+    a = plan.alloc_outputs()
+    plan.execute(*inp, a)
+    torch.cuda.synchronize()
+    plain_evals = plan.eval_count()
+    fa = plan.flat_roots()
+    b = plan.alloc_outputs()
+    S.surrogate_solve(plan, inp, b, mlp=mlp)
+    torch.cuda.synchronize()
+    st = S.surrogate_stats(plan)
+    fb = plan.flat_roots()
+    sur_evals = st["polish_evals"] + st["chain_evals"]
+    sa, sb = a["status"].cpu().numpy(), b["status"].cpu().numpy()
+    same = np.all(np.abs(fa - fb) <= 1e-9 * np.abs(fa), axis=0) & (sa == sb)
+    print(f"plain evaluations {plain_evals}, surrogate {sur_evals} "
+          f"({sur_evals / plain_evals:.3f}); fallback {st['roots_left_to_chain']} of "
+          f"{3 * plan.nent}; entries differing {int((~same).sum())} of {plan.nent}", flush=True)
+    assert sur_evals < 0.5 * plain_evals
+    assert (~same).sum() <= 1e-3 * plan.nent
