@@ -36,71 +36,127 @@ struct Layer {
   int iwat;
 };
 
-// sregn96.f90:827-911 varsv
-JDSG_DEV void varsv(double pr, double pi, double qr, double qi, cplx rp, cplx rsv, int iwat,
-                    double zd, double &cosp, double &cosq, double &rsinp, double &rsinq,
-                    double &sinpr, double &sinqr, double &pex, double &svex) {
+// Vertical wavenumbers of a layer and their layer exponentials.  ra and rb
+// are real (evanescent) or imaginary (propagating), so exp(-ra d) is either
+// exp(-Re(ra) d) or cos(Im(ra) d) - i sin(Im(ra) d): one exp or one sincos
+// per wave type.  varsv, the Dunkin and Haskell matrices and every
+// exponential of the energy integrals (ffunc, gfunc, h1func, h2func) are
+// formed from Ea = exp(-ra d) and Eb = exp(-rb d).
+struct LayerWaves {
+  cplx ra, rb;        // vertical wavenumbers
+  cplx Ea, Eb;        // exp(-ra d), exp(-rb d) (Eb unset in a fluid layer)
+  double pr, pi;      // ra d
+  double qr, qi;      // rb d
+  double ma, mb;      // exp(-Re(ra) d), exp(-Re(rb) d): 1 when imaginary
+};
+
+JDSG_DEV LayerWaves layer_waves(const Layer &L, double omega, double wvno2) {
+  LayerWaves w;
+  double xka = omega / L.za;
+  double xkb = (L.zb > 0.0) ? omega / L.zb : 0.0;
+  w.ra = csqrt_real(wvno2 - xka * xka);
+  w.rb = csqrt_real(wvno2 - xkb * xkb);
+  w.pr = w.ra.re * L.zd;
+  w.pi = w.ra.im * L.zd;
+  w.qr = w.rb.re * L.zd;
+  w.qi = w.rb.im * L.zd;
+  if (w.pi != 0.0) {
+    double s, c;
+    sincos(w.pi, &s, &c);
+    w.Ea = cmk(c, -s);
+    w.ma = 1.0;
+  } else {
+    w.ma = (w.pr == 0.0) ? 1.0 : exp(-w.pr);
+    w.Ea = cmk(w.ma, 0.0);
+  }
+  w.Eb = cmk(0.0, 0.0);
+  w.mb = 1.0;
+  if (L.iwat == 0) {
+    if (w.qi != 0.0) {
+      double s, c;
+      sincos(w.qi, &s, &c);
+      w.Eb = cmk(c, -s);
+    } else {
+      w.mb = (w.qr == 0.0) ? 1.0 : exp(-w.qr);
+      w.Eb = cmk(w.mb, 0.0);
+    }
+  }
+  return w;
+}
+
+// sregn96.f90:827-911 varsv from the layer exponentials: a propagating wave
+// gives cos/sin directly, an evanescent one the exponentially scaled
+// cosh/sinh (exp(-2 Re(ra) d) = Ea^2, dropped past 30 as in the reference).
+JDSG_DEV void varsv_w(const LayerWaves &w, const Layer &L, double &cosp, double &cosq,
+                      double &rsinp, double &rsinq, double &sinpr, double &sinqr, double &pex,
+                      double &svex) {
   const double tiny = (double)1.0e-5f;
-  pex = pr;
+  pex = w.pr;
   svex = 0.0;
-  // rp is real (pi = 0) or imaginary (pr = 0): evaluate only the function
-  // the argument needs (sincos(0) and exp(-0) are exact)
-  double s = 0.0, c = 1.0;
-  if (pi != 0.0) sincos(pi, &s, &c);
-  cplx epp = cmk(c, s) / 2.0;
-  cplx epm = cconj(epp);
-  double pfac = (pr == 0.0) ? 1.0 : (pr < 30.) ? exp(-2. * pr) : 0.0;
-  cosp = (epp + pfac * epm).re;
-  cplx sinp = epp - pfac * epm;
-  rsinp = (rp * sinp).re;
-  if (fabs(pr) < tiny && cabs_(rp) < tiny)
-    sinpr = zd;
-  else
-    sinpr = (sinp / rp).re;
-  if (iwat == 1) {
+  if (w.pi != 0.0) {
+    double c = w.Ea.re, s = -w.Ea.im;
+    cosp = c;
+    rsinp = -(w.ra.im * s);
+    sinpr = (cabs_(w.ra) < tiny) ? L.zd : s / w.ra.im;
+  } else {
+    double pfac = (w.pr == 0.0) ? 1.0 : (w.pr < 30.) ? w.ma * w.ma : 0.0;
+    cosp = 0.5 + pfac * 0.5;
+    double sinp = 0.5 - pfac * 0.5;
+    rsinp = w.ra.re * sinp;
+    sinpr = (fabs(w.pr) < tiny && cabs_(w.ra) < tiny) ? L.zd : sinp / w.ra.re;
+  }
+  if (L.iwat == 1) {
     cosq = 1.0;
     rsinq = 0.0;
     sinqr = 0.0;
     return;
   }
-  svex = qr;
-  s = 0.0;
-  c = 1.0;
-  if (qi != 0.0) sincos(qi, &s, &c);
-  cplx eqp = cmk(c, s) / 2.0;
-  cplx eqm = cconj(eqp);
-  double svfac = (qr == 0.0) ? 1.0 : (qr < 30.) ? exp(-2. * qr) : 0.0;
-  cosq = (eqp + svfac * eqm).re;
-  cplx sinq = eqp - svfac * eqm;
-  rsinq = (rsv * sinq).re;
-  if (fabs(qr) < tiny && cabs_(rsv) < tiny)
-    sinqr = zd;
-  else
-    sinqr = (sinq / rsv).re;
+  svex = w.qr;
+  if (w.qi != 0.0) {
+    double c = w.Eb.re, s = -w.Eb.im;
+    cosq = c;
+    rsinq = -(w.rb.im * s);
+    sinqr = (cabs_(w.rb) < tiny) ? L.zd : s / w.rb.im;
+  } else {
+    double svfac = (w.qr == 0.0) ? 1.0 : (w.qr < 30.) ? w.mb * w.mb : 0.0;
+    cosq = 0.5 + svfac * 0.5;
+    double sinq = 0.5 - svfac * 0.5;
+    rsinq = w.rb.re * sinq;
+    sinqr = (fabs(w.qr) < tiny && cabs_(w.rb) < tiny) ? L.zd : sinq / w.rb.re;
+  }
 }
 
-JDSG_DEV void layer_trig(const Layer &L, double omega, double wvno2, double &cosp,
-                         double &cossv, double &rsinp, double &rsinsv, double &sinpr,
-                         double &sinsvr, double &pex, double &svex) {
-  double xka = omega / L.za;
-  double xkb = (L.zb > 0.0) ? omega / L.zb : 0.0;
-  cplx rp = csqrt_real(wvno2 - xka * xka);
-  cplx rsv = csqrt_real(wvno2 - xkb * xkb);
-  cplx p = rp * L.zd, q = rsv * L.zd;
-  varsv(p.re, p.im, q.re, q.im, rp, rsv, L.iwat, L.zd, cosp, cossv, rsinp, rsinsv, sinpr,
-        sinsvr, pex, svex);
+// Scale a sweep vector by a power of two so that its largest entry lies in
+// [0.5, 1): the sweeps of sregn96.f90 (:475-485 and :1040-1050) divide by
+// the largest entry and add its logarithm to the exponent sum; scaling by
+// 2^-e instead is exact and adds e ln 2.
+template <int K>
+JDSG_DEV double normalize_pow2(double (&v)[K]) {
+  double t1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < K; ++i)
+    if (fabs(v[i]) > t1) t1 = fabs(v[i]);
+  if (t1 < 1.0e-40) return 0.0;
+  int e;
+  (void)frexp(t1, &e);
+#pragma unroll
+  for (int i = 0; i < K; ++i) v[i] = ldexp(v[i], -e);
+  return (double)e * 0.69314718055994530942;
 }
 
 // sregn96.f90:491-647 dnka; returns the 5x5 matrix row-major in ca
 JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, double cossv,
-                     double rsinsv, double sinsvr, float rho, float b, int iwat, double ex,
-                     double exa, double wvno, double wvno2, double om2) {
+                     double rsinsv, double sinsvr, float rho, float b, int iwat,
+                     const LayerWaves &w, double wvno, double wvno2, double om2) {
+  // exp(-ex) and exp(-exa) of the reference (ex = Re(ra) d, exa = ex +
+  // Re(rb) d) are the layer magnitudes ma and ma*mb
+  const double ex = w.pr, exa = w.pr + w.qr;
   if (iwat == 1) {
 #pragma unroll
     for (int j = 0; j < 5; ++j)
 #pragma unroll
       for (int i = 0; i < 5; ++i) ca[i][j] = 0.0;
-    double dfac = (ex > 35.0) ? 0.0 : exp(-ex);
+    double dfac = (ex > 35.0) ? 0.0 : w.ma;
     ca[2][2] = dfac;
     ca[0][0] = cosp;
     ca[4][4] = cosp;
@@ -112,7 +168,7 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
     ca[4][3] = ca[1][0];
     return;
   }
-  double a0 = (exa < 60.0) ? exp(-exa) : 0.0;
+  double a0 = (exa < 60.0) ? w.ma * w.mb : 0.0;
   double cpcq = cosp * cossv, cpy = cosp * sinsvr, cpz = cosp * rsinsv, cqw = cossv * sinpr,
          cqx = cossv * rsinp, xy = rsinp * sinsvr, xz = rsinp * rsinsv, wy = sinpr * sinsvr,
          wz = sinpr * rsinsv;
@@ -168,14 +224,15 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
 
 // sregn96.f90:913-987 hska
 JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, double tcossv,
-                   double trsinsv, double tsinsvr, float rho, float b, int iwat, double pex,
-                   double svex, double wvno, double wvno2, double om2) {
+                   double trsinsv, double tsinsvr, float rho, float b, int iwat,
+                   const LayerWaves &w, double wvno, double wvno2, double om2) {
+  const double pex = w.pr, svex = (iwat == 1) ? 0.0 : w.qr;
   if (iwat == 1) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) AA[i][j] = 0.0;
-    double dfac = (pex > 35.0) ? 0.0 : exp(-pex);
+    double dfac = (pex > 35.0) ? 0.0 : w.ma;
     AA[0][0] = dfac;
     AA[3][3] = dfac;
     AA[1][1] = cosp;
@@ -184,7 +241,8 @@ JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, dou
     AA[2][1] = -(double)rho * om2 * sinpr;
     return;
   }
-  double dfac = ((pex - svex) > 70.0) ? 0.0 : exp(svex - pex);
+  // exp(svex - pex): with a propagating S wave (svex = 0) that is ma
+  double dfac = ((pex - svex) > 70.0) ? 0.0 : (svex == 0.0) ? w.ma : exp(svex - pex);
   double cossv = dfac * tcossv;
   double rsinsv = dfac * trsinsv;
   double sinsvr = dfac * tsinsvr;
@@ -303,21 +361,6 @@ JDSG_DEV cplx gfunc(cplx nub, double dm) {
   if (argcd.re < 75) return cexp_(-argcd) * dm;
   return cmk(0, 0);
 }
-JDSG_DEV cplx h1func(cplx nua, cplx nub, double dm) {
-  if (cabs_(nub + nua) < 1.0e-08) return cmk(dm, 0);
-  cplx argcd = (nua + nub) * dm;
-  cplx exqq = (argcd.re < 40.0) ? cexp_(-argcd) : cmk(0, 0);
-  return (1.0 - exqq) / (nub + nua);
-}
-JDSG_DEV cplx h2func(cplx nua, cplx nub, double dm) {
-  if (cabs_(nub - nua) < 1.0e-08) return cmk(dm, 0);
-  cplx argcd = nua * dm;
-  cplx exqp = (argcd.re < 40.0) ? cexp_(-argcd) : cmk(0, 0);
-  argcd = nub * dm;
-  cplx exqq = (argcd.re < 40.0) ? cexp_(-argcd) : cmk(0, 0);
-  return (exqq - exqp) / (nua - nub);
-}
-
 struct U4 {
   double ur, uz, tz, tr;
 };
@@ -386,38 +429,6 @@ JDSG_DEV cplx Eif(const EScal &s) {
   return I == 0 ? 0.5 * s.ira : -0.5 * s.ira;
 }
 
-// sregn96.f90:1198-1317 intijr for a solid internal layer (typelyr 0).  The
-// ten products of layer functions and wave amplitudes are the same for all
-// six integrals, so they are formed once (SolidBasis) and each integral is a
-// combination of them with coefficients from E.
-struct SolidBasis {
-  cplx t[10];
-};
-JDSG_DEV SolidBasis solid_basis(cplx kmpu, cplx kmsu, cplx km1pd, cplx km1sd, cplx FA, cplx GA,
-                                cplx FB, cplx GB, cplx H1, cplx H2) {
-  SolidBasis b;
-  b.t[0] = kmpu * kmpu * FA;
-  b.t[1] = km1pd * km1pd * FA;
-  b.t[2] = kmsu * kmsu * FB;
-  b.t[3] = km1sd * km1sd * FB;
-  b.t[4] = H1 * kmpu * kmsu;
-  b.t[5] = H1 * km1pd * km1sd;
-  b.t[6] = H2 * kmpu * km1sd;
-  b.t[7] = H2 * km1pd * kmsu;
-  b.t[8] = GA * kmpu * km1pd;
-  b.t[9] = GB * kmsu * km1sd;
-  return b;
-}
-template <int I, int J>
-JDSG_DEV double int_solid(const EScal &s, const SolidBasis &b) {
-  cplx i0 = Es<I, 0>(s), i1 = Es<I, 1>(s), i2 = Es<I, 2>(s), i3 = Es<I, 3>(s);
-  cplx j0 = Es<J, 0>(s), j1 = Es<J, 1>(s), j2 = Es<J, 2>(s), j3 = Es<J, 3>(s);
-  cplx c = (i0 * j0) * b.t[0] + (i2 * j2) * b.t[1] + (i1 * j1) * b.t[2] + (i3 * j3) * b.t[3] +
-           (i0 * j1 + i1 * j0) * b.t[4] + (i2 * j3 + i3 * j2) * b.t[5] +
-           (i0 * j3 + i3 * j0) * b.t[6] + (i1 * j2 + i2 * j1) * b.t[7] +
-           (i0 * j2 + i2 * j0) * b.t[8] + (i1 * j3 + i3 * j1) * b.t[9];
-  return c.re;
-}
 // lower half-space (typelyr 1)
 template <int I, int J>
 JDSG_DEV double int_half(const EScal &s, cplx km1pd, cplx km1sd) {
@@ -440,17 +451,75 @@ JDSG_DEV double int_fluid(const EScal &s, int typelyr, cplx kmpu, cplx km1pd, cp
   return c.re;
 }
 
+// The six integrals of an internal solid layer (sregn96.f90:1198-1317,
+// intijr typelyr 0) from the shared layer exponentials.  Rows 1 and 3 of E
+// are k p + rb q and z(g-1) p + (z g/k) rb q, rows 2 and 4 are ra r + k s
+// and (z g/k) ra r + z(g-1) s, with p = (1,0,1,0), q = (0,1,0,-1),
+// r = (1,0,-1,0), s = (0,1,0,1).  So each INT_IJ = Re(E_I M E_J) -- M the
+// symmetric matrix of the ten layer-function-weighted amplitude products --
+// is a real combination of the six forms pMp, qMq, pMq, rMr, sMs, rMs, and
+// those are sums of four amplitude products.  ra^2 and rb^2 are real.
+JDSG_DEV void solid_ints(const Layer &L, const LayerWaves &w, const U4 &u0, const U4 &u1,
+                         double om, double om2, double wvno, double INT[6]) {
+  const double k = wvno, z = L.zrho * om2, d = L.zd;
+  const double gm = L.zb * wvno / om;
+  const double g = 2.0 * (gm * gm), g1 = g - 1.0;
+  const double gk = z * g / wvno, izh = 0.5 / z;
+  const double hg = 0.5 * g / k, hk = k * izh;
+  // 1/ra, 1/rb of a real or imaginary number
+  const cplx ira = (w.ra.im == 0.0) ? cmk(1.0 / w.ra.re, 0.0) : cmk(0.0, -1.0 / w.ra.im);
+  const cplx irb = (w.rb.im == 0.0) ? cmk(1.0 / w.rb.re, 0.0) : cmk(0.0, -1.0 / w.rb.im);
+  // wave amplitudes E^-1 u at the layer bottom (up-going) and top (down-going)
+  const cplx kmpu = (hg * u1.ur - izh * u1.tz) + (hk * u1.tr - 0.5 * g1 * u1.uz) * ira;
+  const cplx kmsu = (hg * u1.uz - izh * u1.tr) + (hk * u1.tz - 0.5 * g1 * u1.ur) * irb;
+  const cplx km1pd = (hg * u0.ur - izh * u0.tz) + (0.5 * g1 * u0.uz - hk * u0.tr) * ira;
+  const cplx km1sd = (hg * u0.uz - izh * u0.tr) + (0.5 * g1 * u0.ur - hk * u0.tz) * irb;
+  // layer functions (sregn96.f90:1319-1397) with their reference cut-offs
+  const cplx zero = cmk(0.0, 0.0);
+  const cplx FA = (cabs_(w.ra) < 1.0e-08) ? cmk(d, 0.0)
+                  : (0.5 * (1.0 - ((w.pr < 40.0) ? w.Ea * w.Ea : zero))) * ira;
+  const cplx GA = (w.pr < 75.0) ? w.Ea * d : zero;
+  const cplx FB = (cabs_(w.rb) < 1.0e-08) ? cmk(d, 0.0)
+                  : (0.5 * (1.0 - ((w.qr < 40.0) ? w.Eb * w.Eb : zero))) * irb;
+  const cplx GB = (w.qr < 75.0) ? w.Eb * d : zero;
+  const cplx sab = w.rb + w.ra, dab = w.ra - w.rb;
+  const cplx H1 = (cabs_(sab) < 1.0e-08) ? cmk(d, 0.0)
+                  : (1.0 - ((sab.re * d < 40.0) ? w.Ea * w.Eb : zero)) / sab;
+  const cplx H2 = (cabs_(dab) < 1.0e-08) ? cmk(d, 0.0)
+                  : (((w.qr < 40.0) ? w.Eb : zero) - ((w.pr < 40.0) ? w.Ea : zero)) / dab;
+  // the six quadratic forms of M
+  const cplx A2 = kmpu * kmpu + km1pd * km1pd, X = kmpu * km1pd;
+  const cplx B2 = kmsu * kmsu + km1sd * km1sd, Y = kmsu * km1sd;
+  const double fa = (FA * A2).re, ga = 2.0 * (GA * X).re;
+  const double fb = (FB * B2).re, gb = 2.0 * (GB * Y).re;
+  const double P = fa + ga, R = fa - ga;      // pMp, rMr
+  const double Q = fb - gb, S = fb + gb;      // qMq, sMs
+  const cplx h1t = H1 * (kmpu * kmsu - km1pd * km1sd);
+  const cplx h2t = H2 * (km1pd * kmsu - kmpu * km1sd);
+  const cplx PQ = h1t + h2t, RS = h1t - h2t;  // pMq, rMs
+  const double rbPQ = w.rb.re * PQ.re - w.rb.im * PQ.im;  // Re(rb pMq)
+  const double raRS = w.ra.re * RS.re - w.ra.im * RS.im;  // Re(ra rMs)
+  const double rb2 = w.rb.re * w.rb.re - w.rb.im * w.rb.im;
+  const double ra2 = w.ra.re * w.ra.re - w.ra.im * w.ra.im;
+  const double zg1 = z * g1;
+  INT[0] = k * k * P + 2.0 * k * rbPQ + rb2 * Q;                       // INT11
+  INT[1] = k * zg1 * P + (k * gk + zg1) * rbPQ + gk * rb2 * Q;         // INT13
+  INT[2] = ra2 * R + 2.0 * k * raRS + k * k * S;                       // INT22
+  INT[3] = gk * ra2 * R + (zg1 + k * gk) * raRS + k * zg1 * S;         // INT24
+  INT[4] = zg1 * zg1 * P + 2.0 * zg1 * gk * rbPQ + gk * gk * rb2 * Q;  // INT33
+  INT[5] = gk * gk * ra2 * R + 2.0 * gk * zg1 * raRS + zg1 * zg1 * S;  // INT44
+}
+
 struct EnergySums {
   double s0, s1, s2, s3;
 };
 
 // energy contribution of layer m (sregn96.f90:1061-1161 loop body).  Writes
 // the unscaled dc/da, dc/db, dc/drho of layer m to ka/kb/kr.
-JDSG_DEV void layer_energy(const Layer &L, bool allfluid, int typelyr, const U4 &u0,
+JDSG_DEV void layer_energy(const Layer &L, const LayerWaves *lw, int typelyr, const U4 &u0,
                            const U4 &u1, double om, double om2, double wvno, double wvno2,
                            double c, EnergySums &S, double &ka, double &kb, double &kr,
                            double &ur_fluid) {
-  (void)allfluid;
   const EScal s = escal(L, wvno, om, om2, wvno2);
   double ah = L.za, av = L.za, bv = L.zb, rho = L.zrho, eta = 1.0;
   double TA = L.zrho * L.za * L.za, TC = TA;
@@ -488,30 +557,21 @@ JDSG_DEV void layer_energy(const Layer &L, bool allfluid, int typelyr, const U4 
   double TL = L.zrho * L.zb * L.zb, TN = TL;
   double TF = TA - 2. * TN;
   double a12 = -wvno, a14 = 1.0 / TL, a21 = wvno * TF / TC, a23 = 1.0 / TC;
-  cplx km1pd = Eis<2, 0>(s) * u0.ur + Eis<2, 1>(s) * u0.uz + Eis<2, 2>(s) * u0.tz +
-               Eis<2, 3>(s) * u0.tr;
-  cplx km1sd = Eis<3, 0>(s) * u0.ur + Eis<3, 1>(s) * u0.uz + Eis<3, 2>(s) * u0.tz +
-               Eis<3, 3>(s) * u0.tr;
   double INT11, INT13, INT22, INT24, INT33, INT44;
   if (typelyr == 0) {
-    cplx kmpu = Eis<0, 0>(s) * u1.ur + Eis<0, 1>(s) * u1.uz + Eis<0, 2>(s) * u1.tz +
-                Eis<0, 3>(s) * u1.tr;
-    cplx kmsu = Eis<1, 0>(s) * u1.ur + Eis<1, 1>(s) * u1.uz + Eis<1, 2>(s) * u1.tz +
-                Eis<1, 3>(s) * u1.tr;
-    cplx FA = ffunc(s.ra, L.zd);
-    cplx GA = gfunc(s.ra, L.zd);
-    cplx FB = ffunc(s.rb, L.zd);
-    cplx GB = gfunc(s.rb, L.zd);
-    cplx H1 = h1func(s.ra, s.rb, L.zd);
-    cplx H2 = h2func(s.ra, s.rb, L.zd);
-    const SolidBasis B = solid_basis(kmpu, kmsu, km1pd, km1sd, FA, GA, FB, GB, H1, H2);
-    INT11 = int_solid<0, 0>(s, B);
-    INT13 = int_solid<0, 2>(s, B);
-    INT22 = int_solid<1, 1>(s, B);
-    INT24 = int_solid<1, 3>(s, B);
-    INT33 = int_solid<2, 2>(s, B);
-    INT44 = int_solid<3, 3>(s, B);
+    double I6[6];
+    solid_ints(L, *lw, u0, u1, om, om2, wvno, I6);
+    INT11 = I6[0];
+    INT13 = I6[1];
+    INT22 = I6[2];
+    INT24 = I6[3];
+    INT33 = I6[4];
+    INT44 = I6[5];
   } else {
+    cplx km1pd = Eis<2, 0>(s) * u0.ur + Eis<2, 1>(s) * u0.uz + Eis<2, 2>(s) * u0.tz +
+                 Eis<2, 3>(s) * u0.tr;
+    cplx km1sd = Eis<3, 0>(s) * u0.ur + Eis<3, 1>(s) * u0.uz + Eis<3, 2>(s) * u0.tz +
+                 Eis<3, 3>(s) * u0.tr;
     INT11 = int_half<0, 0>(s, km1pd, km1sd);
     INT13 = int_half<0, 2>(s, km1pd, km1sd);
     INT22 = int_half<1, 1>(s, km1pd, km1sd);
@@ -681,11 +741,12 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     double exsum = 0.0;
     for (int m = N - 2; m >= 0; --m) {
       Layer L = load_layer(A, col, m, true);
+      const LayerWaves lw = layer_waves(L, omega, wvno2);
       double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
-      layer_trig(L, omega, wvno2, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
+      varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
       double ca[5][5];
       dnka_r(ca, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat,
-             pex, pex + svex, wvno, wvno2, om2);
+             lw, wvno, wvno2, om2);
       double ee[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
@@ -694,14 +755,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
         for (int j = 0; j < 5; ++j) cr = cr + cd[j] * ca[j][i];
         ee[i] = cr;
       }
-      double t1 = 0.0;
+      double exn = normalize_pow2(ee);
 #pragma unroll
-      for (int i = 0; i < 5; ++i)
-        if (fabs(ee[i]) > t1) t1 = fabs(ee[i]);
-      if (t1 < 1.0e-40) t1 = 1.0;
-#pragma unroll
-      for (int i = 0; i < 5; ++i) cd[i] = ee[i] / t1;
-      double exn = log(t1);
+      for (int i = 0; i < 5; ++i) cd[i] = ee[i];
       exsum = exsum + pex + svex + exn;
 #pragma unroll
       for (int j = 0; j < 5; ++j) scr[((long long)m * 6 + j) * sstride] = cd[j];
@@ -713,6 +769,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   double cd0_2 = scr[(0 * 6 + 2) * sstride];
   double exe0 = scr[(0 * 6 + 5) * sstride];
   double f1213 = -cd0_1;
+  const double r1213 = 1.0 / f1213;
   U4 uprev;
   uprev.ur = cd0_2 / cd0_1;
   uprev.uz = 1.0;
@@ -735,11 +792,12 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   double xmu_m1 = xmu_prev, xlam_m1 = xlam_prev;
   for (int m = 0; m < N - 1; ++m) {
     Layer L = Lprev;  // layer m
+    const LayerWaves lw = layer_waves(L, omega, wvno2);
     double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
-    layer_trig(L, omega, wvno2, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
+    varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
     double AA[4][4];
-    hska(AA, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat, pex,
-         svex, wvno, wvno2, om2);
+    hska(AA, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat, lw,
+         wvno, wvno2, om2);
     double aa0[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -748,14 +806,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
       for (int j = 0; j < 4; ++j) cc = cc + AA[i][j] * vv[j];
       aa0[i] = cc;
     }
-    double t1 = 0.0;
+    double ex2 = normalize_pow2(aa0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (fabs(aa0[i]) > t1) t1 = fabs(aa0[i]);
-    if (t1 < 1.0e-40) t1 = 1.0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) vv[i] = aa0[i] / t1;
-    double ex2 = log(t1);
+    for (int i = 0; i < 4; ++i) vv[i] = aa0[i];
     exsum = exsum + pex + ex2;
     // eigenfunction at the top of layer m+1 (svfunc)
     int i1 = m + 1;
@@ -771,18 +824,18 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     double ext = exsum + exe_i - exe0;
     U4 un;
     if (ext > -80.0 && ext < 80.0) {
-      double fact = exp(ext);
-      un.ur = uu1 * fact / f1213;
-      un.uz = uu2 * fact / f1213;
-      un.tz = uu3 * fact / f1213;
-      un.tr = uu4 * fact / f1213;
+      double fact = exp(ext) * r1213;
+      un.ur = uu1 * fact;
+      un.uz = uu2 * fact;
+      un.tz = uu3 * fact;
+      un.tr = uu4 * fact;
     } else {
       un.ur = un.uz = un.tz = un.tr = 0.0;
     }
     if (i1 < jwat) { un.ur = 0.0; un.tr = 0.0; }
     // energy of layer m (internal)
     double ka, kb, kr, urf;
-    layer_energy(L, allfluid, 0, uprev, un, omega, om2, wvno, wvno2, c, S, ka, kb, kr, urf);
+    layer_energy(L, &lw, 0, uprev, un, omega, om2, wvno, wvno2, c, S, ka, kb, kr, urf);
     long long o = D.base + (long long)m * D.stride;
     D.p[0][o] = ka;
     D.p[1][o] = kb;
@@ -808,7 +861,8 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   // half-space energy
   {
     double ka, kb, kr, urf;
-    layer_energy(Lprev, allfluid, 1, uprev, uprev, omega, om2, wvno, wvno2, c, S, ka, kb, kr, urf);
+    layer_energy(Lprev, nullptr, 1, uprev, uprev, omega, om2, wvno, wvno2, c, S, ka, kb, kr,
+                 urf);
     long long o = D.base + (long long)(N - 1) * D.stride;
     D.p[0][o] = ka;
     D.p[1][o] = kb;
