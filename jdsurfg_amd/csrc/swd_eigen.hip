@@ -36,6 +36,27 @@ struct Layer {
   int iwat;
 };
 
+// Per-pass constants: frequency, wavenumber and the reciprocals the layer
+// loops multiply by (a division costs about ten FP64 instructions).
+struct Wk {
+  double om, om2, k, k2;     // omega, omega^2, wavenumber, wavenumber^2
+  double ik, ik2, ic, kw;    // 1/k, 1/k^2, k/omega = 1/c, k^2/omega^2
+  double iom2;               // 1/omega^2
+};
+JDSG_DEV Wk make_wk(double omega, double wvno) {
+  Wk K;
+  K.om = omega;
+  K.om2 = omega * omega;
+  K.k = wvno;
+  K.k2 = wvno * wvno;
+  K.ik = 1.0 / wvno;
+  K.ik2 = K.ik * K.ik;
+  K.ic = wvno / omega;
+  K.kw = K.ic * K.ic;
+  K.iom2 = 1.0 / K.om2;
+  return K;
+}
+
 // Vertical wavenumbers of a layer and their layer exponentials.  ra and rb
 // are real (evanescent) or imaginary (propagating), so exp(-ra d) is either
 // exp(-Re(ra) d) or cos(Im(ra) d) - i sin(Im(ra) d): one exp or one sincos
@@ -48,6 +69,7 @@ struct LayerWaves {
   double pr, pi;      // ra d
   double qr, qi;      // rb d
   double ma, mb;      // exp(-Re(ra) d), exp(-Re(rb) d): 1 when imaginary
+  double ia, ib;      // 1 / (the nonzero part of ra, rb)
 };
 
 JDSG_DEV LayerWaves layer_waves(const Layer &L, double omega, double wvno2) {
@@ -56,6 +78,8 @@ JDSG_DEV LayerWaves layer_waves(const Layer &L, double omega, double wvno2) {
   double xkb = (L.zb > 0.0) ? omega / L.zb : 0.0;
   w.ra = csqrt_real(wvno2 - xka * xka);
   w.rb = csqrt_real(wvno2 - xkb * xkb);
+  w.ia = 1.0 / ((w.ra.im != 0.0) ? w.ra.im : w.ra.re);
+  w.ib = 1.0 / ((w.rb.im != 0.0) ? w.rb.im : w.rb.re);
   w.pr = w.ra.re * L.zd;
   w.pi = w.ra.im * L.zd;
   w.qr = w.rb.re * L.zd;
@@ -97,13 +121,13 @@ JDSG_DEV void varsv_w(const LayerWaves &w, const Layer &L, double &cosp, double 
     double c = w.Ea.re, s = -w.Ea.im;
     cosp = c;
     rsinp = -(w.ra.im * s);
-    sinpr = (cabs_(w.ra) < tiny) ? L.zd : s / w.ra.im;
+    sinpr = (cabs_(w.ra) < tiny) ? L.zd : s * w.ia;
   } else {
     double pfac = (w.pr == 0.0) ? 1.0 : (w.pr < 30.) ? w.ma * w.ma : 0.0;
     cosp = 0.5 + pfac * 0.5;
     double sinp = 0.5 - pfac * 0.5;
     rsinp = w.ra.re * sinp;
-    sinpr = (fabs(w.pr) < tiny && cabs_(w.ra) < tiny) ? L.zd : sinp / w.ra.re;
+    sinpr = (fabs(w.pr) < tiny && cabs_(w.ra) < tiny) ? L.zd : sinp * w.ia;
   }
   if (L.iwat == 1) {
     cosq = 1.0;
@@ -116,13 +140,13 @@ JDSG_DEV void varsv_w(const LayerWaves &w, const Layer &L, double &cosp, double 
     double c = w.Eb.re, s = -w.Eb.im;
     cosq = c;
     rsinq = -(w.rb.im * s);
-    sinqr = (cabs_(w.rb) < tiny) ? L.zd : s / w.rb.im;
+    sinqr = (cabs_(w.rb) < tiny) ? L.zd : s * w.ib;
   } else {
     double svfac = (w.qr == 0.0) ? 1.0 : (w.qr < 30.) ? w.mb * w.mb : 0.0;
     cosq = 0.5 + svfac * 0.5;
     double sinq = 0.5 - svfac * 0.5;
     rsinq = w.rb.re * sinq;
-    sinqr = (fabs(w.qr) < tiny && cabs_(w.rb) < tiny) ? L.zd : sinq / w.rb.re;
+    sinqr = (fabs(w.qr) < tiny && cabs_(w.rb) < tiny) ? L.zd : sinq * w.ib;
   }
 }
 
@@ -147,7 +171,8 @@ JDSG_DEV double normalize_pow2(double (&v)[K]) {
 // sregn96.f90:491-647 dnka; returns the 5x5 matrix row-major in ca
 JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, double cossv,
                      double rsinsv, double sinsvr, float rho, float b, int iwat,
-                     const LayerWaves &w, double wvno, double wvno2, double om2) {
+                     const LayerWaves &w, const Wk &K) {
+  const double wvno = K.k, wvno2 = K.k2, om2 = K.om2;
   // exp(-ex) and exp(-exa) of the reference (ex = Re(ra) d, exa = ex +
   // Re(rb) d) are the layer magnitudes ma and ma*mb
   const double ex = w.pr, exa = w.pr + w.qr;
@@ -160,7 +185,8 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
     ca[2][2] = dfac;
     ca[0][0] = cosp;
     ca[4][4] = cosp;
-    ca[0][1] = -rsinp / ((double)rho * om2);
+    const double irom = 1.0 / ((double)rho * om2);
+    ca[0][1] = -rsinp * irom;
     ca[1][0] = -(double)rho * sinpr * om2;
     ca[1][1] = cosp;
     ca[3][3] = cosp;
@@ -174,15 +200,15 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
          wz = sinpr * rsinsv;
   float rho2 = rho * rho;
   float bb2 = 2.0f * b * b;
-  double gam = (double)bb2 * wvno2 / om2;
+  double gam = (double)bb2 * K.kw;
   double gam2 = gam * gam;
   double gamm1 = gam - 1.;
   double gamm2 = gamm1 * gamm1;
   double cqww2 = cqw * wvno2;
-  double cqxw2 = cqx / wvno2;
+  double cqxw2 = cqx * K.ik2;
   double gg1 = gam * gamm1;
   double a0c = 2.0 * (a0 - cpcq);
-  double xz2 = xz / wvno2;
+  double xz2 = xz * K.ik2;
   double gxz2 = gam * xz2;
   double g2xz2 = gam2 * xz2;
   double a0cgg1 = a0c * (gam + gamm1);
@@ -190,22 +216,23 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
   double g2wy2 = gamm2 * wy2;
   double g1wy2 = gamm1 * wy2;
   double rom = (double)rho * om2;
+  const double irom = 1.0 / rom;
   double temp = a0c * gg1 + g2xz2 + g2wy2;
   ca[2][2] = a0 + temp + temp;
   ca[0][0] = cpcq - temp;
-  ca[0][1] = (-cqx + wvno2 * cpy) / rom;
+  ca[0][1] = (-cqx + wvno2 * cpy) * irom;
   temp = 0.5 * a0cgg1 + gxz2 + g1wy2;
-  ca[0][2] = wvno * temp / rom;
-  ca[0][3] = (-cqww2 + cpz) / rom;
+  ca[0][2] = wvno * temp * irom;
+  ca[0][3] = (-cqww2 + cpz) * irom;
   temp = wvno2 * (a0c + wy2) + xz;
   ca[0][4] = -temp / ((double)rho2 * om2 * om2);
-  ca[1][0] = (-gamm2 * cqw + gam2 * cpz / wvno2) * (double)rho * om2;
+  ca[1][0] = (-gamm2 * cqw + gam2 * cpz * K.ik2) * (double)rho * om2;
   ca[1][1] = cpcq;
-  ca[1][2] = (gamm1 * cqww2 - gam * cpz) / wvno;
+  ca[1][2] = (gamm1 * cqww2 - gam * cpz) * K.ik;
   ca[1][3] = -wz;
   ca[1][4] = ca[0][3];
   temp = 0.5 * a0cgg1 * gg1 + gam2 * gxz2 + gamm2 * g1wy2;
-  ca[2][0] = -2.0 * temp * (double)rho * om2 / wvno;
+  ca[2][0] = -2.0 * temp * (double)rho * om2 * K.ik;
   ca[2][1] = -wvno * (gam * cqxw2 - gamm1 * cpy) * 2.0;
   ca[2][3] = -2.0 * ca[1][2];
   ca[2][4] = -2.0 * ca[0][2];
@@ -215,7 +242,7 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
   ca[3][3] = ca[1][1];
   ca[3][4] = ca[0][1];
   temp = gamm2 * (a0c * gam2 + g2wy2) + gam2 * g2xz2;
-  ca[4][0] = -(double)rho2 * om2 * om2 * temp / wvno2;
+  ca[4][0] = -(double)rho2 * om2 * om2 * temp * K.ik2;
   ca[4][1] = ca[3][0];
   ca[4][2] = -ca[2][0] / 2.0;
   ca[4][3] = ca[1][0];
@@ -225,7 +252,8 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
 // sregn96.f90:913-987 hska
 JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, double tcossv,
                    double trsinsv, double tsinsvr, float rho, float b, int iwat,
-                   const LayerWaves &w, double wvno, double wvno2, double om2) {
+                   const LayerWaves &w, const Wk &K) {
+  const double wvno = K.k, wvno2 = K.k2, om2 = K.om2;
   const double pex = w.pr, svex = (iwat == 1) ? 0.0 : w.qr;
   if (iwat == 1) {
 #pragma unroll
@@ -237,7 +265,7 @@ JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, dou
     AA[3][3] = dfac;
     AA[1][1] = cosp;
     AA[2][2] = cosp;
-    AA[1][2] = -rsinp / ((double)rho * om2);
+    AA[1][2] = -rsinp * (1.0 / ((double)rho * om2));
     AA[2][1] = -(double)rho * om2 * sinpr;
     return;
   }
@@ -247,22 +275,23 @@ JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, dou
   double rsinsv = dfac * trsinsv;
   double sinsvr = dfac * tsinsvr;
   float bb2 = 2.0f * b * b;
-  double gam = (double)bb2 * wvno2 / om2;
+  double gam = (double)bb2 * K.kw;
   double gamm1 = gam - 1.0;
   double rom = (double)rho * om2;
+  const double irom = 1.0 / rom;
   AA[0][0] = cossv + gam * (cosp - cossv);
-  AA[0][1] = -wvno * gamm1 * sinpr + gam * rsinsv / wvno;
-  AA[0][2] = -wvno * (cosp - cossv) / rom;
-  AA[0][3] = (wvno2 * sinpr - rsinsv) / rom;
-  AA[1][0] = gam * rsinp / wvno - wvno * gamm1 * sinsvr;
+  AA[0][1] = -wvno * gamm1 * sinpr + gam * rsinsv * K.ik;
+  AA[0][2] = -wvno * (cosp - cossv) * irom;
+  AA[0][3] = (wvno2 * sinpr - rsinsv) * irom;
+  AA[1][0] = gam * rsinp * K.ik - wvno * gamm1 * sinsvr;
   AA[1][1] = cosp - gam * (cosp - cossv);
-  AA[1][2] = (-rsinp + wvno2 * sinsvr) / rom;
+  AA[1][2] = (-rsinp + wvno2 * sinsvr) * irom;
   AA[1][3] = -AA[0][2];
-  AA[2][0] = rom * gam * gamm1 * (cosp - cossv) / wvno;
-  AA[2][1] = rom * (-gamm1 * gamm1 * sinpr + gam * gam * rsinsv / wvno2);
+  AA[2][0] = rom * gam * gamm1 * (cosp - cossv) * K.ik;
+  AA[2][1] = rom * (-gamm1 * gamm1 * sinpr + gam * gam * rsinsv * K.ik2);
   AA[2][2] = AA[1][1];
   AA[2][3] = -AA[0][1];
-  AA[3][0] = rom * (gam * gam * rsinp / wvno2 - gamm1 * gamm1 * sinsvr);
+  AA[3][0] = rom * (gam * gam * rsinp * K.ik2 - gamm1 * gamm1 * sinsvr);
   AA[3][1] = -AA[2][0];
   AA[3][2] = -AA[1][0];
   AA[3][3] = AA[0][0];
@@ -460,15 +489,15 @@ JDSG_DEV double int_fluid(const EScal &s, int typelyr, cplx kmpu, cplx km1pd, cp
 // is a real combination of the six forms pMp, qMq, pMq, rMr, sMs, rMs, and
 // those are sums of four amplitude products.  ra^2 and rb^2 are real.
 JDSG_DEV void solid_ints(const Layer &L, const LayerWaves &w, const U4 &u0, const U4 &u1,
-                         double om, double om2, double wvno, double INT[6]) {
-  const double k = wvno, z = L.zrho * om2, d = L.zd;
-  const double gm = L.zb * wvno / om;
+                         const Wk &K, double irz, double INT[6]) {
+  const double k = K.k, z = L.zrho * K.om2, d = L.zd;
+  const double gm = L.zb * K.ic;
   const double g = 2.0 * (gm * gm), g1 = g - 1.0;
-  const double gk = z * g / wvno, izh = 0.5 / z;
-  const double hg = 0.5 * g / k, hk = k * izh;
+  const double gk = z * g * K.ik, izh = 0.5 * irz;
+  const double hg = 0.5 * g * K.ik, hk = k * izh;
   // 1/ra, 1/rb of a real or imaginary number
-  const cplx ira = (w.ra.im == 0.0) ? cmk(1.0 / w.ra.re, 0.0) : cmk(0.0, -1.0 / w.ra.im);
-  const cplx irb = (w.rb.im == 0.0) ? cmk(1.0 / w.rb.re, 0.0) : cmk(0.0, -1.0 / w.rb.im);
+  const cplx ira = (w.ra.im == 0.0) ? cmk(w.ia, 0.0) : cmk(0.0, -w.ia);
+  const cplx irb = (w.rb.im == 0.0) ? cmk(w.ib, 0.0) : cmk(0.0, -w.ib);
   // wave amplitudes E^-1 u at the layer bottom (up-going) and top (down-going)
   const cplx kmpu = (hg * u1.ur - izh * u1.tz) + (hk * u1.tr - 0.5 * g1 * u1.uz) * ira;
   const cplx kmsu = (hg * u1.uz - izh * u1.tr) + (hk * u1.tz - 0.5 * g1 * u1.ur) * irb;
@@ -483,10 +512,16 @@ JDSG_DEV void solid_ints(const Layer &L, const LayerWaves &w, const U4 &u0, cons
                   : (0.5 * (1.0 - ((w.qr < 40.0) ? w.Eb * w.Eb : zero))) * irb;
   const cplx GB = (w.qr < 75.0) ? w.Eb * d : zero;
   const cplx sab = w.rb + w.ra, dab = w.ra - w.rb;
+  // complex quotients as a conj(b) / |b|^2 (|ra|, |rb| are of order
+  // omega / velocity, far from overflow): one division each
+  const double n1 = sab.re * sab.re + sab.im * sab.im;
+  const double n2 = dab.re * dab.re + dab.im * dab.im;
   const cplx H1 = (cabs_(sab) < 1.0e-08) ? cmk(d, 0.0)
-                  : (1.0 - ((sab.re * d < 40.0) ? w.Ea * w.Eb : zero)) / sab;
+                  : ((1.0 - ((sab.re * d < 40.0) ? w.Ea * w.Eb : zero)) * cconj(sab)) *
+                        (1.0 / n1);
   const cplx H2 = (cabs_(dab) < 1.0e-08) ? cmk(d, 0.0)
-                  : (((w.qr < 40.0) ? w.Eb : zero) - ((w.pr < 40.0) ? w.Ea : zero)) / dab;
+                  : ((((w.qr < 40.0) ? w.Eb : zero) - ((w.pr < 40.0) ? w.Ea : zero)) *
+                     cconj(dab)) * (1.0 / n2);
   // the six quadratic forms of M
   const cplx A2 = kmpu * kmpu + km1pd * km1pd, X = kmpu * km1pd;
   const cplx B2 = kmsu * kmsu + km1sd * km1sd, Y = kmsu * km1sd;
@@ -517,13 +552,13 @@ struct EnergySums {
 // energy contribution of layer m (sregn96.f90:1061-1161 loop body).  Writes
 // the unscaled dc/da, dc/db, dc/drho of layer m to ka/kb/kr.
 JDSG_DEV void layer_energy(const Layer &L, const LayerWaves *lw, int typelyr, const U4 &u0,
-                           const U4 &u1, double om, double om2, double wvno, double wvno2,
-                           double c, EnergySums &S, double &ka, double &kb, double &kr,
-                           double &ur_fluid) {
-  const EScal s = escal(L, wvno, om, om2, wvno2);
+                           const U4 &u1, const Wk &K, double c, EnergySums &S, double &ka,
+                           double &kb, double &kr, double &ur_fluid) {
+  const double om = K.om, om2 = K.om2, wvno = K.k, wvno2 = K.k2;
   double ah = L.za, av = L.za, bv = L.zb, rho = L.zrho, eta = 1.0;
   double TA = L.zrho * L.za * L.za, TC = TA;
   if (L.iwat == 1) {
+    const EScal s = escal(L, wvno, om, om2, wvno2);
     double TN = 0.0;
     double TF = TA - 2. * TN;
     double a12 = -(wvno * wvno - om * om / (ah * ah)) / (rho * om * om);
@@ -556,11 +591,12 @@ JDSG_DEV void layer_energy(const Layer &L, const LayerWaves *lw, int typelyr, co
   }
   double TL = L.zrho * L.zb * L.zb, TN = TL;
   double TF = TA - 2. * TN;
-  double a12 = -wvno, a14 = 1.0 / TL, a21 = wvno * TF / TC, a23 = 1.0 / TC;
+  double a12 = -wvno, a14 = 1.0 / TL, a23 = 1.0 / TC, a21 = wvno * TF * a23;
+  const double irho = a23 * (L.za * L.za);
   double INT11, INT13, INT22, INT24, INT33, INT44;
   if (typelyr == 0) {
     double I6[6];
-    solid_ints(L, *lw, u0, u1, om, om2, wvno, I6);
+    solid_ints(L, *lw, u0, u1, K, irho * K.iom2, I6);
     INT11 = I6[0];
     INT13 = I6[1];
     INT22 = I6[2];
@@ -568,6 +604,7 @@ JDSG_DEV void layer_energy(const Layer &L, const LayerWaves *lw, int typelyr, co
     INT33 = I6[4];
     INT44 = I6[5];
   } else {
+    const EScal s = escal(L, wvno, om, om2, wvno2);
     cplx km1pd = Eis<2, 0>(s) * u0.ur + Eis<2, 1>(s) * u0.uz + Eis<2, 2>(s) * u0.tz +
                  Eis<2, 3>(s) * u0.tr;
     cplx km1sd = Eis<3, 0>(s) * u0.ur + Eis<3, 1>(s) * u0.uz + Eis<3, 2>(s) * u0.tz +
@@ -589,21 +626,23 @@ JDSG_DEV void layer_energy(const Layer &L, const LayerWaves *lw, int typelyr, co
   S.s1 = S.s1 + TL * UZUZ + TA * URUR;
   S.s2 = S.s2 + TL * UZDUR - TF * URDUZ;
   S.s3 = S.s3 + TL * DURDUR + TC * DUZDUZ;
-  double facah = rho * ah * (URUR - 2. * eta * URDUZ / wvno);
-  double facav = rho * av * DUZDUZ / wvno2;
+  double facah = rho * ah * (URUR - 2. * eta * URDUZ * K.ik);
+  double facav = rho * av * DUZDUZ * K.ik2;
   double facbh = 0.0;
-  double facbv = rho * bv * (UZUZ + 2. * UZDUR / wvno + DURDUR / wvno2 + 4. * eta * URDUZ / wvno);
+  double facbv = rho * bv * (UZUZ + 2. * UZDUR * K.ik + DURDUR * K.ik2 + 4. * eta * URDUZ * K.ik);
   ka = facah + facav;
   kb = facbv + facbh;
   double facr = -0.5 * c * c * (URUR + UZUZ);
-  kr = 0.5 * (av * facav + ah * facah + bv * facbv) / rho + facr;
+  kr = 0.5 * (av * facav + ah * facah + bv * facbv) * irho + facr;
   ur_fluid = u0.ur;
 }
 
 // sregn96.f90:1430-1529 getdcdh, sum of the six terms (before fac)
+// ixl2, ixmu (and ixl2p, ixmup of the layer above): 1/(lambda + 2 mu) and
+// 1/mu, formed once per layer and reused for the next interface.
 JDSG_DEV double dcdh_terms(int m, const Layer &Lm, const Layer &Lp, double xmu, double xlam,
-                           double xmup, double xlamp, const U4 &u, double om2, double wvno,
-                           double wvno2) {
+                           double xmup, double xlamp, double ixl2, double ixmu, double ixl2p,
+                           double ixmup, const U4 &u, double om2, double wvno, double wvno2) {
   double tuz = u.uz, ttz = u.tz, ttr = u.tr, tur;
   if (Lm.iwat == 1)
     tur = -wvno * ttz / (Lm.zrho * om2);
@@ -616,8 +655,8 @@ JDSG_DEV double dcdh_terms(int m, const Layer &Lm, const Layer &Lp, double xmu, 
     double dlm = xlam - 0.0;
     double dl2mu = dlm + dmu + dmu;
     double xl2mp = xlam + xmu + xmu;
-    double duzdzp = (ttz + wvno * xlam * tur) / xl2mp;
-    double durdzp = (Lm.iwat == 1) ? wvno * tuz : (ttr / xmu) - wvno * tuz;
+    double duzdzp = (ttz + wvno * xlam * tur) * ixl2;
+    double durdzp = (Lm.iwat == 1) ? wvno * tuz : (ttr * ixmu) - wvno * tuz;
     double drur2 = tur * tur * drho;
     double dlur2 = tur * tur * dl2mu;
     gfac1 = om2 * drho * (tuz * tuz);
@@ -633,9 +672,9 @@ JDSG_DEV double dcdh_terms(int m, const Layer &Lm, const Layer &Lp, double xmu, 
     double dl2mu = dlm + dmu + dmu;
     double xl2mp = xlam + xmu + xmu;
     double xl2mm = xlamp + xmup + xmup;
-    double duzdzp = (ttz + wvno * xlam * tur) / xl2mp;
-    double durdzp = (xmu == 0.0) ? wvno * tuz : (ttr / xmu) - wvno * tuz;
-    double durdzm = (xmup == 0.0) ? wvno * tuz : (ttr / xmup) - wvno * tuz;
+    double duzdzp = (ttz + wvno * xlam * tur) * ixl2;
+    double durdzp = (xmu == 0.0) ? wvno * tuz : (ttr * ixmu) - wvno * tuz;
+    double durdzm = (xmup == 0.0) ? wvno * tuz : (ttr * ixmup) - wvno * tuz;
     double drur2, dlur2, duzdzm;
     if (Lp.iwat == 1 && Lm.iwat == 0) {
       double URB = -wvno * u.tz / (Lp.zrho * om2);
@@ -650,7 +689,7 @@ JDSG_DEV double dcdh_terms(int m, const Layer &Lm, const Layer &Lp, double xmu, 
     } else {
       drur2 = tur * tur * drho;
       dlur2 = tur * tur * dl2mu;
-      duzdzm = (ttz + wvno * xlamp * tur) / xl2mm;
+      duzdzm = (ttz + wvno * xlamp * tur) * ixl2p;
     }
     gfac1 = om2 * drho * (tuz * tuz);
     gfac2 = om2 * drur2;
@@ -714,6 +753,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
                             double *scr, long long sstride, const Dst &D, double &ugr,
                             double &sumi0, double &fac, double *eig, long long ebase) {
   double om2 = omega * omega, wvno2 = wvno * wvno;
+  const Wk K = make_wk(omega, wvno);
   bool allfluid = true;
   int jwat = 0;
   bool topwater = true;
@@ -746,7 +786,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
       varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
       double ca[5][5];
       dnka_r(ca, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat,
-             lw, wvno, wvno2, om2);
+             lw, K);
       double ee[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
@@ -783,9 +823,11 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   Layer Lprev = load_layer(A, col, 0, true);
   double xmu_prev = Lprev.zrho * (Lprev.zb * Lprev.zb);
   double xlam_prev = Lprev.zrho * (Lprev.za * Lprev.za) - 2 * xmu_prev;
+  double ixmu_prev = 1.0 / xmu_prev, ixl2_prev = 1.0 / (xlam_prev + xmu_prev + xmu_prev);
   // getdcdh for layer 0
   {
-    double g = dcdh_terms(0, Lprev, Lprev, xmu_prev, xlam_prev, 0.0, 0.0, uprev, om2, wvno, wvno2);
+    double g = dcdh_terms(0, Lprev, Lprev, xmu_prev, xlam_prev, 0.0, 0.0, ixl2_prev, ixmu_prev,
+                          0.0, 0.0, uprev, om2, wvno, wvno2);
     D.p[3][D.base] = g;
   }
   Layer Lm1 = Lprev;  // layer m-1 props for getdcdh
@@ -797,7 +839,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
     double AA[4][4];
     hska(AA, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat, lw,
-         wvno, wvno2, om2);
+         K);
     double aa0[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -835,7 +877,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     if (i1 < jwat) { un.ur = 0.0; un.tr = 0.0; }
     // energy of layer m (internal)
     double ka, kb, kr, urf;
-    layer_energy(L, &lw, 0, uprev, un, omega, om2, wvno, wvno2, c, S, ka, kb, kr, urf);
+    layer_energy(L, &lw, 0, uprev, un, K, c, S, ka, kb, kr, urf);
     long long o = D.base + (long long)m * D.stride;
     D.p[0][o] = ka;
     D.p[1][o] = kb;
@@ -850,19 +892,22 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     Layer Ln = load_layer(A, col, i1, true);
     double xmu_n = Ln.zrho * (Ln.zb * Ln.zb);
     double xlam_n = Ln.zrho * (Ln.za * Ln.za) - 2 * xmu_n;
-    double g = dcdh_terms(i1, Ln, L, xmu_n, xlam_n, xmu_prev, xlam_prev, un, om2, wvno, wvno2);
+    double ixmu_n = 1.0 / xmu_n, ixl2_n = 1.0 / (xlam_n + xmu_n + xmu_n);
+    double g = dcdh_terms(i1, Ln, L, xmu_n, xlam_n, xmu_prev, xlam_prev, ixl2_n, ixmu_n,
+                          ixl2_prev, ixmu_prev, un, om2, wvno, wvno2);
     D.p[3][D.base + (long long)i1 * D.stride] = g;
     uprev = un;
     Lprev = Ln;
     xmu_prev = xmu_n;
     xlam_prev = xlam_n;
+    ixmu_prev = ixmu_n;
+    ixl2_prev = ixl2_n;
     (void)Lm1; (void)xmu_m1; (void)xlam_m1;
   }
   // half-space energy
   {
     double ka, kb, kr, urf;
-    layer_energy(Lprev, nullptr, 1, uprev, uprev, omega, om2, wvno, wvno2, c, S, ka, kb, kr,
-                 urf);
+    layer_energy(Lprev, nullptr, 1, uprev, uprev, K, c, S, ka, kb, kr, urf);
     long long o = D.base + (long long)(N - 1) * D.stride;
     D.p[0][o] = ka;
     D.p[1][o] = kb;
