@@ -151,16 +151,17 @@ JDSG_DEV void varsv_w(const LayerWaves &w, const Layer &L, double &cosp, double 
 }
 
 // Scale a sweep vector by a power of two so that its largest entry lies in
-// [0.5, 1): the sweeps of sregn96.f90 (:475-485 and :1040-1050) divide by
-// the largest entry and add its logarithm to the exponent sum; scaling by
-// 2^-e instead is exact and adds e ln 2.
+// [0.5, 1): the sweeps of sregn96.f90 (:475-485, :1040-1050) and
+// slegn96.f90 (:420-440) divide by the largest entry (left alone below
+// `floor`) and add its logarithm to the exponent sum; scaling by 2^-e
+// instead is exact and adds e ln 2.
 template <int K>
-JDSG_DEV double normalize_pow2(double (&v)[K]) {
+JDSG_DEV double normalize_pow2(double (&v)[K], double floor = 1.0e-40) {
   double t1 = 0.0;
 #pragma unroll
   for (int i = 0; i < K; ++i)
     if (fabs(v[i]) > t1) t1 = fabs(v[i]);
-  if (t1 < 1.0e-40) return 0.0;
+  if (t1 < floor) return 0.0;
   int e;
   (void)frexp(t1, &e);
 #pragma unroll
@@ -826,9 +827,11 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   double ixmu_prev = 1.0 / xmu_prev, ixl2_prev = 1.0 / (xlam_prev + xmu_prev + xmu_prev);
   // getdcdh for layer 0
   {
-    double g = dcdh_terms(0, Lprev, Lprev, xmu_prev, xlam_prev, 0.0, 0.0, ixl2_prev, ixmu_prev,
-                          0.0, 0.0, uprev, om2, wvno, wvno2);
-    D.p[3][D.base] = g;
+    if (D.p[3]) {
+      double g = dcdh_terms(0, Lprev, Lprev, xmu_prev, xlam_prev, 0.0, 0.0, ixl2_prev, ixmu_prev,
+                            0.0, 0.0, uprev, om2, wvno, wvno2);
+      D.p[3][D.base] = g;
+    }
   }
   Layer Lm1 = Lprev;  // layer m-1 props for getdcdh
   double xmu_m1 = xmu_prev, xlam_m1 = xlam_prev;
@@ -879,9 +882,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     double ka, kb, kr, urf;
     layer_energy(L, &lw, 0, uprev, un, K, c, S, ka, kb, kr, urf);
     long long o = D.base + (long long)m * D.stride;
-    D.p[0][o] = ka;
-    D.p[1][o] = kb;
-    D.p[2][o] = kr;
+    if (D.p[0]) D.p[0][o] = ka;
+    if (D.p[1]) D.p[1][o] = kb;
+    if (D.p[2]) D.p[2][o] = kr;
     if (eig) {
       A.eig[0][ebase + m] = urf;
       A.eig[1][ebase + m] = uprev.uz;
@@ -893,9 +896,11 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     double xmu_n = Ln.zrho * (Ln.zb * Ln.zb);
     double xlam_n = Ln.zrho * (Ln.za * Ln.za) - 2 * xmu_n;
     double ixmu_n = 1.0 / xmu_n, ixl2_n = 1.0 / (xlam_n + xmu_n + xmu_n);
-    double g = dcdh_terms(i1, Ln, L, xmu_n, xlam_n, xmu_prev, xlam_prev, ixl2_n, ixmu_n,
-                          ixl2_prev, ixmu_prev, un, om2, wvno, wvno2);
-    D.p[3][D.base + (long long)i1 * D.stride] = g;
+    if (D.p[3]) {
+      double g = dcdh_terms(i1, Ln, L, xmu_n, xlam_n, xmu_prev, xlam_prev, ixl2_n, ixmu_n,
+                            ixl2_prev, ixmu_prev, un, om2, wvno, wvno2);
+      D.p[3][D.base + (long long)i1 * D.stride] = g;
+    }
     uprev = un;
     Lprev = Ln;
     xmu_prev = xmu_n;
@@ -909,9 +914,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     double ka, kb, kr, urf;
     layer_energy(Lprev, nullptr, 1, uprev, uprev, K, c, S, ka, kb, kr, urf);
     long long o = D.base + (long long)(N - 1) * D.stride;
-    D.p[0][o] = ka;
-    D.p[1][o] = kb;
-    D.p[2][o] = kr;
+    if (D.p[0]) D.p[0][o] = ka;
+    if (D.p[1]) D.p[1][o] = kb;
+    if (D.p[2]) D.p[2][o] = kr;
     if (eig) {
       A.eig[0][ebase + N - 1] = urf;
       A.eig[1][ebase + N - 1] = uprev.uz;
@@ -929,6 +934,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
 // slegn96.f90:248-328 varl
 struct VarL {
   double cosq, sinq, yl, zl, mu, rb, xkb, eexl;
+  cplx E1;  // exp(-nub d): (cos q, -sin q) propagating, exp(-q) evanescent
 };
 JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
   VarL v;
@@ -944,13 +950,17 @@ JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
     sincos(q, &v.sinq, &v.cosq);
     v.yl = v.sinq / v.rb;
     v.zl = -v.rb * v.sinq;
+    v.E1 = cmk(v.cosq, -v.sinq);
   } else if (wvno == v.xkb) {
     v.cosq = 1.0;
     v.yl = dpth;
     v.zl = 0.0;
+    v.E1 = cmk(1.0, 0.0);
   } else {
     v.eexl = q;
-    double fac = (q < 18.0) ? exp(-2.0 * q) : 0.0;
+    const double e1 = exp(-q);
+    v.E1 = cmk(e1, 0.0);
+    double fac = (q < 18.0) ? e1 * e1 : 0.0;
     v.cosq = (1.0 + fac) * 0.5;
     v.sinq = (1.0 - fac) * 0.5;
     v.yl = v.sinq / v.rb;
@@ -1006,14 +1016,10 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
       }
       VarL v = varl(L, omega, wvno, L.zd);
       double a11 = v.cosq, a22 = v.cosq, a12 = -(v.yl / v.mu), a21 = -(v.zl * v.mu);
-      double amp0 = a11 * u1 + a12 * (t1v);
-      double str0 = a21 * u1 + a22 * (t1v);
-      double rr = fabs(amp0), ss = fabs(str0);
-      if (ss > rr) rr = ss;
-      if (rr < 1.0e-30) rr = 1.0;
-      EXL(k) = log(rr) + v.eexl;
-      u1 = amp0 / rr;
-      t1v = str0 / rr;
+      double at[2] = {a11 * u1 + a12 * (t1v), a21 * u1 + a22 * (t1v)};
+      EXL(k) = normalize_pow2(at, 1.0e-30) + v.eexl;
+      u1 = at[0];
+      t1v = at[1];
       UU(k) = u1;
       TT(k) = t1v;
     }
@@ -1022,7 +1028,7 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
   // the largest normalised amplitude; the normalised amplitudes are formed
   // where they are used, so the sweep storage is read, not rewritten
   double umax = UU(0);
-  {
+  if (umax == 0.0) {  // the search below ends with umax = UU(0) otherwise
     double ext = 0.0;
     for (int k = 1; k < N; ++k) {
       double uk = 0.0;
@@ -1033,9 +1039,9 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
       }
       if (fabs(uk) > fabs(umax)) umax = uk;
     }
-    if (UU(0) != 0.0) umax = UU(0);
   }
   const bool norm = fabs(umax) > 0.0;
+  const double iumax = norm ? 1.0 / umax : 0.0;
   // energy (slegn96.f90:447-629) fused with the dc/dh terms (:585-610,
   // stored before the factor fac, which needs the finished sums) and the
   // eigenfunctions
@@ -1043,8 +1049,9 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
   double omega2 = omega * omega, wvno2 = wvno * wvno;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   bool solid0 = love_solid(A, col, 0, pu);
-  double u_k = (norm && solid0) ? UU(0) / umax : UU(0);
-  double t_k = (norm && solid0) ? 0.0 / umax : 0.0;
+  double u_k = (norm && solid0) ? UU(0) * iumax : UU(0);
+  double t_k = 0.0;
+  const double ik2 = 1.0 / wvno2;
   double ext = 0.0;
   int llflag = 0;
   double zr_prev = 0.0, xmu_prev = 0.0;
@@ -1056,20 +1063,17 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
     if (k + 1 < N) {
       if (love_solid(A, col, k + 1, pu)) {
         ext = ext + EXL(k);
-        double fact = (ext < 80.0) ? 1. / exp(ext) : 0.0;
+        double fact = (ext < 80.0) ? exp(-ext) : 0.0;
+        if (norm) fact = fact * iumax;
         u_n = UU(k + 1) * fact;
         t_n = TT(k + 1) * fact;
-        if (norm) {
-          u_n = u_n / umax;
-          t_n = t_n / umax;
-        }
       }
     }
     double xmu = L.zrho * (L.zb * L.zb);
     if (L.iwat != 0) {
-      D.p[1][o] = 0.0;
-      D.p[2][o] = 0.0;
-      D.p[3][o] = 0.0;
+      if (D.p[1]) D.p[1][o] = 0.0;
+      if (D.p[2]) D.p[2][o] = 0.0;
+      if (D.p[3]) D.p[3][o] = 0.0;
     } else {
       double TN = L.zrho * L.zb * L.zb, TL = TN;
       double VSHH = L.zb, VSHV = L.zb;
@@ -1084,30 +1088,37 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
         upup = (0.5 / rb) * uk * uk;
         dupdup = (0.5 * rb) * uk * uk;
       } else {
-        cplx nub = (wvno < v.xkb) ? cmk(0.0, rb) : cmk(rb, 0.0);
-        cplx xnub = dmu * nub;
-        cplx ei11 = cmk(0.5 / wvno, 0), ei12 = 0.5 / (wvno * xnub);
-        cplx ei21 = cmk(0.5 / wvno, 0), ei22 = -0.5 / (wvno * xnub);
-        cplx el11 = cmk(wvno, 0), el12 = cmk(wvno, 0);
-        cplx km1dn = ei21 * uk + ei22 * t_k;
-        cplx kmup = ei11 * u_n + ei12 * t_n;
-        cplx f3 = nub * dpth;
-        cplx exqq = (f3.re < 40.0) ? cexp_(-2.0 * f3) : cmk(0, 0);
-        cplx f = (1.0 - exqq) / (2.0 * nub);
-        exqq = (f3.re < 75.0) ? cexp_(-f3) : cmk(0, 0);
+        // nub is i rb or rb, so 1/nub and exp(-nub d) (from varl, unless rb
+        // was raised to the floor above) need no complex division
+        const bool prop = wvno < v.xkb;
+        const double irb = 1.0 / rb;
+        const cplx inub = prop ? cmk(0.0, -irb) : cmk(irb, 0.0);
+        const cplx ixn = (0.5 / (wvno * dmu)) * inub;  // 0.5 / (wvno xnub)
+        const double hk = 0.5 / wvno;
+        const cplx km1dn = hk * uk - ixn * t_k;
+        const cplx kmup = hk * u_n + ixn * t_n;
+        const double f3re = prop ? 0.0 : rb * dpth;
+        const cplx E1 = (v.rb < 1.0e-10) ? cexp_(-(prop ? cmk(0.0, rb) : cmk(rb, 0.0)) * dpth)
+                                         : v.E1;
+        cplx exqq = (f3re < 40.0) ? E1 * E1 : cmk(0, 0);
+        cplx f = (0.5 * (1.0 - exqq)) * inub;
+        exqq = (f3re < 75.0) ? E1 : cmk(0, 0);
         cplx g = dpth * exqq;
-        cplx f1 = f * (el11 * el11 * kmup * kmup + el12 * el12 * km1dn * km1dn);
-        cplx f2 = g * (el11 * el12 + el11 * el12) * kmup * km1dn;
+        const double w2 = wvno * wvno;
+        cplx f1 = f * (w2 * (kmup * kmup + km1dn * km1dn));
+        cplx f2 = g * ((2.0 * w2) * (kmup * km1dn));
         upup = (f1 + f2).re;
-        dupdup = (nub * nub * (f1 - f2)).re;
+        const double nub2 = prop ? -(rb * rb) : rb * rb;
+        dupdup = nub2 * (f1 - f2).re;
       }
       s0 = s0 + drho * upup;
       s1 = s1 + TN * upup;
       s2 = s2 + TL * dupdup;
       double DCDBH = c * drho * VSHH * upup;
-      double DCDBV = c * drho * VSHV * dupdup / wvno2;
-      D.p[1][o] = DCDBH + DCDBV;
-      D.p[2][o] = 0.5 * c * (-c * c * upup + VSHH * VSHH * upup + VSHV * VSHV * dupdup / wvno2);
+      double DCDBV = c * drho * VSHV * dupdup * ik2;
+      if (D.p[1]) D.p[1][o] = DCDBH + DCDBV;
+      if (D.p[2])
+        D.p[2][o] = 0.5 * c * (-c * c * upup + VSHH * VSHH * upup + VSHV * VSHV * dupdup * ik2);
       // dc/dh terms before fac
       double drh, dm, dvdz;
       if (llflag == 0) {
@@ -1119,7 +1130,7 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
         dm = xmu - xmu_prev;
         dvdz = t_k * t_k * (1.0 / xmu - 1.0 / xmu_prev);
       }
-      D.p[3][o] = uk * uk * (omega2 * drh - wvno2 * dm) + dvdz;
+      if (D.p[3]) D.p[3][o] = uk * uk * (omega2 * drh - wvno2 * dm) + dvdz;
       llflag = llflag + 1;
     }
     zr_prev = L.zrho;
@@ -1229,7 +1240,9 @@ jdsg_eigen_pass(EigenArgs A, long long it0, long long lanes, int v) {
   double wvno = omega / A.cflat[(long long)v * A.nent + r.e];
   double *draw = scr + (long long)6 * A.nmax * ss;
   Dst D;
-  for (int a = 0; a < 4; ++a) D.p[a] = draw + (long long)(4 * v + a) * A.nmax * ss;
+  // raw kernels of the requested outputs only (a null pointer skips a type)
+  for (int a = 0; a < 4; ++a)
+    D.p[a] = (A.dc[a] || A.du[a]) ? draw + (long long)(4 * v + a) * A.nmax * ss : nullptr;
   D.base = 0;
   D.stride = ss;
   long long ebase = (A.eig[0] && A.kern_off) ? A.kern_off[r.col] + r.el * r.N : 0;
@@ -1303,20 +1316,21 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
       long long o = (long long)i * ss;
       const double *p0 = draw, *p1 = draw + (long long)A.nmax * ss,
                    *p2 = draw + (long long)2 * A.nmax * ss, *p3 = draw + (long long)3 * A.nmax * ss;
-      double ka = 0.0, kbv, kr, kh;
+      // only the requested types were stored by the passes
+      double ka = 0.0, kbv = 0.0, kr = 0.0, kh = 0.0;
       if (ray) {
         double us0 = ugr * s0;
-        ka = p0[o] / us0;
-        kbv = p1[o] / us0;
-        kr = p2[o] / us0;
-        kh = fac * p3[o];
+        if (A.dc[0]) ka = p0[o] / us0;
+        if (A.dc[1]) kbv = p1[o] / us0;
+        if (A.dc[2]) kr = p2[o] / us0;
+        if (A.dc[3]) kh = fac * p3[o];
         if (fabs(kh) < 1.0e-38) kh = 0.0;
       } else {
         // Love: dcdb, dcdr divided by sumi1, dcdh final
         Layer L = load_layer_love(A, col, i, false);
-        kbv = (L.iwat == 0) ? p1[o] / s0 : 0.0;
-        kr = (L.iwat == 0) ? p2[o] / s0 : 0.0;
-        kh = fac * p3[o];
+        if (A.dc[1]) kbv = (L.iwat == 0) ? p1[o] / s0 : 0.0;
+        if (A.dc[2]) kr = (L.iwat == 0) ? p2[o] / s0 : 0.0;
+        if (A.dc[3]) kh = fac * p3[o];
         if (fabs(kh) < 1.0e-38) kh = 0.0;
       }
       if (A.iflsph > 0) {
@@ -1364,6 +1378,7 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
       if (!ray) solid = (A.cm.zb[q] > 0.0);
       double fct[4] = {vtp, vtp, rtp, dtp};
       for (int a = 0; a < 4; ++a) {
+        if (!A.dc[a] && !A.du[a]) continue;  // not stored by the passes
         double r0 = draw[(long long)(0 + a) * A.nmax * ss + os];
         double r1 = pm_ok ? draw[(long long)(4 + a) * A.nmax * ss + os] : 0.0;
         double r2 = pm_ok ? draw[(long long)(8 + a) * A.nmax * ss + os] : 0.0;

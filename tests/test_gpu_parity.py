@@ -59,3 +59,28 @@ def test_gpu_matches_oracle(name, oracle_rest):
     assert s["u"] <= U_TOL
     for n, v in k.items():
         assert v <= K_TOL, (n, v)
+
+
+@pytest.mark.parametrize("group", [True, False])
+def test_kernel_subsets_match_full_request(group):
+    """The passes store only the requested kernel types: every subset returns
+    exactly the values of a request for all eight."""
+    import torch
+    from jdsurfg_amd import swd
+    w = W.random_columns(10, 24, 11, nmode=(3, 2), group=group, water=False)
+    print(f"case kernel subsets group={group}", flush=True)
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    inp = swd.upload(w)
+    full = plan.alloc_outputs()
+    plan.execute(*inp, full)
+    torch.cuda.synchronize()
+    names = [n for n in swd.KERNEL_NAMES if group or n.startswith("dc")]
+    for sub in (("dcdb",), ("dcdh",), ("dcda", "dcdr"), ("dudb",), ("dcdb", "dudh")):
+        sub = tuple(n for n in sub if n in names)
+        if not sub:
+            continue
+        out = plan.alloc_outputs(sub)
+        plan.execute(*inp, out)
+        torch.cuda.synchronize()
+        for n in ("c", "u", "status") + sub:
+            assert torch.equal(out[n], full[n]), (sub, n)
