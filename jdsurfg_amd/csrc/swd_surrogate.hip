@@ -4,9 +4,10 @@
 // period).  Each root is then bracketed at +-half_width around the prediction
 // (one period-equation evaluation per end; without a sign change the bracket
 // is widened to three half widths, below and then above), accepted only if
-// the period equation changes sign, and polished in FP64 by Illinois false
-// position to
-// |c1-c2| <= tol*c.  A polished root is kept only if it is the root the
+// the period equation changes sign, and polished in FP64 by false position
+// with the Anderson-Bjorck weight until the bracket is within tol*c or the
+// secant correction falls below tol*c right after one below 1e-4*c (the
+// iteration is then superlinear, so the corrected value is within tol).  A polished root is kept only if it is the root the
 // reference's chain would reach: the mode-order pass checks, with the signs
 // already evaluated, that no odd number of roots lies between the previous
 // mode's bracket (or, for the lowest mode, the chain's floor cm) and this
@@ -33,7 +34,7 @@ __device__ __forceinline__ void bracket_ends(double cen, double hw, double cm, d
 // Persistent polish, one launch per wave type (W = 0 Rayleigh, 1 Love).  Each
 // lane walks its items (item = variant-major, then the wave's entries in
 // grid or list order) with a small state machine -- floor sign (lowest mode
-// only), bracket low, bracket high, Illinois steps -- so every trip of the
+// only), bracket low, bracket high, false-position steps -- so every trip of the
 // main loop evaluates exactly one period equation per live lane, and a lane
 // that finishes a root takes its next item at once instead of idling until
 // the slowest lane of its wave is done.
@@ -44,6 +45,7 @@ struct PLane {
   long long e;          // entry
   int col, variant, phase, nit;
   double omega, cm, betmx, a, b, fa, fb, c;
+  double cp;            // the previous iterate
   double a2, b2;        // the widened bracket ends (three half widths)
   double lo, hi;        // the bracket the root was taken from
   signed char s_floor, s_lo;
@@ -94,8 +96,11 @@ JDSG_DEV bool polish_begin(const PolishArgs &A, PLane &S, long long &it, long lo
     S.s_floor = 0;
     S.s_lo = 0;
     S.nit = 0;
+    S.cp = 0.0;
     double cen = A.centers[v][e];
-    bool usable = cen > 0.0 && cen < S.betmx + 0.5;
+    // a root above betmx is rejected, so a centre whose widest bracket starts
+    // above betmx cannot give one: no evaluation is spent on it
+    bool usable = cen > 0.0 && cen * (1.0 - 3.0 * A.hw[v]) < S.betmx;
     if (usable) {
       bracket_ends(cen, A.hw[v], S.cm, &S.a, &S.b);
       bracket_ends(cen, 3.0 * A.hw[v], S.cm, &S.a2, &S.b2);
@@ -212,7 +217,7 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
       }
       polish_finish(A, S, 0.0, (signed char)sign1(F), nf);
       return true;
-    default: {  // Illinois false position step with F = F(S.c)
+    default: {  // false position step with F = F(S.c)
       double fc = F;
       if (fc == 0.0) {
         S.a = S.b = S.c;
@@ -221,21 +226,34 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
           S.a = S.b;
           S.fa = S.fb;
         } else {
-          S.fa = 0.5 * S.fa;
+          // the retained end keeps its place: Anderson-Bjorck scales its
+          // value by 1 - fc/fb (0.5, the Illinois factor, when that is <= 0)
+          double m = 1.0 - fc / S.fb;
+          S.fa = ((m > 0.0) ? m : 0.5) * S.fa;
         }
         S.b = S.c;
         S.fb = fc;
       }
       ++S.nit;
-      if (fc == 0.0 || fabs(S.b - S.a) <= A.tol * fabs(S.b) || S.nit >= 80) {
-        double root = S.b;
-        if (!(root > S.cm) || root > S.betmx) root = 0.0;
-        // the bracket changed sign, so F at its top had the sign opposite to
-        // its bottom: that is the sign the mode-order pass compares
-        polish_finish(A, S, root, (signed char)(-S.s_lo), nf);
-        return true;
+      bool done = fc == 0.0 || fabs(S.b - S.a) <= A.tol * fabs(S.b) || S.nit >= 80;
+      double root = S.b;
+      if (!done) {
+        double cn = (S.a * S.fb - S.b * S.fa) / (S.fb - S.fa);
+        if (!(cn > fmin(S.a, S.b) && cn < fmax(S.a, S.b))) cn = 0.5 * (S.a + S.b);
+        if (fabs(cn - S.c) <= A.tol * fabs(S.c) && fabs(S.c - S.cp) <= 1.0e-4 * fabs(S.c)) {
+          done = true;
+          root = cn;
+        } else {
+          S.cp = S.c;
+          S.c = cn;
+          return false;
+        }
       }
-      break;
+      if (!(root > S.cm) || root > S.betmx) root = 0.0;
+      // the bracket changed sign, so F at its top had the sign opposite to
+      // its bottom: that is the sign the mode-order pass compares
+      polish_finish(A, S, root, (signed char)(-S.s_lo), nf);
+      return true;
     }
   }
   double c = (S.a * S.fb - S.b * S.fa) / (S.fb - S.fa);
