@@ -721,6 +721,11 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   Q.hw[0] = half_width;
   Q.centers[1] = P->cen;
   Q.centers[2] = P->cen + P->nent;
+  // T(1 +- 0.005) centres come from the polished T roots moved along the
+  // slope of the neighbouring periods' roots (about 1e-5 off), but the
+  // bracket stays wider than the chain's dc step (0.005 km/s): a pair of
+  // roots closer than one step is skipped by the chain, and only a bracket
+  // that holds both (no sign change) sends such an entry back to it
   Q.hw[1] = Q.hw[2] = 0.5 * half_width;
   const int pw = default_waves();
   auto polish = [&](int v0, int nv) {

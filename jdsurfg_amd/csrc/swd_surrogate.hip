@@ -383,11 +383,17 @@ extern "C" __global__ void jdsg_pm_centers(const long long *ent_off, const int *
   int k = (int)(el % K);
   const double *T = periods + per_off[col];
   double c0 = known0[e] > 0.0 ? known0[e] : pred[e];
+  // d ln c / d ln T from the neighbouring periods' polished roots (exact, so
+  // only the chord error is left), else from their predictions
   double slope = 0.0;
   int ka = k > 0 ? k - 1 : k, kb = k < K - 1 ? k + 1 : k;
   if (kb > ka) {
+    double ka0 = known0[e - (k - ka)], kb0 = known0[e + (kb - k)];
     double pa = pred[e - (k - ka)], pb = pred[e + (kb - k)];
-    if (pa > 0.0 && pb > 0.0) slope = log(pb / pa) / log(T[kb] / T[ka]);
+    if (ka0 > 0.0 && kb0 > 0.0)
+      slope = log(kb0 / ka0) / log(T[kb] / T[ka]);
+    else if (pa > 0.0 && pb > 0.0)
+      slope = log(pb / pa) / log(T[kb] / T[ka]);
   }
   double l1 = log(1.0 + 0.5 * 0.01), l2 = log(1.0 - 0.5 * 0.01);
   cen1[e] = c0 * exp(slope * l1);
