@@ -159,7 +159,10 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
   P->nchain = (long long)ncol * 2 * P->nvar;
   int dw = default_waves();
   P->chain_waves = (int)std::min<long long>(dw, ((long long)ncol * P->nvar + 63) / 64);
-  P->eigen_waves = (int)std::min<long long>(dw, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
+  // eigen chunks of 32 waves per CU: each pass then runs several rounds of
+  // waves per SIMD, so the last round's tail is a small part of it
+  P->eigen_waves =
+      (int)std::min<long long>(2LL * dw, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
   P->chain_waves = std::max(1, P->chain_waves);
   P->eigen_waves = std::max(1, P->eigen_waves);
   // device descriptors

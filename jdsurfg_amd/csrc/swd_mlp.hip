@@ -20,6 +20,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// two floats to packed bf16 (round to nearest even) in one v_cvt_pk_bf16_f32
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
+  __bf16 ha = (__bf16)a, hb = (__bf16)b;
+  return (uint32_t)__builtin_bit_cast(uint16_t, ha) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+}
 
 // GELU in its tanh form, 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))),
 // with tanh(u) = 1 - 2 / (1 + 2^(2u log2 e)) on the hardware exp2 and
@@ -222,39 +228,46 @@ jdsg_gemm_bf16_128(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni)
         bfv[ni] = *reinterpret_cast<const bf8v *>(&Bs[wn * 64 + ni * 16 + (lane & 15)][kq]);
+      // operands swapped (W^T X^T): a lane's accumulator holds output
+      // (row lane%16, columns 4 (lane/16) .. +3), four consecutive columns
+      // of one row, so the epilogue stores them as one vector
 #pragma unroll
       for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfv[ni], acc[mi][ni], 0,
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfv[ni], af[mi], acc[mi][ni], 0,
                                                                 0, 0);
     }
     __syncthreads();
   }
+  // epilogue: per lane 4 rows (mi) x 4 column quads (ni); the bias quad is
+  // loaded once per ni
 #pragma unroll
-  for (int mi = 0; mi < 4; ++mi)
+  for (int ni = 0; ni < 4; ++ni) {
+    const int j0 = bn + wn * 64 + ni * 16 + 4 * (lane >> 4);
+    v4f bq = v4f{0.f, 0.f, 0.f, 0.f};
+    if (epi == 1 || epi == 2 || epi == 4) bq = *reinterpret_cast<const v4f *>(bias + j0);
 #pragma unroll
-    for (int ni = 0; ni < 4; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int i = bm + wm * 64 + mi * 16 + 4 * (lane >> 4) + r;
-        int j = bn + wn * 64 + ni * 16 + (lane & 15);
-        float v = acc[mi][ni][r];
-        size_t o = (size_t)i * ldc + j;
-        if (epi == 0) {
-          C[o] = v;
-        } else if (epi == 1) {
-          float z = v + bias[j];
-          C[o] = z;
-          Y[(size_t)i * ldy + j] = f2bf(gelu(z));
-        } else if (epi == 2) {
-          C[o] = v + bias[j];
-        } else if (epi == 4) {
-          Y[(size_t)i * ldy + j] = f2bf(gelu(v + bias[j]));
-        } else {
-          C[o] += v;
+    for (int mi = 0; mi < 4; ++mi) {
+      const int i = bm + wm * 64 + mi * 16 + (lane & 15);
+      v4f v = acc[mi][ni];
+      v4f *cp = reinterpret_cast<v4f *>(C + (size_t)i * ldc + j0);
+      if (epi == 0) {
+        *cp = v;
+      } else if (epi == 3) {
+        *cp = *cp + v;
+      } else {
+        v = v + bq;
+        if (epi != 4) *cp = v;
+        if (epi != 2) {
+          uint2 h;
+          h.x = pack_bf2(gelu(v[0]), gelu(v[1]));
+          h.y = pack_bf2(gelu(v[2]), gelu(v[3]));
+          *reinterpret_cast<uint2 *>(Y + (size_t)i * ldy + j0) = h;
         }
       }
+    }
+  }
 }
 
 // dZ = dY * gelu'(Z) -> bf16 (M x N row-major, ld = N)
@@ -349,8 +362,13 @@ int jdsg_gemm_bf16(int ta, int tb, int M, int N, int K, const void *A, int lda, 
   const uint16_t *a = (const uint16_t *)A, *b = (const uint16_t *)B;
   uint16_t *y = (uint16_t *)Y;
   // 16-byte loads need 8-element aligned leading dimensions
+  // the big kernel also stores 4-column vectors: C rows 16-byte, Y rows
+  // 8-byte and the bias 16-byte aligned
   bool big = M % BM2 == 0 && N % BN2 == 0 && K % BK2 == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
-             ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+             ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0 &&
+             (C == nullptr || (ldc % 4 == 0 && ((uintptr_t)C & 15) == 0)) &&
+             (y == nullptr || (ldy % 4 == 0 && ((uintptr_t)y & 7) == 0)) &&
+             (bias == nullptr || ((uintptr_t)bias & 15) == 0);
   if (big) {
     dim3 g((unsigned)((M / BM2) * (N / BN2)));
     if (!ta && tb)
