@@ -1215,6 +1215,27 @@ JDSG_DEV int entry_status(const EigenArgs &A, long long e, bool *pm_ok) {
   return 0;
 }
 
+// The chunk lane a wave works on.  In a grid batch the items of a chunk are
+// C whole mode/period combinations of every column (item = combination * ncol
+// + column), so the C waves over one block of 64 columns read the same column
+// model.  Workgroup w runs on XCD w % 8; the mapping gives the waves that run
+// on one XCD, close in dispatch order, the C combinations of one column block,
+// so those model lines come from that XCD's L2.  A bijection on the chunk's
+// waves; partial chunks and list-ordered batches keep the identity.  The
+// pass and combine kernels use the same mapping, so a lane's scratch slot is
+// the same in both.
+JDSG_DEV long long chunk_lane(const EigenArgs &A, long long it0, long long lanes) {
+  const long long ncol = A.cm.ncol;
+  const long long w = blockIdx.x;
+  if (!A.item_ent && ncol % 512 == 0 && lanes % ncol == 0 && it0 % ncol == 0 &&
+      (long long)gridDim.x * 64 == lanes) {
+    const long long B = ncol / 64, C = lanes / ncol;  // waves per combination, combinations
+    const long long x = w & 7, j = w >> 3, h = j % C, g = j / C;
+    return (h * B + 8 * g + x) * 64 + threadIdx.x;
+  }
+  return w * 64 + threadIdx.x;
+}
+
 JDSG_DEV double entry_period(const EigenArgs &A, const EntryRef &r, int v) {
   if (A.tv) return A.tv[3 * r.e + v];
   double T = A.periods[A.per_off[r.col] + r.k];
@@ -1226,7 +1247,7 @@ JDSG_DEV double entry_period(const EigenArgs &A, const EntryRef &r, int v) {
 template <int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
 jdsg_eigen_pass(EigenArgs A, long long it0, long long lanes, int v) {
-  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long lane = chunk_lane(A, it0, lanes);
   const long long it = it0 + lane;
   if (lane >= lanes || it >= A.nitem) return;
   const EntryRef r = entry_ref(A, it);
@@ -1262,7 +1283,7 @@ jdsg_eigen_pass(EigenArgs A, long long it0, long long lanes, int v) {
 template <int W>
 __global__ void __launch_bounds__(64)
 jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
-  const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long lane = chunk_lane(A, it0, lanes);
   const long long it = it0 + lane;
   if (lane >= lanes || it >= A.nitem) return;
   const bool ray = (W == 0);

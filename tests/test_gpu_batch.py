@@ -101,3 +101,23 @@ def test_graph_model_update_matches_eager_solve():
         torch.cuda.synchronize()
         _same(_host(out), _host(eout))
         print("iteration", it, "max relative Vs step", r, flush=True)
+
+
+def test_xcd_chunk_mapping_matches_identity():
+    """A grid batch of 512 columns whose eigen chunks hold whole mode/period
+    combinations takes the XCD-aware lane mapping; a chunk width that does
+    not divide into columns keeps the identity.  Both give the same bits."""
+    import torch
+    from jdsurfg_amd import swd
+    w = W.grid128(nx=32, ny=16)
+    print(f"case xcd chunk mapping grid ncol={w.ncol}", flush=True)
+    inp = swd.upload(w)
+    res = []
+    for ew in (64, 63):        # 64 waves = 8 combinations x 512 columns
+        plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "grid")
+        plan.set_waves(0, ew)
+        out = plan.alloc_outputs(KERN)
+        plan.execute(*inp, out)
+        torch.cuda.synchronize()
+        res.append(_host(out))
+    _same(res[0], res[1])
