@@ -183,7 +183,7 @@ def main():
         relcp = ((sout["c"] - pout["c"]).abs() / pout["c"].abs())[okp]
         relup = ((sout["u"] - pout["u"]).abs() / pout["u"].abs())[okp]
         # MLP inference: 2 * rows * sum(in*out) flops on the MFMA units
-        mlp_flops = 2.0 * sp.nent * sum(o * i for o, i in S.DIMS)
+        mlp_flops = 2.0 * sp.nent * sum(o * i for o, i in mlp.dims)
         line.update({
             "value": float(cps / (ms_sur * 1e-3)), "ms_per_step": float(ms_sur),
             "speedup_vs_plain": float(ms_plain / ms_sur),

@@ -22,8 +22,7 @@ import numpy as np
 from . import swd
 from . import workloads as W
 
-FEAT, HID, OUTP = 192, 512, 64
-DIMS = [(HID, FEAT), (HID, HID), (HID, HID), (HID, HID), (OUTP, HID)]
+FEAT, HID, OUTP = 192, 512, 64   # inputs, default hidden width, padded output
 C_SHIFT, C_SCALE = float(np.log(3.5)), 0.3   # target: (ln c - C_SHIFT) / C_SCALE
 
 
@@ -60,15 +59,20 @@ def _chk(rc, what):
 
 
 class MLP:
-    """The surrogate network; weights live on the GPU."""
+    """The surrogate network; weights live on the GPU.  `hid` is the width of
+    the four hidden layers (a multiple of 128; the stored weights carry it)."""
 
-    def __init__(self, seed=0):
+    def __init__(self, seed=0, hid=HID):
         import torch
+        if hid % 128:
+            raise ValueError("hidden width must be a multiple of 128")
+        self.hid = hid
+        self.dims = [(hid, FEAT), (hid, hid), (hid, hid), (hid, hid), (OUTP, hid)]
         g = torch.Generator().manual_seed(seed)
         self.W, self.b = [], []
-        for li, (o, i) in enumerate(DIMS):
+        for li, (o, i) in enumerate(self.dims):
             w = torch.randn(o, i, generator=g) * (2.0 / i) ** 0.5
-            if li == len(DIMS) - 1:
+            if li == len(self.dims) - 1:
                 w = w * 0.1
                 w[1:] = 0.0   # padded output rows stay zero
             self.W.append(w.cuda().contiguous())
@@ -89,15 +93,16 @@ class MLP:
         if key in self._buf:
             return self._buf[key]
         bf = torch.bfloat16
-        d = {"Y": [None] + [torch.empty(M, HID, dtype=bf, device="cuda") for _ in range(4)],
+        H = self.hid
+        d = {"Y": [None] + [torch.empty(M, H, dtype=bf, device="cuda") for _ in range(4)],
              "out": torch.zeros(M, OUTP, device="cuda")}
         if train:
-            d["Z"] = [torch.empty(M, HID, device="cuda") for _ in range(4)]
+            d["Z"] = [torch.empty(M, self.hid, device="cuda") for _ in range(4)]
             d["dOut"] = torch.zeros(M, OUTP, device="cuda")
             d["dOutbf"] = torch.zeros(M, OUTP, dtype=bf, device="cuda")
-            d["dYf"] = torch.empty(M, HID, device="cuda")
-            d["dZbf"] = torch.empty(M, HID, dtype=bf, device="cuda")
-            d["dZf"] = torch.empty(M, HID, device="cuda")
+            d["dYf"] = torch.empty(M, self.hid, device="cuda")
+            d["dZbf"] = torch.empty(M, self.hid, dtype=bf, device="cuda")
+            d["dZf"] = torch.empty(M, self.hid, device="cuda")
             d["loss"] = torch.zeros((M + 255) // 256, device="cuda")
         self._buf[key] = d
         return d
@@ -106,21 +111,21 @@ class MLP:
         """X: (M, 192) bf16 CUDA tensor, M a multiple of 64.  Returns the
         (M, 64) fp32 output (column 0 is the prediction)."""
         L = _L()
-        M = X.shape[0]
+        M, H = X.shape[0], self.hid
         d = self._bufs(M, train)
         Y = d["Y"]
         Y[0] = X
         K = FEAT
         for li in range(4):
             if train:
-                _chk(L.jdsg_gemm_bf16(0, 1, M, HID, K, _p(Y[li]), K, _p(self.Wbf[li]), K,
-                                      _p(d["Z"][li]), HID, 1, _p(self.b[li]), _p(Y[li + 1]), HID,
+                _chk(L.jdsg_gemm_bf16(0, 1, M, H, K, _p(Y[li]), K, _p(self.Wbf[li]), K,
+                                      _p(d["Z"][li]), H, 1, _p(self.b[li]), _p(Y[li + 1]), H,
                                       stream), "gemm")
             else:
-                _chk(L.jdsg_gemm_bf16(0, 1, M, HID, K, _p(Y[li]), K, _p(self.Wbf[li]), K, None,
-                                      HID, 4, _p(self.b[li]), _p(Y[li + 1]), HID, stream), "gemm")
-            K = HID
-        _chk(L.jdsg_gemm_bf16(0, 1, M, OUTP, HID, _p(Y[4]), HID, _p(self.Wbf[4]), HID,
+                _chk(L.jdsg_gemm_bf16(0, 1, M, H, K, _p(Y[li]), K, _p(self.Wbf[li]), K, None,
+                                      H, 4, _p(self.b[li]), _p(Y[li + 1]), H, stream), "gemm")
+            K = H
+        _chk(L.jdsg_gemm_bf16(0, 1, M, OUTP, H, _p(Y[4]), H, _p(self.Wbf[4]), H,
                               _p(d["out"]), OUTP, 2, _p(self.b[4]), None, 0, stream), "gemm")
         return d["out"]
 
@@ -129,35 +134,35 @@ class MLP:
         target (M,) fp32; fills the gradients and returns the loss tensor
         (partial sums)."""
         L = _L()
-        M = target.shape[0]
+        M, H = target.shape[0], self.hid
         d = self._bufs(M, True)
         _chk(L.jdsg_mse(M, OUTP, _p(d["out"]), _p(target), _p(weight), float(delta),
                         _p(d["dOut"]), _p(d["loss"]), stream), "loss")
         _chk(L.jdsg_f32_to_bf16(M * OUTP, _p(d["dOut"]), _p(d["dOutbf"]), stream), "cvt")
         Y = d["Y"]
         # output layer
-        _chk(L.jdsg_gemm_bf16(1, 0, OUTP, HID, M, _p(d["dOutbf"]), OUTP, _p(Y[4]), HID,
-                              _p(self.gW[4]), HID, 0, None, None, 0, stream), "gemm dW")
+        _chk(L.jdsg_gemm_bf16(1, 0, OUTP, H, M, _p(d["dOutbf"]), OUTP, _p(Y[4]), H,
+                              _p(self.gW[4]), H, 0, None, None, 0, stream), "gemm dW")
         _chk(L.jdsg_colsum(M, OUTP, _p(d["dOut"]), _p(self.gb[4]), stream), "colsum")
-        _chk(L.jdsg_gemm_bf16(0, 0, M, HID, OUTP, _p(d["dOutbf"]), OUTP, _p(self.Wbf[4]), HID,
-                              _p(d["dYf"]), HID, 0, None, None, 0, stream), "gemm dX")
+        _chk(L.jdsg_gemm_bf16(0, 0, M, H, OUTP, _p(d["dOutbf"]), OUTP, _p(self.Wbf[4]), H,
+                              _p(d["dYf"]), H, 0, None, None, 0, stream), "gemm dX")
         for li in range(3, -1, -1):
-            K = FEAT if li == 0 else HID
-            _chk(L.jdsg_gelu_bwd(M * HID, _p(d["dYf"]), _p(d["Z"][li]), _p(d["dZbf"]),
+            K = FEAT if li == 0 else H
+            _chk(L.jdsg_gelu_bwd(M * H, _p(d["dYf"]), _p(d["Z"][li]), _p(d["dZbf"]),
                                  _p(d["dZf"]), stream), "gelu bwd")
-            _chk(L.jdsg_gemm_bf16(1, 0, HID, K, M, _p(d["dZbf"]), HID, _p(Y[li]), K,
+            _chk(L.jdsg_gemm_bf16(1, 0, H, K, M, _p(d["dZbf"]), H, _p(Y[li]), K,
                                   _p(self.gW[li]), K, 0, None, None, 0, stream), "gemm dW")
-            _chk(L.jdsg_colsum(M, HID, _p(d["dZf"]), _p(self.gb[li]), stream), "colsum")
+            _chk(L.jdsg_colsum(M, H, _p(d["dZf"]), _p(self.gb[li]), stream), "colsum")
             if li > 0:
-                _chk(L.jdsg_gemm_bf16(0, 0, M, HID, HID, _p(d["dZbf"]), HID, _p(self.Wbf[li]),
-                                      HID, _p(d["dYf"]), HID, 0, None, None, 0, stream),
+                _chk(L.jdsg_gemm_bf16(0, 0, M, H, H, _p(d["dZbf"]), H, _p(self.Wbf[li]),
+                                      H, _p(d["dYf"]), H, 0, None, None, 0, stream),
                      "gemm dX")
         return d["loss"]
 
     def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0, stream=None):
         L = _L()
         self.step += 1
-        for li in range(len(DIMS)):
+        for li in range(len(self.dims)):
             _chk(L.jdsg_adam(self.W[li].numel(), _p(self.W[li]), _p(self.mW[li]),
                              _p(self.vW[li]), _p(self.gW[li]), lr, b1, b2, eps, self.step, wd,
                              _p(self.Wbf[li]), stream), "adam")
@@ -175,9 +180,9 @@ class MLP:
     @classmethod
     def load(cls, path):
         import torch
-        m = cls()
         z = np.load(path)
-        for i in range(len(DIMS)):
+        m = cls(hid=int(z["W0"].shape[0]))
+        for i in range(len(m.dims)):
             wb = torch.from_numpy(z[f"W{i}"]).view(torch.bfloat16).cuda()
             m.Wbf[i] = wb.contiguous()
             m.W[i].copy_(wb.float())
@@ -291,7 +296,7 @@ class Features:
 
 
 def train(ncol=200000, steps=3000, batch=8192, lr=1e-3, seed=7, log=print, valid_cols=4000,
-          huber=0.02):
+          huber=0.02, hid=HID):
     """Generate data with the plain GPU solver and train the MLP on the GPU.
     Returns (mlp, report)."""
     import torch
@@ -305,7 +310,7 @@ def train(ncol=200000, steps=3000, batch=8192, lr=1e-3, seed=7, log=print, valid
     tr = ok[ok < e_split]
     va = ok[ok >= e_split]
     feats = Features(plan, inp)
-    mlp = MLP(seed)
+    mlp = MLP(seed, hid=hid)
     g = torch.Generator(device="cuda").manual_seed(seed)
     t1 = time.time()
     losses = []
@@ -489,6 +494,7 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=8192)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=7)
+    t.add_argument("--hid", type=int, default=HID, help="hidden width (multiple of 128)")
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -498,7 +504,7 @@ def main(argv=None):
         raise SystemExit("the surrogate runs on the GPU")
     if a.cmd == "train":
         mlp, rep = train(ncol=a.ncol, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
-                         log=lambda m: print(m, flush=True))
+                         log=lambda m: print(m, flush=True), hid=a.hid)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         mlp.save(a.out)
         rep["weights"] = a.out

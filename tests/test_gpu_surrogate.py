@@ -53,13 +53,15 @@ def test_gelu_forms_agree():
     assert np.abs(_gelu(z) - exact).max() < 5e-4
 
 
-def test_mlp_forward_backward_matches_fp64():
+@pytest.mark.parametrize("hid", [512, 256])
+def test_mlp_forward_backward_matches_fp64(hid):
     """Forward output and every weight/bias gradient of the MFMA MLP against
-    an FP64 host computation of the same bf16-stored network."""
+    an FP64 host computation of the same bf16-stored network (the default
+    width and the shipped network's width)."""
     import torch
     from jdsurfg_amd.surrogate import MLP, OUTP
-    print("case mlp grad M=128", flush=True)
-    mlp = MLP(seed=3)
+    print(f"case mlp grad M=128 hid={hid}", flush=True)
+    mlp = MLP(seed=3, hid=hid)
     g = torch.Generator().manual_seed(5)
     w4 = torch.randn(mlp.W[4].shape, generator=g) * 0.05
     w4[1:] = 0
