@@ -5,9 +5,10 @@
 // (one period-equation evaluation per end; without a sign change the bracket
 // is widened to three half widths, below and then above), accepted only if
 // the period equation changes sign, and polished in FP64 by false position
-// with the Anderson-Bjorck weight until the bracket is within tol*c or the
-// secant correction falls below tol*c right after one below 1e-4*c (the
-// iteration is then superlinear, so the corrected value is within tol).  A polished root is kept only if it is the root the
+// with the Anderson-Bjorck weight until the bracket is within tol*c, or the
+// secant correction falls below tol*c right after one below 1e-4*c, or the
+// error estimate of the corrected value (correction^2 / previous correction)
+// falls below tol*c/10.  A polished root is kept only if it is the root the
 // reference's chain would reach: the mode-order pass checks, with the signs
 // already evaluated, that no odd number of roots lies between the previous
 // mode's bracket (or, for the lowest mode, the chain's floor cm) and this
@@ -240,7 +241,12 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
       if (!done) {
         double cn = (S.a * S.fb - S.b * S.fa) / (S.fb - S.fa);
         if (!(cn > fmin(S.a, S.b) && cn < fmax(S.a, S.b))) cn = 0.5 * (S.a + S.b);
-        if (fabs(cn - S.c) <= A.tol * fabs(S.c) && fabs(S.c - S.cp) <= 1.0e-4 * fabs(S.c)) {
+        // the corrected value's error is about dc^2 / dc_prev for a linearly
+        // converging step and less once convergence is superlinear: accept
+        // it when that estimate is a tenth of the tolerance
+        const double d1 = fabs(cn - S.c), d0 = fabs(S.c - S.cp), cc = fabs(S.c);
+        if ((d1 <= A.tol * cc && d0 <= 1.0e-4 * cc) ||
+            (S.cp != 0.0 && d0 <= 1.0e-3 * cc && d1 * d1 <= 0.1 * A.tol * cc * d0)) {
           done = true;
           root = cn;
         } else {
