@@ -72,6 +72,11 @@ struct LayerWaves {
   double ia, ib;      // 1 / (the nonzero part of ra, rb)
 };
 
+// |z| of a real or imaginary number (one part is zero, so this is exact), and
+// the squared modulus, for the reference's small-modulus tests without hypot.
+JDSG_DEV double abs_ri(cplx z) { return fabs(z.re) + fabs(z.im); }
+JDSG_DEV double norm2(cplx z) { return z.re * z.re + z.im * z.im; }
+
 JDSG_DEV LayerWaves layer_waves(const Layer &L, double omega, double wvno2) {
   LayerWaves w;
   double xka = omega / L.za;
@@ -121,13 +126,13 @@ JDSG_DEV void varsv_w(const LayerWaves &w, const Layer &L, double &cosp, double 
     double c = w.Ea.re, s = -w.Ea.im;
     cosp = c;
     rsinp = -(w.ra.im * s);
-    sinpr = (cabs_(w.ra) < tiny) ? L.zd : s * w.ia;
+    sinpr = (abs_ri(w.ra) < tiny) ? L.zd : s * w.ia;
   } else {
     double pfac = (w.pr == 0.0) ? 1.0 : (w.pr < 30.) ? w.ma * w.ma : 0.0;
     cosp = 0.5 + pfac * 0.5;
     double sinp = 0.5 - pfac * 0.5;
     rsinp = w.ra.re * sinp;
-    sinpr = (fabs(w.pr) < tiny && cabs_(w.ra) < tiny) ? L.zd : sinp * w.ia;
+    sinpr = (fabs(w.pr) < tiny && abs_ri(w.ra) < tiny) ? L.zd : sinp * w.ia;
   }
   if (L.iwat == 1) {
     cosq = 1.0;
@@ -140,13 +145,13 @@ JDSG_DEV void varsv_w(const LayerWaves &w, const Layer &L, double &cosp, double 
     double c = w.Eb.re, s = -w.Eb.im;
     cosq = c;
     rsinq = -(w.rb.im * s);
-    sinqr = (cabs_(w.rb) < tiny) ? L.zd : s * w.ib;
+    sinqr = (abs_ri(w.rb) < tiny) ? L.zd : s * w.ib;
   } else {
     double svfac = (w.qr == 0.0) ? 1.0 : (w.qr < 30.) ? w.mb * w.mb : 0.0;
     cosq = 0.5 + svfac * 0.5;
     double sinq = 0.5 - svfac * 0.5;
     rsinq = w.rb.re * sinq;
-    sinqr = (fabs(w.qr) < tiny && cabs_(w.rb) < tiny) ? L.zd : sinq * w.ib;
+    sinqr = (fabs(w.qr) < tiny && abs_ri(w.rb) < tiny) ? L.zd : sinq * w.ib;
   }
 }
 
@@ -506,21 +511,21 @@ JDSG_DEV void solid_ints(const Layer &L, const LayerWaves &w, const U4 &u0, cons
   const cplx km1sd = (hg * u0.uz - izh * u0.tr) + (0.5 * g1 * u0.ur - hk * u0.tz) * irb;
   // layer functions (sregn96.f90:1319-1397) with their reference cut-offs
   const cplx zero = cmk(0.0, 0.0);
-  const cplx FA = (cabs_(w.ra) < 1.0e-08) ? cmk(d, 0.0)
+  const cplx FA = (abs_ri(w.ra) < 1.0e-08) ? cmk(d, 0.0)
                   : (0.5 * (1.0 - ((w.pr < 40.0) ? w.Ea * w.Ea : zero))) * ira;
   const cplx GA = (w.pr < 75.0) ? w.Ea * d : zero;
-  const cplx FB = (cabs_(w.rb) < 1.0e-08) ? cmk(d, 0.0)
+  const cplx FB = (abs_ri(w.rb) < 1.0e-08) ? cmk(d, 0.0)
                   : (0.5 * (1.0 - ((w.qr < 40.0) ? w.Eb * w.Eb : zero))) * irb;
   const cplx GB = (w.qr < 75.0) ? w.Eb * d : zero;
   const cplx sab = w.rb + w.ra, dab = w.ra - w.rb;
   // complex quotients as a conj(b) / |b|^2 (|ra|, |rb| are of order
-  // omega / velocity, far from overflow): one division each
-  const double n1 = sab.re * sab.re + sab.im * sab.im;
-  const double n2 = dab.re * dab.re + dab.im * dab.im;
-  const cplx H1 = (cabs_(sab) < 1.0e-08) ? cmk(d, 0.0)
+  // omega / velocity, far from overflow): one division each; the reference's
+  // |b| < 1e-8 tests read |b|^2 < 1e-16
+  const double n1 = norm2(sab), n2 = norm2(dab);
+  const cplx H1 = (n1 < 1.0e-16) ? cmk(d, 0.0)
                   : ((1.0 - ((sab.re * d < 40.0) ? w.Ea * w.Eb : zero)) * cconj(sab)) *
                         (1.0 / n1);
-  const cplx H2 = (cabs_(dab) < 1.0e-08) ? cmk(d, 0.0)
+  const cplx H2 = (n2 < 1.0e-16) ? cmk(d, 0.0)
                   : ((((w.qr < 40.0) ? w.Eb : zero) - ((w.pr < 40.0) ? w.Ea : zero)) *
                      cconj(dab)) * (1.0 / n2);
   // the six quadratic forms of M
