@@ -165,6 +165,39 @@ int jdsg_get_1d_kernel(int nx, int ny, int nz, const float *vs, const float *dep
                        int ntLg, int nmLg, double *vc, double *vout, double *kernel,
                        int *status);
 
+/* Fréchet rows and misfit gradient of the travel-time data (replace the row
+ * loops of SurfTime::frechet_matrix, SurfaceWave.cpp:584-672, and of
+ * SurfTime::compute_grad, :695-781, after the eikonal ray tracing).  Device
+ * pointers except where noted.
+ *   kernel     (nx*ny, nz, kmax) column-major, as jdsg_surftime_kernel fills it;
+ *   row_idx    [nrows] period index of each data row (get_period_index order);
+ *   ray_off    [nrows+1], ray_cell, ray_w: the row's ray sensitivity fdm
+ *              (fmst_raypath) in sparse form: the grid cells n = ix + iy*nx
+ *              where it is nonzero, increasing, and their float weights.
+ * Row r holds, in increasing column c = iz*(nx-2)*(ny-2) + (iy-1)*(nx-2) +
+ * (ix-1) over the interior cells (0 <= iz < nz-1), the nonzero values of
+ * float(fdm(ix,iy) * kernel(n, iz, idx)) (:613-641).
+ * jdsg_frechet_rows writes the CSR offsets row_off [nrows+1], col and val
+ * (capacity cap) and the total in *nnz (host); it synchronises `stream` and
+ * fails with the total set when it exceeds cap, or when a row's period index
+ * or ray cells are invalid.
+ * jdsg_frechet_grad: grad[(nx-2)*(ny-2)*(nz-1)] = sum over rows of the row
+ * times resid[r] (t - t_obs), then times vs at the cell (vs (nx,ny,nz)
+ * column-major) (:754-773); asynchronous.
+ * jdsg_write_frechet (host arrays): the reference's frechet.bin, header
+ * (m, n) then per row (row id, nnz, int[nnz] columns, float[nnz] values), as
+ * merge_csr_files (csr_matrix.cpp:205-242) leaves it. */
+int jdsg_frechet_rows(int nx, int ny, int nz, int kmax, const double *kernel, int nrows,
+                      const int *row_idx, const long long *ray_off, const int *ray_cell,
+                      const float *ray_w, long long *row_off, int *col, float *val,
+                      long long cap, long long *nnz, void *stream);
+int jdsg_frechet_grad(int nx, int ny, int nz, int kmax, const double *kernel, int nrows,
+                      const int *row_idx, const long long *ray_off, const int *ray_cell,
+                      const float *ray_w, const float *resid, const float *vs, float *grad,
+                      void *stream);
+int jdsg_write_frechet(const char *path, int m, int n, int nrows, const int *row_id,
+                       const long long *row_off, const int *col, const float *val);
+
 /* Learned-bracket root finding (no counterpart in the reference: an optional
  * accelerated form of the root search of surfdisp96.f:213-339 that returns
  * the same roots in exact-root mode).  An MLP predicts every flat phase

@@ -1265,3 +1265,76 @@ extern "C" int jdsg_get_1d_kernel(int nx, int ny, int nz, const float *vs, const
   jdsg_surftime_destroy(S);
   return rc;
 }
+
+// ---------------------------------------------------------------- Fréchet rows
+// SurfTime::frechet_matrix / compute_grad (SurfaceWave.cpp:554-676, 695-781)
+// over the kernels of jdsg_surftime_kernel; device pointers, see jdsg_swd.h.
+namespace {
+int frechet_ctx(int nx, int ny, int nz, int kmax, int nrows, RowCtx *R) {
+  if (nx < 3 || ny < 3 || nz < 2 || kmax < 1 || nrows < 0)
+    return fail("frechet: need nx, ny >= 3, nz >= 2, kmax >= 1 and nrows >= 0");
+  R->nx = nx;
+  R->ny = ny;
+  R->nz = nz;
+  R->nxy = nx * ny;
+  R->nin = (nx - 2) * (ny - 2);
+  R->kmax = kmax;
+  R->kstride = (long long)R->nxy * nz;
+  return 0;
+}
+}  // namespace
+
+extern "C" int jdsg_frechet_rows(int nx, int ny, int nz, int kmax, const double *kernel,
+                                 int nrows, const int *row_idx, const long long *ray_off,
+                                 const int *ray_cell, const float *ray_w, long long *row_off,
+                                 int *col, float *val, long long cap, long long *nnz,
+                                 void *stream) {
+  RowCtx R;
+  if (frechet_ctx(nx, ny, nz, kmax, nrows, &R)) return 1;
+  hipStream_t st = (hipStream_t)stream;
+  jdsg_launch_frechet_rows(R, kernel, nrows, row_idx, ray_off, ray_cell, ray_w, row_off, col, val,
+                           cap, st);
+  if (hipGetLastError() != hipSuccess) return fail("frechet: kernel launch failed");
+  long long head[2] = {0, 0};
+  if (nrows > 0) {
+    HIPCHK(hipMemcpyAsync(&head[0], row_off, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&head[1], row_off + nrows, sizeof(long long), hipMemcpyDeviceToHost,
+                          st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (head[0] < 0)
+    return fail("frechet: a row has a period index outside [0, kmax) or ray cells that are not "
+                "increasing indices in [0, nx*ny)");
+  if (nnz) *nnz = head[1];
+  if (head[1] > cap) return fail("frechet: more nonzero entries than the output capacity");
+  return 0;
+}
+
+extern "C" int jdsg_frechet_grad(int nx, int ny, int nz, int kmax, const double *kernel,
+                                 int nrows, const int *row_idx, const long long *ray_off,
+                                 const int *ray_cell, const float *ray_w, const float *resid,
+                                 const float *vs, float *grad, void *stream) {
+  RowCtx R;
+  if (frechet_ctx(nx, ny, nz, kmax, nrows, &R)) return 1;
+  jdsg_launch_frechet_grad(R, kernel, nrows, row_idx, ray_off, ray_cell, ray_w, resid, vs, grad,
+                           (hipStream_t)stream);
+  if (hipGetLastError() != hipSuccess) return fail("frechet: kernel launch failed");
+  return 0;
+}
+
+extern "C" int jdsg_write_frechet(const char *path, int m, int n, int nrows, const int *row_id,
+                                  const long long *row_off, const int *col, const float *val) {
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(std::string("frechet: cannot open ") + path);
+  bool ok = fwrite(&m, sizeof(int), 1, f) == 1 && fwrite(&n, sizeof(int), 1, f) == 1;
+  for (int r = 0; ok && r < nrows; ++r) {
+    const long long a = row_off[r], b = row_off[r + 1];
+    const int nar = (int)(b - a);
+    ok = fwrite(&row_id[r], sizeof(int), 1, f) == 1 && fwrite(&nar, sizeof(int), 1, f) == 1 &&
+         fwrite(col + a, sizeof(int), (size_t)nar, f) == (size_t)nar &&
+         fwrite(val + a, sizeof(float), (size_t)nar, f) == (size_t)nar;
+  }
+  if (fclose(f) != 0) ok = false;
+  if (!ok) return fail(std::string("frechet: write to ") + path + " failed");
+  return 0;
+}

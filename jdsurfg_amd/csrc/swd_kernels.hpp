@@ -124,6 +124,21 @@ void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N,
                                    double *vout, double *kernel, int *status, int want_kernel,
                                    hipStream_t st);
 void jdsg_launch_polish(const PolishArgs &A, int wave, int nwaves, hipStream_t st);
+
+// Fréchet rows / misfit gradient (swd_frechet.hip)
+struct RowCtx {
+  int nx, ny, nz, nxy, nin;   // grid; interior cells per depth (nx-2)(ny-2)
+  int kmax;                   // period indices of the kernel array
+  long long kstride;          // nxy * nz: kernel stride between period indices
+};
+void jdsg_launch_frechet_rows(const RowCtx &R, const double *kernel, int nrows, const int *row_idx,
+                              const long long *ray_off, const int *ray_cell, const float *ray_w,
+                              long long *row_off, int *col, float *val, long long cap,
+                              hipStream_t st);
+void jdsg_launch_frechet_grad(const RowCtx &R, const double *kernel, int nrows,
+                              const int *row_idx, const long long *ray_off, const int *ray_cell,
+                              const float *ray_w, const float *resid, const float *vs,
+                              float *grad, hipStream_t st);
 void jdsg_launch_mode_order(const PolishArgs &A, int kmax, hipStream_t st);
 void jdsg_launch_pm_centers(const long long *ent_off, const int *per_off, const double *periods,
                             int ncol, long long nent, const double *known0, const double *pred,

@@ -89,6 +89,11 @@ class Oracle:
                                      C.c_int]
         L.or_restatement_backend.restype = C.c_void_p
         L.or_set_tolerance.argtypes = [C.c_double]
+        _lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        L.or_frechet_rows.argtypes = [C.c_int] * 4 + [_dp, C.c_int, _ip, _fp, _lp, _ip, _fp,
+                                                      C.c_longlong]
+        L.or_frechet_rows.restype = C.c_longlong
+        L.or_frechet_grad.argtypes = [C.c_int] * 4 + [_dp, C.c_int, _ip, _fp, _fp, _fp, _fp]
         if kind == "ref":
             L.ref_backend.restype = C.c_void_p
             self.backend = L.ref_backend()
@@ -239,6 +244,40 @@ class Oracle:
                       res["status"].ctypes.data, res["cflat"].ctypes.data)
         self.lib.or_batch_solve(self._be(), C.byref(b), C.byref(o), nthreads)
         return res
+
+
+    def frechet_rows(self, kernel, row_idx, fdm):
+        """SurfaceWave.cpp:601-641: CSR rows (row_off, col, val) from the
+        kernels (nx*ny, nz, kmax) and dense ray sensitivities fdm (nrows, nx, ny)
+        (fdm[r, ix, iy]), both in the reference's column-major sense."""
+        nrows, nx, ny = fdm.shape
+        kernel = np.asarray(kernel, _d)
+        nz, kmax = kernel.shape[1], kernel.shape[2]
+        kf = np.ascontiguousarray(kernel.ravel(order="F"))
+        ff = np.ascontiguousarray(np.stack([f.ravel(order="F") for f in fdm]).astype(_f))
+        ri = np.ascontiguousarray(row_idx, np.int32)
+        ro = np.zeros(nrows + 1, np.int64)
+        nnz = self.lib.or_frechet_rows(nx, ny, nz, kmax, kf, nrows, ri, ff, ro,
+                                       np.zeros(1, np.int32), np.zeros(1, _f), 0)
+        col = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1), _f)
+        self.lib.or_frechet_rows(nx, ny, nz, kmax, kf, nrows, ri, ff, ro, col, val, nnz)
+        return ro, col[:nnz], val[:nnz]
+
+    def frechet_grad(self, kernel, row_idx, fdm, resid, vs):
+        """SurfaceWave.cpp:738-773 with one thread: the gradient over the
+        interior cells ((nx-2)*(ny-2)*(nz-1), c = iz*(nx-2)*(ny-2)+...)."""
+        nrows, nx, ny = fdm.shape
+        kernel = np.asarray(kernel, _d)
+        nz, kmax = kernel.shape[1], kernel.shape[2]
+        kf = np.ascontiguousarray(kernel.ravel(order="F"))
+        ff = np.ascontiguousarray(np.stack([f.ravel(order="F") for f in fdm]).astype(_f))
+        g = np.zeros((nx - 2) * (ny - 2) * (nz - 1), _f)
+        self.lib.or_frechet_grad(nx, ny, nz, kmax, kf, nrows,
+                                 np.ascontiguousarray(row_idx, np.int32), ff,
+                                 np.ascontiguousarray(resid, _f),
+                                 np.ascontiguousarray(np.asarray(vs, _f).ravel(order="F")), g)
+        return g
 
 
 def empirical_relation(vs):

@@ -155,7 +155,17 @@ void or_batch_offsets(const or_batch *b, long long *ent_off,
 void or_batch_solve(const or_backend *be, const or_batch *b,
                     const or_batch_out *o, int nthreads);
 
+/* SurfaceWave.cpp:601-641 / 738-773: Fréchet rows (CSR, returns the nonzero
+ * count) and the misfit gradient from dense per-row ray sensitivities. */
+long long or_frechet_rows(int nx, int ny, int nz, int kmax, const double *kernel, int nrows,
+                          const int *row_idx, const float *fdm, long long *row_off, int *col,
+                          float *val, long long cap);
+void or_frechet_grad(int nx, int ny, int nz, int kmax, const double *kernel, int nrows,
+                     const int *row_idx, const float *fdm, const float *resid, const float *vs,
+                     float *grad);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

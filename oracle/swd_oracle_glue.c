@@ -464,3 +464,73 @@ void or_batch_solve(const or_backend *be, const or_batch *b,
 #endif
   for (int c = 0; c < b->ncol; ++c) solve_column(be, b, o, c);
 }
+
+/* SurfaceWave.cpp:601-641 (frechet_matrix): row r of the Fréchet matrix from
+ * the dense ray sensitivity fdm_r (nx*ny floats, fdm(ix,iy) at ix + iy*nx,
+ * column-major like fmat2) and the 1-D kernels (nx*ny, nz, kmax) column-major
+ * (dmat3).  frechet(c) = fdm(ix+1,iy+1) * kernel((iy+1)*nx+ix+1, iz, idx) is
+ * stored in a float vector; its nonzero entries (std::abs > 0) are kept in
+ * increasing c.  Writes the CSR rows up to `cap` entries and returns the
+ * total number of nonzero entries. */
+long long or_frechet_rows(int nx, int ny, int nz, int kmax, const double *kernel, int nrows,
+                          const int *row_idx, const float *fdm, long long *row_off, int *col,
+                          float *val, long long cap) {
+  long long nnz = 0;
+  long long nxy = (long long)nx * ny;
+  (void)kmax;
+  row_off[0] = 0;
+  for (int r = 0; r < nrows; ++r) {
+    const float *f = fdm + (long long)r * nxy;
+    int idx = row_idx[r];
+    int c = 0;
+    for (int iz = 0; iz < nz - 1; ++iz)
+      for (int iy = 0; iy < ny - 2; ++iy)
+        for (int ix = 0; ix < nx - 2; ++ix) {
+          long long ii = (long long)(iy + 1) * nx + ix + 1;
+          float tmp = (float)(f[(ix + 1) + (long long)(iy + 1) * nx] *
+                              kernel[ii + (long long)iz * nxy + (long long)idx * nxy * nz]);
+          if (fabsf(tmp) > 0.0f) {
+            if (nnz < cap) {
+              col[nnz] = c;
+              val[nnz] = tmp;
+            }
+            ++nnz;
+          }
+          c += 1;
+        }
+    row_off[r + 1] = nnz;
+  }
+  return nnz;
+}
+
+/* SurfaceWave.cpp:738-773 (compute_grad) with one thread: the rows summed in
+ * row order in float, each times its residual (t - t_obs), then every cell
+ * times vs(ix+1, iy+1, iz) (vs column-major (nx, ny, nz)). */
+void or_frechet_grad(int nx, int ny, int nz, int kmax, const double *kernel, int nrows,
+                     const int *row_idx, const float *fdm, const float *resid, const float *vs,
+                     float *grad) {
+  long long nxy = (long long)nx * ny;
+  long long n = (long long)(nx - 2) * (ny - 2) * (nz - 1);
+  (void)kmax;
+  for (long long i = 0; i < n; ++i) grad[i] = 0.0f;
+  for (int r = 0; r < nrows; ++r) {
+    const float *f = fdm + (long long)r * nxy;
+    int idx = row_idx[r];
+    int c = 0;
+    for (int iz = 0; iz < nz - 1; ++iz)
+      for (int iy = 0; iy < ny - 2; ++iy)
+        for (int ix = 0; ix < nx - 2; ++ix) {
+          long long ii = (long long)(iy + 1) * nx + ix + 1;
+          float fr = (float)(f[(ix + 1) + (long long)(iy + 1) * nx] *
+                             kernel[ii + (long long)iz * nxy + (long long)idx * nxy * nz]);
+          grad[c] += fr * resid[r];
+          c += 1;
+        }
+  }
+  for (int k = 0; k < nz - 1; ++k)
+    for (int j = 0; j < ny - 2; ++j)
+      for (int i = 0; i < nx - 2; ++i) {
+        long long ic = (long long)k * (ny - 2) * (nx - 2) + (long long)j * (nx - 2) + i;
+        grad[ic] *= vs[(i + 1) + (long long)(j + 1) * nx + (long long)k * nxy];
+      }
+}
