@@ -263,6 +263,12 @@ int jdsg_mse(int M, int ldo, const float *out, const float *target, const float 
 int jdsg_adam(long long n, float *w, float *m, float *v, const float *g, float lr, float b1,
               float b2, float eps, int step, float wd, void *wbf, void *stream);
 int jdsg_f32_to_bf16(long long n, const float *x, void *y, void *stream);
+/* Inference output head: pred[e0 + i] = exp((Y_i . w + b)*scale + shift) for
+ * the n rows of the last hidden layer Y (bf16, row stride H, H a multiple of
+ * 64, 16-byte aligned) and the first row w of the output weights (bf16);
+ * fp32 accumulation in a fixed order.  Nonzero on bad sizes or alignment. */
+int jdsg_mlp_head(long long n, int H, const void *Y, const void *w, float b, double scale,
+                  double shift, double *pred, long long e0, void *stream);
 
 const char *jdsg_last_error(void);
 

@@ -348,6 +348,39 @@ __global__ void jdsg_f32_to_bf16(long long n, const float *__restrict__ x, uint1
   if (i < n) y[i] = f2bf(x[i]);
 }
 
+// Output head of the network at inference: only output 0 of the padded last
+// layer is a prediction, so instead of a GEMM over 64 padded outputs a
+// fixed-order dot product of the last hidden layer Y (bf16, row stride H) with
+// the first row w of the last weight matrix, fp32 accumulation, plus bias b,
+// de-standardised: pred[e0 + i] = exp((Y_i . w + b) * scale + shift).  Eight
+// lanes per row, each over H/8 consecutive inputs (16-byte loads), reduced by
+// shuffles in a fixed order.
+__global__ void __launch_bounds__(256)
+jdsg_mlp_head(long long n, int H, const uint16_t *__restrict__ Y, const uint16_t *__restrict__ w,
+              float b, double scale, double shift, double *pred, long long e0) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long row = t >> 3;
+  const int part = (int)(t & 7), seg = H >> 3;
+  float s = 0.f;
+  if (row < n) {
+    const uint4 *y = reinterpret_cast<const uint4 *>(Y + row * H + part * seg);
+    const uint4 *v = reinterpret_cast<const uint4 *>(w + part * seg);
+    for (int q = 0; q < seg / 8; ++q) {
+      const uint4 a = y[q], c = v[q];
+      const uint32_t av[4] = {a.x, a.y, a.z, a.w}, cv[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        s += bf2f((uint16_t)(av[i] & 0xFFFFu)) * bf2f((uint16_t)(cv[i] & 0xFFFFu));
+        s += bf2f((uint16_t)(av[i] >> 16)) * bf2f((uint16_t)(cv[i] >> 16));
+      }
+    }
+  }
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  s += __shfl_xor(s, 4);
+  if (row < n && part == 0) pred[e0 + row] = exp((double)(s + b) * scale + shift);
+}
+
 // ----------------------------------------------------------------------------
 extern "C" {
 
@@ -427,6 +460,16 @@ int jdsg_adam(long long n, float *w, float *m, float *v, const float *g, float l
   hipLaunchKernelGGL(jdsg_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, n, w, m, v, g, lr, b1, b2, eps, c1, c2, wd,
                      (uint16_t *)wbf);
+  return hipGetLastError() != hipSuccess;
+}
+
+int jdsg_mlp_head(long long n, int H, const void *Y, const void *w, float b, double scale,
+                  double shift, double *pred, long long e0, void *stream) {
+  if (H % 64 || ((uintptr_t)Y & 15) || ((uintptr_t)w & 15)) return 1;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(jdsg_mlp_head, dim3((unsigned)((8 * n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n, H, (const uint16_t *)Y, (const uint16_t *)w, b,
+                     scale, shift, pred, e0);
   return hipGetLastError() != hipSuccess;
 }
 

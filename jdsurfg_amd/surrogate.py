@@ -42,6 +42,8 @@ def _L():
     L.jdsg_plan_roots.argtypes = [vp] * 9
     L.jdsg_denorm.argtypes = [C.c_longlong, vp, C.c_int, C.c_double, C.c_double, vp,
                               C.c_longlong, vp]
+    L.jdsg_mlp_head.argtypes = [C.c_longlong, C.c_int, vp, vp, C.c_float, C.c_double, C.c_double,
+                                vp, C.c_longlong, vp]
     L.jdsg_plan_enable_surrogate.argtypes = [vp]
     L.jdsg_plan_execute_surrogate.argtypes = [vp] * 7 + [C.c_double] + [vp] * 12
     L.jdsg_plan_surrogate_stats.argtypes = [vp, C.POINTER(C.c_longlong),
@@ -107,9 +109,10 @@ class MLP:
         self._buf[key] = d
         return d
 
-    def forward(self, X, train=False, stream=None):
+    def forward(self, X, train=False, stream=None, hidden_only=False):
         """X: (M, 192) bf16 CUDA tensor, M a multiple of 64.  Returns the
-        (M, 64) fp32 output (column 0 is the prediction)."""
+        (M, 64) fp32 output (column 0 is the prediction), or with hidden_only
+        the last hidden layer (M, hid) bf16 for the output head."""
         L = _L()
         M, H = X.shape[0], self.hid
         d = self._bufs(M, train)
@@ -125,6 +128,8 @@ class MLP:
                 _chk(L.jdsg_gemm_bf16(0, 1, M, H, K, _p(Y[li]), K, _p(self.Wbf[li]), K, None,
                                       H, 4, _p(self.b[li]), _p(Y[li + 1]), H, stream), "gemm")
             K = H
+        if hidden_only:
+            return Y[4]
         _chk(L.jdsg_gemm_bf16(0, 1, M, OUTP, H, _p(Y[4]), H, _p(self.Wbf[4]), H,
                               _p(d["out"]), OUTP, 2, _p(self.b[4]), None, 0, stream), "gemm")
         return d["out"]
@@ -342,6 +347,7 @@ def predict(mlp, feats, rows=None, e0=0, n=None, chunk=1 << 20):
     if rows is not None:
         n = rows.numel()
     out = torch.empty(n, dtype=torch.float64, device="cuda")
+    b4 = float(mlp.b[4][0].item())   # bias of the output used (the rest is padding)
     Xb = None
     for s in range(0, n, chunk):
         m = min(chunk, n - s)
@@ -355,8 +361,9 @@ def predict(mlp, feats, rows=None, e0=0, n=None, chunk=1 << 20):
             feats.rows(rows=r.contiguous(), X=Xb)
         else:
             feats.rows(e0=e0 + s, n=m, X=Xb[:m])
-        o = mlp.forward(Xb, train=False)
-        _chk(L.jdsg_denorm(m, _p(o), OUTP, C_SCALE, C_SHIFT, _p(out), s, None), "denorm")
+        Yh = mlp.forward(Xb, train=False, hidden_only=True)
+        _chk(L.jdsg_mlp_head(m, mlp.hid, _p(Yh), _p(mlp.Wbf[4]), b4, C_SCALE, C_SHIFT, _p(out),
+                             s, None), "head")
     return out
 
 

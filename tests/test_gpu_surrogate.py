@@ -365,3 +365,25 @@ def test_trained_network_gain():
           f"{3 * plan.nent}; entries differing {int((~same).sum())} of {plan.nent}", flush=True)
     assert sur_evals < 0.5 * plain_evals
     assert (~same).sum() <= 1e-3 * plan.nent
+
+
+def test_output_head_matches_last_gemm():
+    """The inference head (a fixed-order dot product with the one output row
+    that is used) gives the predictions of the padded last-layer GEMM followed
+    by the de-standardisation, up to fp32 summation order."""
+    import torch
+    from jdsurfg_amd import surrogate as S
+    print("case output head vs last GEMM", flush=True)
+    mlp = S.MLP.load(S.DEFAULT_WEIGHTS)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    M = 4096
+    X = torch.randn(M, S.FEAT, device="cuda", generator=g).to(torch.bfloat16)
+    o = mlp.forward(X)
+    want = torch.exp(o[:, 0].double() * S.C_SCALE + S.C_SHIFT)
+    Yh = mlp.forward(X, hidden_only=True)
+    pred = torch.empty(M, dtype=torch.float64, device="cuda")
+    rc = S._L().jdsg_mlp_head(M, mlp.hid, S._p(Yh), S._p(mlp.Wbf[4]), float(mlp.b[4][0].item()),
+                              S.C_SCALE, S.C_SHIFT, S._p(pred), 0, None)
+    torch.cuda.synchronize()
+    assert rc == 0
+    assert ((pred - want).abs() / want).max().item() <= 1e-5
