@@ -319,10 +319,11 @@ extern "C" __global__ void jdsg_mode_order(PolishArgs A, int kmax) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int ncol = A.cm.ncol;
   if (t >= 2LL * ncol * kmax) return;
-  int col = (int)(t % ncol);
-  long long r = t / ncol;
+  // period fastest: a wave reads consecutive entries (e = ... + mode*K + k)
+  int k = (int)(t % kmax);
+  long long r = t / kmax;
   int w = (int)(r & 1);
-  int k = (int)(r >> 1);
+  int col = (int)(r >> 1);
   int K = A.per_off[col + 1] - A.per_off[col];
   if (k >= K) return;
   int nR = A.nmode[2 * col], nL = A.nmode[2 * col + 1];
