@@ -938,7 +938,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
 // ------------------------------------------------------------ Love parts
 // slegn96.f90:248-328 varl
 struct VarL {
-  double cosq, sinq, yl, zl, mu, rb, xkb, eexl;
+  double cosq, sinq, yl, zl, mu, rb, irb, xkb, eexl;
   cplx E1;  // exp(-nub d): (cos q, -sin q) propagating, exp(-q) evanescent
 };
 JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
@@ -951,9 +951,10 @@ JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
   v.mu = L.zrho * L.zb * L.zb;
   v.eexl = 0.0;
   v.sinq = 0.0;
+  v.irb = (v.rb > 0.0) ? 1.0 / v.rb : 0.0;
   if (wvno < v.xkb) {
     sincos(q, &v.sinq, &v.cosq);
-    v.yl = v.sinq / v.rb;
+    v.yl = v.sinq * v.irb;
     v.zl = -v.rb * v.sinq;
     v.E1 = cmk(v.cosq, -v.sinq);
   } else if (wvno == v.xkb) {
@@ -968,7 +969,7 @@ JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
     double fac = (q < 18.0) ? e1 * e1 : 0.0;
     v.cosq = (1.0 + fac) * 0.5;
     v.sinq = (1.0 - fac) * 0.5;
-    v.yl = v.sinq / v.rb;
+    v.yl = v.sinq * v.irb;
     v.zl = v.rb * v.sinq;
   }
   return v;
@@ -1096,7 +1097,7 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
         // nub is i rb or rb, so 1/nub and exp(-nub d) (from varl, unless rb
         // was raised to the floor above) need no complex division
         const bool prop = wvno < v.xkb;
-        const double irb = 1.0 / rb;
+        const double irb = (v.rb >= 1.0e-10) ? v.irb : 1.0 / rb;
         const cplx inub = prop ? cmk(0.0, -irb) : cmk(irb, 0.0);
         const cplx ixn = (0.5 / (wvno * dmu)) * inub;  // 0.5 / (wvno xnub)
         const double hk = 0.5 / wvno;
@@ -1125,17 +1126,19 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
       if (D.p[2])
         D.p[2][o] = 0.5 * c * (-c * c * upup + VSHH * VSHH * upup + VSHV * VSHV * dupdup * ik2);
       // dc/dh terms before fac
-      double drh, dm, dvdz;
-      if (llflag == 0) {
-        drh = L.zrho;
-        dm = xmu;
-        dvdz = 0.0;
-      } else {
-        drh = L.zrho - zr_prev;
-        dm = xmu - xmu_prev;
-        dvdz = t_k * t_k * (1.0 / xmu - 1.0 / xmu_prev);
+      if (D.p[3]) {
+        double drh, dm, dvdz;
+        if (llflag == 0) {
+          drh = L.zrho;
+          dm = xmu;
+          dvdz = 0.0;
+        } else {
+          drh = L.zrho - zr_prev;
+          dm = xmu - xmu_prev;
+          dvdz = t_k * t_k * (1.0 / xmu - 1.0 / xmu_prev);
+        }
+        D.p[3][o] = uk * uk * (omega2 * drh - wvno2 * dm) + dvdz;
       }
-      if (D.p[3]) D.p[3][o] = uk * uk * (omega2 * drh - wvno2 * dm) + dvdz;
       llflag = llflag + 1;
     }
     zr_prev = L.zrho;
@@ -1335,10 +1338,14 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
     }
     // finalise kernels bottom-up with the thickness suffix sums
     double hsum = 0.0;
+    // flattening factors of the requested types only (the stage is bound by
+    // HBM traffic)
+    const bool need_v = A.dc[0] || A.dc[1], need_r = A.dc[2] != nullptr,
+               need_d = A.dc[3] != nullptr;
     for (int i = N - 1; i >= 0; --i) {
       size_t q = (size_t)i * A.cm.ncol_pad + col;
-      double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
-      double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
+      double vtp = need_v ? A.cm.vtp[q] : 0.0, dtp = need_d ? A.cm.dtp[q] : 0.0;
+      double rtp = need_r ? (ray ? A.cm.rtpR[q] : A.cm.rtpL[q]) : 0.0;
       long long o = (long long)i * ss;
       const double *p0 = draw, *p1 = draw + (long long)A.nmax * ss,
                    *p2 = draw + (long long)2 * A.nmax * ss, *p3 = draw + (long long)3 * A.nmax * ss;
@@ -1394,10 +1401,12 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
       tm3 = tm * tm * tm;
     }
     double hsc = 0.0, hsu = 0.0;
+    const bool need_v = A.dc[0] || A.du[0] || A.dc[1] || A.du[1],
+               need_r = A.dc[2] || A.du[2], need_d = A.dc[3] || A.du[3];
     for (int i = N - 1; i >= 0; --i) {
       size_t q = (size_t)i * A.cm.ncol_pad + col;
-      double vtp = A.cm.vtp[q], dtp = A.cm.dtp[q];
-      double rtp = ray ? A.cm.rtpR[q] : A.cm.rtpL[q];
+      double vtp = need_v ? A.cm.vtp[q] : 0.0, dtp = need_d ? A.cm.dtp[q] : 0.0;
+      double rtp = need_r ? (ray ? A.cm.rtpR[q] : A.cm.rtpL[q]) : 0.0;
       long long oo = kb + (long long)i * ls;
       long long os = (long long)i * ss;
       bool solid = true;
