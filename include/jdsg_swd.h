@@ -217,6 +217,18 @@ int jdsg_plan_features(jdsg_plan *plan, const float *thk, const float *vp, const
                        const float *rho, const double *periods, long long e0, long long nrow,
                        const long long *rows, void *X, float *colf, int column_pass,
                        void *stream);
+/* First hidden layer split by what varies: only ln T and the wave and mode
+ * one-hots of an entry's 192 inputs differ between entries of one column.
+ * jdsg_mlp_column_rows writes per-column input rows (bf16 [ncol][192]: the
+ * column features, entry inputs zero) from colf; one GEMM of them with the
+ * first weights gives acol (fp32 [ncol][lda]).  jdsg_plan_mlp_expand then
+ * writes Y = gelu(acol + bf16(lnT) ext[0] + ext[1+wave] + ext[3+mode] +
+ * ext[7]) (bf16 [nrow][H]) for entries e0..e0+nrow-1, ext (fp32 [8][H])
+ * holding the weight columns 144..150 and the bias. */
+int jdsg_mlp_column_rows(int ncol, const float *colf, void *X, void *stream);
+int jdsg_plan_mlp_expand(jdsg_plan *plan, const double *periods, long long e0, long long nrow,
+                         const float *acol, int lda, const float *ext, int H, void *Y,
+                         void *stream);
 /* pred[e0 + r] = exp(out[r*ldo]*scale + shift), r < n (the network predicts
  * a standardised ln c). */
 int jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,

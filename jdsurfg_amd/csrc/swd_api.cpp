@@ -672,6 +672,21 @@ extern "C" int jdsg_plan_features(jdsg_plan *P, const float *thk, const float *v
   return 0;
 }
 
+// First hidden layer of the network for entries e0..e0+nrow from the
+// per-column pre-activations acol (ncol x lda fp32, one GEMM over the column
+// feature rows) and the weight columns of the per-entry inputs (ext, 8 x H):
+// Y (nrow x H, bf16) = gelu(acol + entry terms).
+extern "C" int jdsg_plan_mlp_expand(jdsg_plan *P, const double *periods, long long e0,
+                                    long long nrow, const float *acol, int lda, const float *ext,
+                                    int H, void *Y, void *stream) {
+  if (e0 < 0 || nrow < 0 || e0 + nrow > P->nent) return fail("expand rows out of range");
+  if (H <= 0 || H % 8 || lda < H) return fail("expand: H must be a positive multiple of 8 <= lda");
+  jdsg_launch_mlp_expand(e0, nrow, P->ncol, P->ent_off_d, P->per_off_d, periods, P->nmode_d, acol,
+                         lda, ext, H, (uint16_t *)Y, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // pred[e0 + r] = exp(out[r*ldo]*scale + shift) for r < n (MLP output, a
 // standardised ln c, to flat c)
 extern "C" int jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,

@@ -124,6 +124,10 @@ void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N,
                                    double *vout, double *kernel, int *status, int want_kernel,
                                    hipStream_t st);
 void jdsg_launch_polish(const PolishArgs &A, int wave, int nwaves, hipStream_t st);
+void jdsg_launch_mlp_expand(long long e0, long long nrow, int ncol, const long long *ent_off,
+                            const int *per_off, const double *periods, const int *nmode,
+                            const float *acol, int lda, const float *ext, int H, uint16_t *Y,
+                            hipStream_t st);
 
 // Fréchet rows / misfit gradient (swd_frechet.hip)
 struct RowCtx {

@@ -381,6 +381,81 @@ jdsg_mlp_head(long long n, int H, const uint16_t *__restrict__ Y, const uint16_t
   if (row < n && part == 0) pred[e0 + row] = exp((double)(s + b) * scale + shift);
 }
 
+// First layer of the network at inference, split by what varies.  Of the 192
+// inputs of an entry only ln T, the wave and the mode one-hots depend on the
+// entry; the rest are its column's profile and half-space features.  So the
+// pre-activation is Acol(col) + bf16(ln T) w_144 + w_wave + w_mode + b, with
+// Acol = (column feature row) W^T from one GEMM over the columns (bf16
+// operands, fp32 accumulation, like the full GEMM up to summation order).
+// Row of column features: [0,144) profile, 151-154 half-space, rest zero.
+__global__ void jdsg_mlp_column_rows(int ncol, const float *__restrict__ colf, uint16_t *X) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const float *cf = colf + (size_t)col * 148;
+  uint4 *dst = reinterpret_cast<uint4 *>(X + (size_t)col * 192);
+  for (int g = 0; g < 24; ++g) {
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int f = g * 8 + q;
+      v[q] = f < 144 ? cf[f] : (f >= 151 && f < 155) ? cf[144 + (f - 151)] : 0.f;
+    }
+    dst[g] = make_uint4(pack_bf2(v[0], v[1]), pack_bf2(v[2], v[3]), pack_bf2(v[4], v[5]),
+                        pack_bf2(v[6], v[7]));
+  }
+}
+
+// Y(row, j) = bf16(gelu(Acol(col, j) + bf16(lnT) ext[0][j] + ext[1 + wave][j] +
+// ext[3 + mode][j] + ext[7][j])) for the entries e0 + row (row < nrow); ext
+// holds the weight columns of ln T, the two wave and four mode inputs and
+// the bias.  One thread per 8 outputs of a row (one 16-byte store).
+__global__ void __launch_bounds__(256)
+jdsg_mlp_expand(long long e0, long long nrow, int ncol, const long long *__restrict__ ent_off,
+                const int *__restrict__ per_off, const double *__restrict__ periods,
+                const int *__restrict__ nmode, const float *__restrict__ acol, int lda,
+                const float *__restrict__ ext, int H, uint16_t *Y) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int groups = H >> 3;
+  const long long row = t / groups;
+  if (row >= nrow) return;
+  const int j0 = (int)(t % groups) * 8;
+  const long long e = e0 + row;
+  int lo = 0, hi = ncol - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ent_off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  const int col = lo;
+  const int K = per_off[col + 1] - per_off[col];
+  const long long el = e - ent_off[col];
+  const int wm = (int)(el / K), k = (int)(el % K);
+  const int nR = nmode[2 * col];
+  const int wave = wm < nR ? 0 : 1, mode = wave == 0 ? wm : wm - nR;
+  const float lnT = (float)((log(periods[per_off[col] + k]) - 2.0) / 1.5);
+  const float lt = bf2f(f2bf(lnT));
+  const float *a = acol + (size_t)col * lda + j0;
+  float z[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = j0 + q;
+    z[q] = gelu(a[q] + lt * ext[j] + ext[(1 + wave) * H + j] + ext[(3 + mode) * H + j] +
+                ext[7 * H + j]);
+  }
+  *reinterpret_cast<uint4 *>(Y + row * H + j0) =
+      make_uint4(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]), pack_bf2(z[4], z[5]),
+                 pack_bf2(z[6], z[7]));
+}
+
+void jdsg_launch_mlp_expand(long long e0, long long nrow, int ncol, const long long *ent_off,
+                            const int *per_off, const double *periods, const int *nmode,
+                            const float *acol, int lda, const float *ext, int H, uint16_t *Y,
+                            hipStream_t st) {
+  const long long n = nrow * (H / 8);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(jdsg_mlp_expand, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e0,
+                     nrow, ncol, ent_off, per_off, periods, nmode, acol, lda, ext, H, Y);
+}
+
 // ----------------------------------------------------------------------------
 extern "C" {
 
@@ -460,6 +535,13 @@ int jdsg_adam(long long n, float *w, float *m, float *v, const float *g, float l
   hipLaunchKernelGGL(jdsg_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, n, w, m, v, g, lr, b1, b2, eps, c1, c2, wd,
                      (uint16_t *)wbf);
+  return hipGetLastError() != hipSuccess;
+}
+
+int jdsg_mlp_column_rows(int ncol, const float *colf, void *X, void *stream) {
+  if (ncol <= 0) return 0;
+  hipLaunchKernelGGL(jdsg_mlp_column_rows, dim3((ncol + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, ncol, colf, (uint16_t *)X);
   return hipGetLastError() != hipSuccess;
 }
 

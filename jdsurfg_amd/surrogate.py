@@ -44,6 +44,9 @@ def _L():
                               C.c_longlong, vp]
     L.jdsg_mlp_head.argtypes = [C.c_longlong, C.c_int, vp, vp, C.c_float, C.c_double, C.c_double,
                                 vp, C.c_longlong, vp]
+    L.jdsg_mlp_column_rows.argtypes = [C.c_int, vp, vp, vp]
+    L.jdsg_plan_mlp_expand.argtypes = [vp, vp, C.c_longlong, C.c_longlong, vp, C.c_int, vp,
+                                       C.c_int, vp, vp]
     L.jdsg_plan_enable_surrogate.argtypes = [vp]
     L.jdsg_plan_execute_surrogate.argtypes = [vp] * 7 + [C.c_double] + [vp] * 12
     L.jdsg_plan_surrogate_stats.argtypes = [vp, C.POINTER(C.c_longlong),
@@ -109,17 +112,24 @@ class MLP:
         self._buf[key] = d
         return d
 
-    def forward(self, X, train=False, stream=None, hidden_only=False):
+    def forward(self, X, train=False, stream=None, hidden_only=False, y1=None):
         """X: (M, 192) bf16 CUDA tensor, M a multiple of 64.  Returns the
         (M, 64) fp32 output (column 0 is the prediction), or with hidden_only
-        the last hidden layer (M, hid) bf16 for the output head."""
+        the last hidden layer (M, hid) bf16 for the output head.  With y1
+        (inference only) X is None and the first hidden layer (M, hid) bf16
+        is given."""
         L = _L()
-        M, H = X.shape[0], self.hid
+        H = self.hid
+        M = X.shape[0] if y1 is None else y1.shape[0]
         d = self._bufs(M, train)
         Y = d["Y"]
         Y[0] = X
         K = FEAT
-        for li in range(4):
+        first = 0
+        if y1 is not None:
+            assert not train
+            Y[1], K, first = y1, H, 1
+        for li in range(first, 4):
             if train:
                 _chk(L.jdsg_gemm_bf16(0, 1, M, H, K, _p(Y[li]), K, _p(self.Wbf[li]), K,
                                       _p(d["Z"][li]), H, 1, _p(self.b[li]), _p(Y[li + 1]), H,
@@ -340,14 +350,55 @@ def train(ncol=200000, steps=3000, batch=8192, lr=1e-3, seed=7, log=print, valid
     return mlp, rep
 
 
-def predict(mlp, feats, rows=None, e0=0, n=None, chunk=1 << 20):
-    """Predicted flat c for entries (rows, or e0..e0+n) as a float64 tensor."""
+def first_layer_parts(mlp, feats, stream=None):
+    """Per-column pre-activations of the first hidden layer (ncol x hid fp32:
+    one GEMM over the column feature rows) and the weight columns of the
+    per-entry inputs (8 x hid fp32: ln T, two wave and four mode one-hots,
+    then the bias) -- what jdsg_plan_mlp_expand combines per entry."""
+    import torch
+    L = _L()
+    if feats.first:   # column features only (no entry rows)
+        feats.rows(e0=0, n=0, X=torch.empty(0, FEAT, dtype=torch.bfloat16, device="cuda"),
+                   stream=stream)
+    ncol, H = feats.plan.ncol, mlp.hid
+    ncp = (ncol + 127) // 128 * 128
+    Xc = torch.zeros(ncp, FEAT, dtype=torch.bfloat16, device="cuda")
+    _chk(L.jdsg_mlp_column_rows(ncol, _p(feats.colf), _p(Xc), stream), "column rows")
+    acol = torch.empty(ncp, H, device="cuda")
+    _chk(L.jdsg_gemm_bf16(0, 1, ncp, H, FEAT, _p(Xc), FEAT, _p(mlp.Wbf[0]), FEAT, _p(acol), H,
+                          0, None, None, 0, stream), "gemm")
+    W1 = mlp.Wbf[0].float()
+    ext = torch.cat([W1[:, 144:151].t(), mlp.b[0][None, :]]).contiguous()
+    return acol, ext
+
+
+def predict(mlp, feats, rows=None, e0=0, n=None, chunk=1 << 20, split=True):
+    """Predicted flat c for entries (rows, or e0..e0+n) as a float64 tensor.
+    For a range of entries (split) the first layer is the per-column GEMM
+    plus the per-entry terms; for given rows it is the full GEMM of their
+    input rows."""
     import torch
     L = _L()
     if rows is not None:
         n = rows.numel()
+        split = False
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     b4 = float(mlp.b[4][0].item())   # bias of the output used (the rest is padding)
+    if split:
+        acol, ext = first_layer_parts(mlp, feats)
+        for s in range(0, n, chunk):
+            m = min(chunk, n - s)
+            mp = (m + 63) // 64 * 64
+            d = mlp._bufs(mp, False)
+            if "Y1in" not in d:
+                d["Y1in"] = torch.zeros(mp, mlp.hid, dtype=torch.bfloat16, device="cuda")
+            Y1 = d["Y1in"]
+            _chk(L.jdsg_plan_mlp_expand(feats.plan.h, _p(feats.inp[4]), e0 + s, m, _p(acol),
+                                        mlp.hid, _p(ext), mlp.hid, _p(Y1), None), "expand")
+            Yh = mlp.forward(None, train=False, hidden_only=True, y1=Y1)
+            _chk(L.jdsg_mlp_head(m, mlp.hid, _p(Yh), _p(mlp.Wbf[4]), b4, C_SCALE, C_SHIFT,
+                                 _p(out), s, None), "head")
+        return out
     Xb = None
     for s in range(0, n, chunk):
         m = min(chunk, n - s)

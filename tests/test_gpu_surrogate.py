@@ -387,3 +387,30 @@ def test_output_head_matches_last_gemm():
     torch.cuda.synchronize()
     assert rc == 0
     assert ((pred - want).abs() / want).max().item() <= 1e-5
+
+
+def test_split_first_layer_matches_full_rows():
+    """Predictions with the first layer split into a per-column GEMM plus the
+    per-entry terms (ln T, wave, mode) match those from the full 192-wide
+    input rows: the same bf16 products summed in another fp32 order, which
+    can move a hidden value by one bf16 step.  Chunks of a size that is not
+    a multiple of 64 and a nonzero first entry exercise the row offsets."""
+    import torch
+    from jdsurfg_amd import swd, surrogate as S
+    w = W.grid128().subset(np.arange(0, 16384, 37))
+    print(f"case split first layer, grid128 subset ncol={w.ncol}", flush=True)
+    layout = "grid" if w.grid is not None else "csr"
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
+    inp = swd.upload(w)
+    feats = S.Features(plan, inp)
+    mlp = S.MLP.load(S.DEFAULT_WEIGHTS)
+    e0 = 5
+    n = plan.nent - e0
+    full = S.predict(mlp, feats, e0=e0, n=n, split=False)
+    split = S.predict(mlp, feats, e0=e0, n=n, chunk=4001)
+    torch.cuda.synchronize()
+    rel = ((split - full).abs() / full).cpu().numpy()
+    print(f"split vs full rows: max {rel.max():.3e} median {np.median(rel):.3e}", flush=True)
+    assert np.all(np.isfinite(split.cpu().numpy()))
+    assert rel.max() <= 1e-3
+    assert np.median(rel) <= 1e-5
