@@ -229,6 +229,20 @@ int jdsg_mlp_column_rows(int ncol, const float *colf, void *X, void *stream);
 int jdsg_plan_mlp_expand(jdsg_plan *plan, const double *periods, long long e0, long long nrow,
                          const float *acol, int lda, const float *ext, int H, void *Y,
                          void *stream);
+/* The whole network of hidden width 256 at inference in one kernel: first
+ * layer as jdsg_plan_mlp_expand, hidden layers 2-4 (bf16 weights W2..W4,
+ * 256x256 row-major out x in, fp32 biases b2..b4) with GELU, and the head as
+ * jdsg_mlp_head (w5 the bf16 output row, b5 its bias): pred[p0 + r] for
+ * entries e0 + r, r < nrow, the same bits as those kernels in sequence.
+ * Operands 16-byte aligned, lda >= 256 and a multiple of 4; work is a
+ * device workspace of work_len ints, at least jdsg_mlp_fused_work(nrow). */
+long long jdsg_mlp_fused_work(long long nrow);
+int jdsg_plan_mlp_fused(jdsg_plan *plan, const double *periods, long long e0, long long nrow,
+                        const float *acol, int lda, const float *ext, const void *W2,
+                        const void *W3, const void *W4, const float *b2, const float *b3,
+                        const float *b4, const void *w5, float b5, double scale, double shift,
+                        double *pred, long long p0, int *work, long long work_len,
+                        void *stream);
 /* pred[e0 + r] = exp(out[r*ldo]*scale + shift), r < n (the network predicts
  * a standardised ln c). */
 int jdsg_denorm(long long n, const float *out, int ldo, double scale, double shift,

@@ -680,12 +680,39 @@ extern "C" int jdsg_plan_mlp_expand(jdsg_plan *P, const double *periods, long lo
                                     long long nrow, const float *acol, int lda, const float *ext,
                                     int H, void *Y, void *stream) {
   if (e0 < 0 || nrow < 0 || e0 + nrow > P->nent) return fail("expand rows out of range");
-  if (H <= 0 || H % 8 || lda < H) return fail("expand: H must be a positive multiple of 8 <= lda");
+  if (H <= 0 || H % 8 || lda < H || lda % 4)
+    return fail("expand: H must be a positive multiple of 8 <= lda, lda a multiple of 4");
   jdsg_launch_mlp_expand(e0, nrow, P->ncol, P->ent_off_d, P->per_off_d, periods, P->nmode_d, acol,
                          lda, ext, H, (uint16_t *)Y, (hipStream_t)stream);
   HIPCHK(hipGetLastError());
   return 0;
 }
+
+// The whole width-256 network at inference for entries e0..e0+nrow:
+// pred[p0 + r] = exp(head(hidden layers(first layer)) * scale + shift).
+extern "C" int jdsg_plan_mlp_fused(jdsg_plan *P, const double *periods, long long e0,
+                                   long long nrow, const float *acol, int lda, const float *ext,
+                                   const void *W2, const void *W3, const void *W4,
+                                   const float *b2, const float *b3, const float *b4,
+                                   const void *w5, float b5, double scale, double shift,
+                                   double *pred, long long p0, int *work, long long work_len,
+                                   void *stream) {
+  if (e0 < 0 || nrow < 0 || e0 + nrow > P->nent) return fail("fused network rows out of range");
+  if (nrow > 0 && (!work || work_len < jdsg_mlp_fused_work_len(nrow)))
+    return fail("fused network: workspace missing or shorter than jdsg_mlp_fused_work(nrow)");
+  if (lda < 256 || lda % 4) return fail("fused network: lda must be >= 256 and a multiple of 4");
+  const void *ptrs[] = {acol, ext, W2, W3, W4, b2, b3, b4, w5};
+  for (const void *q : ptrs)
+    if (!q || (reinterpret_cast<uintptr_t>(q) & 15)) return fail("fused network: null or unaligned operand");
+  jdsg_launch_mlp_fused(e0, nrow, P->ncol, P->ent_off_d, P->per_off_d, periods, P->nmode_d, acol,
+                        lda, ext, (const uint16_t *)W2, (const uint16_t *)W3,
+                        (const uint16_t *)W4, b2, b3, b4, (const uint16_t *)w5, b5, scale, shift,
+                        pred, p0, work, (hipStream_t)stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" long long jdsg_mlp_fused_work(long long nrow) { return jdsg_mlp_fused_work_len(nrow); }
 
 // pred[e0 + r] = exp(out[r*ldo]*scale + shift) for r < n (MLP output, a
 // standardised ln c, to flat c)

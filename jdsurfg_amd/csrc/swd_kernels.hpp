@@ -128,6 +128,14 @@ void jdsg_launch_mlp_expand(long long e0, long long nrow, int ncol, const long l
                             const int *per_off, const double *periods, const int *nmode,
                             const float *acol, int lda, const float *ext, int H, uint16_t *Y,
                             hipStream_t st);
+long long jdsg_mlp_fused_work_len(long long nrow);  // ints of tile workspace
+void jdsg_launch_mlp_fused(long long e0, long long nrow, int ncol, const long long *ent_off,
+                           const int *per_off, const double *periods, const int *nmode,
+                           const float *acol, int lda, const float *ext, const uint16_t *W2,
+                           const uint16_t *W3, const uint16_t *W4, const float *b2,
+                           const float *b3, const float *b4, const uint16_t *w5, float b5,
+                           double scale, double shift, double *pred, long long p0, int *tcol,
+                           hipStream_t st);
 
 // Fréchet rows / misfit gradient (swd_frechet.hip)
 struct RowCtx {

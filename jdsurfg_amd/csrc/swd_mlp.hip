@@ -355,6 +355,32 @@ __global__ void jdsg_f32_to_bf16(long long n, const float *__restrict__ x, uint1
 // de-standardised: pred[e0 + i] = exp((Y_i . w + b) * scale + shift).  Eight
 // lanes per row, each over H/8 consecutive inputs (16-byte loads), reduced by
 // shuffles in a fixed order.
+// Partial dot product of the output head: n8 groups of eight bf16 inputs y
+// and weights w, fp32, in order.  Shared by the head kernel and the fused
+// network kernel so both sum the same way.
+__device__ __forceinline__ float head_part(const uint16_t *y, const uint16_t *__restrict__ w,
+                                           int n8) {
+  float s = 0.f;
+  for (int q = 0; q < n8; ++q) {
+    const uint4 a = reinterpret_cast<const uint4 *>(y)[q];
+    const uint4 c = reinterpret_cast<const uint4 *>(w)[q];
+    const uint32_t av[4] = {a.x, a.y, a.z, a.w}, cv[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      s += bf2f((uint16_t)(av[i] & 0xFFFFu)) * bf2f((uint16_t)(cv[i] & 0xFFFFu));
+      s += bf2f((uint16_t)(av[i] >> 16)) * bf2f((uint16_t)(cv[i] >> 16));
+    }
+  }
+  return s;
+}
+// the eight parts of a row sit in eight consecutive lanes
+__device__ __forceinline__ float head_reduce8(float s) {
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  s += __shfl_xor(s, 4);
+  return s;
+}
+
 __global__ void __launch_bounds__(256)
 jdsg_mlp_head(long long n, int H, const uint16_t *__restrict__ Y, const uint16_t *__restrict__ w,
               float b, double scale, double shift, double *pred, long long e0) {
@@ -362,22 +388,8 @@ jdsg_mlp_head(long long n, int H, const uint16_t *__restrict__ Y, const uint16_t
   const long long row = t >> 3;
   const int part = (int)(t & 7), seg = H >> 3;
   float s = 0.f;
-  if (row < n) {
-    const uint4 *y = reinterpret_cast<const uint4 *>(Y + row * H + part * seg);
-    const uint4 *v = reinterpret_cast<const uint4 *>(w + part * seg);
-    for (int q = 0; q < seg / 8; ++q) {
-      const uint4 a = y[q], c = v[q];
-      const uint32_t av[4] = {a.x, a.y, a.z, a.w}, cv[4] = {c.x, c.y, c.z, c.w};
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        s += bf2f((uint16_t)(av[i] & 0xFFFFu)) * bf2f((uint16_t)(cv[i] & 0xFFFFu));
-        s += bf2f((uint16_t)(av[i] >> 16)) * bf2f((uint16_t)(cv[i] >> 16));
-      }
-    }
-  }
-  s += __shfl_xor(s, 1);
-  s += __shfl_xor(s, 2);
-  s += __shfl_xor(s, 4);
+  if (row < n) s = head_part(Y + row * H + part * seg, w + part * seg, seg / 8);
+  s = head_reduce8(s);
   if (row < n && part == 0) pred[e0 + row] = exp((double)(s + b) * scale + shift);
 }
 
@@ -405,10 +417,65 @@ __global__ void jdsg_mlp_column_rows(int ncol, const float *__restrict__ colf, u
   }
 }
 
-// Y(row, j) = bf16(gelu(Acol(col, j) + bf16(lnT) ext[0][j] + ext[1 + wave][j] +
-// ext[3 + mode][j] + ext[7][j])) for the entries e0 + row (row < nrow); ext
-// holds the weight columns of ln T, the two wave and four mode inputs and
-// the bias.  One thread per 8 outputs of a row (one 16-byte store).
+// Column of entry e, searched in [lo, hi] (the entries of column c are
+// ent_off[c] .. ent_off[c+1]-1).
+__device__ __forceinline__ int entry_column(long long e, int lo, int hi,
+                                            const long long *__restrict__ ent_off) {
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ent_off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// What the first layer needs of entry e: its column (in [lo, hi]), wave,
+// mode and the bf16-rounded ln T input (the entry order of the plans: per
+// column, waves R then L, modes, periods fastest).
+struct EntryIn {
+  int col, wave, mode;
+  float lt;
+};
+__device__ __forceinline__ void load8(const float *p, float (&a)[8]) {
+  const float4 u = reinterpret_cast<const float4 *>(p)[0], v = reinterpret_cast<const float4 *>(p)[1];
+  a[0] = u.x; a[1] = u.y; a[2] = u.z; a[3] = u.w;
+  a[4] = v.x; a[5] = v.y; a[6] = v.z; a[7] = v.w;
+}
+__device__ __forceinline__ EntryIn entry_inputs(long long e, int lo, int hi,
+                                                const long long *__restrict__ ent_off,
+                                                const int *__restrict__ per_off,
+                                                const double *__restrict__ periods,
+                                                const int *__restrict__ nmode) {
+  lo = entry_column(e, lo, hi, ent_off);
+  EntryIn r;
+  r.col = lo;
+  const int K = per_off[lo + 1] - per_off[lo];
+  const long long el = e - ent_off[lo];
+  const int wm = (int)(el / K), k = (int)(el % K);
+  const int nR = nmode[2 * lo];
+  r.wave = wm < nR ? 0 : 1;
+  r.mode = r.wave == 0 ? wm : wm - nR;
+  r.lt = bf2f(f2bf((float)((log(periods[per_off[lo] + k]) - 2.0) / 1.5)));
+  return r;
+}
+// Eight first-layer outputs j0..j0+7 of an entry as packed bf16 from its
+// column's pre-activations a[0..7] = acol(col, j0..j0+7):
+// gelu(a + lt ext[0] + ext[1 + wave] + ext[3 + mode] + ext[7]); modes above
+// 3 have no one-hot input, so no mode term.
+__device__ __forceinline__ uint4 first_layer8(const EntryIn &in, const float (&a)[8],
+                                              const float *ext, int H, int j0) {
+  float z[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = j0 + q;
+    const float m = in.mode < 4 ? ext[(3 + in.mode) * H + j] : 0.f;
+    z[q] = gelu(a[q] + in.lt * ext[j] + ext[(1 + in.wave) * H + j] + m + ext[7 * H + j]);
+  }
+  return make_uint4(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]), pack_bf2(z[4], z[5]),
+                    pack_bf2(z[6], z[7]));
+}
+
+// Y(row, j) = first layer of entry e0 + row (row < nrow).  One thread per
+// 8 outputs of a row (one 16-byte store).
 __global__ void __launch_bounds__(256)
 jdsg_mlp_expand(long long e0, long long nrow, int ncol, const long long *__restrict__ ent_off,
                 const int *__restrict__ per_off, const double *__restrict__ periods,
@@ -419,31 +486,169 @@ jdsg_mlp_expand(long long e0, long long nrow, int ncol, const long long *__restr
   const long long row = t / groups;
   if (row >= nrow) return;
   const int j0 = (int)(t % groups) * 8;
-  const long long e = e0 + row;
-  int lo = 0, hi = ncol - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (ent_off[mid] <= e) lo = mid; else hi = mid - 1;
-  }
-  const int col = lo;
-  const int K = per_off[col + 1] - per_off[col];
-  const long long el = e - ent_off[col];
-  const int wm = (int)(el / K), k = (int)(el % K);
-  const int nR = nmode[2 * col];
-  const int wave = wm < nR ? 0 : 1, mode = wave == 0 ? wm : wm - nR;
-  const float lnT = (float)((log(periods[per_off[col] + k]) - 2.0) / 1.5);
-  const float lt = bf2f(f2bf(lnT));
-  const float *a = acol + (size_t)col * lda + j0;
-  float z[8];
+  const EntryIn in = entry_inputs(e0 + row, 0, ncol - 1, ent_off, per_off, periods, nmode);
+  float a[8];
+  load8(acol + (size_t)in.col * lda + j0, a);
+  *reinterpret_cast<uint4 *>(Y + row * H + j0) = first_layer8(in, a, ext, H, j0);
+}
+
+// The whole network at inference for 128 entries per workgroup, hidden width
+// 256: first layer from acol (as jdsg_mlp_expand), three hidden layers on
+// MFMA and the output head (as jdsg_mlp_head), with the activations kept in
+// LDS between layers instead of going through HBM.  Wave w computes
+// features 64w..64w+63 of all 128 rows: acc[ft][rt] is the 16x16 tile of
+// features 64w+16ft.. and rows 16rt..; the weight fragments (16 features x
+// 32 inputs) stream from L2, one K step ahead; the activation fragments are
+// LDS rows (stride 264 bf16 so the sixteen rows of a fragment fall in
+// different banks).  The MFMA operands, their K order and the epilogue are
+// those of jdsg_gemm_bf16_128's inference epilogue, and the first layer and
+// the head share their code with the unfused kernels, so the predictions are
+// the same bits as the unfused path's.  tcol[t] is the column of the first
+// entry of tile t (tcol[ntile]: of the last entry), so a row searches its
+// column among the few columns its tile spans.
+constexpr int FH = 256, FLD = FH + 8;
+constexpr int kFusedRows = 128;  // rows per workgroup (two workgroups per CU)
+
+template <int FR>
+__global__ void jdsg_mlp_tile_cols(long long e0, long long nrow, int ncol,
+                                   const long long *__restrict__ ent_off, int ntile, int *tcol) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntile) return;
+  long long r = (long long)t * FR;
+  if (r > nrow - 1) r = nrow - 1;
+  tcol[t] = entry_column(e0 + r, 0, ncol - 1, ent_off);
+}
+
+template <int FR>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FR == 128 ? 2 : 3)))
+jdsg_mlp_fused(long long e0, long long nrow, int ncol, const long long *__restrict__ ent_off,
+               const int *__restrict__ per_off, const double *__restrict__ periods,
+               const int *__restrict__ nmode, const float *__restrict__ acol, int lda,
+               const float *__restrict__ ext, const uint16_t *__restrict__ W2,
+               const uint16_t *__restrict__ W3, const uint16_t *__restrict__ W4,
+               const float *__restrict__ b2, const float *__restrict__ b3,
+               const float *__restrict__ b4, const uint16_t *__restrict__ w5, float b5,
+               double scale, double shift, double *pred, long long p0,
+               const int *__restrict__ tcol) {
+  __shared__ __attribute__((aligned(16))) uint16_t X[FR][FLD];
+  __shared__ __attribute__((aligned(16))) float sext[8 * FH];
+  __shared__ EntryIn sin_[FR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long long r0 = (long long)blockIdx.x * FR;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int j = j0 + q;
-    z[q] = gelu(a[q] + lt * ext[j] + ext[(1 + wave) * H + j] + ext[(3 + mode) * H + j] +
-                ext[7 * H + j]);
+  for (int q = 0; q < 2; ++q)
+    reinterpret_cast<float4 *>(sext)[q * 256 + tid] =
+        reinterpret_cast<const float4 *>(ext)[q * 256 + tid];
+  if (tid < FR) {
+    long long row = r0 + tid;
+    if (row >= nrow) row = nrow - 1;  // tail rows repeat the last entry, not stored
+    sin_[tid] = entry_inputs(e0 + row, tcol[blockIdx.x], tcol[blockIdx.x + 1], ent_off, per_off,
+                             periods, nmode);
   }
-  *reinterpret_cast<uint4 *>(Y + row * H + j0) =
-      make_uint4(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]), pack_bf2(z[4], z[5]),
-                 pack_bf2(z[6], z[7]));
+  // weight fragments two K steps ahead; step s is layer s/8, K step s%8
+  const uint16_t *const Wl[3] = {W2, W3, W4};
+  const float *const bl[3] = {b2, b3, b4};
+  const size_t wofs = (size_t)(64 * wave + (lane & 15)) * FH + 8 * (lane >> 4);
+  const int xr = lane & 15, xk = 8 * (lane >> 4);
+  bf8v w0[4], w1[4];
+#pragma unroll
+  for (int ft = 0; ft < 4; ++ft) {
+    w0[ft] = *reinterpret_cast<const bf8v *>(W2 + wofs + (size_t)ft * 16 * FH);
+    w1[ft] = *reinterpret_cast<const bf8v *>(W2 + wofs + (size_t)ft * 16 * FH + 32);
+  }
+  __syncthreads();
+  {
+    // a thread keeps features j0..j0+7 and walks rows tid/32 + 8i; rows of
+    // one column share acol, loaded again only when the column changes
+    const int j0 = (tid & 31) * 8;
+    int pc = -1;
+    float a[8];
+    for (int i = 0; i < FR / 8; ++i) {
+      const int row = i * 8 + (tid >> 5);
+      const EntryIn in = sin_[row];
+      if (in.col != pc) {
+        load8(acol + (size_t)in.col * lda + j0, a);
+        pc = in.col;
+      }
+      *reinterpret_cast<uint4 *>(&X[row][j0]) = first_layer8(in, a, sext, FH, j0);
+    }
+  }
+  __syncthreads();
+
+  for (int L = 0; L < 3; ++L) {
+    constexpr int RT = FR / 16;
+    v4f acc[4][RT];
+#pragma unroll
+    for (int ft = 0; ft < 4; ++ft)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) acc[ft][rt] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int ks = 0; ks < 8; ++ks) {
+      const int s2 = L * 8 + ks + 2;
+      bf8v w2[4];
+      if (s2 < 24) {
+        const uint16_t *np = Wl[s2 >> 3] + wofs + (s2 & 7) * 32;
+#pragma unroll
+        for (int ft = 0; ft < 4; ++ft)
+          w2[ft] = *reinterpret_cast<const bf8v *>(np + (size_t)ft * 16 * FH);
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        const bf8v xf = *reinterpret_cast<const bf8v *>(&X[rt * 16 + xr][ks * 32 + xk]);
+#pragma unroll
+        for (int ft = 0; ft < 4; ++ft)
+          acc[ft][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0[ft], xf, acc[ft][rt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int ft = 0; ft < 4; ++ft) {
+        w0[ft] = w1[ft];
+        if (s2 < 24) w1[ft] = w2[ft];
+      }
+    }
+    __syncthreads();  // every wave has read this layer's inputs
+#pragma unroll
+    for (int ft = 0; ft < 4; ++ft) {
+      const int j0 = 64 * wave + 16 * ft + 4 * (lane >> 4);
+      const v4f bq = *reinterpret_cast<const v4f *>(bl[L] + j0);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        const v4f v = acc[ft][rt] + bq;
+        uint2 h;
+        h.x = pack_bf2(gelu(v[0]), gelu(v[1]));
+        h.y = pack_bf2(gelu(v[2]), gelu(v[3]));
+        *reinterpret_cast<uint2 *>(&X[rt * 16 + xr][j0]) = h;
+      }
+    }
+    __syncthreads();
+  }
+  // head: eight lanes per row, FR/32 rows per thread
+#pragma unroll
+  for (int i = 0; i < FR / 32; ++i) {
+    const int row = i * 32 + (tid >> 3), part = tid & 7;
+    float s = head_part(&X[row][part * 32], w5 + part * 32, 4);
+    s = head_reduce8(s);
+    if (part == 0 && r0 + row < nrow)
+      pred[p0 + r0 + row] = exp((double)(s + b5) * scale + shift);
+  }
+}
+
+long long jdsg_mlp_fused_work_len(long long nrow) { return (nrow + kFusedRows - 1) / kFusedRows + 1; }
+
+void jdsg_launch_mlp_fused(long long e0, long long nrow, int ncol, const long long *ent_off,
+                           const int *per_off, const double *periods, const int *nmode,
+                           const float *acol, int lda, const float *ext, const uint16_t *W2,
+                           const uint16_t *W3, const uint16_t *W4, const float *b2,
+                           const float *b3, const float *b4, const uint16_t *w5, float b5,
+                           double scale, double shift, double *pred, long long p0, int *tcol,
+                           hipStream_t st) {
+  if (nrow <= 0) return;
+  constexpr int FR = kFusedRows;
+  const int ntile = (int)((nrow + FR - 1) / FR);
+  hipLaunchKernelGGL(jdsg_mlp_tile_cols<FR>, dim3((ntile + 256) / 256), dim3(256), 0, st, e0,
+                     nrow, ncol, ent_off, ntile, tcol);
+  hipLaunchKernelGGL(jdsg_mlp_fused<FR>, dim3((unsigned)ntile), dim3(256), 0, st, e0, nrow, ncol,
+                     ent_off, per_off, periods, nmode, acol, lda, ext, W2, W3, W4, b2, b3, b4, w5,
+                     b5, scale, shift, pred, p0, tcol);
 }
 
 void jdsg_launch_mlp_expand(long long e0, long long nrow, int ncol, const long long *ent_off,

@@ -47,6 +47,11 @@ def _L():
     L.jdsg_mlp_column_rows.argtypes = [C.c_int, vp, vp, vp]
     L.jdsg_plan_mlp_expand.argtypes = [vp, vp, C.c_longlong, C.c_longlong, vp, C.c_int, vp,
                                        C.c_int, vp, vp]
+    L.jdsg_plan_mlp_fused.argtypes = [vp, vp, C.c_longlong, C.c_longlong, vp, C.c_int] + \
+        [vp] * 7 + [vp, C.c_float, C.c_double, C.c_double, vp, C.c_longlong, vp, C.c_longlong,
+                    vp]
+    L.jdsg_mlp_fused_work.argtypes = [C.c_longlong]
+    L.jdsg_mlp_fused_work.restype = C.c_longlong
     L.jdsg_plan_enable_surrogate.argtypes = [vp]
     L.jdsg_plan_execute_surrogate.argtypes = [vp] * 7 + [C.c_double] + [vp] * 12
     L.jdsg_plan_surrogate_stats.argtypes = [vp, C.POINTER(C.c_longlong),
@@ -372,18 +377,33 @@ def first_layer_parts(mlp, feats, stream=None):
     return acol, ext
 
 
-def predict(mlp, feats, rows=None, e0=0, n=None, chunk=1 << 20, split=True):
+def predict(mlp, feats, rows=None, e0=0, n=None, chunk=1 << 20, split=True, fused=None):
     """Predicted flat c for entries (rows, or e0..e0+n) as a float64 tensor.
     For a range of entries (split) the first layer is the per-column GEMM
-    plus the per-entry terms; for given rows it is the full GEMM of their
-    input rows."""
+    plus the per-entry terms, and with `fused` (default: for width 256) the
+    whole network then runs in one kernel with the same result; for given
+    rows it is the full GEMM of their input rows."""
     import torch
     L = _L()
     if rows is not None:
         n = rows.numel()
         split = False
+    if fused is None:
+        fused = mlp.hid == 256
     out = torch.empty(n, dtype=torch.float64, device="cuda")
     b4 = float(mlp.b[4][0].item())   # bias of the output used (the rest is padding)
+    if split and fused:
+        if mlp.hid != 256:
+            raise ValueError("the fused network kernel is for hidden width 256")
+        acol, ext = first_layer_parts(mlp, feats)
+        nw = int(L.jdsg_mlp_fused_work(n))
+        work = torch.empty(max(nw, 1), dtype=torch.int32, device="cuda")
+        _chk(L.jdsg_plan_mlp_fused(feats.plan.h, _p(feats.inp[4]), e0, n, _p(acol), mlp.hid,
+                                   _p(ext), *[_p(mlp.Wbf[i]) for i in (1, 2, 3)],
+                                   *[_p(mlp.b[i]) for i in (1, 2, 3)], _p(mlp.Wbf[4]), b4,
+                                   C_SCALE, C_SHIFT, _p(out), 0, _p(work), nw, None),
+             "fused network")
+        return out
     if split:
         acol, ext = first_layer_parts(mlp, feats)
         for s in range(0, n, chunk):

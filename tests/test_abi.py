@@ -10,7 +10,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     src = open(os.path.join(ROOT, "include", "jdsg_swd.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char)\s*\*?\s*(\w+)\s*\(", src, flags=re.M)
+    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char|long long)\s*\*?\s*(\w+)\s*\(", src,
+                       flags=re.M)
     return sorted(set(names))
 
 
@@ -26,6 +27,35 @@ def test_library_exports_every_declared_symbol():
     L = ctypes.CDLL(swd.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
+
+
+def declared_arity():
+    """Parameter count of every function the header declares."""
+    src = open(os.path.join(ROOT, "include", "jdsg_swd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?(?:void|int|char|long long)\s*\*?\s*(\w+)\s*\(([^)]*)\)\s*;",
+                         src, flags=re.M):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_python_bindings_match_the_header():
+    """Every argtypes list the Python modules set has the header's parameter
+    count (a missing or extra argument would shift every later one)."""
+    from jdsurfg_amd import frechet, surrogate, swd
+    L = swd.lib()
+    surrogate._L()
+    frechet._L()
+    arity = declared_arity()
+    assert len(arity) >= len(declared_functions()) - 2
+    bad = []
+    for name, n in arity.items():
+        at = getattr(L, name).argtypes
+        if at is not None and len(at) != n:
+            bad.append((name, len(at), n))
+    assert not bad, bad
 
 
 def test_product_does_not_reach_the_oracle():

@@ -394,10 +394,15 @@ def test_split_first_layer_matches_full_rows():
     per-entry terms (ln T, wave, mode) match those from the full 192-wide
     input rows: the same bf16 products summed in another fp32 order, which
     can move a hidden value by one bf16 step.  Chunks of a size that is not
-    a multiple of 64 and a nonzero first entry exercise the row offsets."""
+    a multiple of 64 and a nonzero first entry exercise the row offsets;
+    some columns ask for six Rayleigh modes.
+    The fused network kernel (activations in LDS) gives the same bits as the
+    unfused kernels, tail rows of its last 128-row tile included."""
     import torch
     from jdsurfg_amd import swd, surrogate as S
     w = W.grid128().subset(np.arange(0, 16384, 37))
+    w.nmode = w.nmode.copy()
+    w.nmode[::3] = (6, 2)    # modes above 3 have no one-hot input
     print(f"case split first layer, grid128 subset ncol={w.ncol}", flush=True)
     layout = "grid" if w.grid is not None else "csr"
     plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
@@ -407,8 +412,22 @@ def test_split_first_layer_matches_full_rows():
     e0 = 5
     n = plan.nent - e0
     full = S.predict(mlp, feats, e0=e0, n=n, split=False)
-    split = S.predict(mlp, feats, e0=e0, n=n, chunk=4001)
+    split = S.predict(mlp, feats, e0=e0, n=n, chunk=4001, fused=False)
+    fused = S.predict(mlp, feats, e0=e0, n=n)
     torch.cuda.synchronize()
+    print(f"fused == unfused: {int((fused == split).sum())} of {n}", flush=True)
+    assert torch.equal(fused, split)
+    # a workspace shorter than the tiles need is refused before any launch
+    L = S._L()
+    acol, ext = S.first_layer_parts(mlp, feats)
+    nw = int(L.jdsg_mlp_fused_work(n))
+    work = torch.empty(nw, dtype=torch.int32, device="cuda")
+    with pytest.raises(RuntimeError, match="workspace"):
+        S._chk(L.jdsg_plan_mlp_fused(plan.h, S._p(inp[4]), e0, n, S._p(acol), 256, S._p(ext),
+                                     *[S._p(mlp.Wbf[i]) for i in (1, 2, 3)],
+                                     *[S._p(mlp.b[i]) for i in (1, 2, 3)], S._p(mlp.Wbf[4]),
+                                     0.0, S.C_SCALE, S.C_SHIFT, S._p(fused), 0, S._p(work),
+                                     nw - 1, None), "fused network")
     rel = ((split - full).abs() / full).cpu().numpy()
     print(f"split vs full rows: max {rel.max():.3e} median {np.median(rel):.3e}", flush=True)
     assert np.all(np.isfinite(split.cpu().numpy()))
