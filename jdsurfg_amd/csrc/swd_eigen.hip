@@ -25,6 +25,19 @@ namespace {
 
 constexpr double kPiF = (double)3.1415926535898f;  // REAL literal
 
+// 1/x from the hardware reciprocal estimate refined by two Newton steps:
+// within about an ulp of the IEEE quotient at a little over half its
+// instructions (an IEEE FP64 division is about ten).  For the per-layer
+// quotients of the sweeps, whose denominators are layer constants or
+// vertical wavenumbers: finite and nonzero where the result is used.
+JDSG_DEV double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 JDSG_DEV double powm5d(double x) {
   double x2 = x * x, x4 = x2 * x2, x5 = x4 * x;
   return 1.0 / x5;
@@ -79,12 +92,12 @@ JDSG_DEV double norm2(cplx z) { return z.re * z.re + z.im * z.im; }
 
 JDSG_DEV LayerWaves layer_waves(const Layer &L, double omega, double wvno2) {
   LayerWaves w;
-  double xka = omega / L.za;
-  double xkb = (L.zb > 0.0) ? omega / L.zb : 0.0;
+  double xka = omega * frcp(L.za);
+  double xkb = (L.zb > 0.0) ? omega * frcp(L.zb) : 0.0;
   w.ra = csqrt_real(wvno2 - xka * xka);
   w.rb = csqrt_real(wvno2 - xkb * xkb);
-  w.ia = 1.0 / ((w.ra.im != 0.0) ? w.ra.im : w.ra.re);
-  w.ib = 1.0 / ((w.rb.im != 0.0) ? w.rb.im : w.rb.re);
+  w.ia = frcp((w.ra.im != 0.0) ? w.ra.im : w.ra.re);
+  w.ib = frcp((w.rb.im != 0.0) ? w.rb.im : w.rb.re);
   w.pr = w.ra.re * L.zd;
   w.pi = w.ra.im * L.zd;
   w.qr = w.rb.re * L.zd;
@@ -191,7 +204,7 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
     ca[2][2] = dfac;
     ca[0][0] = cosp;
     ca[4][4] = cosp;
-    const double irom = 1.0 / ((double)rho * om2);
+    const double irom = frcp((double)rho * om2);
     ca[0][1] = -rsinp * irom;
     ca[1][0] = -(double)rho * sinpr * om2;
     ca[1][1] = cosp;
@@ -222,7 +235,7 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
   double g2wy2 = gamm2 * wy2;
   double g1wy2 = gamm1 * wy2;
   double rom = (double)rho * om2;
-  const double irom = 1.0 / rom;
+  const double irom = frcp(rom);
   double temp = a0c * gg1 + g2xz2 + g2wy2;
   ca[2][2] = a0 + temp + temp;
   ca[0][0] = cpcq - temp;
@@ -231,7 +244,7 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
   ca[0][2] = wvno * temp * irom;
   ca[0][3] = (-cqww2 + cpz) * irom;
   temp = wvno2 * (a0c + wy2) + xz;
-  ca[0][4] = -temp / ((double)rho2 * om2 * om2);
+  ca[0][4] = -temp * frcp((double)rho2 * om2 * om2);
   ca[1][0] = (-gamm2 * cqw + gam2 * cpz * K.ik2) * (double)rho * om2;
   ca[1][1] = cpcq;
   ca[1][2] = (gamm1 * cqww2 - gam * cpz) * K.ik;
@@ -271,7 +284,7 @@ JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, dou
     AA[3][3] = dfac;
     AA[1][1] = cosp;
     AA[2][2] = cosp;
-    AA[1][2] = -rsinp * (1.0 / ((double)rho * om2));
+    AA[1][2] = -rsinp * frcp((double)rho * om2);
     AA[2][1] = -(double)rho * om2 * sinpr;
     return;
   }
@@ -284,7 +297,7 @@ JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, dou
   double gam = (double)bb2 * K.kw;
   double gamm1 = gam - 1.0;
   double rom = (double)rho * om2;
-  const double irom = 1.0 / rom;
+  const double irom = frcp(rom);
   AA[0][0] = cossv + gam * (cosp - cossv);
   AA[0][1] = -wvno * gamm1 * sinpr + gam * rsinsv * K.ik;
   AA[0][2] = -wvno * (cosp - cossv) * irom;
@@ -524,10 +537,10 @@ JDSG_DEV void solid_ints(const Layer &L, const LayerWaves &w, const U4 &u0, cons
   const double n1 = norm2(sab), n2 = norm2(dab);
   const cplx H1 = (n1 < 1.0e-16) ? cmk(d, 0.0)
                   : ((1.0 - ((sab.re * d < 40.0) ? w.Ea * w.Eb : zero)) * cconj(sab)) *
-                        (1.0 / n1);
+                        frcp(n1);
   const cplx H2 = (n2 < 1.0e-16) ? cmk(d, 0.0)
                   : ((((w.qr < 40.0) ? w.Eb : zero) - ((w.pr < 40.0) ? w.Ea : zero)) *
-                     cconj(dab)) * (1.0 / n2);
+                     cconj(dab)) * frcp(n2);
   // the six quadratic forms of M
   const cplx A2 = kmpu * kmpu + km1pd * km1pd, X = kmpu * km1pd;
   const cplx B2 = kmsu * kmsu + km1sd * km1sd, Y = kmsu * km1sd;
@@ -597,7 +610,7 @@ JDSG_DEV void layer_energy(const Layer &L, const LayerWaves *lw, int typelyr, co
   }
   double TL = L.zrho * L.zb * L.zb, TN = TL;
   double TF = TA - 2. * TN;
-  double a12 = -wvno, a14 = 1.0 / TL, a23 = 1.0 / TC, a21 = wvno * TF * a23;
+  double a12 = -wvno, a14 = frcp(TL), a23 = frcp(TC), a21 = wvno * TF * a23;
   const double irho = a23 * (L.za * L.za);
   double INT11, INT13, INT22, INT24, INT33, INT44;
   if (typelyr == 0) {
@@ -943,7 +956,7 @@ struct VarL {
 };
 JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
   VarL v;
-  v.xkb = omega / L.zb;
+  v.xkb = omega * frcp(L.zb);
   double wvnop = wvno + v.xkb;
   double wvnom = fabs(wvno - v.xkb);
   v.rb = sqrt(wvnop * wvnom);
@@ -951,7 +964,7 @@ JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
   v.mu = L.zrho * L.zb * L.zb;
   v.eexl = 0.0;
   v.sinq = 0.0;
-  v.irb = (v.rb > 0.0) ? 1.0 / v.rb : 0.0;
+  v.irb = (v.rb > 0.0) ? frcp(v.rb) : 0.0;
   if (wvno < v.xkb) {
     sincos(q, &v.sinq, &v.cosq);
     v.yl = v.sinq * v.irb;
