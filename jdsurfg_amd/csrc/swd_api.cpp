@@ -159,10 +159,12 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
   P->nchain = (long long)ncol * 2 * P->nvar;
   int dw = default_waves();
   P->chain_waves = (int)std::min<long long>(dw, ((long long)ncol * P->nvar + 63) / 64);
-  // eigen chunks of 32 waves per CU: each pass then runs several rounds of
-  // waves per SIMD, so the last round's tail is a small part of it
+  // eigen chunks of 64 waves per CU: each pass then runs eight rounds of two
+  // waves per SIMD, so the last round's tail is a small part of it (measured
+  // on 256x256 columns x 60 periods: 32 waves per CU 201 ms, 64 194 ms, 128
+  // 194 ms, 256 197 ms for the Rayleigh stage; lane scratch 9.9 GB at 64)
   P->eigen_waves =
-      (int)std::min<long long>(2LL * dw, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
+      (int)std::min<long long>(4LL * dw, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
   P->chain_waves = std::max(1, P->chain_waves);
   P->eigen_waves = std::max(1, P->eigen_waves);
   // device descriptors
