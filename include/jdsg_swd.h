@@ -266,6 +266,13 @@ int jdsg_plan_surrogate_roots(const jdsg_plan *plan, double *known, signed char 
  * equation evaluations of its bracket/polish and chain stages. */
 int jdsg_plan_surrogate_stats(const jdsg_plan *plan, long long *nfail, long long *polish_evals,
                               long long *chain_evals);
+/* Long chain scans the last surrogate execute ran as parallel scan jobs: in
+ * its fallback a chain whose upward dc-step scan (surfdisp96.f getsol) has
+ * gone two points without a sign change is suspended, its remaining scan
+ * points (the same c + dc additions) are evaluated 64 at a time by one
+ * wavefront, and the chain resumes at the first sign change or at the end
+ * of the scan, as the sequential scan would. */
+int jdsg_plan_scan_jobs(const jdsg_plan *plan, long long *jobs);
 
 /* BF16 MFMA kernels of the surrogate network (training and inference).
  * C = op(A) op(B), A(i,k) = ta ? A[k*lda+i] : A[i*lda+k], B(k,j) = tb ?

@@ -66,6 +66,11 @@ struct jdsg_plan {
   double *brk = nullptr;
   unsigned *chain_ev = nullptr;  // per-chain evaluation counts (diagnostics)
   unsigned *sfail = nullptr;
+  // suspended long scans of the exact-root fallback
+  char *susp_saved = nullptr;
+  double *susp_cs = nullptr;
+  ScanJob *susp_jobs = nullptr;
+  unsigned *susp_njobs = nullptr;
   const double *known_in = nullptr;
   // explicit per-entry periods for the eigen stage (per-column ABI only)
   double *tv = nullptr;
@@ -731,6 +736,13 @@ extern "C" int jdsg_plan_enable_surrogate(jdsg_plan *P) {
       P->dalloc(&P->sg, 3 * (size_t)P->nvar * P->nent) ||
       P->dalloc(&P->brk, 2 * (size_t)P->nvar * P->nent) || P->dalloc(&P->sfail, 2))
     return 1;
+  // per chain of each wave type: its saved state, its own scratch, a job slot
+  // in each of two lists (C5: about 0.7 GB)
+  const size_t nch = (size_t)P->ncol * P->nvar;
+  if (P->dalloc(&P->susp_saved, 2 * nch * jdsg_chain_state_bytes()) ||
+      P->dalloc(&P->susp_cs, 2 * nch * (2 * (size_t)P->kpad + 22)) ||
+      P->dalloc(&P->susp_jobs, 4 * nch) || P->dalloc(&P->susp_njobs, 6))
+    return 1;
   return 0;
 }
 
@@ -832,7 +844,21 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     A.chain_evals = P->chain_ev;
     A.any_wave[0] = P->nitem[0] > 0;
     A.any_wave[1] = P->nitem[1] > 0;
-    jdsg_launch_chains(A, P->chain_waves, st);
+    // long upward scans (above all the walk to betmx that establishes a
+    // mode's cutoff) go to parallel scan jobs after two sequential steps, up
+    // to 64 suspensions per chain (measured on 128x128 columns x 40 periods:
+    // fallback 375 ms without, 243 ms with 8 after 8 steps, 210 ms with 64
+    // after 2; a cycle without jobs costs two near-empty launches)
+    const size_t nch = (size_t)P->ncol * P->nvar;
+    A.susp.saved = P->susp_saved;
+    A.susp.cs = P->susp_cs;
+    for (int w = 0; w < 2; ++w)
+      for (int l = 0; l < 2; ++l) A.susp.jobs[w][l] = P->susp_jobs + (size_t)(2 * w + l) * nch;
+    A.susp.njobs = P->susp_njobs;
+    A.susp.total = P->susp_njobs + 4;
+    jdsg_launch_fill_u32(A.susp.total, 2, 0u, st);
+    A.susp.min_steps = 2;
+    jdsg_launch_chains_susp(A, P->chain_waves, 64, st);
     rc = hipGetLastError() != hipSuccess;
   }
   P->known_in = nullptr;
@@ -872,6 +898,13 @@ extern "C" int jdsg_plan_surrogate_stats(const jdsg_plan *P, long long *nfail,
   if (nfail) *nfail = f[0];
   if (chain_evals) *chain_evals = (long long)ev[0];
   if (polish_evals) *polish_evals = (long long)ev[1];
+  return 0;
+}
+
+extern "C" int jdsg_plan_scan_jobs(const jdsg_plan *P, long long *jobs) {
+  unsigned t[2] = {0, 0};
+  if (P->susp_njobs) HIPCHK(hipMemcpy(t, P->susp_njobs + 4, sizeof(t), hipMemcpyDeviceToHost));
+  if (jobs) *jobs = (long long)t[0] + t[1];
   return 0;
 }
 

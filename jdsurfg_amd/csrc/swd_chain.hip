@@ -13,6 +13,9 @@
 // Arithmetic follows the Fortran: float model arrays and start values,
 // double recursion, sngl() of the returned root.  FP contraction is off so
 // the recursion rounds like the reference build.
+#include <algorithm>
+#include <type_traits>
+
 #include "swd_kernels.hpp"
 #include "swd_dltar.hpp"
 #include "swd_dltar_fast.hpp"
@@ -174,7 +177,10 @@ enum Phase : int {
   PH_NV_END,
   PH_RETRY_NEXT,
   PH_DEL1ST,
-  PH_DONE
+  PH_DONE,
+  PH_SUSPEND,   // a long upward scan handed to a scan job; the chain is saved
+  PH_SCAN_HIT,  // resumed: the scan job found the sign change
+  PH_SCAN_END   // resumed: the scan job ran to its end without one
 };
 
 struct ChainState {
@@ -201,6 +207,11 @@ struct ChainState {
   // known-root walk: start point of the mode's first period (for del1st)
   double tfirst, cfirst;
   int del1st_ok;
+};
+// state of a chain in the suspension form (exact-root fallback)
+struct ChainStateSusp : ChainState {
+  int nscan;        // evaluations of the current upward scan
+  unsigned evals;   // evaluations of this chain
 };
 
 
@@ -235,7 +246,10 @@ JDSG_DEV void begin_pass(ChainState &S, double *cs, int kpad) {
 
 // Returns true with S.ec set when a period-equation value is needed at
 // phase velocity S.ec (period S.t1); false when the chain is complete.
-__device__ __attribute__((noinline)) bool advance(ChainState &S, const ChainArgs &A, double *cs, double F) {
+template <bool SUSP>
+__device__ __attribute__((noinline)) bool advance(
+    typename std::conditional<SUSP, ChainStateSusp, ChainState>::type &S, const ChainArgs &A,
+    double *cs, double F) {
   const int kpad = A.kmax_pad;
   for (;;) {
     switch (S.phase) {
@@ -324,6 +338,7 @@ __device__ __attribute__((noinline)) bool advance(ChainState &S, const ChainArgs
       }
       case PH_GS_FIRST: {
         S.del1 = F;
+        if constexpr (SUSP) S.nscan = 0;
         if (S.ifirst == 1) {
           S.del1st = S.del1;
           S.del1st_ok = 1;
@@ -334,6 +349,16 @@ __device__ __attribute__((noinline)) bool advance(ChainState &S, const ChainArgs
         continue;
       }
       case PH_SCAN_NEXT: {
+        // exact-root fallback: an upward scan that has gone min_steps points
+        // without a sign change and has more than 64 to go before betmx is
+        // handed to a scan job, which evaluates the same points side by side
+        if constexpr (SUSP) {
+          if (A.susp.allow && S.idir > 0 && S.nscan >= A.susp.min_steps && S.c1 >= S.clow &&
+              S.c1 >= S.cm && ((double)S.betmx + S.dc) - S.c1 > 64.0 * S.dc) {
+            S.phase = PH_SUSPEND;
+            return false;
+          }
+        }
         S.c2 = (S.idir > 0) ? S.c1 + S.dc : S.c1 - S.dc;
         if (S.c2 <= S.clow) {
           S.idir = 1;
@@ -346,6 +371,7 @@ __device__ __attribute__((noinline)) bool advance(ChainState &S, const ChainArgs
       }
       case PH_GS_SCAN: {
         S.del2 = F;
+        if constexpr (SUSP) S.nscan++;
         if (sign1(S.del1) != sign1(S.del2)) {
           S.c3 = 0.5 * (S.c1 + S.c2);
           S.ec = S.c3;
@@ -358,6 +384,18 @@ __device__ __attribute__((noinline)) bool advance(ChainState &S, const ChainArgs
         S.phase = PH_SCAN_NEXT;
         continue;
       }
+      case PH_SCAN_HIT:  // as PH_GS_SCAN on a sign change between c1 and c2
+        if constexpr (SUSP) {
+          S.c3 = 0.5 * (S.c1 + S.c2);
+          S.ec = S.c3;
+          S.phase = PH_NV_INIT;
+          return true;
+        } else {
+          return false;
+        }
+      case PH_SCAN_END:  // as PH_GS_SCAN past betmx: c1 is the scan's last point
+        if constexpr (SUSP) goto getsol_fail;
+        else return false;
       case PH_NV_INIT: {
         S.del3 = F;
         S.nev = 1;
@@ -623,7 +661,7 @@ jdsg_root_chains(ChainArgs A) {
     g = nlanes + (long long)atomicAdd(counter, 1u);
   }
   double F = 0.0;
-  bool need = live ? advance(S, A, cs, F) : false;
+  bool need = live ? advance<false>(S, A, cs, F) : false;
   const float4 *base = (W == 0) ? A.cm.chainR : A.cm.chainL;
   while (live) {
     if (need) {
@@ -639,7 +677,7 @@ jdsg_root_chains(ChainArgs A) {
                      : dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       ++nev;
       ++cev;
-      need = advance(S, A, cs, F);
+      need = advance<false>(S, A, cs, F);
       continue;
     }
     finish_chain(S, A, g, W);
@@ -652,15 +690,198 @@ jdsg_root_chains(ChainArgs A) {
       if (setup_chain(S, A, g, cs, W)) { live = true; break; }
       if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
     }
-    if (live) need = advance(S, A, cs, 0.0);
+    if (live) need = advance<false>(S, A, cs, 0.0);
   }
   if (A.nevals && nev) atomicAdd(A.nevals, nev);
+}
+
+// The exact-root fallback form of jdsg_root_chains<W, true>: a chain that enters a long upward scan
+// is saved and leaves a scan job (PH_SUSPEND); a later launch resumes the
+// chains of a job list from their saved state once jdsg_scan_jobs has run
+// the scans.  Each chain then keeps its scratch in a slot of its own.
+template <int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
+jdsg_root_chains_susp(ChainArgs A) {
+  const long long nlanes = (long long)gridDim.x * blockDim.x;
+  const long long lane_id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long cstride = 2 * A.kmax_pad + 22;
+  double *cs = A.cscr + lane_id * cstride;
+  unsigned *counter = A.counter + W;
+  const long long nch = (long long)A.cm.ncol * A.nvar;
+  // work items: every chain, or the chains of a job list to resume
+  const ScanJob *jin = nullptr;
+  long long nitems = nch;
+  if (A.susp.list_in >= 0) {
+    jin = A.susp.jobs[W][A.susp.list_in];
+    nitems = A.susp.njobs[W * 2 + A.susp.list_in];
+  }
+  ChainStateSusp *saved = reinterpret_cast<ChainStateSusp *>(A.susp.saved) + (long long)W * nch;
+  unsigned long long nev = 0;
+  ChainStateSusp S;
+  long long g = 0;
+  // set up work item i in S (and cs); false when it has no work
+  auto start_item = [&](long long i) -> bool {
+    if (jin) {
+      const ScanJob &J = jin[i];
+      g = J.g;
+      cs = A.susp.cs + ((long long)W * nch + g) * cstride;
+      S = saved[g];
+      if (J.found) {
+        S.c1 = J.c_prev;
+        S.del1 = J.f_prev;
+        S.c2 = J.c_hit;
+        S.del2 = J.f_hit;
+        S.phase = PH_SCAN_HIT;
+      } else {
+        S.c1 = J.c_hit;
+        S.del1 = J.f_hit;
+        S.phase = PH_SCAN_END;
+      }
+      S.evals += J.evals;
+      return true;
+    }
+    g = i;
+    cs = A.susp.cs + ((long long)W * nch + g) * cstride;
+    if (setup_chain(S, A, g, cs, W)) {
+      S.nscan = 0;
+      S.evals = 0;
+      return true;
+    }
+    if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
+    return false;
+  };
+  long long item = lane_id;
+  bool live = false;
+  while (item < nitems) {
+    if (start_item(item)) { live = true; break; }
+    item = nlanes + (long long)atomicAdd(counter, 1u);
+  }
+  double F = 0.0;
+  bool need = live ? advance<true>(S, A, cs, F) : false;
+  const float4 *base = (W == 0) ? A.cm.chainR : A.cm.chainL;
+  while (live) {
+    if (need) {
+      const float4 *st = base + S.col;
+      int mmax = A.cm.nlay[S.col];
+      int llw = A.cm.llw[S.col];
+      double wvno = S.omega / S.ec;
+      F = (W == 0) ? jdsg_pf::dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega)
+                   : jdsg_pf::dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
+      ++nev;
+      ++S.evals;
+      need = advance<true>(S, A, cs, F);
+      continue;
+    }
+    if (S.phase == PH_SUSPEND) {
+      // at most one job per chain and list, so the slot stays below nch
+      const unsigned slot = atomicAdd(&A.susp.njobs[W * 2 + A.susp.list_out], 1u);
+      atomicAdd(&A.susp.total[W], 1u);
+      ScanJob J;
+      J.g = g;
+      J.c0 = S.c1;
+      J.f0 = S.del1;
+      J.omega = S.omega;
+      J.bound = (double)S.betmx + S.dc;
+      J.dc = S.dc;
+      J.col = S.col;
+      J.found = 0;
+      J.evals = 0;
+      J.c_prev = J.f_prev = J.c_hit = J.f_hit = 0.0;
+      A.susp.jobs[W][A.susp.list_out][slot] = J;
+      saved[g] = S;
+    } else {
+      finish_chain(S, A, g, W);
+      if (A.chain_evals) A.chain_evals[(long long)W * nch + g] = S.evals;
+    }
+    live = false;
+    for (;;) {
+      item = nlanes + (long long)atomicAdd(counter, 1u);
+      if (item >= nitems) break;
+      if (start_item(item)) { live = true; break; }
+    }
+    if (live) need = advance<true>(S, A, cs, 0.0);
+  }
+  if (A.nevals && nev) atomicAdd(A.nevals, nev);
+}
+
+// Scan jobs of suspended chains, one wave per job: each round the 64 lanes
+// take the next 64 points of the chain's scan (lane l adds dc l+1 times to
+// the round's base, the additions the chain itself would make), evaluate the
+// period equation where the chain would (its previous point still below the
+// bound), and the first lane with a sign change against its predecessor, or
+// the first point past the bound, ends the scan -- the chain's own outcome.
+// It also clears, for the chain launch that follows, the work counter and
+// the count of the other list (consumed by the previous chain launch).
+template <int W>
+__global__ void __launch_bounds__(64) jdsg_scan_jobs(ChainArgs A, int list) {
+  ScanJob *jobs = A.susp.jobs[W][list];
+  const unsigned n = A.susp.njobs[W * 2 + list];
+  const int lane = threadIdx.x;
+  if (blockIdx.x == 0 && lane == 0) {
+    A.counter[W] = 0u;
+    A.susp.njobs[W * 2 + (list ^ 1)] = 0u;
+  }
+  const float4 *base = (W == 0) ? A.cm.chainR : A.cm.chainL;
+  unsigned long long nev = 0;
+  for (unsigned j = blockIdx.x; j < n; j += gridDim.x) {
+    ScanJob J = jobs[j];
+    const float4 *st = base + J.col;
+    const int mmax = A.cm.nlay[J.col], llw = A.cm.llw[J.col];
+    double cb = J.c0, fb = J.f0;
+    unsigned evals = 0;
+    for (int round = 0;; ++round) {
+      double c = cb;
+      for (int s = 0; s <= lane; ++s) c = c + J.dc;
+      double cprev = __shfl_up(c, 1);
+      if (lane == 0) cprev = cb;
+      const bool eval = cprev < J.bound;
+      double F = 0.0;
+      if (eval) {
+        const double wvno = J.omega / c;
+        F = (W == 0) ? jdsg_pf::dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, J.omega)
+                     : jdsg_pf::dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, J.omega);
+      }
+      double fprev = __shfl_up(F, 1);
+      if (lane == 0) fprev = fb;
+      const bool change = eval && sign1(F) != sign1(fprev);
+      const bool end = eval && !change && c >= J.bound;
+      const unsigned long long me = __ballot(eval);
+      evals += (unsigned)__popcll(me);
+      const unsigned long long ev = __ballot(change) | __ballot(end);
+      if (ev) {
+        const int first = __ffsll((long long)ev) - 1;
+        const double cp = __shfl(cprev, first), fp = __shfl(fprev, first);
+        const double ch = __shfl(c, first), fh = __shfl(F, first);
+        const bool found = (__ballot(change) >> first) & 1ull;
+        if (lane == 0) {
+          J.found = found ? 1 : 0;
+          J.c_prev = cp;
+          J.f_prev = fp;
+          J.c_hit = ch;
+          J.f_hit = fh;
+          J.evals = evals;
+          jobs[j] = J;
+        }
+        break;
+      }
+      // all 64 points inside and no sign change: the next round starts at
+      // the last of them
+      cb = __shfl(c, 63);
+      fb = __shfl(F, 63);
+    }
+    nev += evals;
+  }
+  if (lane == 0 && A.nevals && nev) atomicAdd(A.nevals, nev);
 }
 
 template __global__ void jdsg_root_chains<0, false>(ChainArgs);
 template __global__ void jdsg_root_chains<1, false>(ChainArgs);
 template __global__ void jdsg_root_chains<0, true>(ChainArgs);
 template __global__ void jdsg_root_chains<1, true>(ChainArgs);
+template __global__ void jdsg_root_chains_susp<0>(ChainArgs);
+template __global__ void jdsg_root_chains_susp<1>(ChainArgs);
+
+size_t jdsg_chain_state_bytes() { return sizeof(ChainStateSusp); }
 
 // ----------------------------------------------------------------------------
 // host launchers
@@ -687,6 +908,42 @@ void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st) {
       hipLaunchKernelGGL((jdsg_root_chains<1, true>), dim3(nwaves), dim3(64), 0, st, A);
     else
       hipLaunchKernelGGL((jdsg_root_chains<1, false>), dim3(nwaves), dim3(64), 0, st, A);
+  }
+}
+
+void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st) {
+  A.susp_on = 1;
+  const long long nch = (long long)A.cm.ncol * A.nvar;
+  const unsigned scan_blocks = (unsigned)std::min<long long>(nch, 2048);
+  // after a few cycles only the long chains are left: a smaller grid keeps
+  // the near-empty resume launches cheap (lanes still take every job)
+  const int tail_waves = std::max(64, nwaves / 8);
+  for (int w = 0; w < 2; ++w) {
+    if (!A.any_wave[w]) continue;
+    A.susp.list_in = -1;
+    A.susp.list_out = 0;
+    A.susp.allow = cycles > 0;
+    jdsg_launch_fill_u32(A.counter + w, 1, 0u, st);
+    jdsg_launch_fill_u32(A.susp.njobs + 2 * w, 2, 0u, st);
+    if (w == 0)
+      hipLaunchKernelGGL((jdsg_root_chains_susp<0>), dim3(nwaves), dim3(64), 0, st, A);
+    else
+      hipLaunchKernelGGL((jdsg_root_chains_susp<1>), dim3(nwaves), dim3(64), 0, st, A);
+    for (int c = 0; c < cycles; ++c) {
+      const int lin = c & 1, lout = lin ^ 1;
+      if (w == 0)
+        hipLaunchKernelGGL((jdsg_scan_jobs<0>), dim3(scan_blocks), dim3(64), 0, st, A, lin);
+      else
+        hipLaunchKernelGGL((jdsg_scan_jobs<1>), dim3(scan_blocks), dim3(64), 0, st, A, lin);
+      A.susp.list_in = lin;
+      A.susp.list_out = lout;
+      A.susp.allow = c + 1 < cycles;  // the last pass finishes every chain
+      const int gw = c < 4 ? nwaves : std::min(nwaves, tail_waves);
+      if (w == 0)
+        hipLaunchKernelGGL((jdsg_root_chains_susp<0>), dim3(gw), dim3(64), 0, st, A);
+      else
+        hipLaunchKernelGGL((jdsg_root_chains_susp<1>), dim3(gw), dim3(64), 0, st, A);
+    }
   }
 }
 

@@ -10,6 +10,31 @@ struct PrepOut {
   double *zd, *za, *zb, *rho0, *rtpR, *rtpL, *vtp, *dtp;
 };
 
+// A long upward dc-step scan of a suspended chain (exact-root fallback):
+// the chain's next points are c0 + dc, (c0 + dc) + dc, ... (the same
+// additions the chain makes), the scan stops at the first sign change
+// against the previous point or after the first point >= bound.
+struct ScanJob {
+  long long g;               // chain (per wave type)
+  double c0, f0;             // the last point the chain evaluated and its value
+  double omega, bound, dc;
+  int col, found;            // found: a sign change at c_hit
+  unsigned evals;
+  double c_prev, f_prev;     // the point before c_hit and its value (found)
+  double c_hit, f_hit;       // the sign-change point, or the end point (not found)
+};
+struct ChainSusp {
+  void *saved;               // [2 * nch] chain states, wave-major
+  double *cs;                // [2 * nch * (2 kmax_pad + 22)] per-chain scratch
+  ScanJob *jobs[2][2];       // [wave][list] -> [nch]
+  unsigned *njobs;           // [wave * 2 + list]
+  unsigned *total;           // [wave] scan jobs posted (statistics)
+  int list_in;               // -1: start every chain; else resume the chains of this list
+  int list_out;              // list that receives new suspensions
+  int allow;                 // suspend long scans in this launch
+  int min_steps;             // sequential scan steps before a scan may suspend
+};
+
 struct ChainArgs {
   ColumnModel cm;
   const long long *ent_off;   // [ncol+1]
@@ -32,6 +57,8 @@ struct ChainArgs {
   int any_wave[2];            // some column asks for Rayleigh / Love modes
   long long nchain;           // ncol * 2 * nvar
   unsigned *chain_evals;  // optional [2*ncol*nvar] evaluations per chain
+  int susp_on;            // exact-root fallback: long scans run as parallel scan jobs
+  ChainSusp susp;
 };
 
 struct EigenArgs {
@@ -112,6 +139,10 @@ void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float 
                       const float *vp, const float *vs, const float *rho, int iflsph,
                       const PrepOut &o, hipStream_t st);
 void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st);
+// exact-root fallback with long scans suspended into parallel scan jobs for
+// up to `cycles` rounds (A.susp buffers set; A.susp_on is set here)
+void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st);
+size_t jdsg_chain_state_bytes();
 void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st);
 void jdsg_launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t st);
 void jdsg_launch_grid_to_layers(int col0, int nchunk, int nxy, int nz, int N, int water,

@@ -53,6 +53,7 @@ def _L():
     L.jdsg_mlp_fused_work.argtypes = [C.c_longlong]
     L.jdsg_mlp_fused_work.restype = C.c_longlong
     L.jdsg_plan_enable_surrogate.argtypes = [vp]
+    L.jdsg_plan_scan_jobs.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.jdsg_plan_execute_surrogate.argtypes = [vp] * 7 + [C.c_double] + [vp] * 12
     L.jdsg_plan_surrogate_stats.argtypes = [vp, C.POINTER(C.c_longlong),
                                             C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
@@ -487,7 +488,10 @@ def surrogate_stats(plan):
     L = _L()
     f, pe, ce = C.c_longlong(), C.c_longlong(), C.c_longlong()
     _chk(L.jdsg_plan_surrogate_stats(plan.h, C.byref(f), C.byref(pe), C.byref(ce)), "stats")
-    return {"roots_left_to_chain": f.value, "polish_evals": pe.value, "chain_evals": ce.value}
+    sj = C.c_longlong()
+    _chk(L.jdsg_plan_scan_jobs(plan.h, C.byref(sj)), "scan jobs")
+    return {"roots_left_to_chain": f.value, "polish_evals": pe.value, "chain_evals": ce.value,
+            "scan_jobs": sj.value}
 
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",

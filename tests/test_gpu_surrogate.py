@@ -362,9 +362,13 @@ def test_trained_network_gain():
     same = np.all(np.abs(fa - fb) <= 1e-9 * np.abs(fa), axis=0) & (sa == sb)
     print(f"plain evaluations {plain_evals}, surrogate {sur_evals} "
           f"({sur_evals / plain_evals:.3f}); fallback {st['roots_left_to_chain']} of "
-          f"{3 * plan.nent}; entries differing {int((~same).sum())} of {plan.nent}", flush=True)
+          f"{3 * plan.nent}; entries differing {int((~same).sum())} of {plan.nent}; "
+          f"scan jobs {st['scan_jobs']}", flush=True)
     assert sur_evals < 0.5 * plain_evals
     assert (~same).sum() <= 1e-3 * plan.nent
+    # the fallback ran its long scans (modes past cutoff walk to betmx) as
+    # parallel scan jobs, and the roots above still equal the plain chain's
+    assert st["scan_jobs"] > 0
 
 
 def test_output_head_matches_last_gemm():
