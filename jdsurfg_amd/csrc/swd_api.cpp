@@ -772,7 +772,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   Q.known = P->known;
   Q.sg = P->sg;
   Q.brk = P->brk;
-  Q.nfail = P->sfail;
+  Q.nfail = P->sfail + 1;  // failure events; sfail[0] counts the roots left open at the end
   Q.nevals = P->nevals + 1;
   Q.grid_K = P->uK;
   Q.grid_nR = P->unR;
@@ -803,7 +803,17 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     Q.half_width = Q.hw[v];
     jdsg_launch_mode_order(Q, P->kmax, st);
   };
+  // open T entries next to a kept root of the same mode, continued from it
+  // (the T(1 +- 0.005) brackets then start from those roots through
+  // pm_centers; continuing those variants as well changed statuses where a
+  // T(1 +- 0.005) chain misses a root near cutoff)
+  auto cont = [&](int v) {
+    for (int w = 0; w < 2; ++w)
+      if (P->nitem[w] > 0) jdsg_launch_continue(Q, w, v, JDSG_MAXMODE, st);
+  };
   polish(0, 1);
+  order(0);
+  cont(0);
   order(0);
   if (P->group) {
     jdsg_launch_pm_centers(P->ent_off_d, P->per_off_d, periods, P->ncol, P->nent, P->known, pred,
@@ -812,6 +822,8 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     order(1);
     order(2);
   }
+  jdsg_launch_fill_u32(P->sfail, 1, 0u, st);
+  jdsg_launch_count_open(P->known, (long long)P->nvar * P->nent, P->sfail, st);
   HIPCHK(hipGetLastError());
   if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[1], st));
   // the chains walk in reference order and solve only the roots left open

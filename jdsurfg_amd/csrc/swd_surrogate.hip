@@ -16,6 +16,8 @@
 // each chain in the reference order and solves only those (the `known` input
 // of the chain kernel).  Every lane works on an independent root: no
 // sequential chain.
+#include <algorithm>
+
 #include "swd_kernels.hpp"
 #include "swd_dltar_fast.hpp"
 
@@ -302,6 +304,108 @@ __global__ void __launch_bounds__(64) jdsg_polish_w(PolishArgs A) {
   }
 }
 
+// Continuation of open roots from their polished neighbours, one lane per
+// (column, mode) of one wave type and variant.  An entry left open (its
+// prediction was off, or a check failed) whose neighbouring period's root of
+// the same mode is known is bracketed around that root moved by the
+// network's ratio between the two periods -- the chain's own start, the
+// previous period's root -- and polished like any other entry; the lane
+// walks each run of open entries up the periods, then down, so a run is
+// continued entry by entry.  Only entries whose own prediction lies with
+// its whole widened bracket below betmx are continued (where the network
+// sees no root, or one near the cut-off, the chain may miss one that exists,
+// and a continued root would then differ from the chain's), and a continued
+// root is kept only within 5 % of that prediction.
+// The mode-order pass checks these roots as it checks the others.  Mode 0
+// reuses the floor sign of the first polish.
+template <int W>
+JDSG_DEV bool continue_begin(const PolishArgs &A, PLane &S, long long e, int col, int mode, int k,
+                             int v, double cen) {
+  double T = A.periods[A.per_off[col] + k];
+  if (v == 1) T = T * (1.0 + 0.5 * 0.01);
+  if (v == 2) T = T * (1.0 - 0.5 * 0.01);
+  S.e = e;
+  S.col = col;
+  S.variant = v;
+  S.omega = kTwoPi / T;
+  S.cm = (double)A.cm.cc[col];
+  S.betmx = (double)A.cm.betmx[col];
+  S.nit = 0;
+  S.cp = 0.0;
+  S.s_lo = 0;
+  if (!(cen > 0.0 && cen * (1.0 - 3.0 * A.hw[v]) < S.betmx)) return false;
+  bracket_ends(cen, A.hw[v], S.cm, &S.a, &S.b);
+  bracket_ends(cen, 3.0 * A.hw[v], S.cm, &S.a2, &S.b2);
+  S.lo = S.a;
+  S.hi = S.b;
+  const long long q = (long long)v * A.nent + e;
+  S.s_floor = (mode == 0) ? A.sg[q * 3 + 2] : 0;
+  if (mode == 0 && S.s_floor == 0) return false;  // no floor sign from the first pass
+  S.phase = PP_LO;
+  S.c = S.a;
+  return true;
+}
+
+template <int W>
+__global__ void __launch_bounds__(64) jdsg_continue_w(PolishArgs A, int v, int mmax_modes) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncol = A.cm.ncol;
+  if (t >= (long long)ncol * mmax_modes) return;
+  const int col = (int)(t / mmax_modes), m = (int)(t % mmax_modes);
+  const int nR = A.nmode[2 * col], nL = A.nmode[2 * col + 1];
+  if (m >= (W == 0 ? nR : nL)) return;
+  const int K = A.per_off[col + 1] - A.per_off[col];
+  const long long e0 = A.ent_off[col] + (long long)((W == 0 ? 0 : nR) + m) * K;
+  const double *known = A.known + (long long)v * A.nent;
+  const double *pred = A.centers[0];
+  const float4 *st = (W == 0 ? A.cm.chainR : A.cm.chainL) + col;
+  const int mmax = A.cm.nlay[col], llw = A.cm.llw[col], stride = A.cm.ncol_pad;
+  unsigned long long nev = 0;
+  unsigned nf = 0;
+  for (int dir = 0; dir < 2; ++dir) {
+    for (int i = 1; i < K; ++i) {
+      const int k = dir == 0 ? i : K - 1 - i;
+      const int kn = dir == 0 ? k - 1 : k + 1;
+      const double cn = known[e0 + kn];
+      if (known[e0 + k] != 0.0 || !(cn > 0.0)) continue;
+      const double p1 = pred[e0 + k], p0 = pred[e0 + kn];
+      if (!(p1 > 0.0 && p0 > 0.0) || p1 * (1.0 + 3.0 * A.hw[0]) >= (double)A.cm.betmx[col])
+        continue;
+      const double cen = cn * (p1 / p0);
+      PLane S;
+      if (!continue_begin<W>(A, S, e0 + k, col, m, k, v, cen)) continue;
+      for (;;) {
+        const double wv = S.omega / S.c;
+        const double F = (W == 0) ? jdsg_pf::dltar4(st, stride, mmax, llw, wv, S.omega)
+                                  : jdsg_pf::dltar1(st, stride, mmax, llw, wv, S.omega);
+        ++nev;
+        if (polish_step(A, S, F, nf)) break;
+      }
+      double *kp = A.known + (long long)v * A.nent + e0 + k;
+      if (*kp > 0.0 && fabs(*kp - p1) > 0.05 * p1) *kp = 0.0;
+    }
+  }
+  if (A.nevals && nev) atomicAdd(A.nevals, nev);
+}
+
+// Entries still open after the surrogate stages (known = 0): the roots the
+// chain has to walk to.
+__global__ void __launch_bounds__(256)
+jdsg_count_open(const double *known, long long n, unsigned *out) {
+  __shared__ unsigned part[4];
+  unsigned c = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    c += !(known[i] > 0.0);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned s = part[0] + part[1] + part[2] + part[3];
+    if (s) atomicAdd(out, s);
+  }
+}
+
 // Mode order and root identity, one thread per (column, wave, period).  The
 // chain reaches mode m as the first sign change above mode m-1 (the lowest
 // mode: above its floor cm), with c_m > c_{m-1} + 0.01 dc
@@ -525,6 +629,20 @@ void jdsg_launch_polish(const PolishArgs &A, int wave, int nwaves, hipStream_t s
     hipLaunchKernelGGL(jdsg_polish_w<0>, dim3(g), dim3(64), 0, st, A);
   else
     hipLaunchKernelGGL(jdsg_polish_w<1>, dim3(g), dim3(64), 0, st, A);
+}
+void jdsg_launch_continue(const PolishArgs &A, int wave, int v, int mmax_modes, hipStream_t st) {
+  const long long n = (long long)A.cm.ncol * mmax_modes;
+  if (n <= 0) return;
+  const dim3 g((unsigned)((n + 63) / 64));
+  if (wave == 0)
+    hipLaunchKernelGGL(jdsg_continue_w<0>, g, dim3(64), 0, st, A, v, mmax_modes);
+  else
+    hipLaunchKernelGGL(jdsg_continue_w<1>, g, dim3(64), 0, st, A, v, mmax_modes);
+}
+void jdsg_launch_count_open(const double *known, long long n, unsigned *out, hipStream_t st) {
+  if (n <= 0) return;
+  const long long blocks = std::min<long long>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL(jdsg_count_open, dim3((unsigned)blocks), dim3(256), 0, st, known, n, out);
 }
 void jdsg_launch_mode_order(const PolishArgs &A, int kmax, hipStream_t st) {
   long long n = 2LL * A.cm.ncol * kmax;
