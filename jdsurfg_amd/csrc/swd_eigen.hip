@@ -1416,6 +1416,11 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
     double hsc = 0.0, hsu = 0.0;
     const bool need_v = A.dc[0] || A.du[0] || A.dc[1] || A.du[1],
                need_r = A.dc[2] || A.du[2], need_d = A.dc[3] || A.du[3];
+    // the entry's divisors as reciprocals: the layer loop is bound by its
+    // loads, and a division is about ten instructions between them
+    const double iusc[3] = {1.0 / usc[0], 1.0 / usc[1], 1.0 / usc[2]};
+    const double udu = uc1 * uc1 * (T / dtv), itm3 = 1.0 / tm3;
+#pragma unroll 2
     for (int i = N - 1; i >= 0; --i) {
       size_t q = (size_t)i * A.cm.ncol_pad + col;
       double vtp = need_v ? A.cm.vtp[q] : 0.0, dtp = need_d ? A.cm.dtp[q] : 0.0;
@@ -1436,19 +1441,19 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
           d1 = facv[1] * r1; if (fabs(d1) < 1.0e-38) d1 = 0.0;
           d2 = facv[2] * r2; if (fabs(d2) < 1.0e-38) d2 = 0.0;
         } else if (ray) {
-          d0 = r0 / usc[0];
-          d1 = pm_ok ? r1 / usc[1] : 0.0;
-          d2 = pm_ok ? r2 / usc[2] : 0.0;
+          d0 = r0 * iusc[0];
+          d1 = pm_ok ? r1 * iusc[1] : 0.0;
+          d2 = pm_ok ? r2 * iusc[2] : 0.0;
         } else {
-          d0 = (a != 0 && solid) ? r0 / usc[0] : 0.0;
-          d1 = (a != 0 && solid && pm_ok) ? r1 / usc[1] : 0.0;
-          d2 = (a != 0 && solid && pm_ok) ? r2 / usc[2] : 0.0;
+          d0 = (a != 0 && solid) ? r0 * iusc[0] : 0.0;
+          d1 = (a != 0 && solid && pm_ok) ? r1 * iusc[1] : 0.0;
+          d2 = (a != 0 && solid && pm_ok) ? r2 * iusc[2] : 0.0;
         }
-        double du = uc1 * (2.0 - uc1) * d2 - uc1 * uc1 * T * (d2 - d1) / dtv;
+        double du = uc1 * (2.0 - uc1) * d2 - udu * (d2 - d1);
         double dc = d0;
         if (A.iflsph > 0) {
           du = (tm * du + cg * cp * dc * tm1) * fct[a];
-          dc = dc / tm3 * fct[a];
+          dc = dc * itm3 * fct[a];
         }
         if (!ray && a == 0) { du = 0.0; dc = 0.0; }
         if (!pm_ok) du = 0.0;
