@@ -37,9 +37,14 @@ __device__ __forceinline__ float tanh_fast(float u) {
   float t = __builtin_amdgcn_exp2f(2.8853900817779268f * u);  // exp(2u)
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(t + 1.0f);
 }
+// The same GELU written as z sigmoid(2u) = z / (1 + 2^(-2u log2 e)), with
+// 2u log2 e = z (kS0 + kS1 z^2): one exp2 and one reciprocal in about seven
+// instructions (the epilogues of the inference kernels are VALU-bound next
+// to their MFMA work).  Equal to the tanh form up to fp32 rounding.
+constexpr float kS0 = 2.302208198144325f, kS1 = 0.1029432395800235f;
 __device__ __forceinline__ float gelu(float z) {
-  float u = kG0 * (z + kG1 * z * z * z);
-  return 0.5f * z * (1.0f + tanh_fast(u));
+  const float t = z * fmaf(kS1, z * z, kS0);
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-t));
 }
 __device__ __forceinline__ float gelu_grad(float z) {
   float u = kG0 * (z + kG1 * z * z * z);
