@@ -271,8 +271,11 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
 }
 }  // namespace
 
+// Love lanes fit three waves per SIMD (168 VGPRs, a 32-byte spill); the
+// Rayleigh ones need 214 and slowed down when capped (192 bytes of spill)
 template <int W>
-__global__ void __launch_bounds__(64) jdsg_polish_w(PolishArgs A) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 1 ? 3 : 1)))
+jdsg_polish_w(PolishArgs A) {
   const long long nl = (long long)gridDim.x * blockDim.x;
   long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long nev = 0;
