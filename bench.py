@@ -184,8 +184,12 @@ def main():
         okp = (pout["status"] == 0) & (sout["status"] == 0)
         relcp = ((sout["c"] - pout["c"]).abs() / pout["c"].abs())[okp]
         relup = ((sout["u"] - pout["u"]).abs() / pout["u"].abs())[okp]
-        # MLP inference: 2 * rows * sum(in*out) flops on the MFMA units
-        mlp_flops = 2.0 * sp.nent * sum(o * i for o, i in mlp.dims)
+        # MLP inference as computed: the first layer once per column (its
+        # per-entry part is four vector terms), the hidden layers per entry on
+        # MFMA, and the one output row of the head
+        H = mlp.hid
+        mlp_flops = 2.0 * (sp.ncol * mlp.dims[0][1] * H +
+                           sp.nent * (sum(o * i for o, i in mlp.dims[1:4]) + H))
         line.update({
             "value": float(cps / (ms_sur * 1e-3)), "ms_per_step": float(ms_sur),
             "speedup_vs_plain": float(ms_plain / ms_sur),
