@@ -17,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_REST = os.path.join(HERE, "liboracle.so")
 LIB_REF = os.path.join(HERE, "_ref", "libswdref.so")
+LIB_LSMR_REF = os.path.join(HERE, "_ref", "liblsmrref.so")
 
 _f = np.float32
 _d = np.float64
@@ -278,6 +279,43 @@ class Oracle:
                                  np.ascontiguousarray(resid, _f),
                                  np.ascontiguousarray(np.asarray(vs, _f).ravel(order="F")), g)
         return g
+
+
+def lsmr(m, n, val, ind, ptr, b, damp=0.0, atol=1e-5, btol=1e-5, conlim=1e6, itnlim=None,
+         local_size=10, kind="rest", threads=1):
+    """LSMR on a CSR matrix (lsmrModule_csr.f90): the restatement ('rest') or
+    the compiled reference ('ref', shared/lsmrModule_csr.f90 with
+    shared/parallel.cpp, `threads` OpenMP threads).  Returns x and a dict of
+    istop, itn, normA, condA, normr, normAr, normx."""
+    if itnlim is None:
+        itnlim = 2 * n
+    val = np.ascontiguousarray(val, _f)
+    ind = np.ascontiguousarray(ind, np.int32)
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    b = np.ascontiguousarray(b, _f)
+    x = np.zeros(n, _f)
+    ist, itn = C.c_int(), C.c_int()
+    outs = [C.c_float() for _ in range(5)]
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    if kind == "rest":
+        L = C.CDLL(LIB_REST)
+        fn = L.or_lsmr_csr
+        fn.argtypes = [C.c_int, C.c_int, _fp, ip, ip, _fp] + [C.c_float] * 4 + \
+            [C.c_int, C.c_int, _fp] + [C.POINTER(C.c_int)] * 2 + [C.POINTER(C.c_float)] * 5
+        fn(m, n, val, ind, ptr, b, damp, atol, btol, conlim, itnlim, local_size, x,
+           C.byref(ist), C.byref(itn), *[C.byref(o) for o in outs])
+    else:
+        L = C.CDLL(LIB_LSMR_REF)
+        fn = L.LSMR_csr
+        fn.argtypes = [C.c_int, C.c_int, _fp, ip, ip, _fp] + [C.c_float] * 4 + \
+            [C.c_int, C.c_int, _fp] + [C.POINTER(C.c_int)] * 2 + \
+            [C.POINTER(C.c_float)] * 5 + [C.c_int, C.c_int]
+        fn(m, n, val, ind, ptr, b, damp, atol, btol, conlim, itnlim, local_size, x,
+           C.byref(ist), C.byref(itn), *[C.byref(o) for o in outs], 0, threads)
+    names = ("normA", "condA", "normr", "normAr", "normx")
+    info = {"istop": ist.value, "itn": itn.value}
+    info.update({k: o.value for k, o in zip(names, outs)})
+    return x, info
 
 
 def empirical_relation(vs):

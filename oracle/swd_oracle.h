@@ -163,6 +163,11 @@ long long or_frechet_rows(int nx, int ny, int nz, int kmax, const double *kernel
 void or_frechet_grad(int nx, int ny, int nz, int kmax, const double *kernel, int nrows,
                      const int *row_idx, const float *fdm, const float *resid, const float *vs,
                      float *grad);
+/* lsmrModule_csr.f90 LSMR on a CSR matrix, float, serial products */
+void or_lsmr_csr(int m, int n, const float *val, const int *ind, const int *ptr, const float *b,
+                 float damp, float atol, float btol, float conlim, int itnlim, int localSize,
+                 float *x, int *istop, int *itn, float *normA, float *condA, float *normr,
+                 float *normAr, float *normx);
 
 #ifdef __cplusplus
 }
