@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="grid256")
+    ap.add_argument("--workload", default="grid256",
+                    help="grid256 (the default line) or lsmr (LSMR on Frechet rows)")
     ap.add_argument("--cpu-sample", type=int, default=384,
                     help="columns in the CPU baseline sample (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -78,6 +79,80 @@ def cpu_baseline(w, ncols, nthreads):
     return sub.column_periods() / dt, kind, len(cols), dt
 
 
+def lsmr_main(args, nx=256, ny=256, nz=17, kmax=120, nrows=20000, it_gpu=40, it_cpu=3):
+    """LSMR (lsmrModule_csr.f90) on the Frechet rows of 20000 straight rays
+    over the 256 x 256 x 17 grid (the rows jdsg_frechet_rows builds from random
+    depth kernels; 254*254*16 model cells), b random, damp 0, run for a fixed
+    iteration count (atol = btol = conlim = 0, so istop = 7).  Prints one JSON
+    line: GPU iterations/s from the difference of a 1- and an it_gpu-iteration
+    solve (the one-time transpose reported apart), the algorithmic bytes of an
+    iteration over its time, and the compiled reference LSMR on the host
+    threads over it_cpu iterations, with the relative difference of the two
+    solutions after it_cpu iterations."""
+    import numpy as np
+    import torch
+    from jdsurfg_amd import frechet as F
+    from jdsurfg_amd import lsmr as G
+    rng = np.random.default_rng(0)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    kernel = torch.randn(nx * ny * nz * kmax, dtype=torch.float64, device="cuda", generator=g)
+    ray_off, ray_cell, ray_w = F.straight_rays(nx, ny, nrows, 0)
+    row_idx = rng.integers(0, kmax, nrows).astype(np.int32)
+    ro, col, val = F.frechet_rows(kernel, nx, ny, nz, kmax, row_idx, ray_off, ray_cell, ray_w)
+    del kernel
+    ptr = ro.to(torch.int32)
+    m, n, nnz = nrows, (nx - 2) * (ny - 2) * (nz - 1), int(col.numel())
+    b = torch.randn(m, device="cuda", generator=g)
+    print(f"lsmr: {m} rows, {n} columns, {nnz} nonzeros", file=sys.stderr, flush=True)
+
+    def solve(itn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, info = G.lsmr(m, n, val, col, ptr, b, 0.0, 0.0, 0.0, 0.0, itn, 10)
+        torch.cuda.synchronize()
+        return x, info, time.perf_counter() - t0
+
+    solve(1)                                   # warm the library and the allocator
+    t1 = min(solve(1)[2] for _ in range(3))
+    xs, infos, tk = zip(*[solve(it_gpu) for _ in range(max(1, args.steps // 2))])
+    tk = min(tk)
+    assert all(i["itn"] == it_gpu and i["istop"] == 7 for i in infos)
+    per_it = (tk - t1) / (it_gpu - 1)
+    # an iteration reads each of A and A^T once (value 4 B + index 4 B per
+    # nonzero, 4 B per row pointer), gathers x and writes y once (4 B each),
+    # and streams n-vectors for the norms, the reorthogonalisation against 10
+    # vectors (2 reads of v and q, 1 write of v) and the h/hbar/x update
+    alg = 2 * (8 * nnz) + 4 * (m + 1) + 4 * (n + 1) + 8 * (m + n) * 2 \
+        + 4 * n * (10 * 3 + 7 + 2) + 4 * m * 2
+    rep = {"metric": "LSMR iterations/s on Frechet rows (float, local reorthogonalisation 10)",
+           "value": 1.0 / per_it, "unit": "iterations/s", "n_gpus": 1, "steps": len(xs),
+           "warmup": 1, "ms_per_step": per_it * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"lsmr_frechet_{m}x{n}", "rows": m, "columns": n,
+                      "nnz": nnz, "iterations": it_gpu},
+           "setup_ms": (t1 - per_it) * 1e3,
+           "roofline": {"bound": "hbm", "achieved": alg / per_it / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "frac": alg / per_it / 8e12, "traffic": None,
+                        "note": "whole iteration (both products, norms, host syncs)"}}
+    if not args.no_cpu:
+        from oracle import pyoracle
+        if pyoracle.available("ref") and os.path.exists(pyoracle.LIB_LSMR_REF):
+            kind, lib = "reference", "ref"
+        else:
+            kind, lib = "port", "rest"
+        threads = 16 if lib == "ref" else 1
+        hv, hi, hp, hb = (t.cpu().numpy() for t in (val, col, ptr, b))
+        t0 = time.perf_counter()
+        xr, ir = pyoracle.lsmr(m, n, hv, hi, hp, hb, 0.0, 0.0, 0.0, 0.0, it_cpu, 10, kind=lib,
+                               threads=threads)
+        dt = time.perf_counter() - t0
+        xg = solve(it_cpu)[0].cpu().numpy()
+        rep["cpu_baseline"] = {"value": it_cpu / dt, "unit": "iterations/s", "cores": threads,
+                               "kind": kind, "sample": f"{it_cpu} iterations of the same solve"}
+        rep["x_rel_diff_vs_cpu"] = float(np.abs(xg - xr).max() / np.abs(xr).max())
+    print(json.dumps(rep), flush=True)
+
+
 def timed(fn, steps, warmup):
     """Run fn warmup times, then time `steps` calls between device syncs."""
     import torch
@@ -99,6 +174,8 @@ def main():
                           "(columns are independent: a multi-GPU run would block-shard "
                           "them with no exchange)"}))
         sys.exit(2)
+    if args.workload == "lsmr":
+        return lsmr_main(args)
     import numpy as np
     import torch
     from jdsurfg_amd import swd

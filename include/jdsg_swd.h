@@ -305,6 +305,21 @@ int jdsg_mlp_head(long long n, int H, const void *Y, const void *w, float b, dou
 
 const char *jdsg_last_error(void);
 
+/* ---- least-squares solve of the inversion step ------------------------ */
+/* LSMR on a CSR matrix (m rows, n columns; val, indices, indptr, b and x in
+ * device memory, 0-based, nnz = indptr[m] < 2^31), replacing
+ * LSMR_csr (src/shared/lsmrModule_csr.f90:516, bind(C)) as called by
+ * csr_matrix::lsmr_solver (src/shared/csr_matrix.cpp:114-128): the same
+ * arguments in the same order and meaning, float vectors and scalars, x set
+ * to the solution, istop/itn and the norm estimates returned; verbose and
+ * num_threads give way to a HIP stream.  Nonzero on a bad argument or a HIP
+ * failure (message in jdsg_lsmr_last_error). */
+int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices, const int *indptr,
+                  const float *b, float damp, float atol, float btol, float conlim, int itnlim,
+                  int localSize, float *x, int *istop, int *itn, float *normA, float *condA,
+                  float *normr, float *normAr, float *normx, void *stream);
+const char *jdsg_lsmr_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@ this module, as the checker.  Two builds are bound:
 """
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -310,8 +311,21 @@ def lsmr(m, n, val, ind, ptr, b, damp=0.0, atol=1e-5, btol=1e-5, conlim=1e6, itn
         fn.argtypes = [C.c_int, C.c_int, _fp, ip, ip, _fp] + [C.c_float] * 4 + \
             [C.c_int, C.c_int, _fp] + [C.POINTER(C.c_int)] * 2 + \
             [C.POINTER(C.c_float)] * 5 + [C.c_int, C.c_int]
-        fn(m, n, val, ind, ptr, b, damp, atol, btol, conlim, itnlim, local_size, x,
-           C.byref(ist), C.byref(itn), *[C.byref(o) for o in outs], 0, threads)
+        # LSMR_csr keeps h, hbar, u, v, w and localV(n, localVecs) as
+        # automatic arrays (lsmrModule_csr.f90:789), which flang places on the
+        # stack: call it on a thread whose stack holds them
+        lv = min(local_size, m, n)
+        need = 4 * (5 * n + m + n * lv) + (64 << 20)
+        run = lambda: fn(m, n, val, ind, ptr, b, damp, atol, btol, conlim, itnlim, local_size, x,
+                         C.byref(ist), C.byref(itn), *[C.byref(o) for o in outs], 0, threads)
+        old = threading.stack_size()
+        threading.stack_size(-(-need // 4096) * 4096)
+        try:
+            t = threading.Thread(target=run)
+            t.start()
+        finally:
+            threading.stack_size(old)
+        t.join()
     names = ("normA", "condA", "normr", "normAr", "normx")
     info = {"istop": ist.value, "itn": itn.value}
     info.update({k: o.value for k, o in zip(names, outs)})

@@ -63,3 +63,21 @@ def test_iteration_limit_and_zero_rhs():
     assert info["istop"] == 7 and info["itn"] == 3
     x0, info0 = pyoracle.lsmr(200, 80, val, ind, ptr, np.zeros(200, np.float32))
     assert not x0.any() and info0["itn"] == 0
+
+
+def test_reference_runs_at_frechet_width():
+    """The compiled reference keeps its work vectors on the stack
+    (lsmrModule_csr.f90:789); at a million model cells the wrapper's thread
+    stack must hold them.  The restatement agrees there too."""
+    if not __import__("os").path.exists(pyoracle.LIB_LSMR_REF):
+        pytest.skip("reference LSMR not built")
+    m, n = 3000, 1_032_256
+    rng = np.random.default_rng(11)
+    ptr = np.arange(0, 2 * m + 1, 2, dtype=np.int32)
+    ind = rng.integers(0, n, 2 * m).astype(np.int32)
+    val = rng.normal(size=2 * m).astype(np.float32)
+    b = rng.normal(size=m).astype(np.float32)
+    xr, ir = pyoracle.lsmr(m, n, val, ind, ptr, b, itnlim=4, kind="ref", threads=2)
+    x, info = pyoracle.lsmr(m, n, val, ind, ptr, b, itnlim=4, kind="rest")
+    assert ir["itn"] == info["itn"]
+    assert np.abs(x - xr).max() <= 1e-5 * np.abs(xr).max()
