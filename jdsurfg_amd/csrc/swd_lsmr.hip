@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -99,25 +100,91 @@ lsmr_spmv(int m, const int *__restrict__ ptr, const int *__restrict__ ind,
   if (lane == 0) y[row] = a * y[row] + s;
 }
 
-// lanes per row for an average row length: the smallest of 4..64 with
-// 8 G >= the average
-int spmv_lanes(long long nnz, int rows) {
-  const double avg = rows > 0 ? (double)nnz / rows : 0.0;
+// The same product with value and index read 16 B at a time (val and ind
+// 16-byte aligned): the row's entries before its first 4-aligned position
+// and from its last one on (at most 3 each) are taken one per lane, the
+// aligned quads in between G lanes at a time, two quads in flight per lane.
+template <int G>
+__global__ void __launch_bounds__(256)
+lsmr_spmv4(int m, const int *__restrict__ ptr, const int *__restrict__ ind,
+           const float *__restrict__ val, const float *__restrict__ x, float a, float *y) {
+  const int row = blockIdx.x * (256 / G) + (int)threadIdx.x / G;
+  const int lane = (int)threadIdx.x % G;
+  if (row >= m) return;  // a whole group leaves together
+  const int j0 = ptr[row], j1 = ptr[row + 1];
+  const int ja = (int)min((long long)j0 + 3 - ((long long)j0 + 3) % 4, (long long)j1);
+  const int jb = max(ja, j1 - j1 % 4);
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  if (j0 + lane < ja) s0 = val[j0 + lane] * x[ind[j0 + lane]];
+  if (jb + lane < j1) s1 = val[jb + lane] * x[ind[jb + lane]];
+  const float4 *v4 = reinterpret_cast<const float4 *>(val);
+  const int4 *i4 = reinterpret_cast<const int4 *>(ind);
+  const int qe = jb / 4;
+  int q = ja / 4 + lane;
+  for (; q + G < qe; q += 2 * G) {
+    const float4 va = v4[q], vb = v4[q + G];
+    const int4 ia = i4[q], ib = i4[q + G];
+    s0 += va.x * x[ia.x];
+    s1 += va.y * x[ia.y];
+    s2 += va.z * x[ia.z];
+    s3 += va.w * x[ia.w];
+    s0 += vb.x * x[ib.x];
+    s1 += vb.y * x[ib.y];
+    s2 += vb.z * x[ib.z];
+    s3 += vb.w * x[ib.w];
+  }
+  if (q < qe) {
+    const float4 va = v4[q];
+    const int4 ia = i4[q];
+    s0 += va.x * x[ia.x];
+    s1 += va.y * x[ia.y];
+    s2 += va.z * x[ia.z];
+    s3 += va.w * x[ia.w];
+  }
+  float s = (s0 + s1) + (s2 + s3);
+  if constexpr (G <= 64) {
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+    if (lane == 0) y[row] = a * y[row] + s;
+  } else {  // a block per row: the four wavefront sums combined in order
+    __shared__ float sh[4];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((lane & 63) == 0) sh[lane >> 6] = s;
+    __syncthreads();
+    if (lane == 0) y[row] = a * y[row] + ((sh[0] + sh[1]) + (sh[2] + sh[3]));
+  }
+}
+
+// lanes per row: the smallest of 4..64 with 4 G >= the average number of
+// work items per row (entries, or quads of entries for the 16 B form)
+// (quads: 256, a block per row, for rows of more than 1024 quads)
+int spmv_lanes(long long nnz, int rows, bool quads) {
+  double avg = rows > 0 ? (double)nnz / rows : 0.0;
+  if (quads) avg /= 4.0;
   int g = 4;
-  while (g < 64 && 8.0 * g < avg) g *= 2;
+  while (g < 64 && (quads ? 4.0 : 8.0) * g < avg) g *= 2;
+  if (quads && avg > 1024.0) g = 256;
   return g;
 }
 
-void launch_spmv(int g, int rows, const int *ptr, const int *ind, const float *val,
+template <int G>
+void launch_spmv_g(bool quads, dim3 grd, int rows, const int *ptr, const int *ind,
+                   const float *val, const float *x, float a, float *y, hipStream_t st) {
+  if (quads)
+    hipLaunchKernelGGL(lsmr_spmv4<G>, grd, dim3(256), 0, st, rows, ptr, ind, val, x, a, y);
+  else
+    hipLaunchKernelGGL(lsmr_spmv<G>, grd, dim3(256), 0, st, rows, ptr, ind, val, x, a, y);
+}
+
+void launch_spmv(int g, bool quads, int rows, const int *ptr, const int *ind, const float *val,
                  const float *x, float a, float *y, hipStream_t st) {
-  const dim3 blk(256);
   const dim3 grd((unsigned)((rows + 256 / g - 1) / (256 / g)));
   switch (g) {
-    case 4: hipLaunchKernelGGL(lsmr_spmv<4>, grd, blk, 0, st, rows, ptr, ind, val, x, a, y); break;
-    case 8: hipLaunchKernelGGL(lsmr_spmv<8>, grd, blk, 0, st, rows, ptr, ind, val, x, a, y); break;
-    case 16: hipLaunchKernelGGL(lsmr_spmv<16>, grd, blk, 0, st, rows, ptr, ind, val, x, a, y); break;
-    case 32: hipLaunchKernelGGL(lsmr_spmv<32>, grd, blk, 0, st, rows, ptr, ind, val, x, a, y); break;
-    default: hipLaunchKernelGGL(lsmr_spmv<64>, grd, blk, 0, st, rows, ptr, ind, val, x, a, y); break;
+    case 4: launch_spmv_g<4>(quads, grd, rows, ptr, ind, val, x, a, y, st); break;
+    case 8: launch_spmv_g<8>(quads, grd, rows, ptr, ind, val, x, a, y, st); break;
+    case 16: launch_spmv_g<16>(quads, grd, rows, ptr, ind, val, x, a, y, st); break;
+    case 32: launch_spmv_g<32>(quads, grd, rows, ptr, ind, val, x, a, y, st); break;
+    case 64: launch_spmv_g<64>(quads, grd, rows, ptr, ind, val, x, a, y, st); break;
+    default: hipLaunchKernelGGL(lsmr_spmv4<256>, grd, dim3(256), 0, st, rows, ptr, ind, val, x, a, y);
   }
 }
 
@@ -152,22 +219,46 @@ __global__ void lsmr_scale(int n, float a, float *x) {
   if (i < n) x[i] = a * x[i];
 }
 
-// v -= d q with d = (float) the dot product left by the reduction
-__global__ void lsmr_sub_proj(int n, const double *__restrict__ dd, const float *__restrict__ q,
-                              float *v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = v[i] - (float)(*dd) * q[i];
+// v -= d qp with d = (float) *dd, then part[block] = the block's share of
+// v . qn (v . v if qn is null) over the new v, in double: one pass per
+// vector of the reorthogonalisation (the next dot product, or the norm of v
+// after the last vector, rides on the subtraction)
+__global__ void __launch_bounds__(256)
+lsmr_sub_dot(int n, const double *__restrict__ dd, const float *__restrict__ qp,
+             const float *__restrict__ qn, float *v, double *part) {
+  __shared__ double sh[4];
+  const float d = (float)(*dd);
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float w = v[i] - d * qp[i];
+    v[i] = w;
+    s += (double)w * (double)(qn ? qn[i] : w);
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-// hbar = h - fh hbar;  x = x + fx hbar;  h = v - fn h  (lsmrModule_csr.f90:1003-1006)
-__global__ void lsmr_update(int n, float fh, float fx, float fn, const float *__restrict__ v,
-                            float *h, float *hbar, float *x) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float hb = h[i] - fh * hbar[i];
-  hbar[i] = hb;
-  x[i] = x[i] + fx * hb;
-  h[i] = v[i] - fn * h[i];
+// hbar = h - fh hbar;  x = x + fx hbar;  h = v - fn h  (lsmrModule_csr.f90:1003-1006),
+// with part[block] = the block's share of x . x over the new x
+__global__ void __launch_bounds__(256)
+lsmr_update(int n, float fh, float fx, float fn, const float *__restrict__ v, float *h,
+            float *hbar, float *x, double *part) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float hb = h[i] - fh * hbar[i];
+    hbar[i] = hb;
+    const float xi = x[i] + fx * hb;
+    x[i] = xi;
+    h[i] = v[i] - fn * h[i];
+    s += (double)xi * (double)xi;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 float d2norm(float a, float b) {  // lsmrModule_csr.f90:1151-1163
@@ -288,13 +379,23 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   auto scale = [&](int len, float a, float *p) {
     hipLaunchKernelGGL(lsmr_scale, grid(len), dim3(T), 0, st, len, a, p);
   };
-  const int g_rows = spmv_lanes(nnz, m), g_cols = spmv_lanes(nnz, n);
+  // the 16 B form needs 16-byte aligned value and index arrays (the
+  // transpose's own are)
+  const bool quads_rows = (((uintptr_t)val | (uintptr_t)indices) & 15) == 0;
+  const int g_rows = spmv_lanes(nnz, m, quads_rows), g_cols = spmv_lanes(nnz, n, true);
   // y = a y + A x;  x = a x + A^T y
   auto aprod1 = [&](float a, const float *xx, float *yy) {
-    launch_spmv(g_rows, m, indptr, indices, val, xx, a, yy, st);
+    launch_spmv(g_rows, quads_rows, m, indptr, indices, val, xx, a, yy, st);
   };
   auto aprod2 = [&](float a, const float *yy, float *xx) {
-    launch_spmv(g_cols, n, colptr, crow, cval, yy, a, xx, st);
+    launch_spmv(g_cols, true, n, colptr, crow, cval, yy, a, xx, st);
+  };
+  // the scalar a reduction left in red[k], read back
+  auto read_red = [&](int k, double *out) -> int {
+    if (hipMemcpyAsync(out, red + k, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return 1;
+    return 0;
   };
   // ---- lsmrModule_csr.f90:829-866
   const bool damped = damp > 0.0f;
@@ -347,15 +448,22 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
                               hipMemcpyDeviceToDevice, st));
         }
         aprod2(-beta, u, v);
-        if (localOrtho) {  // localVOrtho
+        if (localOrtho) {  // localVOrtho: v -= (v . q_c) q_c in turn, then ||v||
           const int lim = localFull ? localVecs : localPointer;
-          for (int c = 0; c < lim; ++c) {
-            const float *q = lv + (size_t)c * n;
-            dot(n, v, q, red + 1 + c);
-            hipLaunchKernelGGL(lsmr_sub_proj, grid(n), dim3(T), 0, st, n, red + 1 + c, q, v);
+          dot(n, v, lv, red + 1);
+          for (int c = 1; c <= lim; ++c) {
+            const float *qn = c < lim ? lv + (size_t)c * n : nullptr;
+            double *dst = c < lim ? red + 1 + c : red;
+            hipLaunchKernelGGL(lsmr_sub_dot, dim3(kRedBlocks), dim3(T), 0, st, n, red + c,
+                               lv + (size_t)(c - 1) * n, qn, v, part);
+            hipLaunchKernelGGL(lsmr_reduce_final, dim3(1), dim3(T), 0, st, kRedBlocks, part, dst);
           }
+          double s = 0.0;
+          if (read_red(0, &s)) return fail("lsmr: norm failed");
+          alpha = (float)std::sqrt(s);
+        } else if (norm(n, v, &alpha)) {
+          return fail("lsmr: norm failed");
         }
-        if (norm(n, v, &alpha)) return fail("lsmr: norm failed");
         if (alpha > 0.0f) scale(n, 1.0f / alpha, v);
       }
       // :958-1000 plane rotations
@@ -381,7 +489,9 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
       const float fh = thetabar * rho / (rhoold * rhobarold);
       const float fx = zeta / (rho * rhobar);
       const float fn = thetanew / rho;
-      hipLaunchKernelGGL(lsmr_update, grid(n), dim3(T), 0, st, n, fh, fx, fn, v, h, hbar, x);
+      hipLaunchKernelGGL(lsmr_update, dim3(kRedBlocks), dim3(T), 0, st, n, fh, fx, fn, v, h, hbar,
+                         x, part);
+      hipLaunchKernelGGL(lsmr_reduce_final, dim3(1), dim3(T), 0, st, kRedBlocks, part, red);
       // :1009-1043
       const float betaacute = chat * betadd;
       const float betacheck = -shat * betadd;
@@ -407,7 +517,11 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
       condA = std::max(maxrbar, rhotemp) / std::min(minrbar, rhotemp);
       // :1046-1078
       normAr = fabsf(zetabar);
-      if (norm(n, x, &normx)) return fail("lsmr: norm failed");
+      {
+        double s = 0.0;
+        if (read_red(0, &s)) return fail("lsmr: norm failed");
+        normx = (float)std::sqrt(s);
+      }
       const float test1 = normr / normb;
       const float test2 = normAr / (normA * normr);
       const float test3 = 1.0f / condA;

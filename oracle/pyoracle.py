@@ -284,9 +284,10 @@ class Oracle:
 
 def lsmr(m, n, val, ind, ptr, b, damp=0.0, atol=1e-5, btol=1e-5, conlim=1e6, itnlim=None,
          local_size=10, kind="rest", threads=1):
-    """LSMR on a CSR matrix (lsmrModule_csr.f90): the restatement ('rest') or
-    the compiled reference ('ref', shared/lsmrModule_csr.f90 with
-    shared/parallel.cpp, `threads` OpenMP threads).  Returns x and a dict of
+    """LSMR on a CSR matrix (lsmrModule_csr.f90): the restatement ('rest'),
+    its double twin ('f64', x returned in float64) or the compiled reference
+    ('ref', shared/lsmrModule_csr.f90 with shared/parallel.cpp, `threads`
+    OpenMP threads).  Returns x and a dict of
     istop, itn, normA, condA, normr, normAr, normx."""
     if itnlim is None:
         itnlim = 2 * n
@@ -298,7 +299,17 @@ def lsmr(m, n, val, ind, ptr, b, damp=0.0, atol=1e-5, btol=1e-5, conlim=1e6, itn
     ist, itn = C.c_int(), C.c_int()
     outs = [C.c_float() for _ in range(5)]
     ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
-    if kind == "rest":
+    if kind == "f64":  # the double twin of the restatement (a yardstick)
+        d = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L = C.CDLL(LIB_REST)
+        fn = L.or_lsmr_csr_f64
+        fn.argtypes = [C.c_int, C.c_int, d, ip, ip, d] + [C.c_double] * 4 + \
+            [C.c_int, C.c_int, d] + [C.POINTER(C.c_int)] * 2 + [C.POINTER(C.c_double)] * 5
+        x = np.zeros(n, np.float64)
+        outs = [C.c_double() for _ in range(5)]
+        fn(m, n, val.astype(np.float64), ind, ptr, b.astype(np.float64), damp, atol, btol, conlim,
+           itnlim, local_size, x, C.byref(ist), C.byref(itn), *[C.byref(o) for o in outs])
+    elif kind == "rest":
         L = C.CDLL(LIB_REST)
         fn = L.or_lsmr_csr
         fn.argtypes = [C.c_int, C.c_int, _fp, ip, ip, _fp] + [C.c_float] * 4 + \

@@ -168,6 +168,11 @@ void or_lsmr_csr(int m, int n, const float *val, const int *ind, const int *ptr,
                  float damp, float atol, float btol, float conlim, int itnlim, int localSize,
                  float *x, int *istop, int *itn, float *normA, float *condA, float *normr,
                  float *normAr, float *normx);
+/* the same in double (lsmr_oracle_f64.c) */
+void or_lsmr_csr_f64(int m, int n, const double *val, const int *ind, const int *ptr, const double *b,
+                 double damp, double atol, double btol, double conlim, int itnlim, int localSize,
+                 double *x, int *istop, int *itn, double *normA, double *condA, double *normr,
+                 double *normAr, double *normx);
 
 #ifdef __cplusplus
 }

@@ -33,7 +33,9 @@ def _compare(m, n, val, ind, ptr, b, **kw):
     assert info["istop"] == ir["istop"]
     assert abs(info["itn"] - ir["itn"]) <= 2
     assert err <= 1e-3
-    assert abs(info["normr"] - ir["normr"]) <= 1e-3 * max(ir["normr"], 1e-12) + 1e-6
+    # the residual estimate of a consistent system ends at rounding noise,
+    # which is measured against ||b|| (the scale of the btol test)
+    assert abs(info["normr"] - ir["normr"]) <= 1e-3 * ir["normr"] + 1e-5 * np.linalg.norm(b)
 
 
 @pytest.mark.parametrize("m,n,density,damp,local",
@@ -41,7 +43,9 @@ def _compare(m, n, val, ind, ptr, b, **kw):
                           (80, 200, 0.1, 0.0, 10), (2000, 700, 0.01, 0.1, 10),
                           (2000, 700, 0.01, 0.0, 0),
                           # rows of ~1200 entries (64 lanes per row) / columns of ~1200
-                          (64, 4000, 0.3, 0.0, 10), (4000, 64, 0.3, 0.1, 10)])
+                          (64, 4000, 0.3, 0.0, 10), (4000, 64, 0.3, 0.1, 10),
+                          # rows of ~6000 entries (a block of 256 lanes per row)
+                          (32, 20000, 0.3, 0.0, 10)])
 def test_gpu_lsmr_matches_oracle(m, n, density, damp, local):
     print(f"case lsmr {m}x{n} density {density} damp {damp} local {local}", flush=True)
     A, ptr, ind, val, b = sparse_problem(m, n, density, m + n)
@@ -81,3 +85,68 @@ def test_gpu_lsmr_rejects_bad_matrix():
         _gpu(50, 20, val, bad, ptr, b)
     x, info = _gpu(50, 20, val, ind, ptr, b)   # a good call still works
     assert info["istop"] in (1, 2, 3)
+
+
+def test_gpu_lsmr_unaligned_arrays_take_the_scalar_products():
+    """Value and index arrays that are not 16-byte aligned (views one element
+    into their buffers) are read one entry per lane; the solve matches the
+    oracle as the aligned one does."""
+    import torch
+    from jdsurfg_amd import lsmr as G
+    m, n = 2000, 700
+    print("case lsmr unaligned 2000x700", flush=True)
+    A, ptr, ind, val, b = sparse_problem(m, n, 0.01, 9)
+    vbuf = torch.zeros(val.size + 1, dtype=torch.float32, device="cuda")
+    ibuf = torch.zeros(ind.size + 1, dtype=torch.int32, device="cuda")
+    vbuf[1:] = torch.from_numpy(val).cuda()
+    ibuf[1:] = torch.from_numpy(ind).cuda()
+    vv, iv = vbuf[1:], ibuf[1:]
+    assert vv.data_ptr() % 16 and iv.data_ptr() % 16
+    x, info = G.lsmr(m, n, vv, iv, torch.from_numpy(ptr).cuda(), torch.from_numpy(b).cuda(),
+                     damp=0.1)
+    xr, ir = pyoracle.lsmr(m, n, val, ind, ptr, b, damp=0.1)
+    x = x.cpu().numpy()
+    assert info["istop"] == ir["istop"] and abs(info["itn"] - ir["itn"]) <= 2
+    assert np.abs(x - xr).max() <= 1e-3 * np.abs(xr).max()
+
+
+def test_gpu_lsmr_at_frechet_width_tracks_double():
+    """At a million model cells (254 x 254 x 16, the rows of 1500 straight
+    rays with random depth kernels), three iterations: the GPU iterate stays
+    within 1e-4 of the double-precision iterate, while the float restatement
+    (the reference's own bits) drifts further, its million-term float sums
+    being the larger error."""
+    import torch
+    from jdsurfg_amd import frechet as F
+    nx = ny = 256
+    nz, nrows = 17, 1500
+    print("case lsmr frechet width 1500 x 1032256", flush=True)
+    off, cell, w = F.straight_rays(nx, ny, nrows, 0)
+    rng = np.random.default_rng(0)
+    nin = (nx - 2) * (ny - 2)
+    ptr, inds, vals = [0], [], []
+    for r in range(nrows):
+        c, ww = cell[off[r]:off[r + 1]], w[off[r]:off[r + 1]]
+        ix, iy = c % nx, c // nx
+        ok = (ix >= 1) & (ix <= nx - 2) & (iy >= 1) & (iy <= ny - 2)
+        ix, iy, ww = ix[ok], iy[ok], ww[ok]
+        for iz in range(nz - 1):
+            inds.append(iz * nin + (iy - 1) * (nx - 2) + (ix - 1))
+            vals.append(ww * rng.normal(size=ww.size))
+        ptr.append(ptr[-1] + ww.size * (nz - 1))
+    ind = np.concatenate(inds).astype(np.int32)
+    val = np.concatenate(vals).astype(np.float32)
+    ptr = np.array(ptr, np.int32)
+    m, n = nrows, nin * (nz - 1)
+    b = rng.normal(size=m).astype(np.float32)
+    kw = dict(damp=0.0, atol=0.0, btol=0.0, conlim=0.0, itnlim=3, local_size=10)
+    x64, i64 = pyoracle.lsmr(m, n, val, ind, ptr, b, kind="f64", **kw)
+    xf, _ = pyoracle.lsmr(m, n, val, ind, ptr, b, kind="rest", **kw)
+    x, info = _gpu(m, n, val, ind, ptr, b, **kw)
+    scale = np.abs(x64).max()
+    e_gpu = np.abs(x - x64).max() / scale
+    e_ref = np.abs(xf - x64).max() / scale
+    print(f"vs double: gpu {e_gpu:.2e}, float restatement {e_ref:.2e}", flush=True)
+    assert info["itn"] == 3 and info["istop"] == 7
+    assert e_gpu <= 1e-4
+    assert e_gpu <= e_ref

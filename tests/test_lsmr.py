@@ -81,3 +81,19 @@ def test_reference_runs_at_frechet_width():
     x, info = pyoracle.lsmr(m, n, val, ind, ptr, b, itnlim=4, kind="rest")
     assert ir["itn"] == info["itn"]
     assert np.abs(x - xr).max() <= 1e-5 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("m,n,density,damp", CASES[:3])
+def test_double_twin_reaches_the_exact_solution(m, n, density, damp):
+    """The double twin (a yardstick for float iterates) converges to the
+    least-squares / damped / minimum-norm solution to near double rounding."""
+    A, ptr, ind, val, b = sparse_problem(m, n, density, m + n)
+    x, info = pyoracle.lsmr(m, n, val, ind, ptr, b, damp=damp, atol=1e-14, btol=1e-14,
+                            conlim=1e14, kind="f64")
+    Ad = A.astype(np.float64)
+    if damp > 0:
+        want = np.linalg.solve(Ad.T @ Ad + damp ** 2 * np.eye(n), Ad.T @ b)
+    else:
+        want = np.linalg.lstsq(Ad, b.astype(np.float64), rcond=None)[0]
+    assert x.dtype == np.float64
+    assert np.abs(x - want).max() <= 1e-10 * np.abs(want).max()
