@@ -154,6 +154,57 @@ lsmr_spmv4(int m, const int *__restrict__ ptr, const int *__restrict__ ind,
   }
 }
 
+// The 16 B form for a gathered vector short enough to sit in LDS (the A^T
+// product of a Frechet system gathers its few thousand rows): each block of
+// 1024 threads copies x into LDS once, then walks rows [b R, b R + R),
+// [b R + grid R, ...) with G lanes per row; the per-row arithmetic and its
+// order are those of lsmr_spmv4<G>, with x read from LDS.
+template <int G>
+__global__ void __launch_bounds__(1024)
+lsmr_spmv4_lds(int m, int xlen, const int *__restrict__ ptr, const int *__restrict__ ind,
+               const float *__restrict__ val, const float *__restrict__ x, float a, float *y) {
+  extern __shared__ float xs[];
+  for (int i = threadIdx.x; i < xlen; i += 1024) xs[i] = x[i];
+  __syncthreads();
+  constexpr int R = 1024 / G;
+  const int lane = (int)threadIdx.x % G;
+  const float4 *v4 = reinterpret_cast<const float4 *>(val);
+  const int4 *i4 = reinterpret_cast<const int4 *>(ind);
+  for (int row = blockIdx.x * R + (int)threadIdx.x / G; row < m; row += gridDim.x * R) {
+    const int j0 = ptr[row], j1 = ptr[row + 1];
+    const int ja = (int)min((long long)j0 + 3 - ((long long)j0 + 3) % 4, (long long)j1);
+    const int jb = max(ja, j1 - j1 % 4);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (j0 + lane < ja) s0 = val[j0 + lane] * xs[ind[j0 + lane]];
+    if (jb + lane < j1) s1 = val[jb + lane] * xs[ind[jb + lane]];
+    const int qe = jb / 4;
+    int q = ja / 4 + lane;
+    for (; q + G < qe; q += 2 * G) {
+      const float4 va = v4[q], vb = v4[q + G];
+      const int4 ia = i4[q], ib = i4[q + G];
+      s0 += va.x * xs[ia.x];
+      s1 += va.y * xs[ia.y];
+      s2 += va.z * xs[ia.z];
+      s3 += va.w * xs[ia.w];
+      s0 += vb.x * xs[ib.x];
+      s1 += vb.y * xs[ib.y];
+      s2 += vb.z * xs[ib.z];
+      s3 += vb.w * xs[ib.w];
+    }
+    if (q < qe) {
+      const float4 va = v4[q];
+      const int4 ia = i4[q];
+      s0 += va.x * xs[ia.x];
+      s1 += va.y * xs[ia.y];
+      s2 += va.z * xs[ia.z];
+      s3 += va.w * xs[ia.w];
+    }
+    float s = (s0 + s1) + (s2 + s3);
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+    if (lane == 0) y[row] = a * y[row] + s;
+  }
+}
+
 // lanes per row: the smallest of 4..64 with 4 G >= the average number of
 // work items per row (entries, or quads of entries for the 16 B form)
 // (quads: 256, a block per row, for rows of more than 1024 quads)
@@ -173,6 +224,57 @@ void launch_spmv_g(bool quads, dim3 grd, int rows, const int *ptr, const int *in
     hipLaunchKernelGGL(lsmr_spmv4<G>, grd, dim3(256), 0, st, rows, ptr, ind, val, x, a, y);
   else
     hipLaunchKernelGGL(lsmr_spmv<G>, grd, dim3(256), 0, st, rows, ptr, ind, val, x, a, y);
+}
+
+// LDS launch shape of lsmr_spmv4_lds for a gathered vector of xlen floats:
+// blocks = CUs x the blocks whose LDS fits a CU, or 0 when x does not fit
+// one block's LDS (then the product gathers from memory)
+struct LdsShape {
+  int blocks = 0;
+  size_t bytes = 0;
+};
+
+LdsShape lds_shape(int xlen) {
+  LdsShape sh;
+  int dev = 0, per_block = 0, per_cu = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return sh;
+  const size_t bytes = sizeof(float) * (size_t)std::max(xlen, 1);
+  if (bytes > (size_t)per_block || cus <= 0) return sh;
+  const int fit = std::max(1, std::min(2, per_cu / (int)bytes));
+  sh.blocks = cus * fit;
+  sh.bytes = bytes;
+  return sh;
+}
+
+template <int G>
+hipError_t launch_lds_g(const LdsShape &sh, int rows, int xlen, const int *ptr, const int *ind,
+                        const float *val, const float *x, float a, float *y, hipStream_t st) {
+  const hipError_t e = hipFuncSetAttribute((const void *)lsmr_spmv4_lds<G>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sh.bytes);
+  if (e != hipSuccess) return e;
+  const int need = (rows + 1024 / G - 1) / (1024 / G);
+  hipLaunchKernelGGL(lsmr_spmv4_lds<G>, dim3((unsigned)std::max(1, std::min(sh.blocks, need))),
+                     dim3(1024), sh.bytes, st, rows, xlen, ptr, ind, val, x, a, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_spmv_lds(int g, const LdsShape &sh, int rows, int xlen, const int *ptr,
+                           const int *ind, const float *val, const float *x, float a, float *y,
+                           hipStream_t st) {
+  switch (g) {
+    case 4: return launch_lds_g<4>(sh, rows, xlen, ptr, ind, val, x, a, y, st);
+    case 8: return launch_lds_g<8>(sh, rows, xlen, ptr, ind, val, x, a, y, st);
+    case 16: return launch_lds_g<16>(sh, rows, xlen, ptr, ind, val, x, a, y, st);
+    case 32: return launch_lds_g<32>(sh, rows, xlen, ptr, ind, val, x, a, y, st);
+    default: return launch_lds_g<64>(sh, rows, xlen, ptr, ind, val, x, a, y, st);
+  }
 }
 
 void launch_spmv(int g, bool quads, int rows, const int *ptr, const int *ind, const float *val,
@@ -387,8 +489,19 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   auto aprod1 = [&](float a, const float *xx, float *yy) {
     launch_spmv(g_rows, quads_rows, m, indptr, indices, val, xx, a, yy, st);
   };
+  // A^T gathers a vector of m floats: from LDS when it fits a block's LDS
+  // (the columns then get at most 64 lanes)
+  const LdsShape lds_cols = lds_shape(m);
+  const int g_cols_lds = std::min(g_cols, 64);
+  hipError_t lds_err = hipSuccess;
   auto aprod2 = [&](float a, const float *yy, float *xx) {
-    launch_spmv(g_cols, true, n, colptr, crow, cval, yy, a, xx, st);
+    if (lds_cols.blocks > 0) {
+      const hipError_t e =
+          launch_spmv_lds(g_cols_lds, lds_cols, n, m, colptr, crow, cval, yy, a, xx, st);
+      if (e != hipSuccess) lds_err = e;
+    } else {
+      launch_spmv(g_cols, true, n, colptr, crow, cval, yy, a, xx, st);
+    }
   };
   // the scalar a reduction left in red[k], read back
   auto read_red = [&](int k, double *out) -> int {
@@ -408,6 +521,7 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   if (beta > 0.0f) {
     scale(m, 1.0f / beta, u);
     aprod2(0.0f, u, v);
+    if (lds_err != hipSuccess) return fail("lsmr: LDS product launch failed");
     if (norm(n, v, &alpha)) return fail("lsmr: norm failed");
   }
   if (alpha > 0.0f) scale(n, 1.0f / alpha, v);
@@ -538,6 +652,7 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
     }
   }
 #undef LCHK
+  if (lds_err != hipSuccess) return fail("lsmr: LDS product launch failed");
   if (damped && istop == 2) istop = 3;
   if (hipStreamSynchronize(st) != hipSuccess) return fail("lsmr: stream failed");
   if (istop_out) *istop_out = istop;
