@@ -84,8 +84,10 @@ def lsmr_main(args, nx=256, ny=256, nz=17, kmax=120, nrows=20000, it_gpu=40, it_
     over the 256 x 256 x 17 grid (the rows jdsg_frechet_rows builds from random
     depth kernels; 254*254*16 model cells), b random, damp 0, run for a fixed
     iteration count (atol = btol = conlim = 0, so istop = 7).  Prints one JSON
-    line: GPU iterations/s from the difference of a 1- and an it_gpu-iteration
-    solve (the one-time transpose reported apart), the algorithmic bytes of an
+    line: GPU iterations/s of the cell-major solve (jdsg_lsmr_csr_layers, 16
+    layers) from the difference of a 1- and an it_gpu-iteration solve (the
+    one-time set-up reported apart; the solve in the caller's layer-major
+    numbering rides along), the algorithmic bytes of an
     iteration over its time, and the compiled reference LSMR on the host
     threads over it_cpu iterations, with the relative difference of the two
     solutions after it_cpu iterations."""
@@ -105,19 +107,24 @@ def lsmr_main(args, nx=256, ny=256, nz=17, kmax=120, nrows=20000, it_gpu=40, it_
     b = torch.randn(m, device="cuda", generator=g)
     print(f"lsmr: {m} rows, {n} columns, {nnz} nonzeros", file=sys.stderr, flush=True)
 
-    def solve(itn):
+    def solve(itn, layers):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        x, info = G.lsmr(m, n, val, col, ptr, b, 0.0, 0.0, 0.0, 0.0, itn, 10)
+        x, info = G.lsmr(m, n, val, col, ptr, b, 0.0, 0.0, 0.0, 0.0, itn, 10, layers=layers)
         torch.cuda.synchronize()
         return x, info, time.perf_counter() - t0
 
-    solve(1)                                   # warm the library and the allocator
-    t1 = min(solve(1)[2] for _ in range(3))
-    xs, infos, tk = zip(*[solve(it_gpu) for _ in range(max(1, args.steps // 2))])
-    tk = min(tk)
-    assert all(i["itn"] == it_gpu and i["istop"] == 7 for i in infos)
-    per_it = (tk - t1) / (it_gpu - 1)
+    def timing(layers):
+        solve(1, layers)                       # warm the library and the allocator
+        t1 = min(solve(1, layers)[2] for _ in range(3))
+        xs, infos, tk = zip(*[solve(it_gpu, layers) for _ in range(max(1, args.steps // 2))])
+        assert all(i["itn"] == it_gpu and i["istop"] == 7 for i in infos)
+        per_it = (min(tk) - t1) / (it_gpu - 1)
+        return per_it, t1 - per_it, xs[0]
+
+    # the caller's (layer-major) numbering, and the cell-major solve
+    per_it_plain, setup_plain, x_plain = timing(1)
+    per_it, setup, x_cell = timing(nz - 1)
     # an iteration reads each of A and A^T once (value 4 B + index 4 B per
     # nonzero, 4 B per row pointer), gathers x and writes y once (4 B each),
     # and streams n-vectors for the norms, the reorthogonalisation against 10
@@ -125,12 +132,15 @@ def lsmr_main(args, nx=256, ny=256, nz=17, kmax=120, nrows=20000, it_gpu=40, it_
     alg = 2 * (8 * nnz) + 4 * (m + 1) + 4 * (n + 1) + 8 * (m + n) * 2 \
         + 4 * n * (10 * 3 + 7 + 2) + 4 * m * 2
     rep = {"metric": "LSMR iterations/s on Frechet rows (float, local reorthogonalisation 10)",
-           "value": 1.0 / per_it, "unit": "iterations/s", "n_gpus": 1, "steps": len(xs),
+           "value": 1.0 / per_it, "unit": "iterations/s", "n_gpus": 1, "steps": max(1, args.steps // 2),
            "warmup": 1, "ms_per_step": per_it * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"lsmr_frechet_{m}x{n}", "rows": m, "columns": n,
-                      "nnz": nnz, "iterations": it_gpu},
-           "setup_ms": (t1 - per_it) * 1e3,
+                      "nnz": nnz, "iterations": it_gpu, "layers": nz - 1},
+           "setup_ms": setup * 1e3,
+           "layer_major": {"ms_per_step": per_it_plain * 1e3, "setup_ms": setup_plain * 1e3},
+           "x_rel_diff_cell_vs_layer_major": float((x_cell - x_plain).abs().max() /
+                                                   x_plain.abs().max()),
            "roofline": {"bound": "hbm", "achieved": alg / per_it / 1e9, "peak": 8000.0,
                         "unit": "GB/s", "frac": alg / per_it / 8e12, "traffic": None,
                         "note": "whole iteration (both products, norms, host syncs)"}}
@@ -146,7 +156,7 @@ def lsmr_main(args, nx=256, ny=256, nz=17, kmax=120, nrows=20000, it_gpu=40, it_
         xr, ir = pyoracle.lsmr(m, n, hv, hi, hp, hb, 0.0, 0.0, 0.0, 0.0, it_cpu, 10, kind=lib,
                                threads=threads)
         dt = time.perf_counter() - t0
-        xg = solve(it_cpu)[0].cpu().numpy()
+        xg = solve(it_cpu, nz - 1)[0].cpu().numpy()
         rep["cpu_baseline"] = {"value": it_cpu / dt, "unit": "iterations/s", "cores": threads,
                                "kind": kind, "sample": f"{it_cpu} iterations of the same solve"}
         rep["x_rel_diff_vs_cpu"] = float(np.abs(xg - xr).max() / np.abs(xr).max())

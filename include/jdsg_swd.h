@@ -318,6 +318,19 @@ int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices, const int 
                   const float *b, float damp, float atol, float btol, float conlim, int itnlim,
                   int localSize, float *x, int *istop, int *itn, float *normA, float *condA,
                   float *normr, float *normAr, float *normx, void *stream);
+/* The same solve for a matrix whose columns are numbered layer-major,
+ * c = layer * (n / layers) + cell, as SurfTime::frechet_matrix numbers them
+ * (src/SWD/SurfaceWave.cpp:630-637: depth outermost, then y, then x): the
+ * iterations run in the cell-major numbering (cell * layers + layer), where
+ * the layers of one cell share a cache line of x, and x comes back in the
+ * caller's numbering.  No counterpart in the reference (LSMR's iterates do
+ * not depend on the column order; only the rounding order changes).  layers
+ * must divide n. */
+int jdsg_lsmr_csr_layers(int m, int n, const float *val, const int *indices, const int *indptr,
+                         const float *b, float damp, float atol, float btol, float conlim,
+                         int itnlim, int localSize, float *x, int *istop, int *itn, float *normA,
+                         float *condA, float *normr, float *normAr, float *normx, int layers,
+                         void *stream);
 const char *jdsg_lsmr_last_error(void);
 
 #ifdef __cplusplus

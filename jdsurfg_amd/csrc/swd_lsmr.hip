@@ -363,6 +363,42 @@ lsmr_update(int n, float fh, float fx, float fn, const float *__restrict__ v, fl
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// Cell-major renumbering of a layer-major column numbering: column
+// c = l * nc + cell (l < layers) becomes cell * layers + l, so the layers of
+// one cell sit side by side in x.
+__device__ __forceinline__ int cell_major(int c, int nc, int layers) {
+  return (c % nc) * layers + c / nc;
+}
+
+// key[k] = (row of entry k) << nb | its cell-major column: sorting these
+// keys orders the entries by row, then by renumbered column
+__global__ void lsmr_layer_keys(long long nnz, int nc, int layers, int nb,
+                                const int *__restrict__ rowidx, const int *__restrict__ ind,
+                                unsigned long long *key) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  key[k] = ((unsigned long long)rowidx[k] << nb) |
+           (unsigned long long)cell_major(ind[k], nc, layers);
+}
+
+// the renumbered matrix: column from the sorted key, value from the entry
+// the sort moved there
+__global__ void lsmr_layer_take(long long nnz, unsigned long long mask,
+                                const unsigned long long *__restrict__ key,
+                                const int *__restrict__ perm, const float *__restrict__ val,
+                                int *lind, float *lval) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  lind[k] = (int)(key[k] & mask);
+  lval[k] = val[perm[k]];
+}
+
+// x in the caller's numbering from the cell-major solution
+__global__ void lsmr_layer_out(int n, int nc, int layers, const float *__restrict__ xs, float *x) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) x[c] = xs[cell_major(c, nc, layers)];
+}
+
 float d2norm(float a, float b) {  // lsmrModule_csr.f90:1151-1163
   const float scale = fabsf(a) + fabsf(b);
   if (scale == 0.0f) return 0.0f;
@@ -396,14 +432,14 @@ static thread_local std::string g_lsmr_err;
 extern "C" const char *jdsg_lsmr_last_error(void) { return g_lsmr_err.c_str(); }
 
 // LSMR_csr (lsmrModule_csr.f90:516-1148) with the matrix, b and x in device
-// memory.  Returns 0, or nonzero with a message on a bad argument or a HIP
+// memory; layers > 1 solves in the cell-major numbering of a layer-major
+// one.  Returns 0, or nonzero with a message on a bad argument or a HIP
 // failure.
-extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
-                             const int *indptr, const float *b, float damp, float atol,
-                             float btol, float conlim, int itnlim, int localSize, float *x,
-                             int *istop_out, int *itn_out, float *normA_out, float *condA_out,
-                             float *normr_out, float *normAr_out, float *normx_out,
-                             void *stream) {
+static int lsmr_impl(int layers, int m, int n, const float *val, const int *indices,
+                     const int *indptr, const float *b, float damp, float atol, float btol,
+                     float conlim, int itnlim, int localSize, float *x, int *istop_out,
+                     int *itn_out, float *normA_out, float *condA_out, float *normr_out,
+                     float *normAr_out, float *normx_out, void *stream) {
   auto fail = [](const char *msg) {
     g_lsmr_err = msg;
     return 1;
@@ -411,6 +447,7 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   if (m <= 0 || n <= 0 || !val || !indices || !indptr || !b || !x)
     return fail("lsmr: empty matrix or null pointer");
   if (localSize < 0) return fail("lsmr: localSize < 0");
+  if (layers < 1 || n % layers != 0) return fail("lsmr: layers must be >= 1 and divide n");
   hipStream_t st = (hipStream_t)stream;
   int nnz_i = 0;
   if (hipMemcpyAsync(&nnz_i, indptr + m, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -448,21 +485,51 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
       return fail("lsmr: indptr[0] != 0, row pointers not increasing or a column index outside "
                   "[0, n)");
   }
-  // ---- transpose: entries sorted by column, stable (row order kept)
   hipLaunchKernelGGL(lsmr_row_index, dim3((unsigned)((m + 3) / 4)), dim3(T), 0, st, m, indptr,
                      rowidx);
   hipLaunchKernelGGL(lsmr_iota, grid(nnz), dim3(T), 0, st, nnz, perm);
-  {
+  const int nbits = 32 - __builtin_clz((unsigned)std::max(n, 1));
+  // ---- the matrix the iterations use: the caller's, or its cell-major
+  // renumbering with each row's entries sorted by the new column (the row
+  // pointers stay, and so does the row of every position)
+  const float *aval = val;
+  const int *aind = indices;
+  float *xs = x;
+  if (layers > 1) {
+    const int nc = n / layers;
+    const int mbits = 32 - __builtin_clz((unsigned)std::max(m, 1));
+    unsigned long long *k_in = W.get<unsigned long long>(nnz),
+                       *k_out = W.get<unsigned long long>(nnz);
+    int *lperm = W.get<int>(nnz), *lind = W.get<int>(nnz);
+    float *lval = W.get<float>(nnz);
+    xs = W.get<float>(n);
+    if (W.err) return fail("lsmr: out of device memory");
+    hipLaunchKernelGGL(lsmr_layer_keys, grid(nnz), dim3(T), 0, st, nnz, nc, layers, nbits, rowidx,
+                       indices, k_in);
     size_t tb = 0;
-    LCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, indices, key2, perm, perm2, (int)nnz, 0,
-                                            32 - __builtin_clz((unsigned)std::max(n, 1)), st));
+    LCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k_in, k_out, perm, lperm, (int)nnz, 0,
+                                            nbits + mbits, st));
     void *tmp = W.get<char>(tb);
     if (W.err) return fail("lsmr: out of device memory");
-    LCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, indices, key2, perm, perm2, (int)nnz, 0,
-                                            32 - __builtin_clz((unsigned)std::max(n, 1)), st));
+    LCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k_in, k_out, perm, lperm, (int)nnz, 0,
+                                            nbits + mbits, st));
+    hipLaunchKernelGGL(lsmr_layer_take, grid(nnz), dim3(T), 0, st, nnz, (1ull << nbits) - 1ull,
+                       k_out, lperm, val, lind, lval);
+    aval = lval;
+    aind = lind;
+  }
+  // ---- transpose: entries sorted by column, stable (row order kept)
+  {
+    size_t tb = 0;
+    LCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, aind, key2, perm, perm2, (int)nnz, 0,
+                                            nbits, st));
+    void *tmp = W.get<char>(tb);
+    if (W.err) return fail("lsmr: out of device memory");
+    LCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, aind, key2, perm, perm2, (int)nnz, 0, nbits,
+                                            st));
   }
   hipLaunchKernelGGL(lsmr_col_bounds, grid(nnz + 1), dim3(T), 0, st, nnz, n, key2, colptr);
-  hipLaunchKernelGGL(lsmr_gather, grid(nnz), dim3(T), 0, st, nnz, perm2, val, rowidx, cval, crow);
+  hipLaunchKernelGGL(lsmr_gather, grid(nnz), dim3(T), 0, st, nnz, perm2, aval, rowidx, cval, crow);
   LCHK(hipGetLastError());
   // ---- helpers
   auto dot = [&](int len, const float *p, const float *q, double *dst) {
@@ -483,11 +550,11 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   };
   // the 16 B form needs 16-byte aligned value and index arrays (the
   // transpose's own are)
-  const bool quads_rows = (((uintptr_t)val | (uintptr_t)indices) & 15) == 0;
+  const bool quads_rows = (((uintptr_t)aval | (uintptr_t)aind) & 15) == 0;
   const int g_rows = spmv_lanes(nnz, m, quads_rows), g_cols = spmv_lanes(nnz, n, true);
   // y = a y + A x;  x = a x + A^T y
   auto aprod1 = [&](float a, const float *xx, float *yy) {
-    launch_spmv(g_rows, quads_rows, m, indptr, indices, val, xx, a, yy, st);
+    launch_spmv(g_rows, quads_rows, m, indptr, aind, aval, xx, a, yy, st);
   };
   // A^T gathers a vector of m floats: from LDS when it fits a block's LDS
   // (the columns then get at most 64 lanes)
@@ -513,7 +580,7 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   // ---- lsmrModule_csr.f90:829-866
   const bool damped = damp > 0.0f;
   LCHK(hipMemcpyAsync(u, b, sizeof(float) * (size_t)m, hipMemcpyDeviceToDevice, st));
-  LCHK(hipMemsetAsync(x, 0, sizeof(float) * (size_t)n, st));
+  LCHK(hipMemsetAsync(xs, 0, sizeof(float) * (size_t)n, st));
   LCHK(hipMemsetAsync(v, 0, sizeof(float) * (size_t)n, st));
   LCHK(hipMemsetAsync(hbar, 0, sizeof(float) * (size_t)n, st));
   float alpha = 0.0f, beta = 0.0f;
@@ -604,7 +671,7 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
       const float fx = zeta / (rho * rhobar);
       const float fn = thetanew / rho;
       hipLaunchKernelGGL(lsmr_update, dim3(kRedBlocks), dim3(T), 0, st, n, fh, fx, fn, v, h, hbar,
-                         x, part);
+                         xs, part);
       hipLaunchKernelGGL(lsmr_reduce_final, dim3(1), dim3(T), 0, st, kRedBlocks, part, red);
       // :1009-1043
       const float betaacute = chat * betadd;
@@ -653,6 +720,8 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   }
 #undef LCHK
   if (lds_err != hipSuccess) return fail("lsmr: LDS product launch failed");
+  if (layers > 1)
+    hipLaunchKernelGGL(lsmr_layer_out, grid(n), dim3(T), 0, st, n, n / layers, layers, xs, x);
   if (damped && istop == 2) istop = 3;
   if (hipStreamSynchronize(st) != hipSuccess) return fail("lsmr: stream failed");
   if (istop_out) *istop_out = istop;
@@ -663,4 +732,23 @@ extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
   if (normAr_out) *normAr_out = normAr;
   if (normx_out) *normx_out = normx;
   return 0;
+}
+
+extern "C" int jdsg_lsmr_csr(int m, int n, const float *val, const int *indices,
+                             const int *indptr, const float *b, float damp, float atol,
+                             float btol, float conlim, int itnlim, int localSize, float *x,
+                             int *istop, int *itn, float *normA, float *condA, float *normr,
+                             float *normAr, float *normx, void *stream) {
+  return lsmr_impl(1, m, n, val, indices, indptr, b, damp, atol, btol, conlim, itnlim, localSize,
+                   x, istop, itn, normA, condA, normr, normAr, normx, stream);
+}
+
+extern "C" int jdsg_lsmr_csr_layers(int m, int n, const float *val, const int *indices,
+                                    const int *indptr, const float *b, float damp, float atol,
+                                    float btol, float conlim, int itnlim, int localSize,
+                                    float *x, int *istop, int *itn, float *normA, float *condA,
+                                    float *normr, float *normAr, float *normx, int layers,
+                                    void *stream) {
+  return lsmr_impl(layers, m, n, val, indices, indptr, b, damp, atol, btol, conlim, itnlim,
+                   localSize, x, istop, itn, normA, condA, normr, normAr, normx, stream);
 }

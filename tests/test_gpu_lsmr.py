@@ -26,7 +26,7 @@ def _gpu(m, n, val, ind, ptr, b, **kw):
 def _compare(m, n, val, ind, ptr, b, **kw):
     x, info = _gpu(m, n, val, ind, ptr, b, **kw)
     xr, ir = pyoracle.lsmr(m, n, val, ind, ptr, b, **{k: v for k, v in kw.items()
-                                                    if k != "local_size"},
+                                                    if k not in ("local_size", "layers")},
                            local_size=kw.get("local_size", 10))
     err = np.abs(x - xr).max() / np.abs(xr).max()
     print(f"gpu {info}\noracle {ir}\nmax |dx| / max |x| {err:.2e}", flush=True)
@@ -52,13 +52,14 @@ def test_gpu_lsmr_matches_oracle(m, n, density, damp, local):
     _compare(m, n, val, ind, ptr, b, damp=damp, local_size=local)
 
 
-def test_gpu_lsmr_on_frechet_rows():
+@pytest.mark.parametrize("layers", [1, 5])
+def test_gpu_lsmr_on_frechet_rows(layers):
     """Rows of straight rays through a 16x14x6 grid (the Frechet assembly of
     the GPU path), b from a known model plus noise, damped."""
     from jdsurfg_amd import frechet as F
     import torch
     nx, ny, nz, kmax, nrows = 16, 14, 6, 4, 400
-    print("case lsmr on frechet rows", flush=True)
+    print(f"case lsmr on frechet rows, layers {layers}", flush=True)
     rng = np.random.default_rng(5)
     K = rng.normal(size=(nx * ny, nz, kmax)) * 0.1
     off, cell, w = F.straight_rays(nx, ny, nrows, seed=6)
@@ -74,7 +75,8 @@ def test_gpu_lsmr_on_frechet_rows():
     for r in range(nrows):
         np.add.at(A[r], ind[ptr[r]:ptr[r + 1]], v[ptr[r]:ptr[r + 1]])
     b = (A @ xt + 0.01 * rng.normal(size=nrows)).astype(np.float32)
-    _compare(nrows, n, v, ind, ptr, b, damp=0.05, local_size=10)
+    # layers = nz - 1: the solve runs cell-major, x comes back layer-major
+    _compare(nrows, n, v, ind, ptr, b, damp=0.05, local_size=10, layers=layers)
 
 
 def test_gpu_lsmr_rejects_bad_matrix():
@@ -142,11 +144,19 @@ def test_gpu_lsmr_at_frechet_width_tracks_double():
     kw = dict(damp=0.0, atol=0.0, btol=0.0, conlim=0.0, itnlim=3, local_size=10)
     x64, i64 = pyoracle.lsmr(m, n, val, ind, ptr, b, kind="f64", **kw)
     xf, _ = pyoracle.lsmr(m, n, val, ind, ptr, b, kind="rest", **kw)
-    x, info = _gpu(m, n, val, ind, ptr, b, **kw)
     scale = np.abs(x64).max()
-    e_gpu = np.abs(x - x64).max() / scale
     e_ref = np.abs(xf - x64).max() / scale
-    print(f"vs double: gpu {e_gpu:.2e}, float restatement {e_ref:.2e}", flush=True)
-    assert info["itn"] == 3 and info["istop"] == 7
-    assert e_gpu <= 1e-4
-    assert e_gpu <= e_ref
+    for layers in (1, nz - 1):
+        x, info = _gpu(m, n, val, ind, ptr, b, layers=layers, **kw)
+        e_gpu = np.abs(x - x64).max() / scale
+        print(f"vs double: gpu (layers {layers}) {e_gpu:.2e}, float restatement {e_ref:.2e}",
+              flush=True)
+        assert info["itn"] == 3 and info["istop"] == 7
+        assert e_gpu <= 1e-4
+        assert e_gpu <= e_ref
+
+
+def test_gpu_lsmr_layers_must_divide_columns():
+    A, ptr, ind, val, b = sparse_problem(50, 21, 0.1, 4)
+    with pytest.raises(RuntimeError, match="layers"):
+        _gpu(50, 21, val, ind, ptr, b, layers=4)
