@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="grid256",
-                    help="grid256 (the default line) or lsmr (LSMR on Frechet rows)")
+                    help="grid256 (the default line), lsmr (LSMR on Frechet rows) or smooth "
+                         "(gradient smoothing)")
     ap.add_argument("--cpu-sample", type=int, default=384,
                     help="columns in the CPU baseline sample (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -163,6 +164,72 @@ def lsmr_main(args, nx=256, ny=256, nz=17, kmax=120, nrows=20000, it_gpu=40, it_
     print(json.dumps(rep), flush=True)
 
 
+def smooth_main(args, nx=256, ny=256):
+    """DSurfTomo::smoothing (surftomo.cpp:147-178) of a random gradient over a
+    256 x 256-node model with 17 irregular depths 0-100 km (finest step 1 km,
+    so 80 regular depths): the spherical form (sigma_h 20 km, sigma_v 5 km,
+    0.1 degree nodes) and the Cartesian form (sigma_h 3, sigma_v 2 nodes).
+    Prints one JSON line: node-steps/s of the spherical driver (each explicit
+    diffusion step updates every node of the 254 x 254 x 80 regular grid), the
+    Cartesian driver beside it, and the oracle restatement on the host (one
+    thread) over a few steps of the same grid."""
+    import numpy as np
+    import torch
+    from jdsurfg_amd import smooth as S
+    dep = np.array([0, 1, 2, 4, 6, 8, 10, 15, 20, 25, 30, 40, 50, 60, 70, 80, 100], np.float32)
+    lat = (40.0 - 0.1 * np.arange(nx)).astype(np.float32)
+    lon = (100.0 + 0.1 * np.arange(ny)).astype(np.float32)
+    g0 = torch.randn((dep.size - 1, ny - 2, nx - 2), generator=torch.Generator().manual_seed(0))
+    g0 = g0.cuda()
+    nz1 = int((dep[-2] - dep[0]) / (1.0 * 0.99))
+    nodes = (nx - 2) * (ny - 2) * nz1
+
+    def run(in_km, sh, sv):
+        g = g0.clone()
+        S.smooth_gradient(g, dep, lat, lon, in_km, sh, sv)      # warm-up
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(1, args.steps // 2)):
+            g = g0.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            S.smooth_gradient(g, dep, lat, lon, in_km, sh, sv)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    from oracle import pyoracle as P
+    dz = float(np.float32((dep[-2] - dep[0]) / np.float32(nz1 - 1)))
+    geo = P.sph_geometry(nx - 2, ny - 2, nz1, 0.1, 0.1, dz, float(lat[1]), float(lon[1]), 0.0,
+                         20.0, 5.0)
+    nstep_sph = geo["nstep"]
+    nstep_cart = int(float(np.float32(9.0)) / (2.0 * float(np.float32(1.0 / 12))))
+    t_sph = run(1, 20.0, 5.0)
+    t_cart = run(0, 3.0, 2.0)
+    rep = {"metric": "gradient smoothing node-steps/s (spherical PDE smoother, 254x254x80)",
+           "value": nodes * nstep_sph / t_sph, "unit": "node-steps/s", "n_gpus": 1,
+           "steps": max(1, args.steps // 2), "warmup": 1, "ms_per_step": t_sph * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
+           "data": "synthetic",
+           "config": {"workload": "smooth_gradient_256x256x17", "regular_depths": nz1,
+                      "sph_steps": nstep_sph, "cart_steps": nstep_cart},
+           "cartesian": {"ms": t_cart * 1e3, "node_steps_per_s": nodes * nstep_cart / t_cart}}
+    if not args.no_cpu:
+        g = np.random.default_rng(0).normal(size=(nz1, ny - 2, nx - 2)).astype(np.float32)
+        # sigma_v = sigma_h giving 4 steps of the same grid
+        h2 = float(geo["Ch"]) / float(np.float32(1.0 / 7)) * (20.0 ** 2 / 20.0 ** 2)
+        sig = float(np.sqrt(4.5 * 2 * (1.0 / 7) * h2))
+        ns = P.sph_geometry(nx - 2, ny - 2, nz1, 0.1, 0.1, dz, float(lat[1]), float(lon[1]), 0.0,
+                            sig, sig)["nstep"]
+        t0 = time.perf_counter()
+        P.smooth_sph_pde(g, 0.1, 0.1, dz, float(lat[1]), float(lon[1]), 0.0, sig, sig)
+        dt = time.perf_counter() - t0
+        rep["cpu_baseline"] = {"value": nodes * ns / dt, "unit": "node-steps/s", "cores": 1,
+                               "kind": "port", "sample": f"{ns} steps of the spherical smoother "
+                               "on the same grid (the reference needs Eigen, absent)"}
+    print(json.dumps(rep), flush=True)
+
+
 def timed(fn, steps, warmup):
     """Run fn warmup times, then time `steps` calls between device syncs."""
     import torch
@@ -186,6 +253,8 @@ def main():
         sys.exit(2)
     if args.workload == "lsmr":
         return lsmr_main(args)
+    if args.workload == "smooth":
+        return smooth_main(args)
     import numpy as np
     import torch
     from jdsurfg_amd import swd

@@ -333,6 +333,42 @@ int jdsg_lsmr_csr_layers(int m, int n, const float *val, const int *indices, con
                          void *stream);
 const char *jdsg_lsmr_last_error(void);
 
+/* ---- model smoothing and regularisation rows (device arrays, column-major,
+ * x fastest; 0 on success, else nonzero with jdsg_smooth_last_error) ---- */
+
+/* smooth_cart_pde (src/shared/smoothing.cpp:177-249): Gaussian smoothing by
+ * explicit diffusion steps on a grid of unit spacing, in place; the
+ * finite-difference coefficients are the exact central-difference rationals
+ * (the reference solves for them with a float LU). */
+int jdsg_smooth_cart_pde(float *grad, int nx, int ny, int nz, float sigma_h, float sigma_v,
+                         void *stream);
+/* smooth_sph_pde (smoothing.cpp:256-362): the same in spherical coordinates,
+ * nodes (colatitude from lat0, longitude from lon0, depth from z0) spaced
+ * dx, dy degrees and dz km, in place. */
+int jdsg_smooth_sph_pde(float *grad, int nx, int ny, int nz, float dx, float dy, float dz,
+                        float lat0, float lon0, float z0, float sigma_h, float sigma_v,
+                        void *stream);
+/* interp_irregular_z (smoothing.cpp:122-170): fwd != 0 fills gradr (nx, ny,
+ * nz1) from grad (nx, ny, nz) on the increasing host depths dep[nz]; else
+ * fills grad from gradr. */
+int jdsg_interp_irregular_z(float *grad, float *gradr, int nx, int ny, int nz, int nz1,
+                            const float *dep, int fwd, void *stream);
+/* DSurfTomo::smoothing (src/surftomo/surftomo.cpp:147-178): grad of shape
+ * (nx-2, ny-2, nz-1) for a model of nx x ny x nz nodes with host axes lat[nx],
+ * lon[ny], dep[nz]: resampled to a regular depth grid, smoothed (spherical
+ * when smooth_in_km, else Cartesian), resampled back, in place. */
+int jdsg_smooth_gradient(float *grad, int nx, int ny, int nz, const float *dep, const float *lat,
+                         const float *lon, int smooth_in_km, float sigma_h, float sigma_v,
+                         void *stream);
+/* add_regularization (src/shared/csr_matrix.cpp:268-328): appends the
+ * nx*ny*nz Laplacian rows (2w on a face node; 6w and six -w inside) after the
+ * m rows of a CSR matrix in device memory, entries from ptr[m] on; ptr holds
+ * m + nx*ny*nz + 1 ints and col / val cap entries.  *nnz = the new entry
+ * count; fails, as the reference exits, when cap is too small. */
+int jdsg_add_regularization(int m, int *ptr, int *col, float *val, long long cap, float weight,
+                            int nx, int ny, int nz, long long *nnz, void *stream);
+const char *jdsg_smooth_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -355,3 +355,91 @@ def empirical_relation(vs):
         vp.ravel()[i] = a.value
         rho.ravel()[i] = b.value
     return vp, rho
+
+
+# ---- model smoothing and regularisation rows (smooth_oracle.c)
+def _smooth_lib():
+    L = C.CDLL(LIB_REST)
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    L.or_fdcoefs.argtypes = [C.c_int, C.c_int, fp]
+    L.or_interp_irregular_z.argtypes = [fp, fp] + [C.c_int] * 4 + [fp, C.c_int]
+    L.or_smooth_cart_pde.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float]
+    L.or_sph_geometry.argtypes = [C.c_int] * 3 + [C.c_float] * 8 + [fp] * 5
+    L.or_sph_geometry.restype = C.c_int
+    L.or_smooth_sph_pde.argtypes = [fp, C.c_int, C.c_int, C.c_int] + [C.c_float] * 8
+    L.or_add_regularization.argtypes = [C.c_int, ip, ip, fp, C.c_float, C.c_int, C.c_int, C.c_int]
+    L.or_add_regularization.restype = C.c_longlong
+    L.or_smooth_gradient.argtypes = [fp, C.c_int, C.c_int, C.c_int, fp, fp, fp, C.c_int,
+                                     C.c_float, C.c_float]
+    L.or_smooth_gradient.restype = C.c_int
+    return L
+
+
+def fdcoefs(acc, deriv):
+    """Central finite-difference coefficients (smoothing.cpp:84-110)."""
+    c = np.zeros(acc + 1, np.float32)
+    _smooth_lib().or_fdcoefs(acc, deriv, c)
+    return c
+
+
+def interp_irregular_z(grad, nz1, dep, fwd, gradr=None):
+    """grad: (nz, ny, nx) float32 (x fastest).  fwd: returns gradr (nz1, ny, nx);
+    else fills and returns grad from gradr (smoothing.cpp:122-170)."""
+    nz, ny, nx = grad.shape
+    grad = np.ascontiguousarray(grad, np.float32).copy()
+    gradr = np.zeros((nz1, ny, nx), np.float32) if gradr is None else \
+        np.ascontiguousarray(gradr, np.float32).copy()
+    _smooth_lib().or_interp_irregular_z(grad, gradr, nx, ny, nz, nz1,
+                                        np.ascontiguousarray(dep, np.float32), int(fwd))
+    return gradr if fwd else grad
+
+
+def smooth_cart_pde(grad, sigma_h, sigma_v):
+    """grad: (nz, ny, nx) float32; returns the smoothed copy (smoothing.cpp:177-249)."""
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    nz, ny, nx = g.shape
+    _smooth_lib().or_smooth_cart_pde(g, nx, ny, nz, sigma_h, sigma_v)
+    return g
+
+
+def sph_geometry(nx, ny, nz, dx, dy, dz, lat0, lon0, z0, sigma_h, sigma_v):
+    """Padded-grid radius, colatitude, sin, cos, (Ch, Cv, dr, dth, dphi) and
+    the step count of smooth_sph_pde."""
+    radi = np.zeros(nz + 6, np.float32)
+    th, s, c = (np.zeros(nx + 6, np.float32) for _ in range(3))
+    sc = np.zeros(5, np.float32)
+    nstep = _smooth_lib().or_sph_geometry(nx, ny, nz, dx, dy, dz, lat0, lon0, z0, sigma_h,
+                                          sigma_v, radi, th, s, c, sc)
+    return {"radi": radi, "theta": th, "sin": s, "cos": c, "Ch": sc[0], "Cv": sc[1],
+            "dr": sc[2], "dth": sc[3], "dphi": sc[4], "nstep": nstep}
+
+
+def smooth_sph_pde(grad, dx, dy, dz, lat0, lon0, z0, sigma_h, sigma_v):
+    """grad: (nz, ny, nx) float32 over (r, lon, colat); returns the smoothed
+    copy (smoothing.cpp:256-362)."""
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    nz, ny, nx = g.shape
+    _smooth_lib().or_smooth_sph_pde(g, nx, ny, nz, dx, dy, dz, lat0, lon0, z0, sigma_h, sigma_v)
+    return g
+
+
+def add_regularization(ptr, col, val, m, weight, nx, ny, nz):
+    """Append the nx*ny*nz regularisation rows after m rows of a CSR whose
+    arrays have room (csr_matrix.cpp:268-328); returns (ptr, col, val, nnz)."""
+    ptr = np.ascontiguousarray(ptr, np.int32).copy()
+    col = np.ascontiguousarray(col, np.int32).copy()
+    val = np.ascontiguousarray(val, np.float32).copy()
+    nnz = _smooth_lib().or_add_regularization(m, ptr, col, val, weight, nx, ny, nz)
+    return ptr, col, val, int(nnz)
+
+
+def smooth_gradient(grad, nx, ny, nz, dep, lat, lon, smooth_in_km, sigma_h, sigma_v):
+    """DSurfTomo::smoothing (surftomo.cpp:147-178) on grad of shape
+    (nz-1, ny-2, nx-2) (x fastest); returns the smoothed copy."""
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    if _smooth_lib().or_smooth_gradient(g, nx, ny, nz, f(dep), f(lat), f(lon), int(smooth_in_km),
+                                        sigma_h, sigma_v):
+        raise ValueError("depths do not give a regular grid")
+    return g

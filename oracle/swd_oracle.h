@@ -174,6 +174,22 @@ void or_lsmr_csr_f64(int m, int n, const double *val, const int *ind, const int 
                  double *x, int *istop, int *itn, double *normA, double *condA, double *normr,
                  double *normAr, double *normx);
 
+/* smooth_oracle.c: model smoothing (smoothing.cpp) and regularisation rows
+ * (csr_matrix.cpp:268-328) */
+void or_fdcoefs(int acc, int deriv, float *coefs);
+void or_interp_irregular_z(float *grad, float *gradr, int nx, int ny, int nz, int nz1,
+                           const float *dep, int fwd);
+void or_smooth_cart_pde(float *grad, int nx, int ny, int nz, float sigma_h, float sigma_v);
+int or_sph_geometry(int nx, int ny, int nz, float dx, float dy, float dz, float lat0, float lon0,
+                    float z0, float sigma_h, float sigma_v, float *radi, float *theta,
+                    float *sinth, float *costh, float *scal);
+void or_smooth_sph_pde(float *grad, int nx, int ny, int nz, float dx, float dy, float dz,
+                       float lat0, float lon0, float z0, float sigma_h, float sigma_v);
+long long or_add_regularization(int m, int *ptr, int *col, float *val, float weight, int nx,
+                                int ny, int nz);
+int or_smooth_gradient(float *grad, int nx, int ny, int nz, const float *dep, const float *lat,
+                       const float *lon, int smooth_in_km, float sigma_h, float sigma_v);
+
 #ifdef __cplusplus
 }
 #endif
