@@ -373,6 +373,7 @@ def _smooth_lib():
     L.or_smooth_gradient.argtypes = [fp, C.c_int, C.c_int, C.c_int, fp, fp, fp, C.c_int,
                                      C.c_float, C.c_float]
     L.or_smooth_gradient.restype = C.c_int
+    L.or_smooth_sph_conv.argtypes = [fp, C.c_int, C.c_int, C.c_int] + [C.c_float] * 8
     return L
 
 
@@ -442,4 +443,13 @@ def smooth_gradient(grad, nx, ny, nz, dep, lat, lon, smooth_in_km, sigma_h, sigm
     if _smooth_lib().or_smooth_gradient(g, nx, ny, nz, f(dep), f(lat), f(lon), int(smooth_in_km),
                                         sigma_h, sigma_v):
         raise ValueError("depths do not give a regular grid")
+    return g
+
+
+def smooth_sph_conv(grad, dx, dy, dz, lat0, lon0, z0, sigma_h, sigma_v):
+    """The direct Gaussian convolution smooth_sph (smoothing.cpp:21-63), in
+    double; grad (nz, ny, nx) float32; returns the smoothed copy."""
+    g = np.ascontiguousarray(grad, np.float32).copy()
+    nz, ny, nx = g.shape
+    _smooth_lib().or_smooth_sph_conv(g, nx, ny, nz, dx, dy, dz, lat0, lon0, z0, sigma_h, sigma_v)
     return g

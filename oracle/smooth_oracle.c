@@ -301,3 +301,57 @@ int or_smooth_gradient(float *grad, int nx, int ny, int nz, const float *dep, co
   free(gr);
   return 0;
 }
+
+/* spherical.cpp:9-26 delsph: great-circle distance (km, R = 6371) between
+ * two (colatitude, longitude) points in radians, by the haversine formula;
+ * in double here */
+static double delsph_d(double c1, double l1, double c2, double l2) {
+  const double dlat = c2 - c1, dlon = l2 - l1, la1 = M_PI * 0.5 - c1, la2 = M_PI * 0.5 - c2;
+  const double a = sin(dlat * 0.5) * sin(dlat * 0.5) +
+                   sin(dlon * 0.5) * sin(dlon * 0.5) * cos(la1) * cos(la2);
+  return 6371.0 * 2.0 * atan2(sqrt(a), sqrt(1 - a));
+}
+
+/* smoothing.cpp:21-63 smooth_sph: the direct Gaussian convolution the PDE
+ * smoother approximates, grad[n] = sum_m g[m] w(n, m) / sum_m w(n, m) with
+ * w = exp(-(delta r_m)^2 / 2 sigma_h^2) exp(-(r_n - r_m)^2 / 2 sigma_v^2),
+ * delta the angular distance; in double (a yardstick, not the reference's
+ * float bits), the horizontal factor taken once per pair of columns */
+void or_smooth_sph_conv(float *grad, int nx, int ny, int nz, float dx, float dy, float dz,
+                        float lat0, float lon0, float z0, float sigma_h, float sigma_v) {
+  const double R = 6371.0, d2r = M_PI / 180.0;
+  const size_t nxy = (size_t)nx * ny, n = nxy * nz;
+  float *g1 = malloc(sizeof(float) * n);
+  memcpy(g1, grad, sizeof(float) * n);
+  double *r = malloc(sizeof(double) * nz), *V = malloc(sizeof(double) * nz * nz);
+  double *sk = malloc(sizeof(double) * nz), *sn = malloc(sizeof(double) * nz);
+  double *h = malloc(sizeof(double) * nz);
+  for (int k = 0; k < nz; ++k) r[k] = R - (z0 + k * dz);
+  for (int k = 0; k < nz; ++k)
+    for (int k1 = 0; k1 < nz; ++k1)
+      V[k * nz + k1] = exp(-(r[k] - r[k1]) * (r[k] - r[k1]) * 0.5 / ((double)sigma_v * sigma_v));
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double c = d2r * (90. - lat0 + i * dx), l = d2r * (lon0 + j * dy);
+      for (int k = 0; k < nz; ++k) sk[k] = sn[k] = 0.0;
+      for (int j1 = 0; j1 < ny; ++j1)
+        for (int i1 = 0; i1 < nx; ++i1) {
+          const double del = delsph_d(c, l, d2r * (90. - lat0 + i1 * dx), d2r * (lon0 + j1 * dy)) / R;
+          for (int k1 = 0; k1 < nz; ++k1)
+            h[k1] = exp(-(del * r[k1]) * (del * r[k1]) * 0.5 / ((double)sigma_h * sigma_h));
+          for (int k = 0; k < nz; ++k)
+            for (int k1 = 0; k1 < nz; ++k1) {
+              const double w = h[k1] * V[k * nz + k1];
+              sk[k] += g1[k1 * nxy + (size_t)j1 * nx + i1] * w;
+              sn[k] += w;
+            }
+        }
+      for (int k = 0; k < nz; ++k) grad[k * nxy + (size_t)j * nx + i] = (float)(sk[k] / sn[k]);
+    }
+  free(g1);
+  free(r);
+  free(V);
+  free(sk);
+  free(sn);
+  free(h);
+}

@@ -187,6 +187,8 @@ void or_smooth_sph_pde(float *grad, int nx, int ny, int nz, float dx, float dy, 
                        float lat0, float lon0, float z0, float sigma_h, float sigma_v);
 long long or_add_regularization(int m, int *ptr, int *col, float *val, float weight, int nx,
                                 int ny, int nz);
+void or_smooth_sph_conv(float *grad, int nx, int ny, int nz, float dx, float dy, float dz,
+                        float lat0, float lon0, float z0, float sigma_h, float sigma_v);
 int or_smooth_gradient(float *grad, int nx, int ny, int nz, const float *dep, const float *lat,
                        const float *lon, int smooth_in_km, float sigma_h, float sigma_v);
 

@@ -99,3 +99,17 @@ def test_gpu_regularization_rows_are_the_oracle_rows():
     small = torch.zeros(nnz - 1, dtype=torch.int32, device="cuda")
     with pytest.raises(RuntimeError, match="sparse ratio"):
         S.add_regularization(dp, small, dv, m, w, nx, ny, nz)
+
+
+def test_gpu_sph_smoother_on_the_reference_test_case():
+    """The reference's own smoothing case (test/smooth): the GPU smoother gives
+    the oracle's bits and stays within 1 % of the direct convolution."""
+    from jdsurfg_amd import smooth as S
+    from tests.test_smooth import REF_CASE, box_field
+    print("case smooth reference test/smooth", flush=True)
+    g = box_field()
+    d = _cuda(g)
+    S.smooth_sph_pde(d, *REF_CASE)
+    got = d.cpu().numpy()
+    assert np.array_equal(got, pyoracle.smooth_sph_pde(g, *REF_CASE))
+    assert np.abs(got - pyoracle.smooth_sph_conv(g, *REF_CASE)).max() <= 0.02
