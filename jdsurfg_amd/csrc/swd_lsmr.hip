@@ -321,15 +321,29 @@ __global__ void lsmr_scale(int n, float a, float *x) {
   if (i < n) x[i] = a * x[i];
 }
 
-// v -= d qp with d = (float) *dd, then part[block] = the block's share of
-// v . qn (v . v if qn is null) over the new v, in double: one pass per
-// vector of the reorthogonalisation (the next dot product, or the norm of v
-// after the last vector, rides on the subtraction)
+// the sum of part[0..nb) in a fixed order, taken by a whole block (every
+// block that calls it gets the same bits)
+__device__ double block_sum_fixed(int nb, const double *__restrict__ part, double *sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return t;
+}
+
+// v -= d qp with d = (float) (the fixed-order sum of the previous pass's
+// partials part_in[0..nb)), then part[block] = the block's share of v . qn
+// (v . v if qn is null) over the new v, in double: one pass per vector of
+// the reorthogonalisation (the next dot product, or the norm of v after the
+// last vector, rides on the subtraction)
 __global__ void __launch_bounds__(256)
-lsmr_sub_dot(int n, const double *__restrict__ dd, const float *__restrict__ qp,
+lsmr_sub_dot(int n, int nb, const double *__restrict__ part_in, const float *__restrict__ qp,
              const float *__restrict__ qn, float *v, double *part) {
   __shared__ double sh[4];
-  const float d = (float)(*dd);
+  const float d = (float)block_sum_fixed(nb, part_in, sh);
   double s = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float w = v[i] - d * qp[i];
@@ -461,7 +475,8 @@ static int lsmr_impl(int layers, int m, int n, const float *val, const int *indi
   float *u = W.get<float>(m), *v = W.get<float>(n), *h = W.get<float>(n),
         *hbar = W.get<float>(n);
   float *lv = W.get<float>((size_t)n * std::max(localVecs, 1));
-  double *part = W.get<double>(kRedBlocks), *red = W.get<double>(1 + (size_t)localVecs);
+  double *part = W.get<double>(kRedBlocks), *part2 = W.get<double>(kRedBlocks),
+         *red = W.get<double>(1 + (size_t)localVecs);
   int *rowidx = W.get<int>(nnz), *perm = W.get<int>(nnz), *perm2 = W.get<int>(nnz),
       *key2 = W.get<int>(nnz), *colptr = W.get<int>((size_t)n + 1), *crow = W.get<int>(nnz),
       *cnt = W.get<int>(1);
@@ -631,14 +646,17 @@ static int lsmr_impl(int layers, int m, int n, const float *val, const int *indi
         aprod2(-beta, u, v);
         if (localOrtho) {  // localVOrtho: v -= (v . q_c) q_c in turn, then ||v||
           const int lim = localFull ? localVecs : localPointer;
-          dot(n, v, lv, red + 1);
+          // the partials of v . q_c alternate between part and part2; each
+          // pass sums the previous one's partials itself
+          double *pin = part, *pout = part2;
+          hipLaunchKernelGGL(lsmr_reduce_f, dim3(kRedBlocks), dim3(T), 0, st, n, v, lv, pin);
           for (int c = 1; c <= lim; ++c) {
             const float *qn = c < lim ? lv + (size_t)c * n : nullptr;
-            double *dst = c < lim ? red + 1 + c : red;
-            hipLaunchKernelGGL(lsmr_sub_dot, dim3(kRedBlocks), dim3(T), 0, st, n, red + c,
-                               lv + (size_t)(c - 1) * n, qn, v, part);
-            hipLaunchKernelGGL(lsmr_reduce_final, dim3(1), dim3(T), 0, st, kRedBlocks, part, dst);
+            hipLaunchKernelGGL(lsmr_sub_dot, dim3(kRedBlocks), dim3(T), 0, st, n, kRedBlocks, pin,
+                               lv + (size_t)(c - 1) * n, qn, v, pout);
+            std::swap(pin, pout);
           }
+          hipLaunchKernelGGL(lsmr_reduce_final, dim3(1), dim3(T), 0, st, kRedBlocks, pin, red);
           double s = 0.0;
           if (read_red(0, &s)) return fail("lsmr: norm failed");
           alpha = (float)std::sqrt(s);
