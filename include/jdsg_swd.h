@@ -369,6 +369,26 @@ int jdsg_add_regularization(int m, int *ptr, int *col, float *val, long long cap
                             int nx, int ny, int nz, long long *nnz, void *stream);
 const char *jdsg_smooth_last_error(void);
 
+/* ---- surface-wave data file (host code) ---- */
+
+/* SurfTime::read_swd_data (src/SWD/io.cpp:29-238, read_receiver :6-27): the
+ * period tables and the station pairs of surfdataSC.dat.  Wave type codes:
+ * 0 Rc, 1 Rg, 2 Lc, 3 Lg; periods 0-based; positions as colatitude and
+ * longitude in radians; distances (km, haversine) and observed times
+ * (distance / velocity) as float.  0 on success, else nonzero with
+ * jdsg_io_last_error (the reference prints and exits). */
+typedef struct jdsg_swd_data jdsg_swd_data;
+int jdsg_read_swd_data(const char *path, jdsg_swd_data **out);
+void jdsg_swd_data_sizes(const jdsg_swd_data *d, int *nmode, int *nt, long long *npairs,
+                         long long *nobs);
+/* nmode and nt hold 4 ints (Rc, Rg, Lc, Lg) */
+int jdsg_swd_data_table(const jdsg_swd_data *d, int wave, float *t, int *kmax);
+int jdsg_swd_data_arrays(const jdsg_swd_data *d, int *pair_off, int *wave, int *mode, int *period,
+                         float *src_colat, float *src_lon, float *rcv_colat, float *rcv_lon,
+                         float *dist, float *obst);
+void jdsg_swd_data_free(jdsg_swd_data *d);
+const char *jdsg_io_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,208 @@
+// swd_io.cpp -- the surface-wave data file (surfdataSC.dat) read into the
+// arrays the GPU path consumes: SurfTime::read_swd_data
+// (src/SWD/io.cpp:29-238) with read_receiver (:6-27).
+//
+// Layout of the file: four header blocks (Rayleigh phase, Rayleigh group,
+// Love phase, Love group), each a line "nmode k_0 .. k_{nmode-1}" followed by
+// nmode lines of periods; then, for every source station, a line
+// "# lat lon mode period wavetype veltype" followed by its receivers
+// "lat lon velocity".  Positions are stored as colatitude and longitude in
+// radians, distances in km (haversine, R = 6371), observed times as
+// distance / velocity, all float as in the reference.  Host code only.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Table {
+  int nmode = 0, nt = 0;
+  std::vector<int> kmax;  // periods per mode
+  std::vector<float> t;   // nt x nmode, column-major, -1 where a mode has fewer
+};
+
+thread_local std::string g_err;
+
+// spherical.cpp:9-26 delsph, in float
+float delsph(float c1, float l1, float c2, float l2) {
+  const float R = 6371.0f;
+  const float dlat = c2 - c1, dlon = l2 - l1;
+  const float la1 = (float)(M_PI * 0.5 - c1), la2 = (float)(M_PI * 0.5 - c2);
+  const float a = std::sin(dlat * 0.5f) * std::sin(dlat * 0.5f) +
+                  std::sin(dlon * 0.5f) * std::sin(dlon * 0.5f) * std::cos(la1) * std::cos(la2);
+  const float c = 2.0f * std::atan2(std::sqrt(a), std::sqrt(1 - a));
+  return R * c;
+}
+
+// io.cpp:57-87: "nmode k..." then nmode lines of periods (strtof, as the
+// reference parses them; missing numbers read as 0)
+bool read_table(std::ifstream &in, Table &T) {
+  std::string line;
+  if (!std::getline(in, line)) return false;
+  const char *p = line.c_str();
+  char *e = nullptr;
+  T.nmode = (int)std::strtof(p, &e);
+  p = e;
+  if (T.nmode < 0 || T.nmode > 1000) return false;
+  T.kmax.assign(T.nmode, 0);
+  T.nt = 0;
+  for (int i = 0; i < T.nmode; ++i) {
+    T.kmax[i] = (int)std::strtof(p, &e);
+    p = e;
+    T.nt = std::max(T.nt, T.kmax[i]);
+  }
+  if (T.nt < 0 || T.nt > 100000) return false;
+  T.t.assign((size_t)T.nt * T.nmode, -1.0f);
+  for (int im = 0; im < T.nmode; ++im) {
+    if (!std::getline(in, line)) return false;
+    p = line.c_str();
+    for (int it = 0; it < T.nt; ++it) {
+      T.t[(size_t)im * T.nt + it] = std::strtof(p, &e);
+      p = e;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+struct jdsg_swd_data {
+  Table tab[4];                 // Rc, Rg, Lc, Lg
+  std::vector<int> pair_off;    // receivers of pair p: [pair_off[p], pair_off[p+1])
+  std::vector<int> wave, mode, period;
+  std::vector<float> src_colat, src_lon;
+  std::vector<float> rcv_colat, rcv_lon, dist, obst;
+};
+
+extern "C" const char *jdsg_io_last_error(void) { return g_err.c_str(); }
+
+// SurfTime::read_swd_data (io.cpp:29-238).  wave type codes: 0 Rc, 1 Rg,
+// 2 Lc, 3 Lg (wavetype 2 / veltype 0 is Rc, 2 / 1 Rg, 1 / 0 Lc, else Lg);
+// period is the 0-based index into the mode's periods.  Blank lines are
+// skipped.
+extern "C" int jdsg_read_swd_data(const char *path, jdsg_swd_data **out) {
+  if (!path || !out) {
+    g_err = "read_swd_data: null argument";
+    return 1;
+  }
+  std::ifstream in(path);
+  if (!in.is_open()) {
+    g_err = std::string("cannot open file ") + path;
+    return 1;
+  }
+  const float DEG2RAD = (float)(M_PI / 180.);
+  auto *D = new jdsg_swd_data;
+  for (int w = 0; w < 4; ++w)
+    if (!read_table(in, D->tab[w])) {
+      g_err = "read_swd_data: bad period header";
+      delete D;
+      return 1;
+    }
+  D->pair_off.push_back(0);
+  std::string line;
+  bool have = false;
+  while (std::getline(in, line)) {
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    have = true;
+    break;
+  }
+  while (have) {
+    char dummy = 0;
+    float lat = 0, lon = 0;
+    int mode = 0, period = 0, wavetp = 0, veltp = 0;
+    if (line[0] != '#' ||
+        std::sscanf(line.c_str(), "%c%f%f%d%d%d%d", &dummy, &lat, &lon, &mode, &period, &wavetp,
+                    &veltp) != 7) {
+      g_err = "read_swd_data: expected '# lat lon mode period wavetype veltype', got: " + line;
+      delete D;
+      return 1;
+    }
+    const float c1 = (90.0f - lat) * DEG2RAD, l1 = lon * DEG2RAD;
+    const int code = (wavetp == 2 && veltp == 0)   ? 0
+                     : (wavetp == 2 && veltp == 1) ? 1
+                     : (wavetp == 1 && veltp == 0) ? 2
+                                                   : 3;
+    have = false;
+    while (std::getline(in, line)) {  // io.cpp:6-27 read_receiver
+      if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+      if (line[0] == '#') {
+        have = true;
+        break;
+      }
+      float rlat = 0, rlon = 0, v = 0;
+      if (std::sscanf(line.c_str(), "%f%f%f", &rlat, &rlon, &v) != 3) {
+        g_err = "read_swd_data: expected 'lat lon velocity', got: " + line;
+        delete D;
+        return 1;
+      }
+      const float c2 = (90.0f - rlat) * DEG2RAD, l2 = rlon * DEG2RAD;
+      const float d = delsph(c1, l1, c2, l2);
+      D->rcv_colat.push_back(c2);
+      D->rcv_lon.push_back(l2);
+      D->dist.push_back(d);
+      D->obst.push_back(d / v);
+    }
+    D->wave.push_back(code);
+    D->mode.push_back(mode);
+    D->period.push_back(period - 1);
+    D->src_colat.push_back(c1);
+    D->src_lon.push_back(l1);
+    D->pair_off.push_back((int)D->dist.size());
+  }
+  *out = D;
+  return 0;
+}
+
+extern "C" void jdsg_swd_data_sizes(const jdsg_swd_data *D, int *nmode, int *nt,
+                                    long long *npairs, long long *nobs) {
+  for (int w = 0; w < 4; ++w) {
+    if (nmode) nmode[w] = D->tab[w].nmode;
+    if (nt) nt[w] = D->tab[w].nt;
+  }
+  if (npairs) *npairs = (long long)D->wave.size();
+  if (nobs) *nobs = (long long)D->dist.size();
+}
+
+// the period table of wave type w (0 Rc .. 3 Lg): nt x nmode floats,
+// column-major, -1 past a mode's count; kmax[nmode] the counts
+extern "C" int jdsg_swd_data_table(const jdsg_swd_data *D, int w, float *t, int *kmax) {
+  if (w < 0 || w > 3) {
+    g_err = "swd_data_table: wave type must be 0..3";
+    return 1;
+  }
+  const Table &T = D->tab[w];
+  if (t) std::memcpy(t, T.t.data(), sizeof(float) * T.t.size());
+  if (kmax) std::memcpy(kmax, T.kmax.data(), sizeof(int) * T.kmax.size());
+  return 0;
+}
+
+// per pair: receiver offsets (npairs + 1), wave code, mode, 0-based period
+// index, source colatitude / longitude (radians); per measurement: receiver
+// colatitude / longitude, distance (km), observed time (s).  Any pointer may
+// be null.
+extern "C" int jdsg_swd_data_arrays(const jdsg_swd_data *D, int *pair_off, int *wave, int *mode,
+                                    int *period, float *src_colat, float *src_lon,
+                                    float *rcv_colat, float *rcv_lon, float *dist, float *obst) {
+  auto cp = [](void *dst, const void *src, size_t bytes) {
+    if (dst && bytes) std::memcpy(dst, src, bytes);
+  };
+  const size_t np = D->wave.size(), no = D->dist.size();
+  cp(pair_off, D->pair_off.data(), sizeof(int) * (np + 1));
+  cp(wave, D->wave.data(), sizeof(int) * np);
+  cp(mode, D->mode.data(), sizeof(int) * np);
+  cp(period, D->period.data(), sizeof(int) * np);
+  cp(src_colat, D->src_colat.data(), sizeof(float) * np);
+  cp(src_lon, D->src_lon.data(), sizeof(float) * np);
+  cp(rcv_colat, D->rcv_colat.data(), sizeof(float) * no);
+  cp(rcv_lon, D->rcv_lon.data(), sizeof(float) * no);
+  cp(dist, D->dist.data(), sizeof(float) * no);
+  cp(obst, D->obst.data(), sizeof(float) * no);
+  return 0;
+}
+
+extern "C" void jdsg_swd_data_free(jdsg_swd_data *D) { delete D; }
