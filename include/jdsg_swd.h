@@ -389,6 +389,24 @@ int jdsg_swd_data_arrays(const jdsg_swd_data *d, int *pair_off, int *wave, int *
 void jdsg_swd_data_free(jdsg_swd_data *d);
 const char *jdsg_io_last_error(void);
 
+/* ---- the model file (host code) ---- */
+
+/* DSurfTomo::read_model (src/surftomo/io.cpp:27-114; JSurfGTomo::read_model,
+ * src/JSurfGTomo/JSurfGtomo.cpp:57-) split in two so the caller sizes the
+ * buffers: the header gives nx (nlat), ny (nlon), nz, the origin goxd (lat),
+ * gozd (lon) and the spacing dvxd, dvzd; then dep[nz] and the column-major
+ * vs[nx*ny*nz] (i fastest), the layout jdsg_surftime_* take.  0 on success,
+ * else nonzero with jdsg_io_last_error (the reference prints and exits; a
+ * file with fewer values than the grid is an error here). */
+int jdsg_read_model_header(const char *path, int *nx, int *ny, int *nz, float *goxd, float *gozd,
+                           float *dvxd, float *dvzd);
+int jdsg_read_model(const char *path, int nx, int ny, int nz, float *dep, float *vs);
+/* The per-iteration model snapshot "lon lat dep vs" (%g), k slowest then j
+ * then i, with lat[i] = goxd - i*dvxd and lon[j] = gozd + j*dvzd
+ * (src/surftomo/main.cpp:53-59, :113-120). */
+int jdsg_write_model(const char *path, int nx, int ny, int nz, float goxd, float gozd, float dvxd,
+                     float dvzd, const float *dep, const float *vs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -206,3 +206,131 @@ extern "C" int jdsg_swd_data_arrays(const jdsg_swd_data *D, int *pair_off, int *
 }
 
 extern "C" void jdsg_swd_data_free(jdsg_swd_data *D) { delete D; }
+
+// ---- the model file (MOD) ----
+//
+// DSurfTomo::read_model (src/surftomo/io.cpp:27-85; JSurfGTomo::read_model,
+// src/JSurfGTomo/JSurfGtomo.cpp:57- reads the same layout): line 1
+// "nx ny nz" (nlat nlon nz), line 2 "goxd gozd" (origin lat, lon), line 3
+// "dvxd dvzd" (spacing in lat, lon), line 4 the nz depths (strtof), then the
+// nx*ny*nz velocities read as whitespace-separated floats with i (lat)
+// fastest, then j (lon), then k (depth): the column-major fmat3 vs(nx,ny,nz)
+// that the GPU batch takes as it is.  Grid: lat[i] = goxd - i*dvxd,
+// lon[j] = gozd + j*dvzd (io.cpp:84-89), both float.  Anything after a header line's
+// numbers (the example's "# nlat nlon nz" comments) is ignored, as sscanf
+// ignores it there.
+
+namespace {
+
+bool read_header(std::ifstream &in, const char *path, int *nx, int *ny, int *nz, float *g) {
+  std::string l1, l2, l3;
+  if (!std::getline(in, l1) || !std::getline(in, l2) || !std::getline(in, l3)) {
+    g_err = std::string("read_model: header shorter than three lines in ") + path;
+    return false;
+  }
+  if (std::sscanf(l1.c_str(), "%d%d%d", nx, ny, nz) != 3 ||
+      std::sscanf(l2.c_str(), "%f%f", &g[0], &g[1]) != 2 ||
+      std::sscanf(l3.c_str(), "%f%f", &g[2], &g[3]) != 2) {
+    g_err = std::string("read_model: bad header in ") + path;
+    return false;
+  }
+  if (*nx <= 0 || *ny <= 0 || *nz <= 0 || (long long)*nx * *ny * *nz > (1LL << 34)) {
+    g_err = "read_model: grid dimensions must be positive";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int jdsg_read_model_header(const char *path, int *nx, int *ny, int *nz, float *goxd,
+                                      float *gozd, float *dvxd, float *dvzd) {
+  if (!path || !nx || !ny || !nz || !goxd || !gozd || !dvxd || !dvzd) {
+    g_err = "read_model_header: null argument";
+    return 1;
+  }
+  std::ifstream in(path);
+  if (!in.is_open()) {
+    g_err = std::string("cannot open ") + path;
+    return 1;
+  }
+  float g[4];
+  if (!read_header(in, path, nx, ny, nz, g)) return 1;
+  *goxd = g[0];
+  *gozd = g[1];
+  *dvxd = g[2];
+  *dvzd = g[3];
+  return 0;
+}
+
+extern "C" int jdsg_read_model(const char *path, int nx, int ny, int nz, float *dep, float *vs) {
+  if (!path || !dep || !vs) {
+    g_err = "read_model: null argument";
+    return 1;
+  }
+  std::ifstream in(path);
+  if (!in.is_open()) {
+    g_err = std::string("cannot open ") + path;
+    return 1;
+  }
+  int fx, fy, fz;
+  float g[4];
+  if (!read_header(in, path, &fx, &fy, &fz, g)) return 1;
+  if (fx != nx || fy != ny || fz != nz) {
+    g_err = "read_model: the file's grid is not the one the buffers were sized for";
+    return 1;
+  }
+  // io.cpp:59-70: strtof over the depth line; a missing number reads as 0
+  std::string line;
+  if (!std::getline(in, line)) {
+    g_err = "read_model: no depth line";
+    return 1;
+  }
+  const char *p = line.c_str();
+  char *e = nullptr;
+  for (int k = 0; k < nz; ++k) {
+    dep[k] = std::strtof(p, &e);
+    p = e;
+  }
+  // io.cpp:73-78: i fastest; the reference leaves missing values
+  // uninitialised, here a short file is an error
+  const long long n = (long long)nx * ny * nz;
+  for (long long q = 0; q < n; ++q) {
+    if (!(in >> vs[q])) {
+      g_err = "read_model: expected " + std::to_string(n) + " velocities, read " +
+              std::to_string(q);
+      return 1;
+    }
+  }
+  return 0;
+}
+
+// The model snapshot the drivers write per iteration (results/mod_iter*.dat,
+// src/surftomo/main.cpp:53-59 and :113-120, JSurfGTomo/main.cpp:66-71): one
+// line "lon lat dep vs" per node, "%g" each, k slowest, then j, then i.
+extern "C" int jdsg_write_model(const char *path, int nx, int ny, int nz, float goxd, float gozd,
+                                float dvxd, float dvzd, const float *dep, const float *vs) {
+  if (!path || !dep || !vs || nx <= 0 || ny <= 0 || nz <= 0) {
+    g_err = "write_model: bad argument";
+    return 1;
+  }
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) {
+    g_err = std::string("cannot open ") + path + " for writing";
+    return 1;
+  }
+  std::vector<float> lat(nx), lon(ny);
+  for (int i = 0; i < nx; ++i) lat[i] = goxd - i * dvxd;
+  for (int j = 0; j < ny; ++j) lon[j] = gozd + j * dvzd;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        std::fprintf(fp, "%g %g %g %g\n", lon[j], lat[i], dep[k],
+                     vs[i + (size_t)nx * (j + (size_t)ny * k)]);
+  const bool ok = std::ferror(fp) == 0;
+  if (std::fclose(fp) != 0 || !ok) {
+    g_err = std::string("write_model: write failed on ") + path;
+    return 1;
+  }
+  return 0;
+}

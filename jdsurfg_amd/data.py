@@ -19,6 +19,9 @@ def _L():
     L.jdsg_swd_data_arrays.argtypes = [vp] * 11
     L.jdsg_swd_data_free.argtypes = [vp]
     L.jdsg_io_last_error.restype = C.c_char_p
+    L.jdsg_read_model_header.argtypes = [C.c_char_p] + [vp] * 7
+    L.jdsg_read_model.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.jdsg_write_model.argtypes = [C.c_char_p] + [C.c_int] * 3 + [C.c_float] * 4 + [vp, vp]
     return L
 
 
@@ -55,3 +58,40 @@ def read_swd_data(path):
         return out
     finally:
         L.jdsg_swd_data_free(h)
+
+
+def read_model(path):
+    """DSurfTomo::read_model (src/surftomo/io.cpp:27-114): returns a dict with
+    nx, ny, nz, goxd, gozd, dvxd, dvzd, dep (float32[nz]) and vs, a float32
+    array of shape (nx, ny, nz) in Fortran order (i fastest), as the batch
+    calls take it; lat = goxd - i*dvxd, lon = gozd + j*dvzd (float32)."""
+    L = _L()
+    n = np.zeros(3, np.int32)
+    g = np.zeros(4, np.float32)
+    ptr = lambda a, i=0: C.c_void_p(a.ctypes.data + i * a.itemsize)
+    if L.jdsg_read_model_header(str(path).encode(), ptr(n, 0), ptr(n, 1), ptr(n, 2),
+                                ptr(g, 0), ptr(g, 1), ptr(g, 2), ptr(g, 3)):
+        raise RuntimeError(L.jdsg_io_last_error().decode())
+    nx, ny, nz = (int(v) for v in n)
+    dep = np.zeros(nz, np.float32)
+    vs = np.zeros((nx, ny, nz), np.float32, order="F")
+    if L.jdsg_read_model(str(path).encode(), nx, ny, nz, ptr(dep), ptr(vs)):
+        raise RuntimeError(L.jdsg_io_last_error().decode())
+    i, j = np.arange(nx, dtype=np.float32), np.arange(ny, dtype=np.float32)
+    return {"nx": nx, "ny": ny, "nz": nz, "goxd": g[0], "gozd": g[1], "dvxd": g[2],
+            "dvzd": g[3], "dep": dep, "vs": vs,
+            "lat": (g[0] - i * g[2]).astype(np.float32), "lon": (g[1] + j * g[3]).astype(np.float32)}
+
+
+def write_model(path, vs, dep, goxd, gozd, dvxd, dvzd):
+    """The per-iteration snapshot results/mod_iter*.dat of
+    src/surftomo/main.cpp:53-59: "lon lat dep vs" per node."""
+    L = _L()
+    vs = np.asfortranarray(vs, np.float32)
+    dep = np.ascontiguousarray(dep, np.float32)
+    nx, ny, nz = vs.shape
+    if dep.shape != (nz,):
+        raise ValueError("dep must hold nz depths")
+    if L.jdsg_write_model(str(path).encode(), nx, ny, nz, goxd, gozd, dvxd, dvzd,
+                          dep.ctypes.data_as(C.c_void_p), vs.ctypes.data_as(C.c_void_p)):
+        raise RuntimeError(L.jdsg_io_last_error().decode())

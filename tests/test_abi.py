@@ -44,10 +44,11 @@ def declared_arity():
 def test_python_bindings_match_the_header():
     """Every argtypes list the Python modules set has the header's parameter
     count (a missing or extra argument would shift every later one)."""
-    from jdsurfg_amd import frechet, surrogate, swd
+    from jdsurfg_amd import data, frechet, surrogate, swd
     L = swd.lib()
     surrogate._L()
     frechet._L()
+    data._L()
     arity = declared_arity()
     assert len(arity) >= len(declared_functions()) - 2
     bad = []
