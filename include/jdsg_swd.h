@@ -407,6 +407,43 @@ int jdsg_read_model(const char *path, int nx, int ny, int nz, float *dep, float 
 int jdsg_write_model(const char *path, int nx, int ny, int nz, float goxd, float gozd, float dvxd,
                      float dvzd, const float *dep, const float *vs);
 
+/* ---- the nonlinear model update (device-resident NonlinOPT) ---- */
+
+/* NonlinOPT<T> (include/nonlinear_opt.hpp:15-390): CG (Polak-Ribiere) or
+ * L-BFGS (5 pairs) directions in log-model space, the trial step capped at
+ * max_step relative change, x1 = x * exp(step * direc), and the bisecting
+ * Wolfe line search.  The history the reference keeps in grad/direc/logmd/
+ * syn .bin files is a ring in device memory.  The model object's calls are
+ * callbacks on device buffers, issued on the handle's stream:
+ *   grad   -- T::compute_grad(x, dsyn, grad): n floats in, m + n out
+ *   misfit -- T::compute_misfit(dsyn)
+ *   smooth -- T::smoothing(direc), in place (may be null: no smoothing).
+ * A callback returns nonzero to abort the update.  jdsg_nlopt_update is
+ * NonlinOPT::update (:271-389): x (n device floats) is replaced by the
+ * accepted model, dsyn (m device floats) holds the synthetics of the model
+ * on entry; method 0 "CG", 1 "LBFGS".  0 on success; 2 when 100 trial
+ * steps meet no Wolfe condition (the reference loops on); else nonzero,
+ * with jdsg_nlopt_last_error. */
+typedef struct jdsg_nlopt jdsg_nlopt;
+typedef int (*jdsg_nlopt_grad_fn)(void *user, const float *x, float *dsyn, float *grad, void *stream);
+typedef float (*jdsg_nlopt_misfit_fn)(void *user, const float *dsyn, void *stream);
+typedef int (*jdsg_nlopt_smooth_fn)(void *user, float *direc, void *stream);
+/* NonlinOPT(outdir, max_step, iter_start, iter_current) (:36-48).  work: null
+ * (the library allocates one block) or jdsg_nlopt_work_floats(n, m) device
+ * floats, 8-byte aligned, kept by the caller until jdsg_nlopt_destroy; every
+ * buffer a callback sees lies in it (or is the caller's x / dsyn). */
+long long jdsg_nlopt_work_floats(int n, int m);
+int jdsg_nlopt_create(int n, int m, float max_step, int iter_start, int iter_current, float *work,
+                      void *stream, jdsg_nlopt **out);
+int jdsg_nlopt_update(jdsg_nlopt *h, float *x, float *dsyn, int method, jdsg_nlopt_grad_fn grad,
+                      jdsg_nlopt_misfit_fn misfit, jdsg_nlopt_smooth_fn smooth, void *user);
+/* iteration, ITER_START, trial steps of the last update, STEP_FAC, and the
+ * misfits at the start and at the last trial of the last update */
+void jdsg_nlopt_state(const jdsg_nlopt *h, int *iter, int *iter_start, int *trials,
+                      float *step_fac, float *fcost, float *fcost1);
+void jdsg_nlopt_destroy(jdsg_nlopt *h);
+const char *jdsg_nlopt_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

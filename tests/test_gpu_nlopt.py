@@ -1,0 +1,35 @@
+"""The device-resident optimiser (jdsg_nlopt_*, NonlinOPT of
+include/nonlinear_opt.hpp:15-390) against the oracle on the same
+log-space least-squares problem: after every update the models agree to
+float rounding, and the iteration counters, the line-search trial counts
+and the restarts (ITER_START) are the same."""
+import numpy as np
+import pytest
+
+from oracle.nlopt_oracle import NonlinOPTOracle
+from tests.nlopt_problem import LogQuad, TorchLogQuad
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("method,max_step,iters", [("CG", 0.03, 8), ("LBFGS", 0.03, 8),
+                                                    ("LBFGS", 0.3, 12), ("CG", 0.3, 12)])
+def test_device_optimiser_matches_the_oracle(method, max_step, iters):
+    import torch
+    from jdsurfg_amd.nlopt import NonlinOPT
+    print(f"nlopt {method} max_step={max_step} iters={iters}", flush=True)
+    p = LogQuad()
+    o = NonlinOPTOracle(max_step, 0, 0)
+    dev = NonlinOPT(p.n, p.m, max_step, 0, 0)
+    tp = TorchLogQuad(p)
+    xo = p.x0.copy()
+    xd = torch.from_numpy(p.x0).cuda()
+    dd = torch.zeros(p.m, dtype=torch.float32, device="cuda")
+    for k in range(iters):
+        xo, dso = o.update(p, xo, method)
+        st = dev.update(tp, xd, dd, method)
+        torch.cuda.synchronize()
+        assert (st["iter"], st["iter_start"], st["trials"]) == (o.ITER, o.ITER_START, o.trials), k
+        assert np.allclose(xd.cpu().numpy(), xo, rtol=2e-5, atol=0), k
+        assert np.allclose(dd.cpu().numpy(), dso, rtol=1e-4, atol=1e-4), k
+        assert abs(st["step_fac"] - o.STEP_FAC) <= 1e-4 * abs(o.STEP_FAC), k
