@@ -12,8 +12,8 @@ from tests.nlopt_problem import LogQuad, TorchLogQuad
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("method,max_step,iters", [("CG", 0.03, 8), ("LBFGS", 0.03, 8),
-                                                    ("LBFGS", 0.3, 12), ("CG", 0.3, 12)])
+@pytest.mark.parametrize("method,max_step,iters", [("CG", 0.3, 12), ("CG", 1.0, 12),
+                                                    ("LBFGS", 0.03, 10), ("LBFGS", 1.0, 12)])
 def test_device_optimiser_matches_the_oracle(method, max_step, iters):
     import torch
     from jdsurfg_amd.nlopt import NonlinOPT
@@ -33,3 +33,22 @@ def test_device_optimiser_matches_the_oracle(method, max_step, iters):
         assert np.allclose(xd.cpu().numpy(), xo, rtol=2e-5, atol=0), k
         assert np.allclose(dd.cpu().numpy(), dso, rtol=1e-4, atol=1e-4), k
         assert abs(st["step_fac"] - o.STEP_FAC) <= 1e-4 * abs(o.STEP_FAC), k
+
+
+def test_capped_step_that_fails_the_curvature_condition_stops_both():
+    """CG with a 3 % cap on this problem: the capped step never meets the
+    curvature condition and the reference's line search would not end; the
+    oracle and the device path both stop after 100 trials."""
+    import torch
+    from jdsurfg_amd.nlopt import NonlinOPT
+    print("nlopt CG max_step=0.03 stuck line search", flush=True)
+    p = LogQuad()
+    o = NonlinOPTOracle(0.03, 0, 0)
+    with pytest.raises(RuntimeError, match="Wolfe"):
+        o.update(p, p.x0.copy(), "CG")
+    dev = NonlinOPT(p.n, p.m, 0.03, 0, 0)
+    xd = torch.from_numpy(p.x0).cuda()
+    dd = torch.zeros(p.m, dtype=torch.float32, device="cuda")
+    with pytest.raises(RuntimeError, match="Wolfe"):
+        dev.update(TorchLogQuad(p), xd, dd, "CG")
+    assert torch.equal(xd.cpu(), torch.from_numpy(p.x0))

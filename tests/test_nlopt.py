@@ -90,3 +90,13 @@ def test_library_exports_the_optimiser():
     for name in ("jdsg_nlopt_create", "jdsg_nlopt_update", "jdsg_nlopt_state",
                  "jdsg_nlopt_destroy", "jdsg_nlopt_last_error"):
         assert hasattr(L, name)
+
+
+def test_capped_step_that_never_meets_the_curvature_condition_stops():
+    # the step grows tenfold after a curvature failure and is capped back
+    # at max_step / dmax: the reference's loop would not end
+    p = LogQuad()
+    o = NonlinOPTOracle(0.03, 0, 0)
+    with pytest.raises(RuntimeError, match="Wolfe"):
+        o.update(p, p.x0.copy(), "CG")
+    assert o.trials == 101 and o.STEP_FAC * np.float32(10) > 0
