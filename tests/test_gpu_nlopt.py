@@ -2,7 +2,12 @@
 include/nonlinear_opt.hpp:15-390) against the oracle on the same
 log-space least-squares problem: after every update the models agree to
 float rounding, and the iteration counters, the line-search trial counts
-and the restarts (ITER_START) are the same."""
+and the restarts (ITER_START) are the same.  Device expf/logf and numpy's
+differ by an ulp here and there; the cases run only as many updates as the
+oracle itself stays within these tolerances when its starting model moves
+by one ulp (CG with a 100 % cap on this problem amplifies such a change to
+1e-3 in the trial step by its seventh update, and splits after that).
+Tolerances: x 2e-5 relative, dsyn 1e-4, trial step 1e-4."""
 import numpy as np
 import pytest
 
@@ -12,7 +17,7 @@ from tests.nlopt_problem import LogQuad, TorchLogQuad
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("method,max_step,iters", [("CG", 0.3, 12), ("CG", 1.0, 12),
+@pytest.mark.parametrize("method,max_step,iters", [("CG", 0.3, 12), ("CG", 1.0, 6),
                                                     ("LBFGS", 0.03, 10), ("LBFGS", 1.0, 12)])
 def test_device_optimiser_matches_the_oracle(method, max_step, iters):
     import torch
