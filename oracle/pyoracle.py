@@ -89,6 +89,8 @@ class Oracle:
         L.or_batch_offsets.argtypes = [C.POINTER(_Batch), C.c_void_p, C.c_void_p]
         L.or_batch_solve.argtypes = [C.c_void_p, C.POINTER(_Batch), C.POINTER(_BatchOut),
                                      C.c_int]
+        L.or_batch_eigen.argtypes = [C.c_void_p, C.POINTER(_Batch), C.POINTER(_BatchOut),
+                                     C.c_void_p, C.c_int]
         L.or_restatement_backend.restype = C.c_void_p
         L.or_set_tolerance.argtypes = [C.c_double]
         _lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -221,7 +223,10 @@ class Oracle:
 
     # ---------------- layer-stack batch ----------------
     def batch_solve(self, layer_off, thk, vp, vs, rho, per_off, periods, nmode, group,
-                    nthreads=1, kernels=True):
+                    nthreads=1, kernels=True, roots=None):
+        """The layer-stack batch (kernel1D's call sequence per column).  With
+        roots ([3*nent] flat roots T, T(1+.005), T(1-.005)) the chains are
+        skipped and the eigen stage is evaluated at those roots."""
         layer_off = np.ascontiguousarray(layer_off, np.int32)
         per_off = np.ascontiguousarray(per_off, np.int32)
         nmode = np.ascontiguousarray(nmode, np.int32)
@@ -244,7 +249,15 @@ class Oracle:
                       res["u"].ctypes.data,
                       *[(res[n].ctypes.data if kernels else None) for n in names],
                       res["status"].ctypes.data, res["cflat"].ctypes.data)
-        self.lib.or_batch_solve(self._be(), C.byref(b), C.byref(o), nthreads)
+        if roots is None:
+            self.lib.or_batch_solve(self._be(), C.byref(b), C.byref(o), nthreads)
+        else:
+            roots = np.ascontiguousarray(roots, _d)
+            if roots.size != 3 * ne:
+                raise ValueError("roots must hold 3*nent flat roots")
+            res["cflat"][:] = roots
+            self.lib.or_batch_eigen(self._be(), C.byref(b), C.byref(o), roots.ctypes.data,
+                                    nthreads)
         return res
 
 

@@ -154,6 +154,11 @@ void or_batch_offsets(const or_batch *b, long long *ent_off,
                       long long *kern_off);
 void or_batch_solve(const or_backend *be, const or_batch *b,
                     const or_batch_out *o, int nthreads);
+/* The eigen stage of or_batch_solve at given flat roots (roots[3*nent] in
+ * the order of or_batch_out.cflat; 0 = no root): what the reference's
+ * sregn96_/sregnpu_/slegn96_/slegnpu_ return for those roots. */
+void or_batch_eigen(const or_backend *be, const or_batch *b, const or_batch_out *o,
+                    const double *roots, int nthreads);
 
 /* SurfaceWave.cpp:601-641 / 738-773: Fréchet rows (CSR, returns the nonzero
  * count) and the misfit gradient from dense per-row ray sensitivities. */

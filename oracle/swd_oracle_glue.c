@@ -380,8 +380,13 @@ static void zero_entry(const or_batch_out *o, long long e, long long q, int N,
     if (arr[a]) memset(arr[a] + q, 0, sizeof(double) * (size_t)N);
 }
 
+/* One column of or_batch_solve.  With roots non-null ([3*nent], T, T(1+.005),
+ * T(1-.005) flat roots as or_batch_out.cflat holds them) the root chains are
+ * not run: the eigen stage is evaluated at those roots (status from the
+ * roots alone: 0 where all needed roots are nonzero, 1 where the T root is 0,
+ * 2 where a T(1+-.005) root is 0). */
 static void solve_column(const or_backend *be, const or_batch *b,
-                         const or_batch_out *o, int col) {
+                         const or_batch_out *o, int col, const double *roots) {
   int l0 = b->layer_off[col], N = b->layer_off[col + 1] - l0;
   int p0 = b->per_off[col], K = b->per_off[col + 1] - p0;
   const float *thk = b->thk + l0, *vp = b->vp + l0, *vs = b->vs + l0,
@@ -403,11 +408,20 @@ static void solve_column(const or_backend *be, const or_batch *b,
     int code = (w == 0) ? 0 : 2;
     for (int m = 0; m < M; ++m) {
       long long ebase = o->ent_off[col] + (long long)((w == 0 ? 0 : nR) + m) * K;
-      int err = be->surfdisp(thk, vp, vs, rho, N, T, cp, K, code, m, 1, 1);
-      int errpm = 0;
-      if (b->group) {
-        errpm |= be->surfdisp(thk, vp, vs, rho, N, t1, c1, K, code, m, 1, 1);
-        errpm |= be->surfdisp(thk, vp, vs, rho, N, t2, c2, K, code, m, 1, 1);
+      int err = 0, errpm = 0;
+      if (roots) {
+        long long ne = o->ent_off[b->ncol];
+        for (int k = 0; k < K; ++k) {
+          cp[k] = roots[ebase + k];
+          c1[k] = b->group ? roots[ne + ebase + k] : 0.0;
+          c2[k] = b->group ? roots[2 * ne + ebase + k] : 0.0;
+        }
+      } else {
+        err = be->surfdisp(thk, vp, vs, rho, N, T, cp, K, code, m, 1, 1);
+        if (b->group) {
+          errpm |= be->surfdisp(thk, vp, vs, rho, N, t1, c1, K, code, m, 1, 1);
+          errpm |= be->surfdisp(thk, vp, vs, rho, N, t2, c2, K, code, m, 1, 1);
+        }
       }
       for (int k = 0; k < K; ++k) {
         long long e = ebase + k;
@@ -462,7 +476,16 @@ void or_batch_solve(const or_backend *be, const or_batch *b,
   if (nthreads <= 0) nthreads = 1;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
 #endif
-  for (int c = 0; c < b->ncol; ++c) solve_column(be, b, o, c);
+  for (int c = 0; c < b->ncol; ++c) solve_column(be, b, o, c, NULL);
+}
+
+void or_batch_eigen(const or_backend *be, const or_batch *b, const or_batch_out *o,
+                    const double *roots, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+#endif
+  for (int c = 0; c < b->ncol; ++c) solve_column(be, b, o, c, roots);
 }
 
 /* SurfaceWave.cpp:601-641 (frechet_matrix): row r of the Fréchet matrix from
