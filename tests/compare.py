@@ -75,3 +75,31 @@ def scalar_errors(a, b, mask=None):
             res[n] = 0.0
             res[n + "_exact"] = 1.0
     return res
+
+
+def at_gpu_roots(oracle, w, a, nthreads=4):
+    """The oracle's eigen stage evaluated at the GPU's flat roots: what the
+    reference's sregn96_/sregnpu_/slegn96_/slegnpu_ return for those roots."""
+    r = np.zeros(3 * len(a["c"]))
+    r[:len(a["cflat"])] = a["cflat"]
+    return oracle.batch_solve(w.layer_off, w.thk, w.vp, w.vs, w.rho, w.per_off, w.periods,
+                              w.nmode.ravel(), w.group, nthreads=nthreads, roots=r)
+
+
+def differing_root_report(oracle, w, a, b, nthreads=4):
+    """Entries (status 0) whose flat roots differ between the GPU (a) and the
+    oracle (b): U and kernel errors of the GPU against the oracle evaluated
+    at the GPU's roots (the bar), and against the oracle's own roots (the
+    spread the reference's 1e-6 stop band allows)."""
+    same = same_roots(a, b, w)
+    diff = (~same) & (a["status"] == 0) & (b["status"] == 0)
+    rep = {"n": int(diff.sum())}
+    if not diff.any():
+        return rep, diff
+    g = at_gpu_roots(oracle, w, a, nthreads)
+    assert np.array_equal(g["status"][diff], a["status"][diff])
+    rep["at_gpu_roots"] = scalar_errors(a, g, mask=diff)
+    rep["at_gpu_roots"].update(kernel_errors(a, g, w, mask=diff))
+    rep["vs_own_roots"] = scalar_errors(a, b, mask=diff)
+    rep["vs_own_roots"].update(kernel_errors(a, b, w, mask=diff))
+    return rep, diff

@@ -1,0 +1,266 @@
+"""GPU parity against the reference's own numbers: the fixtures the compiled
+reference wrote (tests/golden/make_golden.py, from src/SWD built from source)
+compared directly with the HIP path, through the batch plan and through the
+per-column reference ABI (swd.hpp:19-95) the library exports.
+
+Bars, against the reference's outputs:
+* status identical, every c within 1e-6 relative and at least 99.9 % of them
+  bit-identical;
+* U within 1e-6 and kernels within 1e-4 of the entry's largest layer value
+  where the flat roots are the reference's; where a root differs in its last
+  bits (a Neville iterate within rounding of the root), against the
+  reference's eigen stage evaluated at the GPU's root.  The reference's flat
+  roots are the restatement's, which equal them bit for bit on every fixture
+  (tests/test_oracle.py, asserted here again).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from jdsurfg_amd import workloads as W
+from tests.compare import KERNELS, at_gpu_roots, gpu_solve, kernel_errors, scalar_errors
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIXTURES = ["batch_grid20.npz", "batch_higher_modes.npz", "batch_sediment_lvz.npz",
+            "batch_water_top.npz", "batch_thin16_thick128.npz", "batch_grid128_sub.npz"]
+
+
+def _load(name):
+    return dict(np.load(os.path.join(GOLD, name)))
+
+
+def _workload(g):
+    return W.Workload("golden", g["layer_off"], g["thk"], g["vp"], g["vs"], g["rho"],
+                      g["per_off"], g["periods"], g["nmode"], bool(g["group"]))
+
+
+def _golden_as_solution(g, ref_roots):
+    """The fixture as a full solution dict (kernels only for its kernel
+    columns; other columns' kernel entries are masked out by the caller)."""
+    nk = int(g["kern_off"][-1])
+    out = {"c": g["c"], "u": g["u"], "status": g["status"], "ent_off": g["ent_off"],
+           "kern_off": g["kern_off"], "cflat": ref_roots}
+    q = 0
+    for n in KERNELS:
+        out[n] = np.zeros(nk)
+    for c in g["kern_cols"]:
+        a, b = int(g["kern_off"][c]), int(g["kern_off"][c + 1])
+        for n in KERNELS:
+            out[n][a:b] = g[n][q:q + (b - a)]
+        q += b - a
+    return out
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_plan_matches_reference_fixture(name, oracle_rest):
+    g = _load(name)
+    w = _workload(g)
+    print(f"case {name}: ncol={w.ncol} group={w.group}", flush=True)
+    rest = oracle_rest.batch_solve(w.layer_off, w.thk, w.vp, w.vs, w.rho, w.per_off,
+                                   w.periods, w.nmode.ravel(), w.group, nthreads=4)
+    ok = g["status"] == 0
+    assert np.array_equal(rest["status"], g["status"]) and np.array_equal(rest["c"][ok],
+                                                                          g["c"][ok])
+    nv = 3 if w.group else 1
+    ref = _golden_as_solution(g, rest["cflat"][:nv * len(g["c"])])
+    a = gpu_solve(w, layout="csr")
+    assert np.array_equal(a["status"], g["status"]), \
+        f"status mismatch at {np.nonzero(a['status'] != g['status'])[0][:10]}"
+    same = np.ones(len(g["c"]), bool)
+    ne = len(g["c"])
+    for v in range(nv):
+        same &= a["cflat"][v * ne:(v + 1) * ne] == ref["cflat"][v * ne:(v + 1) * ne]
+    frac = float(same[ok].mean())
+    s_all = scalar_errors(a, ref)
+    kcol = np.zeros(ne, bool)
+    for c in g["kern_cols"]:
+        kcol[g["ent_off"][c]:g["ent_off"][c + 1]] = True
+    s = scalar_errors(a, ref, mask=same)
+    k = kernel_errors(a, ref, w, mask=same & kcol)
+    print(name, f"identical roots {frac:.5f}", s_all, s, k, flush=True)
+    assert s_all["c"] <= 1e-6
+    assert frac >= 0.999
+    assert s["u"] <= 1e-6
+    for n, v in k.items():
+        assert v <= 1e-4, (n, v)
+    diff = ~same & ok
+    if diff.any():
+        at = at_gpu_roots(oracle_rest, w, a)
+        su = scalar_errors(a, at, mask=diff)
+        ku = kernel_errors(a, at, w, mask=diff)
+        print(name, f"{int(diff.sum())} entries with differing roots, at the GPU roots:", su, ku,
+              "against the reference's own:", scalar_errors(a, ref, mask=diff), flush=True)
+        assert su["u"] <= 1e-6
+        for n, v in ku.items():
+            assert v <= 1e-4, (n, v)
+
+
+# ------------------------------------------------------- per-column ABI (B1)
+def _abi():
+    from jdsurfg_amd import swd
+    L = swd.lib()
+    f, d, i = C.POINTER(C.c_float), C.POINTER(C.c_double), C.c_int
+    L.surfdisp96_.argtypes = [f, f, f, f, i, i, i, i, i, i, d, d, C.POINTER(C.c_int)]
+    L.surfdisp96_.restype = None
+    L.sregn96_.argtypes = [f] * 4 + [i] + [d] * 11 + [i]
+    L.sregn96_.restype = None
+    L.sregnpu_.argtypes = [f] * 4 + [i] + [d] * 19 + [i]
+    L.sregnpu_.restype = None
+    L.slegn96_.argtypes = [f] * 3 + [i] + [d] * 8 + [i]
+    L.slegn96_.restype = None
+    L.slegnpu_.argtypes = [f] * 3 + [i] + [d] * 15 + [i]
+    L.slegnpu_.restype = None
+    return L
+
+
+def _p(a):
+    t = C.POINTER(C.c_float) if a.dtype == np.float32 else C.POINTER(C.c_double)
+    return a.ctypes.data_as(t)
+
+
+def test_surfdisp96_matches_reference_calls():
+    """surfdisp96_ (swd.hpp:52-54) on the reference's fixture: flat and
+    spherical Earth, Love and Rayleigh, modes 1-2, igr 0 and the
+    finite-difference group velocity igr 1 (surfdisp96.f:241-248, 291-319),
+    and a one-period call of mode 3."""
+    g = _load("surfdisp96_calls.npz")
+    L = _abi()
+    thk, vp, vs, rho = (np.ascontiguousarray(g[k], np.float32) for k in ("thk", "vp", "vs", "rho"))
+    per = np.ascontiguousarray(g["periods"])
+    n, kmax = len(thk), len(per)
+    nid = ntot = 0
+    for key, ref in zip(g["keys"], g["cg"]):
+        iflsph, iwave, mode, igr, ierr = (int(v) for v in key)
+        print("case surfdisp96_", key, flush=True)
+        cg = np.zeros(kmax)
+        e = C.c_int(-1)
+        L.surfdisp96_(_p(thk), _p(vp), _p(vs), _p(rho), n, iflsph, iwave, mode, igr, kmax,
+                      _p(per), _p(cg), C.byref(e))
+        assert e.value == ierr, (key, e.value)
+        nz = ref != 0
+        assert np.array_equal(cg == 0, ref == 0), key
+        # the finite-difference group velocity divides the roots' 1e-6 stop
+        # band by the 1 % period step (surfdisp96.f:312), hence 1e-4 where a
+        # root differs in its last bits
+        assert np.abs(cg[nz] / ref[nz] - 1).max() <= (1e-6 if igr == 0 else 5e-4), key
+        nid += int((cg == ref).sum())
+        ntot += kmax
+    one = np.zeros(1)
+    e = C.c_int(-1)
+    L.surfdisp96_(_p(thk), _p(vp), _p(vs), _p(rho), n, 1, 2, 3, 0, 1, _p(per[5:6].copy()),
+                  _p(one), C.byref(e))
+    assert e.value == int(g["single_ierr"])
+    assert abs(one[0] / g["single"][0] - 1) <= 1e-6
+    print("surfdisp96_: bit-identical", nid, "of", ntot, flush=True)
+    assert nid >= 0.99 * ntot
+
+
+@pytest.mark.parametrize("name", ["batch_higher_modes.npz", "batch_water_top.npz",
+                                  "batch_sediment_lvz.npz"])
+def test_eigen_entry_points_match_reference(name, oracle_rest):
+    """sregn96_/sregnpu_/slegn96_/slegnpu_ (swd.hpp:19-95) per column, at the
+    reference's flat roots, against the reference's spherical c, U and
+    kernels (the fixture's kernel columns)."""
+    g = _load(name)
+    w = _workload(g)
+    L = _abi()
+    rest = oracle_rest.batch_solve(w.layer_off, w.thk, w.vp, w.vs, w.rho, w.per_off,
+                                   w.periods, w.nmode.ravel(), w.group, nthreads=2)
+    ok = g["status"] == 0
+    assert np.array_equal(rest["c"][ok], g["c"][ok])
+    ne = len(g["c"])
+    worst = {"c": 0.0, "u": 0.0, "k": 0.0}
+    q = 0
+    for c in range(w.ncol):
+        l0, l1 = w.layer_off[c], w.layer_off[c + 1]
+        N = int(l1 - l0)
+        thk, vp, vs, rho = (np.ascontiguousarray(a[l0:l1], np.float32)
+                            for a in (w.thk, w.vp, w.vs, w.rho))
+        T = w.periods[w.per_off[c]:w.per_off[c + 1]]
+        K = len(T)
+        nR, nL = (int(x) for x in w.nmode[c])
+        kc = c in set(int(x) for x in g["kern_cols"])
+        for wave, M in ((0, nR), (1, nL)):
+            for m in range(M):
+                print(f"case {name} col {c} wave {wave} mode {m}", flush=True)
+                for k in range(K):
+                    j = (wave * nR + m) * K + k
+                    e = int(g["ent_off"][c] + j)
+                    if g["status"][e] != 0:
+                        continue
+                    t = np.array([T[k]])
+                    cp = np.array([rest["cflat"][e]])
+                    cg = np.zeros(1)
+                    arr = {n: np.zeros(N) for n in KERNELS + ("e1", "e2", "e3", "e4")}
+                    if w.group:
+                        t1, t2 = np.array([T[k] * 1.005]), np.array([T[k] * 0.995])
+                        c1 = np.array([rest["cflat"][ne + e]])
+                        c2 = np.array([rest["cflat"][2 * ne + e]])
+                        if wave == 0:
+                            L.sregnpu_(_p(thk), _p(vp), _p(vs), _p(rho), N, _p(t), _p(cp),
+                                       _p(cg), *(_p(arr[x]) for x in ("e1", "e2", "e3", "e4")),
+                                       _p(t1), _p(c1), _p(t2), _p(c2),
+                                       *(_p(arr[x]) for x in ("dcda", "dcdb", "dcdh", "dcdr",
+                                                              "duda", "dudb", "dudh", "dudr")),
+                                       1)
+                        else:
+                            L.slegnpu_(_p(thk), _p(vs), _p(rho), N, _p(t), _p(cp), _p(cg),
+                                       _p(arr["e1"]), _p(arr["e2"]), _p(t1), _p(c1), _p(t2),
+                                       _p(c2), *(_p(arr[x]) for x in ("dcdb", "dcdh", "dcdr",
+                                                                      "dudb", "dudh", "dudr")),
+                                       1)
+                    elif wave == 0:
+                        L.sregn96_(_p(thk), _p(vp), _p(vs), _p(rho), N, _p(t), _p(cp), _p(cg),
+                                   *(_p(arr[x]) for x in ("e1", "e2", "e3", "e4", "dcda",
+                                                          "dcdb", "dcdh", "dcdr")), 1)
+                    else:
+                        L.slegn96_(_p(thk), _p(vs), _p(rho), N, _p(t), _p(cp), _p(cg),
+                                   _p(arr["e1"]), _p(arr["e2"]),
+                                   *(_p(arr[x]) for x in ("dcdb", "dcdh", "dcdr")), 1)
+                    worst["c"] = max(worst["c"], abs(cp[0] / g["c"][e] - 1))
+                    worst["u"] = max(worst["u"], abs(cg[0] / g["u"][e] - 1))
+                    if kc:
+                        qq = int(g["kern_off"][c] - g["kern_off"][c]) + j * N
+                        for n in KERNELS:
+                            if not w.group and n.startswith("du"):
+                                continue
+                            ref = g[n][q + qq:q + qq + N]
+                            sc = np.abs(ref).max()
+                            if sc > 0:
+                                worst["k"] = max(worst["k"], np.abs(arr[n] - ref).max() / sc)
+        if kc:
+            q += int(g["kern_off"][c + 1] - g["kern_off"][c])
+    print(name, "worst", worst, flush=True)
+    assert worst["c"] <= 1e-12
+    assert worst["u"] <= 1e-6
+    assert worst["k"] <= 1e-4
+
+
+@pytest.mark.parametrize("wave", [0, 1, 2, 3])
+def test_jdsg_surfdisp_matches_oracle(wave, oracle_rest):
+    """jdsg_surfdisp (surfdisp, surfdisp.cpp:64-116) in its sphere/keep_flat
+    forms, modes 0 and 1, against the oracle's surfdisp."""
+    from jdsurfg_amd import swd
+    L = swd.lib()
+    w = W.random_columns(1, 30, 9, nmode=(1, 1))
+    thk, vp, vs, rho = (np.ascontiguousarray(a, np.float32) for a in (w.thk, w.vp, w.vs, w.rho))
+    per = np.ascontiguousarray(w.periods)
+    names = ["Rc", "Rg", "Lc", "Lg"]
+    for mode in (0, 1):
+        for sphere, keep in ((0, 1), (1, 1), (1, 0)):
+            print(f"case jdsg_surfdisp {names[wave]} mode {mode} sphere {sphere} keep {keep}",
+                  flush=True)
+            cg = np.zeros(len(per))
+            rc = L.jdsg_surfdisp(thk.ctypes.data, vp.ctypes.data, vs.ctypes.data,
+                                 rho.ctypes.data, len(thk), per.ctypes.data, cg.ctypes.data,
+                                 len(per), wave, mode, sphere, keep)
+            ref, rrc = oracle_rest.surfdisp(thk, vp, vs, rho, per, names[wave], mode,
+                                            bool(sphere), bool(keep))
+            assert rc == rrc
+            nz = ref != 0
+            assert np.array_equal(cg == 0, ~nz)
+            tol = 1e-6 if wave in (0, 2) else 5e-4
+            assert np.abs(cg[nz] / ref[nz] - 1).max() <= tol

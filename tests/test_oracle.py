@@ -268,3 +268,22 @@ def test_param_convert_constant_kernel(oracle_rest):
     expect[0] = 2.0
     expect[-1] = 2.0 + 7.0 / 8.0 + 4.0
     np.testing.assert_allclose(out, expect, rtol=1e-15)
+
+
+@pytest.mark.parametrize("group", [True, False])
+def test_eigen_at_given_roots_reproduces_the_solve(oracle_rest, group):
+    """The eigen stage at given roots (used to check GPU kernels where a root
+    differs in its last bits) returns the solve's own outputs when given the
+    solve's own roots."""
+    from tests.compare import KERNELS, at_gpu_roots
+    w = W.random_columns(5, 20, 4, nmode=(2, 2), group=group)
+    b = oracle_rest.batch_solve(w.layer_off, w.thk, w.vp, w.vs, w.rho, w.per_off, w.periods,
+                                w.nmode.ravel(), w.group, nthreads=2)
+    a = dict(b)
+    if not group:
+        a["cflat"] = b["cflat"][:len(b["c"])]
+    g = at_gpu_roots(oracle_rest, w, a, nthreads=2)
+    ok = b["status"] != 4
+    assert np.array_equal(g["status"][ok], b["status"][ok])
+    for n in ("c", "u") + KERNELS:
+        assert np.array_equal(g[n], b[n]), n
