@@ -132,38 +132,54 @@ int jdsg_plan_capture(jdsg_plan *plan, float *thk, float *vp, float *vs, float *
 int jdsg_graph_launch(jdsg_graph *graph, void *stream);
 void jdsg_graph_destroy(jdsg_graph *graph);
 
-/* SurfTime grid form (replaces SurfTime::get_1d_kernel, SurfaceWave.hpp:103,
- * SurfaceWave.cpp:458-502; depth grid and period tables as set_model /
- * read_swd_data give them, SurfaceWave.hpp:114-115).
+/* SurfTime grid forms (replace SurfTime::get_1d_kernel and
+ * SurfTime::get_2d_map, SurfaceWave.hpp:102-103, SurfaceWave.cpp:420-502;
+ * depth grid and period tables as set_model / read_swd_data give them,
+ * SurfaceWave.hpp:114-115).
  *   dep[nz]      node depths (km);
- *   water        1 if vs(:,:,0) == 0 for every column (water top), else 0;
  *   t{Rc,Rg,Lc,Lg}  column-major (nt x nmode) period tables padded with -1
  *                (SurfaceWave.cpp:157-169); nmode 0 or a null table = absent.
+ * A column whose top node has vs == 0 has a water top, decided per column
+ * at every call as grid2LayerModel / _ParamConvert do (SurfaceWave.cpp:120,
+ * 71, 495); a grid may mix water and land columns.
  * Output index order is SurfaceWave.cpp:14-48 get_period_index: Rc modes,
  * Rg modes, Lc modes, Lg modes, each mode's periods contiguous. */
 typedef struct jdsg_surftime jdsg_surftime;
-int jdsg_surftime_create(int nx, int ny, int nz, const float *dep, int water,
-                         const double *tRc, int ntRc, int nmRc, const double *tRg, int ntRg,
-                         int nmRg, const double *tLc, int ntLc, int nmLc, const double *tLg,
-                         int ntLg, int nmLg, jdsg_surftime **st);
+int jdsg_surftime_create(int nx, int ny, int nz, const float *dep, const double *tRc, int ntRc,
+                         int nmRc, const double *tRg, int ntRg, int nmRg, const double *tLc,
+                         int ntLc, int nmLc, const double *tLg, int ntLg, int nmLg,
+                         jdsg_surftime **st);
 int jdsg_surftime_kmax(const jdsg_surftime *st, int *kmax);
-/* Device pointers: vs (nx,ny,nz) column-major float; vc, vout, status
- * (nx*ny, kmax) and kernel (nx*ny, nz, kmax) column-major (kernel may be
- * null: the get_2d_map-style outputs only).  Rc/Lc entries hold the
- * spherical phase velocity in vc and vout and dc/dVs in kernel; Rg/Lg
- * entries hold c in vc, U in vout and dU/dVs in kernel (SurfaceWave.cpp
- * :296-405, chain rule :492-500).  status as for plans; entries with a
- * nonzero status hold zeros where the reference would compute from c = 0.
- * Synchronises `stream` at the end (water-flag check). */
+/* get_1d_kernel.  Device pointers: vs (nx,ny,nz) column-major float; vc,
+ * vout, status (nx*ny, kmax) and kernel (nx*ny, nz, kmax) column-major
+ * (kernel may be null).  Rc/Lc entries hold the spherical phase velocity in
+ * vc and vout and dc/dVs in kernel; Rg/Lg entries hold c in vc, U in vout
+ * and dU/dVs in kernel (SurfaceWave.cpp:296-405, chain rule :492-500).  A
+ * water column's top node is not written (:495).  status as for plans;
+ * entries with a nonzero status hold zeros where the reference would
+ * compute from c = 0.  Synchronises `stream` once at the start (it reads
+ * the water-top mask of vs), then runs asynchronously. */
 int jdsg_surftime_kernel(jdsg_surftime *st, const float *vs, double *vc, double *vout,
                          double *kernel, int *status, void *stream);
+/* get_2d_map (disp1D, SurfaceWave.cpp:177-256): vc is the spherical phase
+ * velocity from one root chain per (wave, mode) and _flat2sphere
+ * (surfdisp.cpp:17-51); vout is vc for Rc/Lc and the analytic group
+ * velocity of groupvel_r / groupvel_l for Rg/Lg (surfdisp.cpp:126-174; the
+ * Love one on the chain groupvel_l runs with Vp = 1.732 Vs).  No kernels.
+ * Device pointers and synchronisation as jdsg_surftime_kernel. */
+int jdsg_surftime_map(jdsg_surftime *st, const float *vs, double *vc, double *vout, int *status,
+                      void *stream);
 void jdsg_surftime_destroy(jdsg_surftime *st);
-/* Host-array convenience form of the above (allocates, copies, solves). */
+/* Host-array convenience forms of the above (allocate, copy, solve). */
 int jdsg_get_1d_kernel(int nx, int ny, int nz, const float *vs, const float *dep,
                        const double *tRc, int ntRc, int nmRc, const double *tRg, int ntRg,
                        int nmRg, const double *tLc, int ntLc, int nmLc, const double *tLg,
                        int ntLg, int nmLg, double *vc, double *vout, double *kernel,
                        int *status);
+int jdsg_get_2d_map(int nx, int ny, int nz, const float *vs, const float *dep,
+                    const double *tRc, int ntRc, int nmRc, const double *tRg, int ntRg,
+                    int nmRg, const double *tLc, int ntLc, int nmLc, const double *tLg,
+                    int ntLg, int nmLg, double *vc, double *vout, int *status);
 
 /* Fréchet rows and misfit gradient of the travel-time data (replace the row
  * loops of SurfTime::frechet_matrix, SurfaceWave.cpp:584-672, and of

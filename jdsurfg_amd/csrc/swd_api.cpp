@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -1076,20 +1077,31 @@ extern "C" int jdsg_graph_launch(jdsg_graph *G, void *stream) {
 extern "C" void jdsg_graph_destroy(jdsg_graph *G) { delete G; }
 
 // ============================================================================
-// SurfTime grid entry point (SurfaceWave.cpp:458-502 get_1d_kernel).
+// SurfTime grid entry points: get_1d_kernel (SurfaceWave.cpp:458-502 over
+// kernel1D :266-412) and get_2d_map (:420-450 over disp1D :177-256).
+//
+// Columns are split by their water top (vs(i,j,0) == 0, decided per column
+// as grid2LayerModel and _ParamConvert do, :120, :71, :495) into two
+// classes with 4nz-3 and 4nz-6 layers.  Each class runs in chunks of
+// columns through uniform layer plans, one plan per job (wave type, period
+// list, mode count, form); an epilogue writes SurfTime's column-major
+// outputs through the class's column list.
 struct jdsg_surftime {
-  int nx = 0, ny = 0, nz = 0, nxy = 0, water = 0, N = 0, kmax = 0, chunk = 0;
+  int nx = 0, ny = 0, nz = 0, nxy = 0, kmax = 0;
+  std::vector<double> tab[4];
+  int ntm[4] = {0, 0, 0, 0}, nmd[4] = {0, 0, 0, 0};
   struct Job {
-    int wave, group, M;
+    int wave;   // 0 Rayleigh, 1 Love
+    int group;  // kernel form: Rg/Lg (three chains); map form: eigen stage for U
+    int vp2;    // map form: the chain runs on Vp = 1.732 Vs (groupvel_l)
+    int M;
     std::vector<double> list;
   };
-  std::vector<Job> jobs;
-  std::vector<StEntry> ents;
   struct Set {
-    int ncol = 0;
+    int cls = 0, ncol = 0, N = 0;
     std::vector<jdsg_plan *> plans;
     std::vector<void *> allocs;
-    float *thk = nullptr, *vp = nullptr, *vs = nullptr, *rho = nullptr;
+    float *thk = nullptr, *vp = nullptr, *vs = nullptr, *rho = nullptr, *vp2 = nullptr;
     std::vector<double *> periods;
     std::vector<StJob> hjobs;
     std::vector<double *> out_c, out_u;
@@ -1110,26 +1122,160 @@ struct jdsg_surftime {
       return 0;
     }
   };
-  Set sets[2];
+  struct Mode {
+    int map = 0;
+    std::vector<Job> jobs;
+    std::vector<StEntry> ents;
+    std::vector<StMapEntry> ments;
+    void *ents_d = nullptr;
+    std::vector<std::unique_ptr<Set>> sets;
+    bool built = false;
+  };
+  Mode modes[2];  // 0: get_1d_kernel, 1: get_2d_map
   std::vector<void *> allocs;
   float *dep_d = nullptr, *node_vp = nullptr, *node_vs = nullptr;
-  int *err_d = nullptr;
-  StEntry *ents_d = nullptr;
+  int *mask_d = nullptr, *cols_d = nullptr;
+  double *tlist_d = nullptr;
+  std::vector<int> mask_h;
+  int ncls[2] = {0, 0};
+  bool have_mask = false;
   ~jdsg_surftime() {
+    modes[0].sets.clear();
+    modes[1].sets.clear();
     for (void *q : allocs) (void)hipFree(q);
+  }
+  template <class T>
+  int dalloc(T **p, size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+      return fail("hipMalloc (surftime)");
+    allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+  }
+  int layers(int cls) const { return cls ? 4 * nz - 6 : 4 * nz - 3; }
+  int ntrow(int w, int m) const {  // SurfaceWave.cpp:157-168
+    const double *col = tab[w].data() + (size_t)m * ntm[w];
+    int n = 0;
+    while (n < ntm[w] && col[n] != -1) ++n;
+    return n;
+  }
+  std::vector<double> list(int w, int m) const {
+    const double *col = tab[w].data() + (size_t)m * ntm[w];
+    return std::vector<double>(col, col + ntrow(w, m));
   }
 };
 
 namespace {
 
-int build_set(jdsg_surftime *S, jdsg_surftime::Set &T, int ncol) {
+// Jobs and entries of one form.  Output index order is get_period_index's
+// (SurfaceWave.cpp:14-48): Rc modes, Rg modes, Lc modes, Lg modes.
+int build_mode(jdsg_surftime *S, jdsg_surftime::Mode &Md) {
+  auto &jobs = Md.jobs;
+  // a job serves a request when wave, list and Vp form agree and it runs at
+  // least the stages needed
+  auto find_job = [&](int wave, int group, int vp2, const std::vector<double> &L) {
+    for (size_t j = 0; j < jobs.size(); ++j)
+      if (jobs[j].wave == wave && jobs[j].list == L && jobs[j].vp2 == vp2 &&
+          (jobs[j].group || !group))
+        return (int)j;
+    jobs.push_back({wave, group, vp2, 0, L});
+    return (int)jobs.size() - 1;
+  };
+  // job per (table, mode, part): part 0 gives the value / root, part 1 U
+  std::vector<std::vector<std::array<int, 2>>> jobof(4);
+  for (int w : {1, 3, 0, 2}) {  // group tables first so phase tables reuse them
+    int wave = w >= 2 ? 1 : 0, grp = (w == 1 || w == 3) ? 1 : 0;
+    jobof[w].assign(std::max(S->nmd[w], 0), {-1, -1});
+    for (int m = 0; m < S->nmd[w]; ++m) {
+      std::vector<double> L = S->list(w, m);
+      if (L.empty()) continue;
+      std::array<int, 2> jj{-1, -1};
+      if (!Md.map) {
+        jj[0] = find_job(wave, grp, 0, L);
+      } else if (grp && wave == 0) {  // Rg: one chain gives vc and, with sregn96, U
+        jj[1] = find_job(0, 1, 0, L);
+        jj[0] = jj[1];
+      } else if (grp) {  // Lg: vc from the chain on the model's Vp, U from Vp = 1.732 Vs
+        jj[1] = find_job(1, 1, 1, L);
+        jj[0] = find_job(1, 0, 0, L);
+      } else {
+        jj[0] = find_job(wave, 0, 0, L);
+      }
+      for (int p : jj)
+        if (p >= 0) {
+          jobs[p].M = std::max(jobs[p].M, m + 1);
+          if (jobs[p].M > JDSG_MAXMODE) return fail("at most 8 modes per table");
+        }
+      jobof[w][m] = jj;
+    }
+  }
+  std::vector<double> tl;
+  for (int w = 0; w < 4; ++w)
+    for (int m = 0; m < S->nmd[w]; ++m) {
+      int nt = S->ntrow(w, m);
+      int grp = (w == 1 || w == 3) ? 1 : 0, love = w >= 2 ? 1 : 0;
+      for (int k = 0; k < nt; ++k) {
+        if (!Md.map)
+          Md.ents.push_back({jobof[w][m][0], m, k, grp, love});
+        else
+          Md.ments.push_back({jobof[w][m][0], jobof[w][m][1], m, k, love});
+        tl.push_back(S->tab[w][(size_t)m * S->ntm[w] + k]);
+      }
+    }
+  size_t ne = Md.map ? Md.ments.size() : Md.ents.size();
+  if ((int)ne != S->kmax) return fail("surftime: inconsistent period index");
+  if (Md.map) {
+    StMapEntry *d = nullptr;
+    if (S->dalloc(&d, ne)) return 1;
+    if (ne && hipMemcpy(d, Md.ments.data(), sizeof(StMapEntry) * ne, hipMemcpyHostToDevice) !=
+                  hipSuccess)
+      return fail("surftime upload");
+    Md.ents_d = d;
+    if (!S->tlist_d) {
+      if (S->dalloc(&S->tlist_d, ne)) return 1;
+      if (ne && hipMemcpy(S->tlist_d, tl.data(), sizeof(double) * ne, hipMemcpyHostToDevice) !=
+                    hipSuccess)
+        return fail("surftime upload");
+    }
+  } else {
+    StEntry *d = nullptr;
+    if (S->dalloc(&d, ne)) return 1;
+    if (ne && hipMemcpy(d, Md.ents.data(), sizeof(StEntry) * ne, hipMemcpyHostToDevice) !=
+                  hipSuccess)
+      return fail("surftime upload");
+    Md.ents_d = d;
+  }
+  Md.built = true;
+  return 0;
+}
+
+// Device bytes of one column's intermediate results in a form.
+double column_bytes(const jdsg_surftime *S, const jdsg_surftime::Mode &Md, int cls) {
+  double b = 16.0 * S->layers(cls) + (Md.map ? 4.0 * S->layers(cls) : 0.0);
+  for (auto &J : Md.jobs) {
+    double ne = (double)J.M * J.list.size();
+    b += ne * 20.0 + (J.group ? 3.0 : 1.0) * ne * 12.0;
+    if (!Md.map) b += ne * S->layers(cls) * 8.0 * (J.group ? 6 : 3);
+  }
+  return b;
+}
+
+int build_set(jdsg_surftime *S, jdsg_surftime::Mode &Md, jdsg_surftime::Set &T, int cls,
+              int ncol) {
+  T.cls = cls;
   T.ncol = ncol;
-  size_t nl = (size_t)ncol * S->N;
+  T.N = S->layers(cls);
+  const int N = T.N;
+  size_t nl = (size_t)ncol * N;
   if (T.dalloc(&T.thk, nl) || T.dalloc(&T.vp, nl) || T.dalloc(&T.vs, nl) || T.dalloc(&T.rho, nl))
     return 1;
+  bool need_vp2 = false;
+  for (auto &J : Md.jobs) need_vp2 |= J.vp2 != 0;
+  if (need_vp2 && T.dalloc(&T.vp2, nl)) return 1;
   std::vector<int> lo(ncol + 1);
-  for (int c = 0; c <= ncol; ++c) lo[c] = c * S->N;
-  for (auto &J : S->jobs) {
+  for (int c = 0; c <= ncol; ++c) lo[c] = c * N;
+  for (auto &J : Md.jobs) {
     int K = (int)J.list.size();
     std::vector<int> po(ncol + 1), nm(2 * (size_t)ncol);
     for (int c = 0; c <= ncol; ++c) po[c] = c * K;
@@ -1138,7 +1284,8 @@ int build_set(jdsg_surftime *S, jdsg_surftime::Set &T, int ncol) {
       nm[2 * c + 1] = J.wave == 1 ? J.M : 0;
     }
     jdsg_plan *P = nullptr;
-    if (jdsg_plan_create(ncol, lo.data(), po.data(), nm.data(), J.group, 0, 1, &P)) return 1;
+    int pg = Md.map ? 0 : J.group;
+    if (jdsg_plan_create(ncol, lo.data(), po.data(), nm.data(), pg, 0, 1, &P)) return 1;
     T.plans.push_back(P);
     std::vector<double> per((size_t)ncol * K);
     for (int c = 0; c < ncol; ++c)
@@ -1151,9 +1298,11 @@ int build_set(jdsg_surftime *S, jdsg_surftime::Set &T, int ncol) {
     T.periods.push_back(pd);
     double *c = nullptr, *u = nullptr;
     int *st = nullptr;
-    if (T.dalloc(&c, P->nent) || T.dalloc(&u, P->nent) || T.dalloc(&st, P->nent)) return 1;
     std::array<double *, 6> k{};
-    int nk = J.group ? 6 : 3;
+    bool eig = !Md.map || J.group;
+    if (eig && (T.dalloc(&c, P->nent) || T.dalloc(&u, P->nent) || T.dalloc(&st, P->nent)))
+      return 1;
+    int nk = Md.map ? 0 : (J.group ? 6 : 3);
     for (int a = 0; a < nk; ++a)
       if (T.dalloc(&k[a], P->nkern)) return 1;
     T.out_c.push_back(c);
@@ -1166,121 +1315,144 @@ int build_set(jdsg_surftime *S, jdsg_surftime::Set &T, int ncol) {
     h.status = st;
     for (int a = 0; a < 3; ++a) {
       h.dc[a] = k[a];
-      h.du[a] = J.group ? k[3 + a] : nullptr;
+      h.du[a] = (!Md.map && J.group) ? k[3 + a] : nullptr;
     }
+    h.cflat = P->cflat;
+    h.cstat = P->cstat;
     h.K = K;
     h.nR = J.wave == 0 ? J.M : 0;
     h.nL = J.wave == 1 ? J.M : 0;
     T.hjobs.push_back(h);
   }
-  if (T.dalloc(&T.djobs, T.hjobs.size())) return 1;
-  if (hipMemcpy(T.djobs, T.hjobs.data(), sizeof(StJob) * T.hjobs.size(),
+  if (T.dalloc(&T.djobs, std::max<size_t>(T.hjobs.size(), 1))) return 1;
+  if (!T.hjobs.empty() &&
+      hipMemcpy(T.djobs, T.hjobs.data(), sizeof(StJob) * T.hjobs.size(),
                 hipMemcpyHostToDevice) != hipSuccess)
     return fail("surftime job upload");
   return 0;
 }
 
+jdsg_surftime::Set *get_set(jdsg_surftime *S, jdsg_surftime::Mode &Md, int cls, int ncol) {
+  for (auto &p : Md.sets)
+    if (p->cls == cls && p->ncol == ncol) return p.get();
+  auto T = std::make_unique<jdsg_surftime::Set>();
+  if (build_set(S, Md, *T, cls, ncol)) return nullptr;
+  Md.sets.push_back(std::move(T));
+  return Md.sets.back().get();
+}
+
+// Reads the water-top mask of vs (synchronises st) and, when it changed,
+// rebuilds the two column lists (land columns first, then water columns).
+int update_mask(jdsg_surftime *S, const float *vs, hipStream_t st) {
+  jdsg_launch_water_mask(S->nxy, vs, S->mask_d, st);
+  HIPCHK(hipGetLastError());
+  std::vector<int> m(S->nxy);
+  HIPCHK(hipMemcpyAsync(m.data(), S->mask_d, sizeof(int) * S->nxy, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (S->have_mask && m == S->mask_h) return 0;
+  std::vector<int> cols;
+  cols.reserve(S->nxy);
+  for (int n = 0; n < S->nxy; ++n)
+    if (!m[n]) cols.push_back(n);
+  S->ncls[0] = (int)cols.size();
+  for (int n = 0; n < S->nxy; ++n)
+    if (m[n]) cols.push_back(n);
+  S->ncls[1] = S->nxy - S->ncls[0];
+  if (S->ncls[1] && S->nz < 3) return fail("a water top needs nz >= 3");
+  HIPCHK(hipMemcpy(S->cols_d, cols.data(), sizeof(int) * S->nxy, hipMemcpyHostToDevice));
+  S->mask_h.swap(m);
+  S->have_mask = true;
+  for (auto &Md : S->modes) Md.sets.clear();  // chunk sizes follow the class sizes
+  return 0;
+}
+
+int surftime_run(jdsg_surftime *S, int mapform, const float *vs, double *vc, double *vout,
+                 double *kernel, int *status, hipStream_t st) {
+  if (S->kmax == 0) return 0;
+  auto &Md = S->modes[mapform];
+  if (!Md.built && build_mode(S, Md)) return 1;
+  if (update_mask(S, vs, st)) return 1;
+  double budget = 12e9;
+  if (const char *b = getenv("JDSG_SURFTIME_BUDGET_GB")) budget = atof(b) * 1e9;
+  for (int cls = 0; cls < 2; ++cls) {
+    int nc = S->ncls[cls];
+    if (nc == 0) continue;
+    const int *cols = S->cols_d + (cls ? S->ncls[0] : 0);
+    double share = budget * nc / S->nxy;
+    int chunk = (int)std::max(1.0, std::min<double>(nc, share / column_bytes(S, Md, cls)));
+    for (int col0 = 0; col0 < nc; col0 += chunk) {
+      int nch = std::min(chunk, nc - col0);
+      jdsg_surftime::Set *T = get_set(S, Md, cls, nch);
+      if (!T) return 1;
+      jdsg_launch_grid_to_layers(col0, nch, S->nxy, S->nz, T->N, cls, cols, vs, S->dep_d, T->thk,
+                                 T->vp, T->vs, T->rho, T->vp2, S->node_vp, S->node_vs, st);
+      for (size_t j = 0; j < Md.jobs.size(); ++j) {
+        auto &J = Md.jobs[j];
+        jdsg_plan *P = T->plans[j];
+        const float *vpj = J.vp2 ? T->vp2 : T->vp;
+        if (!mapform) {
+          auto &k = T->out_k[j];
+          bool g = J.group;
+          if (jdsg_plan_execute(P, T->thk, vpj, T->vs, T->rho, T->periods[j], T->out_c[j],
+                                T->out_u[j], T->out_st[j], k[1], k[0], k[2], nullptr,
+                                g ? k[4] : nullptr, g ? k[3] : nullptr, g ? k[5] : nullptr,
+                                nullptr, st))
+            return 1;
+        } else if (J.group) {
+          if (jdsg_plan_execute(P, T->thk, vpj, T->vs, T->rho, T->periods[j], T->out_c[j],
+                                T->out_u[j], T->out_st[j], nullptr, nullptr, nullptr, nullptr,
+                                nullptr, nullptr, nullptr, nullptr, st))
+            return 1;
+        } else if (run_front(P, T->thk, vpj, T->vs, T->rho, T->periods[j], 0, 1, st)) {
+          return 1;
+        }
+      }
+      if (!mapform)
+        jdsg_launch_surftime_epilogue(col0, nch, S->nxy, S->nz, T->N, cls, S->kmax, cols,
+                                      Md.ents_d, T->djobs, S->node_vp, S->node_vs, vc, vout,
+                                      kernel, status, kernel != nullptr, st);
+      else
+        jdsg_launch_surftime_map_epilogue(col0, nch, S->nxy, S->kmax, cols, Md.ents_d, T->djobs,
+                                          S->tlist_d, vc, vout, status, st);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
-extern "C" int jdsg_surftime_create(int nx, int ny, int nz, const float *dep, int water,
-                                    const double *tRc, int ntRc, int nmRc, const double *tRg,
-                                    int ntRg, int nmRg, const double *tLc, int ntLc, int nmLc,
-                                    const double *tLg, int ntLg, int nmLg, jdsg_surftime **out) {
+extern "C" int jdsg_surftime_create(int nx, int ny, int nz, const float *dep, const double *tRc,
+                                    int ntRc, int nmRc, const double *tRg, int ntRg, int nmRg,
+                                    const double *tLc, int ntLc, int nmLc, const double *tLg,
+                                    int ntLg, int nmLg, jdsg_surftime **out) {
   *out = nullptr;
   if (nx <= 0 || ny <= 0 || nz < 2) return fail("bad grid dimensions");
-  if (water && nz < 3) return fail("a water top needs nz >= 3");
-  auto *S = new jdsg_surftime();
+  auto S = std::make_unique<jdsg_surftime>();
   S->nx = nx;
   S->ny = ny;
   S->nz = nz;
   S->nxy = nx * ny;
-  S->water = water ? 1 : 0;
-  S->N = water ? (nz - 1 + (nz - 2) * 3 + 1) : (nz + (nz - 1) * 3);
   const double *tab[4] = {tRc, tRg, tLc, tLg};
   int ntm[4] = {ntRc, ntRg, ntLc, ntLg}, nmd[4] = {nmRc, nmRg, nmLc, nmLg};
-  auto ntrow = [&](int w, int m) {
-    const double *col = tab[w] + (size_t)m * ntm[w];
-    int n = 0;
-    while (n < ntm[w] && col[n] != -1) ++n;  // SurfaceWave.cpp:157-168
-    return n;
-  };
-  auto find_job = [&](int wave, int group_needed, const std::vector<double> &L) -> int {
-    for (size_t j = 0; j < S->jobs.size(); ++j) {
-      auto &J = S->jobs[j];
-      if (J.wave == wave && J.list == L && (J.group || !group_needed)) return (int)j;
-    }
-    return -1;
-  };
-  // group tables first so a phase table with the same periods reuses them
-  std::vector<std::vector<int>> jobof(4);
-  for (int w : {1, 3, 0, 2}) {
-    int wave = w >= 2 ? 1 : 0, group = (w == 1 || w == 3) ? 1 : 0;
-    if (!tab[w] || nmd[w] <= 0) continue;
-    jobof[w].assign(nmd[w], -1);
-    for (int m = 0; m < nmd[w]; ++m) {
-      int nt = ntrow(w, m);
-      if (nt <= 0) continue;
-      std::vector<double> L(tab[w] + (size_t)m * ntm[w], tab[w] + (size_t)m * ntm[w] + nt);
-      int j = find_job(wave, group, L);
-      if (j < 0) {
-        S->jobs.push_back({wave, group, 0, L});
-        j = (int)S->jobs.size() - 1;
-      }
-      S->jobs[j].M = std::max(S->jobs[j].M, m + 1);
-      if (S->jobs[j].M > JDSG_MAXMODE) {
-        delete S;
-        return fail("at most 8 modes per table");
-      }
-      jobof[w][m] = j;
-    }
-  }
   for (int w = 0; w < 4; ++w) {
-    if (!tab[w] || nmd[w] <= 0) continue;
-    for (int m = 0; m < nmd[w]; ++m) {
-      int nt = ntrow(w, m);
-      for (int k = 0; k < nt; ++k)
-        S->ents.push_back({jobof[w][m], m, k, (w == 1 || w == 3) ? 1 : 0, w >= 2 ? 1 : 0});
-    }
+    if (!tab[w] || nmd[w] <= 0 || ntm[w] <= 0) continue;
+    S->ntm[w] = ntm[w];
+    S->nmd[w] = nmd[w];
+    S->tab[w].assign(tab[w], tab[w] + (size_t)ntm[w] * nmd[w]);
   }
-  S->kmax = (int)S->ents.size();
-  // memory per column of the intermediate layer kernels
-  double per_col = 0;
-  for (auto &J : S->jobs)
-    per_col += (double)J.M * J.list.size() * S->N * 8.0 * (J.group ? 6 : 3) + 64.0 * J.list.size();
-  double budget = 12e9;
-  if (const char *b = getenv("JDSG_SURFTIME_BUDGET_GB")) budget = atof(b) * 1e9;
-  S->chunk = (int)std::max(1.0, std::min<double>(S->nxy, per_col > 0 ? budget / per_col : S->nxy));
-  auto alloc = [&](auto **p, size_t n) {
-    void *q = nullptr;
-    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(**p)) != hipSuccess) return 1;
-    S->allocs.push_back(q);
-    *p = (std::remove_reference_t<decltype(*p)>)q;
-    return 0;
-  };
-  if (alloc(&S->dep_d, nz) || alloc(&S->node_vp, (size_t)nz * S->nxy) ||
-      alloc(&S->node_vs, (size_t)nz * S->nxy) || alloc(&S->err_d, 1) ||
-      alloc(&S->ents_d, S->ents.size())) {
-    delete S;
-    return fail("hipMalloc (surftime)");
-  }
-  if (hipMemcpy(S->dep_d, dep, sizeof(float) * nz, hipMemcpyHostToDevice) != hipSuccess ||
-      (!S->ents.empty() && hipMemcpy(S->ents_d, S->ents.data(), sizeof(StEntry) * S->ents.size(),
-                                     hipMemcpyHostToDevice) != hipSuccess)) {
-    delete S;
+  S->kmax = 0;
+  for (int w = 0; w < 4; ++w)
+    for (int m = 0; m < S->nmd[w]; ++m) S->kmax += S->ntrow(w, m);
+  S->modes[1].map = 1;
+  if (S->dalloc(&S->dep_d, nz) || S->dalloc(&S->node_vp, (size_t)nz * S->nxy) ||
+      S->dalloc(&S->node_vs, (size_t)nz * S->nxy) || S->dalloc(&S->mask_d, S->nxy) ||
+      S->dalloc(&S->cols_d, S->nxy))
+    return 1;
+  if (hipMemcpy(S->dep_d, dep, sizeof(float) * nz, hipMemcpyHostToDevice) != hipSuccess)
     return fail("surftime upload");
-  }
-  if (!S->jobs.empty()) {
-    if (build_set(S, S->sets[0], S->chunk)) {
-      delete S;
-      return 1;
-    }
-    int tail = S->nxy % S->chunk;
-    if (tail && build_set(S, S->sets[1], tail)) {
-      delete S;
-      return 1;
-    }
-  }
-  *out = S;
+  if (build_mode(S.get(), S->modes[0])) return 1;
+  *out = S.release();
   return 0;
 }
 
@@ -1293,66 +1465,58 @@ extern "C" void jdsg_surftime_destroy(jdsg_surftime *S) { delete S; }
 
 extern "C" int jdsg_surftime_kernel(jdsg_surftime *S, const float *vs, double *vc, double *vout,
                                     double *kernel, int *status, void *stream) {
-  hipStream_t st = (hipStream_t)stream;
-  if (S->kmax == 0) return 0;
-  jdsg_launch_fill_u32((unsigned *)S->err_d, 1, 0u, st);
-  for (int col0 = 0; col0 < S->nxy; col0 += S->chunk) {
-    int nch = std::min(S->chunk, S->nxy - col0);
-    jdsg_surftime::Set &T = (nch == S->sets[0].ncol) ? S->sets[0] : S->sets[1];
-    jdsg_launch_grid_to_layers(col0, nch, S->nxy, S->nz, S->N, S->water, vs, S->dep_d, T.thk,
-                               T.vp, T.vs, T.rho, S->node_vp, S->node_vs, S->err_d, st);
-    for (size_t j = 0; j < S->jobs.size(); ++j) {
-      auto &k = T.out_k[j];
-      bool g = S->jobs[j].group;
-      if (jdsg_plan_execute(T.plans[j], T.thk, T.vp, T.vs, T.rho, T.periods[j], T.out_c[j],
-                            T.out_u[j], T.out_st[j], k[1], k[0], k[2], nullptr,
-                            g ? k[4] : nullptr, g ? k[3] : nullptr, g ? k[5] : nullptr,
-                            nullptr, st))
-        return 1;
-    }
-    jdsg_launch_surftime_epilogue(col0, nch, S->nxy, S->nz, S->N, S->water, S->kmax, S->ents_d,
-                                  T.djobs, S->node_vp, S->node_vs, vc, vout, kernel, status,
-                                  kernel != nullptr, st);
-    HIPCHK(hipGetLastError());
-  }
-  int err = 0;
-  HIPCHK(hipMemcpyAsync(&err, S->err_d, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  if (err) return fail("vs(0)==0 (water top) must hold for all columns or none, as declared");
+  return surftime_run(S, 0, vs, vc, vout, kernel, status, (hipStream_t)stream);
+}
+
+extern "C" int jdsg_surftime_map(jdsg_surftime *S, const float *vs, double *vc, double *vout,
+                                 int *status, void *stream) {
+  return surftime_run(S, 1, vs, vc, vout, nullptr, status, (hipStream_t)stream);
+}
+
+namespace {
+int host_grid_call(int mapform, int nx, int ny, int nz, const float *vs, const float *dep,
+                   const double *tRc, int ntRc, int nmRc, const double *tRg, int ntRg, int nmRg,
+                   const double *tLc, int ntLc, int nmLc, const double *tLg, int ntLg, int nmLg,
+                   double *vc, double *vout, double *kernel, int *status) {
+  int nxy = nx * ny;
+  jdsg_surftime *S = nullptr;
+  if (jdsg_surftime_create(nx, ny, nz, dep, tRc, ntRc, nmRc, tRg, ntRg, nmRg, tLc, ntLc, nmLc,
+                           tLg, ntLg, nmLg, &S))
+    return 1;
+  std::unique_ptr<jdsg_surftime> own(S);
+  DevBuf B;
+  size_t nk = (size_t)nxy * S->kmax;
+  float *dvs = B.up(vs, (size_t)nxy * nz);
+  double *dvc = B.up<double>(nullptr, nk), *dvo = B.up<double>(nullptr, nk);
+  double *dker = (kernel && !mapform) ? B.up<double>(nullptr, nk * nz) : nullptr;
+  int *dst = B.up<int>(nullptr, nk);
+  if (!dvs || !dvc || !dvo || !dst || (kernel && !mapform && !dker))
+    return fail("device allocation");
+  if (dker) HIPCHK(hipMemset(dker, 0, sizeof(double) * nk * nz));
+  if (surftime_run(S, mapform, dvs, dvc, dvo, dker, dst, nullptr)) return 1;
+  HIPCHK(hipMemcpy(vc, dvc, sizeof(double) * nk, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vout, dvo, sizeof(double) * nk, hipMemcpyDeviceToHost));
+  if (dker) HIPCHK(hipMemcpy(kernel, dker, sizeof(double) * nk * nz, hipMemcpyDeviceToHost));
+  if (status) HIPCHK(hipMemcpy(status, dst, sizeof(int) * nk, hipMemcpyDeviceToHost));
   return 0;
 }
+}  // namespace
 
 extern "C" int jdsg_get_1d_kernel(int nx, int ny, int nz, const float *vs, const float *dep,
                                   const double *tRc, int ntRc, int nmRc, const double *tRg,
                                   int ntRg, int nmRg, const double *tLc, int ntLc, int nmLc,
                                   const double *tLg, int ntLg, int nmLg, double *vc,
                                   double *vout, double *kernel, int *status) {
-  int nxy = nx * ny;
-  int water = (nxy > 0 && vs[0] == 0.0f) ? 1 : 0;
-  jdsg_surftime *S = nullptr;
-  if (jdsg_surftime_create(nx, ny, nz, dep, water, tRc, ntRc, nmRc, tRg, ntRg, nmRg, tLc, ntLc,
-                           nmLc, tLg, ntLg, nmLg, &S))
-    return 1;
-  DevBuf B;
-  size_t nk = (size_t)nxy * S->kmax;
-  float *dvs = B.up(vs, (size_t)nxy * nz);
-  double *dvc = B.up<double>(nullptr, nk), *dvo = B.up<double>(nullptr, nk);
-  double *dker = kernel ? B.up<double>(nullptr, nk * nz) : nullptr;
-  int *dst = B.up<int>(nullptr, nk);
-  int rc = (!dvs || !dvc || !dvo || !dst || (kernel && !dker)) ? fail("device allocation") : 0;
-  if (!rc && kernel) (void)hipMemset(dker, 0, sizeof(double) * nk * nz);
-  if (!rc) rc = jdsg_surftime_kernel(S, dvs, dvc, dvo, dker, dst, nullptr);
-  if (!rc) {
-    hipError_t e = hipMemcpy(vc, dvc, sizeof(double) * nk, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(vout, dvo, sizeof(double) * nk, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && kernel)
-      e = hipMemcpy(kernel, dker, sizeof(double) * nk * nz, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && status)
-      e = hipMemcpy(status, dst, sizeof(int) * nk, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = fail("copy back");
-  }
-  jdsg_surftime_destroy(S);
-  return rc;
+  return host_grid_call(0, nx, ny, nz, vs, dep, tRc, ntRc, nmRc, tRg, ntRg, nmRg, tLc, ntLc,
+                        nmLc, tLg, ntLg, nmLg, vc, vout, kernel, status);
+}
+
+extern "C" int jdsg_get_2d_map(int nx, int ny, int nz, const float *vs, const float *dep,
+                               const double *tRc, int ntRc, int nmRc, const double *tRg, int ntRg,
+                               int nmRg, const double *tLc, int ntLc, int nmLc, const double *tLg,
+                               int ntLg, int nmLg, double *vc, double *vout, int *status) {
+  return host_grid_call(1, nx, ny, nz, vs, dep, tRc, ntRc, nmRc, tRg, ntRg, nmRg, tLc, ntLc,
+                        nmLc, tLg, ntLg, nmLg, vc, vout, nullptr, status);
 }
 
 // ---------------------------------------------------------------- Fréchet rows

@@ -105,7 +105,14 @@ struct StJob {
   const double *c, *u;
   const int *status;
   const double *dc[3], *du[3];  // b, a, rho
+  const double *cflat;          // the plan's flat roots (T variant first)
+  const int *cstat;             // the plan's chain status bits
   int K, nR, nL;
+};
+// get_2d_map output entry: the job whose chain gives vc, the job giving U
+// for Rg/Lg (-1 for Rc/Lc), mode and period index within the jobs' lists
+struct StMapEntry {
+  int job_c, job_u, mode, k, love;
 };
 
 
@@ -145,15 +152,19 @@ void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st
 size_t jdsg_chain_state_bytes();
 void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st);
 void jdsg_launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t st);
+void jdsg_launch_water_mask(int nxy, const float *vs, int *mask, hipStream_t st);
 void jdsg_launch_grid_to_layers(int col0, int nchunk, int nxy, int nz, int N, int water,
-                                const float *vs, const float *dep, float *thk, float *vp,
-                                float *vsl, float *rho, float *node_vp, float *node_vs, int *err,
-                                hipStream_t st);
+                                const int *cols, const float *vs, const float *dep, float *thk,
+                                float *vp, float *vsl, float *rho, float *vp2, float *node_vp,
+                                float *node_vs, hipStream_t st);
 void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N, int water,
-                                   int kmax, const void *ents, const void *jobs,
+                                   int kmax, const int *cols, const void *ents, const void *jobs,
                                    const float *node_vp, const float *node_vs, double *vc,
                                    double *vout, double *kernel, int *status, int want_kernel,
                                    hipStream_t st);
+void jdsg_launch_surftime_map_epilogue(int col0, int nchunk, int nxy, int kmax, const int *cols,
+                                       const void *ents, const void *jobs, const double *tlist,
+                                       double *vc, double *vout, int *status, hipStream_t st);
 void jdsg_launch_polish(const PolishArgs &A, int wave, int nwaves, hipStream_t st);
 void jdsg_launch_mlp_expand(long long e0, long long nrow, int ncol, const long long *ent_off,
                             const int *per_off, const double *periods, const int *nmode,

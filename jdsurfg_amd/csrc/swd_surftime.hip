@@ -30,21 +30,33 @@ JDSG_DEV void brocher_deriv(float vp, float vs, float &drda, float &dadb) {
 
 }  // namespace
 
-// One thread per column of a chunk [col0, col0+nchunk).  vs is (nx, ny, nz)
-// column-major; column n = i + j*nx (SurfaceWave.cpp:433-434).  The chunk's
-// layer stacks are written column-contiguous (column nl at nl*N); with a
-// water top (vs(0) = 0, uniform over the grid) N = 4nz-6, else 4nz-3.
-// node_vp/node_vs keep the node values for the chain rule.
+// mask[n] = 1 where column n has a water top (vs(i,j,0) == 0), else 0:
+// grid2LayerModel and _ParamConvert decide this per column
+// (SurfaceWave.cpp:120, 71, 495).
+extern "C" __global__ void __launch_bounds__(256)
+jdsg_water_mask(int nxy, const float *__restrict__ vs, int *__restrict__ mask) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < nxy) mask[n] = vs[n] == 0.0f ? 1 : 0;
+}
+
+// One thread per column of a chunk: chunk column nl is grid column
+// n = cols[col0 + nl] (all columns of a chunk share the water class).  vs is
+// (nx, ny, nz) column-major; column n = i + j*nx (SurfaceWave.cpp:433-434).
+// The chunk's layer stacks are written column-contiguous (column nl at
+// nl*N); with a water top N = 4nz-6, else 4nz-3.  vp2, when not null,
+// receives 1.732 Vs per layer: the Vp that groupvel_l hands to its root
+// chain (surfdisp.cpp:139).  node_vp/node_vs keep the node values for the
+// chain rule.
 extern "C" __global__ void __launch_bounds__(256)
 jdsg_grid_to_layers(int col0, int nchunk, int nxy, int nz, int N, int water,
-                    const float *__restrict__ vs, const float *__restrict__ dep, float *thk,
-                    float *vp, float *vsl, float *rho, float *node_vp, float *node_vs, int *err) {
+                    const int *__restrict__ cols, const float *__restrict__ vs,
+                    const float *__restrict__ dep, float *thk, float *vp, float *vsl, float *rho,
+                    float *vp2, float *node_vp, float *node_vs) {
   int nl = blockIdx.x * blockDim.x + threadIdx.x;
   if (nl >= nchunk) return;
-  int n = col0 + nl;
+  int n = cols[col0 + nl];
   const int sub = 3, ns = sub + 1;
   float vs0 = vs[n];
-  if ((vs0 == 0.0f) != (water != 0)) atomicExch(err, 1);
   for (int k = 0; k < nz; ++k) {
     float a, b;
     float s = vs[n + (size_t)k * nxy];
@@ -86,19 +98,22 @@ jdsg_grid_to_layers(int col0, int nchunk, int nxy, int nz, int N, int water,
   vp[base + k] = pb;
   vsl[base + k] = sb;
   rho[base + k] = rb;
+  if (vp2)
+    for (int i = 0; i < N; ++i) vp2[base + i] = (float)(1.732 * (double)vsl[base + i]);
 }
 
 // One thread per (column of the chunk, SurfTime index idx).
 extern "C" __global__ void __launch_bounds__(256)
 jdsg_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N, int water, int kmax,
-                       const StEntry *__restrict__ ents, const StJob *__restrict__ jobs,
-                       const float *__restrict__ node_vp, const float *__restrict__ node_vs,
-                       double *vc, double *vout, double *kernel, int *status, int want_kernel) {
+                       const int *__restrict__ cols, const StEntry *__restrict__ ents,
+                       const StJob *__restrict__ jobs, const float *__restrict__ node_vp,
+                       const float *__restrict__ node_vs, double *vc, double *vout,
+                       double *kernel, int *status, int want_kernel) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nchunk * kmax) return;
   int nl = (int)(t % nchunk);
   int idx = (int)(t / nchunk);
-  int n = col0 + nl;
+  int n = cols[col0 + nl];
   StEntry E = ents[idx];
   const StJob &J = jobs[E.job];
   int woff = E.love ? J.nR : 0;
@@ -154,24 +169,79 @@ jdsg_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N, int water, 
   }
 }
 
+// get_2d_map (SurfaceWave.cpp:420-450 over disp1D :177-256): one thread per
+// (column of the chunk, SurfTime index).  vc is the root of the entry's
+// chain taken to the sphere by _flat2sphere (surfdisp.cpp:17-51: R = 6371 km,
+// the Rayleigh factor 0.5 for Rc/Rg and the Love factor 1.5 for Lc/Lg, phase
+// form); vout is vc for Rc/Lc and U of the group job (groupvel_r/_l,
+// surfdisp.cpp:126-174) for Rg/Lg.  status: the vc chain's (4 on the exit
+// path, 1 without a root), else the group job's.
+extern "C" __global__ void __launch_bounds__(256)
+jdsg_surftime_map_epilogue(int col0, int nchunk, int nxy, int kmax, const int *__restrict__ cols,
+                           const StMapEntry *__restrict__ ents, const StJob *__restrict__ jobs,
+                           const double *__restrict__ tlist, double *vc, double *vout,
+                           int *status) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nchunk * kmax) return;
+  int nl = (int)(t % nchunk);
+  int idx = (int)(t / nchunk);
+  int n = cols[col0 + nl];
+  StMapEntry E = ents[idx];
+  const StJob &J = jobs[E.job_c];
+  long long e = (long long)nl * (J.nR + J.nL) * J.K + (long long)E.mode * J.K + E.k;
+  double c = J.cflat[e];
+  int st = (J.cstat[e] & JDSG_EXIT_PATH) ? JDSG_EXIT_PATH : (c == 0.0 ? JDSG_NO_ROOT : 0);
+  const double ar = 6371.0;
+  double T = tlist[idx];
+  double omega = 2.0 * M_PI / T;
+  double x = (E.love ? 1.5 : 0.5) * c / (ar * omega);
+  double tm = sqrt(1. + x * x);
+  double cs = c / tm;
+  size_t o = (size_t)n + (size_t)idx * nxy;
+  vc[o] = cs;
+  double v = cs;
+  if (E.job_u >= 0) {
+    const StJob &G = jobs[E.job_u];
+    long long eg = (long long)nl * (G.nR + G.nL) * G.K + (long long)E.mode * G.K + E.k;
+    v = G.u[eg];
+    if (st == 0) st = G.status[eg];
+  }
+  vout[o] = v;
+  status[o] = st;
+}
+
+void jdsg_launch_water_mask(int nxy, const float *vs, int *mask, hipStream_t st) {
+  hipLaunchKernelGGL(jdsg_water_mask, dim3((nxy + 255) / 256), dim3(256), 0, st, nxy, vs, mask);
+}
+
 void jdsg_launch_grid_to_layers(int col0, int nchunk, int nxy, int nz, int N, int water,
-                                const float *vs, const float *dep, float *thk, float *vp,
-                                float *vsl, float *rho, float *node_vp, float *node_vs, int *err,
-                                hipStream_t st) {
+                                const int *cols, const float *vs, const float *dep, float *thk,
+                                float *vp, float *vsl, float *rho, float *vp2, float *node_vp,
+                                float *node_vs, hipStream_t st) {
   hipLaunchKernelGGL(jdsg_grid_to_layers, dim3((nchunk + 255) / 256), dim3(256), 0, st, col0,
-                     nchunk, nxy, nz, N, water, vs, dep, thk, vp, vsl, rho, node_vp, node_vs,
-                     err);
+                     nchunk, nxy, nz, N, water, cols, vs, dep, thk, vp, vsl, rho, vp2, node_vp,
+                     node_vs);
 }
 
 void jdsg_launch_surftime_epilogue(int col0, int nchunk, int nxy, int nz, int N, int water,
-                                   int kmax, const void *ents, const void *jobs,
+                                   int kmax, const int *cols, const void *ents, const void *jobs,
                                    const float *node_vp, const float *node_vs, double *vc,
                                    double *vout, double *kernel, int *status, int want_kernel,
                                    hipStream_t st) {
   long long n = (long long)nchunk * kmax;
   if (n <= 0) return;
   hipLaunchKernelGGL(jdsg_surftime_epilogue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     st, col0, nchunk, nxy, nz, N, water, kmax, (const StEntry *)ents,
+                     st, col0, nchunk, nxy, nz, N, water, kmax, cols, (const StEntry *)ents,
                      (const StJob *)jobs, node_vp, node_vs, vc, vout, kernel, status,
                      want_kernel);
+}
+
+void jdsg_launch_surftime_map_epilogue(int col0, int nchunk, int nxy, int kmax, const int *cols,
+                                       const void *ents, const void *jobs, const double *tlist,
+                                       double *vc, double *vout, int *status, hipStream_t st) {
+  long long n = (long long)nchunk * kmax;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(jdsg_surftime_map_epilogue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     st, col0, nchunk, nxy, kmax, cols, (const StMapEntry *)ents,
+                     (const StJob *)jobs, tlist, vc, vout, status);
 }

@@ -240,7 +240,94 @@ def grid_to_csr(plan, arr, N):
     return np.ascontiguousarray(a.transpose(2, 0, 1)).reshape(-1)
 
 
-def get_1d_kernel(vs, dep, tables, water=None):
+class SurfTime:
+    """The SurfTime grid forms on the GPU for a fixed depth grid and period
+    tables (SurfaceWave.hpp:102-115): kernel() is get_1d_kernel, map() is
+    get_2d_map, both on a device-resident (nx, ny, nz) Vs model.
+
+    tables: dict with keys 'Rc','Rg','Lc','Lg' -> (nt, nmode) period arrays
+    padded with -1.  A column whose top node has Vs = 0 has a water top."""
+
+    def __init__(self, nx, ny, nz, dep, tables):
+        L = lib()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        args, self._keep = [], []
+        for w in ("Rc", "Rg", "Lc", "Lg"):
+            a = tables.get(w)
+            if a is None or np.size(a) == 0:
+                args += [None, 0, 0]
+            else:
+                a = np.asfortranarray(np.asarray(a, np.float64))
+                self._keep.append(a)
+                args += [C.c_void_p(a.ctypes.data), a.shape[0], a.shape[1]]
+        h = C.c_void_p()
+        depa = np.ascontiguousarray(dep, np.float32)
+        L.jdsg_surftime_create.argtypes = [C.c_int] * 3 + [C.c_void_p] + \
+            [C.c_void_p, C.c_int, C.c_int] * 4 + [C.POINTER(C.c_void_p)]
+        if L.jdsg_surftime_create(nx, ny, nz, depa.ctypes.data, *args, C.byref(h)):
+            raise RuntimeError("jdsg_surftime_create: " + last_error())
+        self.h = h
+        km = C.c_int()
+        L.jdsg_surftime_kmax.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.jdsg_surftime_kmax(h, C.byref(km))
+        self.kmax = km.value
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.jdsg_surftime_destroy.argtypes = [C.c_void_p]
+            _lib.jdsg_surftime_destroy(self.h)
+            self.h = None
+
+    def alloc(self, kernel=True):
+        import torch
+        nxy = self.nx * self.ny
+        o = {"vc": torch.zeros(nxy * self.kmax, dtype=torch.float64, device="cuda"),
+             "vout": torch.zeros(nxy * self.kmax, dtype=torch.float64, device="cuda"),
+             "status": torch.zeros(nxy * self.kmax, dtype=torch.int32, device="cuda")}
+        if kernel:
+            o["kernel"] = torch.zeros(nxy * self.nz * self.kmax, dtype=torch.float64,
+                                      device="cuda")
+        return o
+
+    def kernel(self, vs, out, stream=None):
+        """get_1d_kernel on vs (CUDA float32, (nx, ny, nz) column-major
+        flattened); fills out from alloc()."""
+        import torch
+        L = lib()
+        L.jdsg_surftime_kernel.argtypes = [C.c_void_p] * 7
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        if L.jdsg_surftime_kernel(self.h, _ptr(vs), _ptr(out["vc"]), _ptr(out["vout"]),
+                                  _ptr(out.get("kernel")), _ptr(out["status"]), C.c_void_p(st)):
+            raise RuntimeError("jdsg_surftime_kernel: " + last_error())
+        return out
+
+    def map(self, vs, out, stream=None):
+        """get_2d_map on vs; fills vc, vout, status of out."""
+        import torch
+        L = lib()
+        L.jdsg_surftime_map.argtypes = [C.c_void_p] * 6
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        if L.jdsg_surftime_map(self.h, _ptr(vs), _ptr(out["vc"]), _ptr(out["vout"]),
+                               _ptr(out["status"]), C.c_void_p(st)):
+            raise RuntimeError("jdsg_surftime_map: " + last_error())
+        return out
+
+    def to_host(self, out):
+        nxy = self.nx * self.ny
+        f = lambda t, shp: t.cpu().numpy().reshape(shp, order="F")
+        r = [f(out["vc"], (nxy, self.kmax)), f(out["vout"], (nxy, self.kmax))]
+        if "kernel" in out:
+            r.append(f(out["kernel"], (nxy, self.nz, self.kmax)))
+        r.append(f(out["status"], (nxy, self.kmax)))
+        return tuple(r)
+
+
+def _upload_vs(vs):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(vs, np.float32).ravel(order="F"))).cuda()
+
+
+def get_1d_kernel(vs, dep, tables):
     """SurfTime::get_1d_kernel on the GPU (host arrays in, host arrays out).
 
     vs: (nx, ny, nz) float array; dep: (nz,) node depths; tables: dict with
@@ -248,47 +335,18 @@ def get_1d_kernel(vs, dep, tables, water=None):
     Returns vc, vout (nx*ny, kmax), kernel (nx*ny, nz, kmax) in the reference's
     column-major order, and status (nx*ny, kmax)."""
     import torch
-    L = lib()
     nx, ny, nz = vs.shape
-    vsf = torch.from_numpy(np.ascontiguousarray(np.asarray(vs, np.float32).ravel(order="F"))).cuda()
-    args = []
-    keep = []
-    for w in ("Rc", "Rg", "Lc", "Lg"):
-        a = tables.get(w)
-        if a is None or np.size(a) == 0:
-            args += [None, 0, 0]
-        else:
-            a = np.asfortranarray(np.asarray(a, np.float64))
-            keep.append(a)
-            args += [C.c_void_p(a.ctypes.data), a.shape[0], a.shape[1]]
-    if water is None:
-        water = bool(np.asarray(vs)[0, 0, 0] == 0.0)
-    h = C.c_void_p()
-    depa = np.ascontiguousarray(dep, np.float32)
-    L.jdsg_surftime_create.argtypes = [C.c_int] * 3 + [C.c_void_p, C.c_int] + \
-        [C.c_void_p, C.c_int, C.c_int] * 4 + [C.POINTER(C.c_void_p)]
-    rc = L.jdsg_surftime_create(nx, ny, nz, depa.ctypes.data, int(water), *args, C.byref(h))
-    if rc:
-        raise RuntimeError("jdsg_surftime_create: " + last_error())
-    try:
-        km = C.c_int()
-        L.jdsg_surftime_kmax.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
-        L.jdsg_surftime_kmax(h, C.byref(km))
-        kmax = km.value
-        nxy = nx * ny
-        vc = torch.zeros(nxy * kmax, dtype=torch.float64, device="cuda")
-        vout = torch.zeros_like(vc)
-        ker = torch.zeros(nxy * nz * kmax, dtype=torch.float64, device="cuda")
-        st = torch.zeros(nxy * kmax, dtype=torch.int32, device="cuda")
-        L.jdsg_surftime_kernel.argtypes = [C.c_void_p] * 7
-        rc = L.jdsg_surftime_kernel(h, _ptr(vsf), _ptr(vc), _ptr(vout), _ptr(ker), _ptr(st),
-                                    C.c_void_p(torch.cuda.current_stream().cuda_stream))
-        if rc:
-            raise RuntimeError("jdsg_surftime_kernel: " + last_error())
-        torch.cuda.synchronize()
-        f = lambda t, shp: t.cpu().numpy().reshape(shp, order="F")
-        return (f(vc, (nxy, kmax)), f(vout, (nxy, kmax)), f(ker, (nxy, nz, kmax)),
-                f(st, (nxy, kmax)))
-    finally:
-        L.jdsg_surftime_destroy.argtypes = [C.c_void_p]
-        L.jdsg_surftime_destroy(h)
+    S = SurfTime(nx, ny, nz, dep, tables)
+    out = S.kernel(_upload_vs(vs), S.alloc(kernel=True))
+    torch.cuda.synchronize()
+    return S.to_host(out)
+
+
+def get_2d_map(vs, dep, tables):
+    """SurfTime::get_2d_map on the GPU: vc, vout (nx*ny, kmax) and status."""
+    import torch
+    nx, ny, nz = vs.shape
+    S = SurfTime(nx, ny, nz, dep, tables)
+    out = S.map(_upload_vs(vs), S.alloc(kernel=False))
+    torch.cuda.synchronize()
+    return S.to_host(out)

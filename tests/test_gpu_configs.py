@@ -14,9 +14,8 @@ Bars: status identical; c and U within 1e-6 and kernels within 1e-4 (of
 the entry's largest layer value) of the oracle at the reference tolerance,
 U and kernels checked at the GPU's roots where a root differs in its last
 bits; exact-root mode and the surrogate against the exact-root oracle
-(tolerance 1e-10): c within 1e-8, U within 1e-7, dc/dm within 1e-4, and
-dU/dm within 1e-4 except where an entry's T(1+-0.005) roots sit within 1e-6
-of a mode osculation (reported).
+(tolerance 1e-12 for both): c within 1e-8, U within 1e-7, dc/dm and dU/dm
+within 1e-4.
 """
 import numpy as np
 import pytest
@@ -26,7 +25,11 @@ from tests.compare import (KERNELS, differing_root_report, gpu_solve, kernel_err
                            same_roots, scalar_errors)
 
 pytestmark = pytest.mark.gpu
-EXACT_TOL = 1e-10
+# exact-root stop rule: at 1e-10 the oracle's own dU/dm still moves by
+# 2e-4 on the C4 sediment columns when the tolerance is tightened (the
+# difference quotient over T(1+-0.005) amplifies the root's last digits near
+# mode osculations); at 1e-12 it moves by at most 5e-5
+EXACT_TOL = 1e-12
 
 
 def _c5():
@@ -126,7 +129,6 @@ def test_exact_and_surrogate_match_exact_oracle(name, oracle_rest):
           f"({st['roots_left_to_chain'] / nroot:.4f})", st, flush=True)
     ks = _check_exact(name, "surrogate", _gpu_csr(plan, sout, w), ref, w)
     # dU/dm: the difference quotient over T(1+-0.005) (sregn96.f90:1831-1834)
-    # divides root differences of ~1e-10 by 0.01 of the period; both exact
-    # solves stay within 1e-4 of the oracle on these subsets
+    # divides root differences by 0.01 of the period
     for n in ("duda", "dudb", "dudr", "dudh"):
         assert ke[n] <= 1e-4 and ks[n] <= 1e-4, (n, ke[n], ks[n])

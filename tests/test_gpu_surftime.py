@@ -47,3 +47,85 @@ def test_grid_with_two_mode_tables(oracle_rest):
     assert ok.sum() > 0.9 * ok.size
     _check(a, b[:3], ok)
     _check(a, (g["g_vc"], g["g_vout"], g["g_kernel"]), ok)
+
+
+def test_map_matches_reference_fixture():
+    """get_2d_map (disp1D: one chain per wave and mode, _flat2sphere, the
+    analytic group velocity of groupvel_r/_l) against the reference's map of
+    test/swd/model.txt and of the 3x2 grid with two-mode tables."""
+    from jdsurfg_amd import swd
+    g = dict(np.load(os.path.join(GOLD, "surftime_model_txt.npz")))
+    t = g["periods"]
+    tabs = {w: t.reshape(-1, 1) for w in ("Rc", "Rg", "Lc", "Lg")}
+    print("case get_2d_map model.txt", flush=True)
+    vc, vout, st = swd.get_2d_map(g["vs"].reshape(1, 1, -1), g["z"], tabs)
+    assert (st == 0).all()
+    assert np.abs(vc / g["map_vc"] - 1).max() <= 1e-6
+    assert np.abs(vout / g["map_vout"] - 1).max() <= 1e-6
+    print("map model.txt: vc identical", float((vc == g["map_vc"]).mean()), flush=True)
+    tabs3 = {"Rc": g["tR"], "Rg": g["tR"], "Lc": g["tL"], "Lg": g["tL"]}
+    print("case get_2d_map 3x2 grid", flush=True)
+    vc, vout, st = swd.get_2d_map(g["vs3"], g["z"], tabs3)
+    ok = st == 0
+    assert ok.sum() > 0.9 * ok.size
+    assert np.array_equal(g["g_map_vc"][~ok], np.zeros((~ok).sum()))
+    assert np.abs(vc[ok] / g["g_map_vc"][ok] - 1).max() <= 1e-6
+    assert np.abs(vout[ok] / g["g_map_vout"][ok] - 1).max() <= 1e-6
+
+
+def _coastal_grid():
+    """A 4x3 grid on 21 nodes (0-40 km): land columns with gradients and a
+    low-velocity zone, and water columns (vs = 0 at the top node) over slow
+    sediments, mixed in one grid."""
+    rng = np.random.default_rng(17)
+    dep = (2.0 * np.arange(21)).astype(np.float32)
+    base = 2.6 + 0.04 * dep
+    vs = np.zeros((4, 3, 21), np.float32)
+    for i in range(4):
+        for j in range(3):
+            v = base + rng.normal(0, 0.05, 21)
+            if (i + j) % 3 == 0:
+                v[0] = 0.0
+                v[1:3] = rng.uniform(1.0, 1.8, 2)
+            vs[i, j] = v
+    return vs, dep
+
+
+def test_mixed_water_and_land_grid(oracle_rest):
+    """A grid mixing water-top and land columns: get_1d_kernel and
+    get_2d_map against the oracle's glue, Rayleigh modes 0-1 (phase and
+    group).  Love waves are compared on the land columns: below water the
+    reference's slegn96 reads an unset value (uu(1) of a fluid top layer)."""
+    from jdsurfg_amd import swd
+    vs, dep = _coastal_grid()
+    t = np.array([4.0, 6.0, 9.0, 14.0, 22.0, 35.0])
+    two = np.full((6, 2), -1.0)
+    two[:, 0] = t
+    two[:4, 1] = t[:4]
+    tabs = {"Rc": two, "Rg": two, "Lc": t.reshape(-1, 1), "Lg": t.reshape(-1, 1)}
+    nxy = 12
+    water = (vs[:, :, 0] == 0).ravel(order="F")
+    print("case coastal grid get_1d_kernel", flush=True)
+    a = swd.get_1d_kernel(vs, dep, tabs)
+    b = oracle_rest.get_1d_kernel(vs, dep, tabs)
+    nR = 6 + 4
+    love = np.zeros(a[0].shape, bool)
+    love[:, 2 * nR:] = True
+    ok = (a[3] == 0) & ~(love & water[:, None])
+    assert ok.sum() > 0.8 * ok.size
+    assert water.sum() >= 3 and (~water).sum() >= 3
+    # the water column's top node carries no kernel (SurfaceWave.cpp:495)
+    kb = b[2].copy()
+    kb[water, 0, :] = 0.0
+    _check(a, (b[0], b[1], kb), ok)
+    print("case coastal grid get_2d_map", flush=True)
+    m = swd.get_2d_map(vs, dep, tabs)
+    mo = oracle_rest.get_2d_map(vs, dep, tabs)
+    okm = (m[2] == 0) & ~(love & water[:, None])
+    assert np.abs(m[0][okm] / mo[0][okm] - 1).max() <= 1e-6
+    assert np.abs(m[1][okm] / mo[1][okm] - 1).max() <= 1e-6
+    # the map's phase velocities are the kernel form's (same chains)
+    both = okm & ok
+    assert np.abs(m[0][both] / a[0][both] - 1).max() <= 1e-12
+    print("coastal: Love in water columns (kernel form) GPU", a[1][water][:, 2 * nR:][:2],
+          "oracle", b[1][water][:, 2 * nR:][:2], flush=True)
