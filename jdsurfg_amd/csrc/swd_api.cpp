@@ -116,13 +116,15 @@ struct jdsg_plan {
 
 extern "C" const char *jdsg_last_error(void) { return g_err.c_str(); }
 
-extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_off,
-                                const int *nmode, int group, int layout, int iflsph,
-                                jdsg_plan **out) {
-  *out = nullptr;
+namespace {
+
+// Host part of a plan: entry and kernel offsets, sizes, launch widths and
+// (for uneven batches) the per-wave item lists.  No device work.
+int plan_host(jdsg_plan *P, int ncol, const int *layer_off, const int *per_off,
+              const int *nmode, int group, int layout, int iflsph,
+              std::vector<long long> items[2]) {
   if (ncol <= 0) return fail("ncol must be positive");
   if (layer_off[0] != 0 || per_off[0] != 0) return fail("offsets must start at 0");
-  jdsg_plan *P = new jdsg_plan();
   P->ncol = ncol;
   P->ncol_pad = (ncol + 63) / 64 * 64;
   P->group = group ? 1 : 0;
@@ -138,9 +140,9 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
     int N = layer_off[c + 1] - layer_off[c];
     int K = per_off[c + 1] - per_off[c];
     int nR = nmode[2 * c], nL = nmode[2 * c + 1];
-    if (N < 2) { delete P; return fail("every column needs at least one layer and a half-space"); }
-    if (K < 0 || nR < 0 || nL < 0) { delete P; return fail("negative period or mode count"); }
-    if (nR > JDSG_MAXMODE || nL > JDSG_MAXMODE) { delete P; return fail("at most 8 modes per wave"); }
+    if (N < 2) return fail("every column needs at least one layer and a half-space");
+    if (K < 0 || nR < 0 || nL < 0) return fail("negative period or mode count");
+    if (nR > JDSG_MAXMODE || nL > JDSG_MAXMODE) return fail("at most 8 modes per wave");
     P->nmax = std::max(P->nmax, N);
     P->kmax = std::max(P->kmax, K);
     P->ent_off[c] = e;
@@ -160,7 +162,7 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
   P->uK = K0;
   P->unR = nmode[0];
   P->unL = nmode[1];
-  if (layout == 0 && !uni) { delete P; return fail("grid layout needs a uniform batch"); }
+  if (layout == 0 && !uni) return fail("grid layout needs a uniform batch");
   P->kpad = std::max(1, P->kmax);
   P->nchain = (long long)ncol * 2 * P->nvar;
   int dw = default_waves();
@@ -173,62 +175,68 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
       (int)std::min<long long>(4LL * dw, (std::max(P->nitem[0], P->nitem[1]) + 63) / 64);
   P->chain_waves = std::max(1, P->chain_waves);
   P->eigen_waves = std::max(1, P->eigen_waves);
-  // device descriptors
-  size_t cp = (size_t)P->ncol_pad * P->nmax;
-  if (P->dalloc(&P->layer_off_d, ncol + 1) || P->dalloc(&P->per_off_d, ncol + 1) ||
-      P->dalloc(&P->nmode_d, 2 * ncol) || P->dalloc(&P->ent_off_d, ncol + 1) ||
-      P->dalloc(&P->kern_off_d, ncol + 1) || P->dalloc(&P->prep.chainR, cp) ||
-      P->dalloc(&P->prep.chainL, cp) || P->dalloc(&P->prep.cc, ncol) ||
-      P->dalloc(&P->prep.betmx, ncol) || P->dalloc(&P->prep.llw, ncol) ||
-      P->dalloc(&P->prep.nlay, ncol) || P->dalloc(&P->prep.zd, cp) ||
-      P->dalloc(&P->prep.za, cp) || P->dalloc(&P->prep.zb, cp) ||
-      P->dalloc(&P->prep.rho0, cp) || P->dalloc(&P->prep.rtpR, cp) ||
-      P->dalloc(&P->prep.rtpL, cp) || P->dalloc(&P->prep.vtp, cp) ||
-      P->dalloc(&P->prep.dtp, cp) || P->dalloc(&P->cflat, (size_t)P->nvar * P->nent) ||
-      P->dalloc(&P->cstat, (size_t)P->nvar * P->nent) || P->dalloc(&P->ierr, P->nchain) ||
-      P->dalloc(&P->counters, 4) || P->dalloc(&P->nevals, 4) || P->size_scratch()) {
-    delete P;
-    return 1;
-  }
   P->nlay_total = layer_off[ncol];
-  std::vector<int> lo(layer_off, layer_off + ncol + 1), po(per_off, per_off + ncol + 1),
-      nm(nmode, nmode + 2 * ncol);
-  hipError_t err = hipSuccess;
-  err = hipMemcpy(P->layer_off_d, lo.data(), sizeof(int) * (ncol + 1), hipMemcpyHostToDevice);
-  if (err == hipSuccess)
-    err = hipMemcpy(P->per_off_d, po.data(), sizeof(int) * (ncol + 1), hipMemcpyHostToDevice);
-  if (err == hipSuccess)
-    err = hipMemcpy(P->nmode_d, nm.data(), sizeof(int) * 2 * ncol, hipMemcpyHostToDevice);
-  if (err == hipSuccess)
-    err = hipMemcpy(P->ent_off_d, P->ent_off.data(), sizeof(long long) * (ncol + 1),
-                    hipMemcpyHostToDevice);
-  if (err == hipSuccess)
-    err = hipMemcpy(P->kern_off_d, P->kern_off.data(), sizeof(long long) * (ncol + 1),
-                    hipMemcpyHostToDevice);
-  if (err != hipSuccess) {
-    delete P;
-    return fail(std::string("plan upload: ") + hipGetErrorString(err));
-  }
   if (!uni) {
     for (int w = 0; w < 2; ++w) {
-      std::vector<long long> items;
-      items.reserve(P->nitem[w]);
+      items[w].clear();
+      items[w].reserve(P->nitem[w]);
       for (int c = 0; c < ncol; ++c) {
         int K = per_off[c + 1] - per_off[c];
         int nR = nmode[2 * c], nL = nmode[2 * c + 1];
         int M = w == 0 ? nR : nL, off = w == 0 ? 0 : nR;
         for (int m = 0; m < M; ++m)
-          for (int k = 0; k < K; ++k) items.push_back(P->ent_off[c] + (long long)(off + m) * K + k);
-      }
-      if (P->dalloc(&P->items_d[w], items.size())) { delete P; return 1; }
-      if (!items.empty() &&
-          hipMemcpy(P->items_d[w], items.data(), sizeof(long long) * items.size(),
-                    hipMemcpyHostToDevice) != hipSuccess) {
-        delete P;
-        return fail("item upload failed");
+          for (int k = 0; k < K; ++k)
+            items[w].push_back(P->ent_off[c] + (long long)(off + m) * K + k);
       }
     }
   }
+  return 0;
+}
+
+// Device arrays of a plan placed by `take(pointer, count)` -- hipMalloc in
+// jdsg_plan_create, a carve of the static scratch in the allocation-free
+// form.  Returns nonzero when an array cannot be placed.
+template <class Take>
+int plan_place(jdsg_plan *P, bool uneven, Take take) {
+  size_t cp = (size_t)P->ncol_pad * P->nmax;
+  int ncol = P->ncol;
+  if (take(&P->layer_off_d, ncol + 1) || take(&P->per_off_d, ncol + 1) ||
+      take(&P->nmode_d, 2 * ncol) || take(&P->ent_off_d, ncol + 1) ||
+      take(&P->kern_off_d, ncol + 1) || take(&P->prep.chainR, cp) ||
+      take(&P->prep.chainL, cp) || take(&P->prep.cc, ncol) || take(&P->prep.betmx, ncol) ||
+      take(&P->prep.llw, ncol) || take(&P->prep.nlay, ncol) || take(&P->prep.zd, cp) ||
+      take(&P->prep.za, cp) || take(&P->prep.zb, cp) || take(&P->prep.rho0, cp) ||
+      take(&P->prep.rtpR, cp) || take(&P->prep.rtpL, cp) || take(&P->prep.vtp, cp) ||
+      take(&P->prep.dtp, cp) || take(&P->cflat, (size_t)P->nvar * P->nent) ||
+      take(&P->cstat, (size_t)P->nvar * P->nent) || take(&P->ierr, P->nchain) ||
+      take(&P->counters, 4) || take(&P->nevals, 4))
+    return 1;
+  if (uneven)
+    for (int w = 0; w < 2; ++w)
+      if (take(&P->items_d[w], P->nitem[w])) return 1;
+  return 0;
+}
+
+// Copies the descriptors to the device arrays and binds the column model.
+int plan_upload(jdsg_plan *P, const int *layer_off, const int *per_off, const int *nmode,
+                const std::vector<long long> items[2], hipStream_t st) {
+  int ncol = P->ncol;
+  HIPCHK(hipMemcpyAsync(P->layer_off_d, layer_off, sizeof(int) * (ncol + 1),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P->per_off_d, per_off, sizeof(int) * (ncol + 1), hipMemcpyHostToDevice,
+                        st));
+  HIPCHK(hipMemcpyAsync(P->nmode_d, nmode, sizeof(int) * 2 * ncol, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P->ent_off_d, P->ent_off.data(), sizeof(long long) * (ncol + 1),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(P->kern_off_d, P->kern_off.data(), sizeof(long long) * (ncol + 1),
+                        hipMemcpyHostToDevice, st));
+  if (!P->uniform)
+    for (int w = 0; w < 2; ++w)
+      if (!items[w].empty())
+        HIPCHK(hipMemcpyAsync(P->items_d[w], items[w].data(), sizeof(long long) * items[w].size(),
+                              hipMemcpyHostToDevice, st));
+  // the host arrays are the caller's and this function's: wait for the copies
+  HIPCHK(hipStreamSynchronize(st));
   P->cm.ncol = ncol;
   P->cm.ncol_pad = P->ncol_pad;
   P->cm.nmax = P->nmax;
@@ -246,12 +254,29 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
   P->cm.rtpL = P->prep.rtpL;
   P->cm.vtp = P->prep.vtp;
   P->cm.dtp = P->prep.dtp;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_off,
+                                const int *nmode, int group, int layout, int iflsph,
+                                jdsg_plan **out) {
+  *out = nullptr;
+  auto P = std::make_unique<jdsg_plan>();
+  std::vector<long long> items[2];
+  if (plan_host(P.get(), ncol, layer_off, per_off, nmode, group, layout, iflsph, items)) return 1;
+  jdsg_plan *Q = P.get();
+  if (plan_place(Q, !Q->uniform, [Q](auto **p, size_t n) { return Q->dalloc(p, n); }) ||
+      Q->size_scratch())
+    return 1;
+  if (plan_upload(Q, layer_off, per_off, nmode, items, nullptr)) return 1;
   const char *tenv = getenv("JDSG_TIMING");
   if (tenv && tenv[0] == '1') {
     P->timing = true;
     for (auto &ev : P->ev) (void)hipEventCreate(&ev);
   }
-  *out = P;
+  *out = P.release();
   return 0;
 }
 
@@ -467,33 +492,104 @@ void report(const char *where) {
   fprintf(stderr, "libjdsg_swd: %s failed: %s\n", where, g_err.c_str());
 }
 
+// Per-thread cache for the per-column entry points: the reference calls
+// them from its OpenMP column loop (SurfaceWave.cpp:469), so each host
+// thread keeps its own small plans (keyed by the column's sizes) and one
+// growing device arena, and a call allocates nothing once its sizes have
+// been seen.  The cache lives until the process ends.
+struct ColumnCache {
+  struct Entry {
+    long long key;
+    jdsg_plan *plan;
+    unsigned long long used;
+  };
+  std::vector<Entry> plans;
+  unsigned long long tick = 0;
+  void *arena = nullptr;
+  size_t cap = 0;
+  jdsg_plan *get(int N, int K, int nR, int nL, int group, int iflsph) {
+    long long key = ((((long long)N * 4096 + K) * 16 + nR) * 16 + nL) * 4 + group * 2 + iflsph;
+    ++tick;
+    for (auto &e : plans)
+      if (e.key == key) {
+        e.used = tick;
+        return e.plan;
+      }
+    int lo[2] = {0, N}, po[2] = {0, K}, nm[2] = {nR, nL};
+    jdsg_plan *P = nullptr;
+    if (jdsg_plan_create(1, lo, po, nm, group, 1, iflsph, &P)) return nullptr;
+    if (plans.size() >= 16) {  // drop the least recently used plan
+      auto it = std::min_element(plans.begin(), plans.end(),
+                                 [](const Entry &a, const Entry &b) { return a.used < b.used; });
+      jdsg_plan_destroy(it->plan);
+      plans.erase(it);
+    }
+    plans.push_back({key, P, tick});
+    return P;
+  }
+  // device arena of at least n bytes (contents not kept across growth)
+  char *space(size_t n) {
+    if (n > cap) {
+      if (arena) (void)hipFree(arena);
+      arena = nullptr;
+      cap = 0;
+      size_t want = std::max<size_t>(n, 1 << 16);
+      if (hipMalloc(&arena, want) != hipSuccess) return nullptr;
+      cap = want;
+    }
+    return (char *)arena;
+  }
+};
+
+ColumnCache &column_cache() {
+  thread_local ColumnCache *c = new ColumnCache();
+  return *c;
+}
+
+// Places arrays one after another (256-byte aligned) in an arena.
+struct Carve {
+  char *base;
+  size_t off = 0;
+  template <class T>
+  T *take(size_t n) {
+    T *p = (T *)(base + off);
+    off += (n * sizeof(T) + 255) / 256 * 256;
+    return p;
+  }
+};
+size_t carve_size(std::initializer_list<size_t> bytes) {
+  size_t s = 0;
+  for (size_t b : bytes) s += (b + 255) / 256 * 256;
+  return s;
+}
+
 // One column, chains only: returns flat roots for modes 1..mode (mode-major)
 int column_chains(const float *thk, const float *vp, const float *vs, const float *rho, int N,
                   int iflsph, int iwave, int mode, int igr, int retry, int kmax,
                   const double *t, std::vector<double> &cg_all, std::vector<int> &st_all,
                   int *ierr) {
-  int lo[2] = {0, N}, po[2] = {0, kmax};
-  int nm[2] = {iwave == 2 ? mode : 0, iwave == 1 ? mode : 0};
-  jdsg_plan *P = nullptr;
-  if (jdsg_plan_create(1, lo, po, nm, 0, 1, iflsph, &P)) return 1;
-  DevBuf B;
-  float *dthk = B.up(thk, N), *dvp = B.up(vp, N), *dvs = B.up(vs, N), *drho = B.up(rho, N);
-  double *dt = B.up(t, kmax);
-  int rc = run_front(P, dthk, dvp, dvs, drho, dt, igr, retry, nullptr);
-  if (!rc) {
-    cg_all.assign((size_t)mode * kmax, 0.0);
-    st_all.assign((size_t)mode * kmax, 0);
-    hipError_t e = hipMemcpy(cg_all.data(), P->cflat, sizeof(double) * mode * kmax,
-                             hipMemcpyDeviceToHost);
-    if (e == hipSuccess)
-      e = hipMemcpy(st_all.data(), P->cstat, sizeof(int) * mode * kmax, hipMemcpyDeviceToHost);
-    int ie = 0;
-    if (e == hipSuccess) e = hipMemcpy(&ie, P->ierr, sizeof(int), hipMemcpyDeviceToHost);
-    if (ierr) *ierr = ie;
-    if (e != hipSuccess) rc = fail(std::string("column copy: ") + hipGetErrorString(e));
-  }
-  jdsg_plan_destroy(P);
-  return rc;
+  ColumnCache &CC = column_cache();
+  jdsg_plan *P = CC.get(N, kmax, iwave == 2 ? mode : 0, iwave == 1 ? mode : 0, 0, iflsph);
+  if (!P) return 1;
+  char *a = CC.space(carve_size({4 * sizeof(float) * N, sizeof(double) * kmax}));
+  if (!a) return fail("device allocation");
+  Carve cv{a};
+  float *d[4] = {cv.take<float>(N), cv.take<float>(N), cv.take<float>(N), cv.take<float>(N)};
+  double *dt = cv.take<double>(kmax);
+  const float *h[4] = {thk, vp, vs, rho};
+  for (int i = 0; i < 4; ++i) HIPCHK(hipMemcpy(d[i], h[i], sizeof(float) * N, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dt, t, sizeof(double) * kmax, hipMemcpyHostToDevice));
+  if (run_front(P, d[0], d[1], d[2], d[3], dt, igr, retry, nullptr)) return 1;
+  cg_all.assign((size_t)mode * kmax, 0.0);
+  st_all.assign((size_t)mode * kmax, 0);
+  HIPCHK(hipMemcpy(cg_all.data(), P->cflat, sizeof(double) * mode * kmax, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(st_all.data(), P->cstat, sizeof(int) * mode * kmax, hipMemcpyDeviceToHost));
+  // the chain kernel of wave W writes chain g's ierr at W * ncol * nvar + g
+  int ie = 0;
+  HIPCHK(hipMemcpy(&ie, P->ierr + (iwave == 1 ? P->nchain / 2 : 0), sizeof(int),
+                   hipMemcpyDeviceToHost));
+  if (ierr) *ierr = ie;
+  return 0;
 }
 
 // One column, eigen stage from given flat roots.  nv = 1 (sregn96/slegn96)
@@ -501,49 +597,47 @@ int column_chains(const float *thk, const float *vp, const float *vs, const floa
 int column_eigen(int wave, const float *thk, const float *vp, const float *vs, const float *rho,
                  int N, int iflsph, const double *tv, const double *cv, int nv, double *cp,
                  double *cg, double *eig[4], double *dc[4], double *du[4]) {
-  int lo[2] = {0, N}, po[2] = {0, 1};
-  int nm[2] = {wave == 0 ? 1 : 0, wave == 1 ? 1 : 0};
-  jdsg_plan *P = nullptr;
-  if (jdsg_plan_create(1, lo, po, nm, nv == 3, 1, iflsph, &P)) return 1;
-  DevBuf B;
-  float *dthk = B.up(thk, N), *dvp = B.up(vp ? vp : vs, N), *dvs = B.up(vs, N),
-        *drho = B.up(rho, N);
-  double *dt = B.up(tv, 1);
-  double *dtv = B.up(tv, 3);
-  double *dout = B.up<double>(nullptr, 2 + 12 * (size_t)N);
-  int *dst = B.up<int>(nullptr, 1);
-  int rc = 0;
-  if (!dthk || !dvp || !dvs || !drho || !dt || !dout || !dst) rc = fail("device allocation");
-  if (!rc) {
-    for (int a = 0; a < 4; ++a) P->eig[a] = dout + 2 + (size_t)(8 + a) * N;
-    P->tv = (nv == 3) ? dtv : nullptr;
-    jdsg_launch_prep(1, P->ncol_pad, P->layer_off_d, dthk, dvp, dvs, drho, iflsph, P->prep,
-                     nullptr);
-    jdsg_launch_fill_u32(P->counters, 4, 0u, nullptr);
-    jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, nullptr);
-    for (int v = 0; v < nv; ++v)
-      (void)hipMemcpy(P->cflat + v, cv + v, sizeof(double), hipMemcpyHostToDevice);
-    double *ddc[4], *ddu[4];
-    for (int a = 0; a < 4; ++a) {
-      ddc[a] = dout + 2 + (size_t)a * N;
-      ddu[a] = dout + 2 + (size_t)(4 + a) * N;
-    }
-    rc = run_eigen(P, dt, dout, dout + 1, dst, ddc, ddu, nullptr);
-    if (!rc) {
-      std::vector<double> h(2 + 12 * (size_t)N);
-      hipError_t e = hipMemcpy(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost);
-      if (e != hipSuccess) rc = fail("copy back");
-      *cp = h[0];
-      *cg = h[1];
-      for (int a = 0; a < 4; ++a) {
-        if (dc[a]) memcpy(dc[a], &h[2 + a * N], sizeof(double) * N);
-        if (du[a]) memcpy(du[a], &h[2 + (4 + a) * N], sizeof(double) * N);
-        if (eig[a]) memcpy(eig[a], &h[2 + (8 + a) * N], sizeof(double) * N);
-      }
-    }
+  ColumnCache &CC = column_cache();
+  jdsg_plan *P = CC.get(N, 1, wave == 0 ? 1 : 0, wave == 1 ? 1 : 0, nv == 3, iflsph);
+  if (!P) return 1;
+  size_t nout = 2 + 12 * (size_t)N;
+  char *a = CC.space(carve_size({4 * sizeof(float) * N, 3 * sizeof(double),
+                                 sizeof(double) * nout, sizeof(int)}));
+  if (!a) return fail("device allocation");
+  Carve c{a};
+  float *d[4] = {c.take<float>(N), c.take<float>(N), c.take<float>(N), c.take<float>(N)};
+  double *dtv = c.take<double>(3);
+  double *dout = c.take<double>(nout);
+  int *dst = c.take<int>(1);
+  const float *h[4] = {thk, vp ? vp : vs, vs, rho};
+  for (int i = 0; i < 4; ++i) HIPCHK(hipMemcpy(d[i], h[i], sizeof(float) * N, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dtv, tv, sizeof(double) * 3, hipMemcpyHostToDevice));
+  for (int q = 0; q < 4; ++q) P->eig[q] = dout + 2 + (size_t)(8 + q) * N;
+  P->tv = (nv == 3) ? dtv : nullptr;
+  jdsg_launch_prep(1, P->ncol_pad, P->layer_off_d, d[0], d[1], d[2], d[3], iflsph, P->prep,
+                   nullptr);
+  jdsg_launch_fill_u32(P->counters, 4, 0u, nullptr);
+  jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, nullptr);
+  HIPCHK(hipMemcpy(P->cflat, cv, sizeof(double) * nv, hipMemcpyHostToDevice));
+  double *ddc[4], *ddu[4];
+  for (int q = 0; q < 4; ++q) {
+    ddc[q] = dout + 2 + (size_t)q * N;
+    ddu[q] = dout + 2 + (size_t)(4 + q) * N;
   }
-  jdsg_plan_destroy(P);
-  return rc;
+  int rc = run_eigen(P, dtv, dout, dout + 1, dst, ddc, ddu, nullptr);
+  P->tv = nullptr;
+  for (int q = 0; q < 4; ++q) P->eig[q] = nullptr;
+  if (rc) return rc;
+  std::vector<double> hb(nout);
+  HIPCHK(hipMemcpy(hb.data(), dout, sizeof(double) * nout, hipMemcpyDeviceToHost));
+  *cp = hb[0];
+  *cg = hb[1];
+  for (int q = 0; q < 4; ++q) {
+    if (dc[q]) memcpy(dc[q], &hb[2 + q * N], sizeof(double) * N);
+    if (du[q]) memcpy(du[q], &hb[2 + (4 + q) * N], sizeof(double) * N);
+    if (eig[q]) memcpy(eig[q], &hb[2 + (8 + q) * N], sizeof(double) * N);
+  }
+  return 0;
 }
 
 }  // namespace

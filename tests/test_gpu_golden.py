@@ -264,3 +264,44 @@ def test_jdsg_surfdisp_matches_oracle(wave, oracle_rest):
             assert np.array_equal(cg == 0, ~nz)
             tol = 1e-6 if wave in (0, 2) else 5e-4
             assert np.abs(cg[nz] / ref[nz] - 1).max() <= tol
+
+
+def test_per_column_calls_from_threads():
+    """The reference calls the per-column entry points from an OpenMP column
+    loop (SurfaceWave.cpp:469): calls from several host threads at once give
+    the single-threaded results (each thread keeps its own plans and device
+    arena)."""
+    import threading
+    L = _abi()
+    w = W.random_columns(8, 30, 21, nmode=(2, 1))
+    per = np.ascontiguousarray(w.periods[:12])
+    cols = []
+    for c in range(w.ncol):
+        s = slice(w.layer_off[c], w.layer_off[c + 1])
+        cols.append([np.ascontiguousarray(a[s], np.float32) for a in (w.thk, w.vp, w.vs, w.rho)])
+
+    def run(c, iwave, mode):
+        thk, vp, vs, rho = cols[c]
+        cg = np.zeros(len(per))
+        e = C.c_int(-1)
+        L.surfdisp96_(_p(thk), _p(vp), _p(vs), _p(rho), len(thk), 1, iwave, mode, 0, len(per),
+                      _p(per), _p(cg), C.byref(e))
+        return cg, e.value
+
+    jobs = [(c, iw, m) for c in range(w.ncol) for iw in (1, 2) for m in (1, 2)]
+    print("case per-column calls from 4 threads:", len(jobs), "calls", flush=True)
+    ref = {j: run(*j) for j in jobs}
+    got = {}
+
+    def worker(part):
+        for _ in range(3):
+            for j in part:
+                got[j] = run(*j)
+
+    ts = [threading.Thread(target=worker, args=(jobs[i::4],)) for i in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for j in jobs:
+        assert got[j][1] == ref[j][1] and np.array_equal(got[j][0], ref[j][0]), j
