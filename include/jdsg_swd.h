@@ -83,6 +83,25 @@ int jdsg_plan_execute(jdsg_plan *plan, const float *thk, const float *vp, const 
                       const float *rho, const double *periods, double *c, double *u,
                       int *status, double *dcda, double *dcdb, double *dcdr, double *dcdh,
                       double *duda, double *dudb, double *dudr, double *dudh, void *stream);
+/* Allocation-free form of jdsg_plan_create + jdsg_plan_execute + destroy (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as the planned solve, the
+ * descriptors (layer_off, per_off, nmode) as host arrays, everything else
+ * device memory.  Every device array of the plan lives in one static
+ * device array of jdsg_static_scratch_bytes() (64 MiB) per GPU, held until
+ * the process ends; a batch whose plan does not fit is refused with an
+ * error (nonzero return, jdsg_last_error()).  Calls allocate nothing; they
+ * synchronise `stream` once (descriptor upload).  Calls that overlap on one
+ * GPU share that array and race: serialise them. */
+int jdsg_plan_execute_static(int ncol, const int *layer_off, const int *per_off,
+                             const int *nmode, int group, int layout, int iflsph,
+                             const float *thk, const float *vp, const float *vs,
+                             const float *rho, const double *periods, double *c, double *u,
+                             int *status, double *dcda, double *dcdb, double *dcdr,
+                             double *dcdh, double *duda, double *dudb, double *dudr,
+                             double *dudh, void *stream);
+long long jdsg_static_scratch_bytes(void);
 /* Per-kernel launch tuning: waves for the chain and eigen launches (0 keeps
  * the default). */
 int jdsg_plan_set_waves(jdsg_plan *plan, int chain_waves, int eigen_waves);

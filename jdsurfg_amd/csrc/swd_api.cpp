@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -467,6 +468,93 @@ extern "C" int jdsg_plan_execute(jdsg_plan *P, const float *thk, const float *vp
     for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
   }
   return 0;
+}
+
+// ============================================================================
+// Allocation-free batch solve: jdsg_plan_create + jdsg_plan_execute with
+// every device array of the plan carved from one file-scope device array.
+namespace {
+constexpr size_t kStaticScratchBytes = 64ull << 20;
+}  // namespace
+
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+int static_scratch(void **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("static scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(std::string("static scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" long long jdsg_static_scratch_bytes(void) { return (long long)kStaticScratchBytes; }
+
+extern "C" int jdsg_plan_execute_static(int ncol, const int *layer_off, const int *per_off,
+                                        const int *nmode, int group, int layout, int iflsph,
+                                        const float *thk, const float *vp, const float *vs,
+                                        const float *rho, const double *periods, double *c,
+                                        double *u, int *status, double *dcda, double *dcdb,
+                                        double *dcdr, double *dcdh, double *duda, double *dudb,
+                                        double *dudr, double *dudh, void *stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!thk || !vp || !vs || !rho || !periods || !c || !u || !status)
+    return fail("null input or scalar output pointer");
+  jdsg_plan P;  // host descriptor; its device arrays live in the static scratch
+  std::vector<long long> items[2];
+  if (plan_host(&P, ncol, layer_off, per_off, nmode, group, layout, iflsph, items)) return 1;
+  void *base = nullptr;
+  if (static_scratch(&base)) return 1;
+  size_t off = 0;
+  auto take = [&](auto **p, size_t n) -> int {
+    using T = std::remove_reference_t<decltype(**p)>;
+    size_t b = (std::max<size_t>(n, 1) * sizeof(T) + 255) / 256 * 256;
+    if (off + b > kStaticScratchBytes) return 1;
+    *p = (T *)((char *)base + off);
+    off += b;
+    return 0;
+  };
+  auto too_big = [&]() {
+    return fail("batch too large for the allocation-free solve: its plan needs more than the " +
+                std::to_string(kStaticScratchBytes >> 20) +
+                " MiB static scratch (use jdsg_plan_create / jdsg_plan_execute)");
+  };
+  if (plan_place(&P, !P.uniform, take)) return too_big();
+  // lane scratch of the chain and eigen launches from what is left, at most
+  // the plan's default widths (results do not depend on the widths: chains
+  // and eigen entries are taken one lane each from work counters)
+  size_t cwb = (size_t)64 * (2 * P.kpad + 22) * sizeof(double) + 256;
+  size_t ewb = (size_t)64 * (18 * (size_t)P.nmax + 9) * sizeof(double) + 256;
+  size_t left = kStaticScratchBytes - off;
+  long long cw = std::min<long long>(P.chain_waves, (long long)(left / 2 / cwb));
+  long long ew = cw > 0 ? std::min<long long>(P.eigen_waves, (long long)((left - cw * cwb) / ewb))
+                        : 0;
+  if (cw < 1 || ew < 1) return too_big();
+  P.chain_waves = (int)cw;
+  P.eigen_waves = (int)ew;
+  if (take(&P.cscr, (size_t)cw * 64 * (2 * P.kpad + 22)) ||
+      take(&P.escr, (size_t)ew * 64 * (18 * (size_t)P.nmax + 9)))
+    return too_big();
+  if (plan_upload(&P, layer_off, per_off, nmode, items, st)) return 1;
+  if (run_front(&P, thk, vp, vs, rho, periods, 0, 1, st)) return 1;
+  double *dc[4] = {dcda, dcdb, dcdr, dcdh};
+  double *du[4] = {duda, dudb, dudr, dudh};
+  return run_eigen(&P, periods, c, u, status, dc, du, st);
 }
 
 // ============================================================================
