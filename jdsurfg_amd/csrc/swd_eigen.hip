@@ -17,7 +17,7 @@
 //   base(entry) + layer * lstride.
 #include "swd_kernels.hpp"
 
-#pragma clang fp contract(off)
+#pragma clang fp contract(fast)
 
 namespace {
 
@@ -768,9 +768,47 @@ struct Dst {
   long long base, stride;
 };
 
+// The Dunkin up-sweep is kept as checkpoints: its vector (5 entries and the
+// exponent sum) at layers 0, S, 2S, ... and at the half-space, in the lane's
+// global scratch; the down-sweep recomputes each segment of S layers from
+// the checkpoint below it into LDS (S x 6 doubles per lane) and reads it
+// there.  The recomputation repeats the first sweep's operations from the
+// same stored values, so every vector is the one the full sweep gives; the
+// global traffic per lane drops from 6 doubles per layer to 6 per S layers.
+constexpr int kUpSeg = 6;
+JDSG_DEV int up_slot(int m, int N) {
+  return (m == N - 1) ? (N - 2) / kUpSeg + 1 : m / kUpSeg;
+}
+
+// One step of the up-sweep (sregn96.f90:455-487): cd, the vector at the top
+// of layer m + 1, becomes the vector at the top of layer m.
+JDSG_DEV void dunkin_step(const EigenArgs &A, int col, int m, double omega, double wvno2,
+                          const Wk &K, double (&cd)[5], double &exsum) {
+  Layer L = load_layer(A, col, m, true);
+  const LayerWaves lw = layer_waves(L, omega, wvno2);
+  double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
+  varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
+  double ca[5][5];
+  dnka_r(ca, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat, lw,
+         K);
+  double ee[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    double cr = 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) cr = cr + cd[j] * ca[j][i];
+    ee[i] = cr;
+  }
+  double exn = normalize_pow2(ee);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) cd[i] = ee[i];
+  exsum = exsum + pex + svex + exn;
+}
+
 JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, double wvno,
                             double *scr, long long sstride, const Dst &D, double &ugr,
-                            double &sumi0, double &fac, double *eig, long long ebase) {
+                            double &sumi0, double &fac, double *eig, long long ebase,
+                            double *seg) {
   double om2 = omega * omega, wvno2 = wvno * wvno;
   const Wk K = make_wk(omega, wvno);
   bool allfluid = true;
@@ -784,7 +822,11 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     }
   }
   if (allfluid) jwat = 0;
-  // ---- up: Dunkin vectors from the half-space (sregn96.f90:401-489)
+  auto ck = [&](int m, int q) -> double & {
+    return scr[((long long)up_slot(m, N) * 6 + q) * sstride];
+  };
+  // ---- up: Dunkin vectors from the half-space (sregn96.f90:401-489),
+  // checkpoints only
   {
     Layer H = load_layer(A, col, N - 1, true);
     EMat M;
@@ -794,39 +836,23 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
       cd[j] = g[j];
-      scr[((long long)(N - 1) * 6 + j) * sstride] = cd[j];
+      ck(N - 1, j) = cd[j];
     }
-    scr[((long long)(N - 1) * 6 + 5) * sstride] = 0.0;
+    ck(N - 1, 5) = 0.0;
     double exsum = 0.0;
     for (int m = N - 2; m >= 0; --m) {
-      Layer L = load_layer(A, col, m, true);
-      const LayerWaves lw = layer_waves(L, omega, wvno2);
-      double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
-      varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
-      double ca[5][5];
-      dnka_r(ca, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat,
-             lw, K);
-      double ee[5];
+      dunkin_step(A, col, m, omega, wvno2, K, cd, exsum);
+      if (m % kUpSeg == 0) {
 #pragma unroll
-      for (int i = 0; i < 5; ++i) {
-        double cr = 0.0;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) cr = cr + cd[j] * ca[j][i];
-        ee[i] = cr;
+        for (int j = 0; j < 5; ++j) ck(m, j) = cd[j];
+        ck(m, 5) = exsum;
       }
-      double exn = normalize_pow2(ee);
-#pragma unroll
-      for (int i = 0; i < 5; ++i) cd[i] = ee[i];
-      exsum = exsum + pex + svex + exn;
-#pragma unroll
-      for (int j = 0; j < 5; ++j) scr[((long long)m * 6 + j) * sstride] = cd[j];
-      scr[((long long)m * 6 + 5) * sstride] = exsum;
     }
   }
   // ---- down + svfunc + energy + getdcdh, top to bottom
-  double cd0_1 = scr[(0 * 6 + 1) * sstride];
-  double cd0_2 = scr[(0 * 6 + 2) * sstride];
-  double exe0 = scr[(0 * 6 + 5) * sstride];
+  double cd0_1 = ck(0, 1);
+  double cd0_2 = ck(0, 2);
+  double exe0 = ck(0, 5);
   double f1213 = -cd0_1;
   const double r1213 = 1.0 / f1213;
   U4 uprev;
@@ -875,10 +901,30 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     exsum = exsum + pex + ex2;
     // eigenfunction at the top of layer m+1 (svfunc)
     int i1 = m + 1;
-    long long so = (long long)i1 * 6;
-    double cd1 = scr[(so + 0) * sstride], cd2 = scr[(so + 1) * sstride],
-           cd3 = scr[(so + 2) * sstride], cd4 = -cd3, cd5 = scr[(so + 3) * sstride],
-           cd6 = scr[(so + 4) * sstride], exe_i = scr[(so + 5) * sstride];
+    const int top = ((i1 - 1) / kUpSeg) * kUpSeg + 1;
+    if (i1 == top) {
+      // recompute the up-sweep vectors of layers top..bot from the
+      // checkpoint at bot (a multiple of kUpSeg, or the half-space)
+      const int bot = min(top + kUpSeg - 1, N - 1);
+      double cu[5], eu;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) cu[j] = ck(bot, j);
+      eu = ck(bot, 5);
+      double *sb = seg + (long long)(bot - top) * 6 * 64;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) sb[j * 64] = cu[j];
+      sb[5 * 64] = eu;
+      for (int mm = bot - 1; mm >= top; --mm) {
+        dunkin_step(A, col, mm, omega, wvno2, K, cu, eu);
+        double *sm = seg + (long long)(mm - top) * 6 * 64;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) sm[j * 64] = cu[j];
+        sm[5 * 64] = eu;
+      }
+    }
+    const double *sv = seg + (long long)(i1 - top) * 6 * 64;
+    double cd1 = sv[0], cd2 = sv[64], cd3 = sv[2 * 64], cd4 = -cd3, cd5 = sv[3 * 64],
+           cd6 = sv[4 * 64], exe_i = sv[5 * 64];
     double tz1 = -vv[3], tz2 = -vv[2], tz3 = vv[1], tz4 = vv[0];
     double uu1 = tz2 * cd6 - tz3 * cd5 + tz4 * cd4;
     double uu2 = -tz1 * cd6 + tz3 * cd3 - tz4 * cd2;
@@ -1289,12 +1335,14 @@ jdsg_eigen_pass(EigenArgs A, long long it0, long long lanes, int v) {
   D.stride = ss;
   long long ebase = (A.eig[0] && A.kern_off) ? A.kern_off[r.col] + r.el * r.N : 0;
   double ugr, s0, fac;
-  if (W == 0)
+  if constexpr (W == 0) {
+    __shared__ double seg[kUpSeg * 6 * 64];
     rayleigh_pass(A, r.col, r.N, omega, wvno, scr, ss, D, ugr, s0, fac,
-                  (v == 0 && A.eig[0]) ? A.eig[0] : nullptr, ebase);
-  else
+                  (v == 0 && A.eig[0]) ? A.eig[0] : nullptr, ebase, seg + threadIdx.x);
+  } else {
     love_pass(A, r.col, r.N, omega, wvno, A.group != 0, scr, ss, D, ugr, s0, fac,
               v == 0 && A.eig[0] != nullptr, ebase);
+  }
   double *sc = scr + ((long long)18 * A.nmax + 3 * v) * ss;
   sc[0] = ugr;
   sc[ss] = s0;
