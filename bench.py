@@ -40,44 +40,75 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="grid256",
-                    help="grid256 (the default line), lsmr (LSMR on Frechet rows) or smooth "
+                    help="grid256 (the default line), grid128, uneven (the 50,000-column "
+                         "batch, replayed graph), lsmr (LSMR on Frechet rows) or smooth "
                          "(gradient smoothing)")
     ap.add_argument("--cpu-sample", type=int, default=384,
                     help="columns in the CPU baseline sample (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surftime-steps", type=int, default=2,
+                    help="timed calls of the drop-in SurfTime leg (0 skips it)")
     return ap.parse_args()
 
 
-def flop_model(w):
-    """Algorithmic flops of the two eigen launches; the frozen model."""
-    import numpy as np
+def flop_model():
+    """The frozen algorithmic flop model (op-counted on the oracle
+    restatement, oracle/flopcount/freeze.py)."""
     with open(os.path.join(ROOT, "jdsurfg_amd", "flops_model.json")) as f:
-        fm = json.load(f)
-    N = np.diff(w.layer_off).astype(np.float64)
-    K = np.diff(w.per_off).astype(np.float64)
-    neig = 3 if w.group else 1
-    eigR = (w.nmode[:, 0] * K * N).sum() * neig * fm["f_eig"]["rayleigh"]
-    eigL = (w.nmode[:, 1] * K * N).sum() * neig * fm["f_eig"]["love"]
-    return fm, eigR, eigL
+        return json.load(f)
+
+
+def eigen_pass_flops(w, status, fm):
+    """Algorithmic flops of the Rayleigh eigen pass launches of one solve:
+    every Rayleigh entry with a T root takes the T pass (status 0, 2, 8) and,
+    with group kernels, those with T(1+-0.005) roots (status 0) two more;
+    N layer entries x f_eig per pass."""
+    import numpy as np
+    N = np.diff(w.layer_off).astype(np.int64)
+    K = np.diff(w.per_off).astype(np.int64)
+    nR = w.nmode[:, 0].astype(np.int64)
+    ne = (w.nmode.sum(axis=1) * K).astype(np.int64)
+    col = np.repeat(np.arange(w.ncol), ne)
+    j = np.arange(len(status)) - np.repeat(np.concatenate([[0], np.cumsum(ne)[:-1]]), ne)
+    ray = j < (nR * K)[col]
+    t_pass = ray & np.isin(status, (0, 2, 8))
+    pm_pass = ray & (status == 0) if w.group else np.zeros_like(ray)
+    lay = N[col].astype(np.float64)
+    passes = float((lay * t_pass).sum() + 2.0 * (lay * pm_pass).sum())
+    return passes * fm["f_eig"]["rayleigh"], int(t_pass.sum()), int(pm_pass.sum())
+
+
+def kernel_counters(name):
+    """Per-dispatch HBM bytes and hardware FP64 flops of a kernel from the
+    committed rocprofv3 counter runs (jdsurfg_amd/traffic_model.json)."""
+    p = os.path.join(ROOT, "jdsurfg_amd", "traffic_model.json")
+    if not os.path.exists(p):
+        return None, None
+    with open(p) as f:
+        d = json.load(f)
+    return d["kernels"].get(name), d.get("source")
 
 
 def cpu_baseline(w, ncols, nthreads):
     """Time the reference numerics (oracle/_ref, compiled from the reference
-    sources) or, if that build is absent, the plain-C restatement, on a
-    strided sample of columns; returns column-periods/s and the kind."""
+    sources) and the project's restatement (oracle/liboracle.so) on the same
+    strided sample of columns, OpenMP over columns on nthreads host threads;
+    returns both rates in column-periods/s."""
     import numpy as np
     from oracle.pyoracle import Oracle, available
-    kind = "reference" if available("ref") else "port"
-    o = Oracle("ref" if kind == "reference" else "rest")
     step = max(1, w.ncol // ncols)
     cols = np.arange(0, w.ncol, step)[:ncols]
     sub = w.subset(cols)
-    t0 = time.perf_counter()
-    o.batch_solve(sub.layer_off, sub.thk, sub.vp, sub.vs, sub.rho, sub.per_off, sub.periods,
-                  sub.nmode.ravel(), sub.group, nthreads=nthreads, kernels=False)
-    dt = time.perf_counter() - t0
-    return sub.column_periods() / dt, kind, len(cols), dt
+    out = {}
+    for kind in (("ref", "rest") if available("ref") else ("rest",)):
+        o = Oracle(kind)
+        t0 = time.perf_counter()
+        o.batch_solve(sub.layer_off, sub.thk, sub.vp, sub.vs, sub.rho, sub.per_off, sub.periods,
+                      sub.nmode.ravel(), sub.group, nthreads=nthreads, kernels=False)
+        out[kind] = (sub.column_periods() / (time.perf_counter() - t0),
+                     time.perf_counter() - t0)
+    return out, len(cols), sub.column_periods()
 
 
 def lsmr_main(args, nx=256, ny=256, nz=17, kmax=120, nrows=20000, it_gpu=40, it_cpu=3):
@@ -244,6 +275,66 @@ def timed(fn, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps, per
 
 
+def uneven_main(args):
+    """The batched uneven workload (50,000 columns, 16-128 layers of
+    0.5-4 km, 8-64 periods in 1-80 s, 1-3 modes per wave type, group kernels):
+    one plan, autotuned launch widths, one inversion iteration (1 % Vs update
+    with Brocher Vp/rho, then the whole solve) captured in a hipGraph and
+    replayed `steps` times.  Prints one JSON line: column-periods/s of a
+    replay (mean over the replays)."""
+    from jdsurfg_amd import batch as B
+    from jdsurfg_amd import workloads as W
+    w = W.uneven()
+    rep = B.run_iterations(w, iters=max(2, args.steps * 2), amp=0.01,
+                           log=lambda m: print(m, file=sys.stderr, flush=True))
+    line = {"metric": "column·periods/s per replayed inversion iteration (uneven batch)",
+            "value": rep["column_periods_per_s"], "unit": "column·periods/s", "n_gpus": 1,
+            "steps": len(rep["graph_ms"]), "warmup": 1, "ms_per_step": rep["graph_ms_mean"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"uneven: {w.ncol} columns, 16-128 layers, 8-64 periods, "
+                                   "1-3 modes per wave, group kernels, hipGraph replay",
+                       "columns": w.ncol, "column_periods": rep["column_periods"]},
+            "detail": {k: rep[k] for k in ("eager_ms", "autotune_ms", "chain_waves",
+                                            "eigen_waves", "graph_ms", "status_counts",
+                                            "max_vs_drift") if k in rep}}
+    print(json.dumps(line), flush=True)
+
+
+def surftime_leg(steps):
+    """The drop-in SurfTime form (get_1d_kernel, SurfaceWave.cpp:458-502) on a
+    C5-equivalent grid: 256 x 256 columns of 17 nodes 0-64 km (4 km apart,
+    refined to 65 layer entries), the C5 Vs (trend with the +-8 % checkerboard
+    at the node depths), Rg tables of 4 modes and Lg tables of 2 modes at the
+    60 periods 1-80 s; Vp and rho by Brocher inside, the a/b/rho kernels,
+    the chain rule and the grid epilogue; Vs and outputs resident in HBM."""
+    import numpy as np
+    import torch
+    from jdsurfg_amd import swd
+    nx = ny = 256
+    dep = (4.0 * np.arange(17)).astype(np.float32)
+    x = (np.arange(nx) + 0.5) * 10.0
+    sgn = (-1.0) ** ((np.floor(x / 40.0)[:, None, None] + np.floor(x / 40.0)[None, :, None] +
+                      np.floor(dep / 40.0)[None, None, :]) % 2)
+    vs = ((2.6 + 0.035 * dep)[None, None, :] * (1.0 + 0.08 * sgn)).astype(np.float32)
+    per = np.geomspace(1.0, 80.0, 60)
+    tabs = {"Rg": np.tile(per[:, None], (1, 4)), "Lg": np.tile(per[:, None], (1, 2))}
+    S = swd.SurfTime(nx, ny, 17, dep, tabs)
+    dvs = torch.from_numpy(np.ascontiguousarray(vs.ravel(order="F"))).cuda()
+    out = S.alloc(kernel=True)
+
+    def run():
+        S.kernel(dvs, out)
+    ms, _ = timed(run, steps, 1)
+    cps = nx * ny * len(per)
+    st = out["status"].cpu().numpy()
+    return {"value": float(cps / (ms * 1e-3)), "unit": "column·periods/s",
+            "ms_per_step": float(ms), "steps": steps,
+            "config": "SurfTime get_1d_kernel, 256x256 columns x 17 nodes (65 layer entries), "
+                      "Rg modes 0-3 + Lg modes 0-1, 60 periods 1-80 s, reference tolerance",
+            "status_counts": {str(i): int(c) for i, c in enumerate(np.bincount(st)) if c}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -255,6 +346,8 @@ def main():
         return lsmr_main(args)
     if args.workload == "smooth":
         return smooth_main(args)
+    if args.workload == "uneven":
+        return uneven_main(args)
     import numpy as np
     import torch
     from jdsurfg_amd import swd
@@ -267,7 +360,7 @@ def main():
     inp = swd.upload(w)
     cps = w.column_periods()
     N = float(np.mean(np.diff(w.layer_off)))
-    fm, eigR, eigL = flop_model(w)
+    fm = flop_model()
     peak = fm["peak_fp64_tflops"]
 
     # ---- plain leg (the reference method)
@@ -276,9 +369,9 @@ def main():
 
     def run_plain():
         plain.execute(*inp, pout)   # synchronises on its last event (timing on)
-        return plain.timings()
+        return list(plain.timings()) + [float(plain.pass_timings()[0][0])]
     ms_plain, st_plain = timed(run_plain, args.steps, args.warmup)
-    sm_plain = np.mean(np.array(st_plain), axis=0)  # prep, chains, eigR, eigL
+    sm_plain = np.mean(np.array(st_plain), axis=0)  # prep, chains, eigR, eigL, R passes
     nevals = plain.eval_count()
     status = pout["status"].cpu().numpy()
     line = {
@@ -300,13 +393,16 @@ def main():
                   "status_counts": {str(i): int(c) for i, c in enumerate(np.bincount(status))
                                     if c}},
     }
-    # roofline of the plain leg's chain kernel (the plain method's dominant launch)
+    # roofline of the plain leg's chain launches (the plain method's dominant
+    # kernel): E evaluations x N x f_disp, Rayleigh and Love apart would need
+    # per-wave counts, so the Rayleigh figure bounds it from above
     chain_flops = nevals * N * fm["f_disp"]["rayleigh"]
     line["plain"]["chain_roofline"] = {
         "achieved": float(chain_flops / (sm_plain[1] * 1e-3) / 1e12), "peak": peak,
-        "unit": "TFLOP/s"}
+        "unit": "TFLOP/s", "note": "upper bound: every evaluation counted at the Rayleigh f_disp"}
 
-    dom_ms, dom_name, dom_flops = float(sm_plain[2]), "jdsg_eigen_pass<0> (Rayleigh eigen stage)", eigR
+    dom_name = "jdsg_eigen_pass<0>"
+    dom_status, dom_ms, dom_n = status, float(sm_plain[4]), int(plain.pass_timings()[1][0])
     if not args.no_surrogate:
         from jdsurfg_amd import surrogate as S
         mlp = S.MLP.load(S.DEFAULT_WEIGHTS)
@@ -323,9 +419,10 @@ def main():
             ev[1].record()
             S.surrogate_solve(sp, inp, sout, pred=pred)
             t = sp.timings()
-            return [ev[0].elapsed_time(ev[1])] + [float(x) for x in t]
+            return [ev[0].elapsed_time(ev[1])] + [float(x) for x in t] + \
+                [float(sp.pass_timings()[0][0])]
         ms_sur, st_sur = timed(run_sur, args.steps, args.warmup)
-        sm = np.mean(np.array(st_sur), axis=0)  # predict, prep+polish, chains, eigR, eigL
+        sm = np.mean(np.array(st_sur), axis=0)  # predict, prep+polish, chains, eigR, eigL, passes
         stats = S.surrogate_stats(sp)
         nvar = 3 if w.group else 1
         # accuracy against the plain method in exact-root mode, same inputs
@@ -369,31 +466,49 @@ def main():
                           "mlp_mfma_frac_bf16": float(mlp_flops / (sm[0] * 1e-3) / 2.5e15),
                           "weights": os.path.relpath(S.DEFAULT_WEIGHTS, ROOT)},
         })
-        dom_ms = float(sm[3])
+        dom_status = sout["status"].cpu().numpy()
+        dom_ms, dom_n = float(sm[5]), int(sp.pass_timings()[1][0])
     else:
         line.update({"value": line["plain"]["value"], "ms_per_step": float(ms_plain),
                      "metric": "column-periods/s (c, U, dc/dVs, dU/dVs), plain FP64 method"})
-    achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    # HBM bytes per launch of the dominant kernel, from the committed
-    # rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a device copy)
-    traffic = None
-    tm_path = os.path.join(ROOT, "jdsurfg_amd", "traffic_model.json")
-    if os.path.exists(tm_path):
-        with open(tm_path) as f:
-            k = json.load(f)["kernels"].get("jdsg_eigen_pass<0>")
-        if k:
-            traffic = k["hbm_read_bytes_per_dispatch"] + k["hbm_write_bytes_per_dispatch"]
-    line["roofline"] = {"bound": "fp64-valu", "kernel": dom_name, "achieved": achieved,
-                        "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                        "traffic": traffic}
+    # roofline of the dominant kernel: the Rayleigh eigen pass launches'
+    # algorithmic flops (valid passes only) over their own event-timed time
+    flops, n_t, n_pm = eigen_pass_flops(w, dom_status, fm)
+    achieved = flops / (dom_ms * 1e-3) / 1e12
+    kc, ksrc = kernel_counters(dom_name)
+    line["roofline"] = {
+        "bound": "fp64-valu", "kernel": dom_name, "achieved": achieved, "peak": peak,
+        "unit": "TFLOP/s", "frac": achieved / peak,
+        "traffic": (kc["hbm_read_bytes_per_dispatch"] + kc["hbm_write_bytes_per_dispatch"])
+        if kc else None,
+        "launches": dom_n, "avg_launch_ms": dom_ms / max(dom_n, 1),
+        "algorithmic_flops_per_launch": flops / max(dom_n, 1),
+        "valid_passes": {"T": n_t, "T_pm": 2 * n_pm},
+        "f_eig_per_layer_pass": fm["f_eig"]["rayleigh"],
+        "source": "flops: jdsurfg_amd/flops_model.json (op-counted restatement); time: HIP "
+                  "events around each pass launch on its stream; traffic: " + (ksrc or "none")}
+    if kc and kc.get("hw_fp64_flops_per_dispatch"):
+        hw = kc["hw_fp64_flops_per_dispatch"] * dom_n / (dom_ms * 1e-3) / 1e12
+        line["roofline"]["hw_fp64_tflops"] = hw
+        line["roofline"]["hw_frac"] = hw / peak
+    if args.surftime_steps > 0:
+        line["surftime"] = surftime_leg(args.surftime_steps)
     if not args.no_cpu and args.cpu_sample > 0:
-        nthr = min(16, os.cpu_count() or 1)
-        v, kind, ncols, dt = cpu_baseline(w, args.cpu_sample, nthr)
-        line["cpu_baseline"] = {"value": float(v), "unit": "column·periods/s", "cores": nthr,
-                                "kind": kind,
-                                "sample": f"{ncols} strided columns of the same workload "
-                                          f"({ncols * int(np.diff(w.per_off)[0])} "
-                                          f"column-periods, {dt:.1f} s, OpenMP over columns)"}
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        res, ncols, ncp = cpu_baseline(w, args.cpu_sample, nthr)
+        kind = "reference" if "ref" in res else "port"
+        v, dt = res["ref" if kind == "reference" else "rest"]
+        line["cpu_baseline"] = {
+            "value": float(v), "unit": "column·periods/s", "cores": nthr, "kind": kind,
+            "sample": f"{ncols} strided columns of the same workload ({ncp} column-periods, "
+                      f"{dt:.1f} s, OpenMP over columns on all {nthr} host threads this process "
+                      "may use)"}
+        if kind == "reference":
+            line["cpu_baseline"]["restatement_value"] = float(res["rest"][0])
+            line["cpu_baseline"]["restatement_over_reference"] = float(res["rest"][0] / v)
+            line["cpu_baseline"]["note"] = (
+                "reference timed only: its fixed-form surfdisp96.f leaves the threadprivate "
+                "lines of del1st/dhalf as comments, so OpenMP columns race on them")
     print(json.dumps(line), flush=True)
 
 

@@ -125,6 +125,11 @@ int jdsg_plan_chain_stats(jdsg_plan *plan, unsigned *evals, int *ierr);
 /* Per-stage milliseconds (prep, chains, eigen Rayleigh, eigen Love) of the
  * last execute, when JDSG_TIMING=1 was set at plan creation. */
 int jdsg_plan_timings(const jdsg_plan *plan, float *ms4);
+/* The eigen pass kernels' own time (ms, summed over their launches) and
+ * launch count per wave type (Rayleigh, Love) in the last execute, when
+ * JDSG_TIMING=1 was set at plan creation (events around each pass launch,
+ * on the stream of the execute). */
+int jdsg_plan_pass_timings(const jdsg_plan *plan, float *ms2, int *n2);
 
 /* Iterated batches (no counterpart in the reference, whose inversion loop
  * re-runs get_1d_kernel each iteration, surftomo.cpp:27-138).

@@ -31,6 +31,12 @@ int fail(const std::string &m) {
     if (e_ != hipSuccess) return fail(std::string(#x ": ") + hipGetErrorString(e_)); \
   } while (0)
 
+#define HIPCHK_P(x)                                                            \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) return fail(std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
 int default_waves() {
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
@@ -82,6 +88,31 @@ struct jdsg_plan {
   bool capturing = false;
   hipEvent_t ev[5] = {};
   float last_ms[4] = {0, 0, 0, 0};
+  // the eigen pass kernels' own time per wave type (timing on)
+  std::vector<hipEvent_t> pev[2];
+  int pass_n[2] = {0, 0};
+  float pass_ms[2] = {0, 0};
+  int pass_events(int w, int n) {
+    while ((int)pev[w].size() < 2 * n) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return fail("hipEventCreate");
+      pev[w].push_back(e);
+    }
+    return 0;
+  }
+  int finish_timing() {
+    HIPCHK_P(hipEventSynchronize(ev[4]));
+    for (int i = 0; i < 4; ++i) HIPCHK_P(hipEventElapsedTime(&last_ms[i], ev[i], ev[i + 1]));
+    for (int w = 0; w < 2; ++w) {
+      pass_ms[w] = 0.0f;
+      for (int i = 0; i < pass_n[w]; ++i) {
+        float t = 0.0f;
+        HIPCHK_P(hipEventElapsedTime(&t, pev[w][2 * i], pev[w][2 * i + 1]));
+        pass_ms[w] += t;
+      }
+    }
+    return 0;
+  }
 
   template <class T>
   int dalloc(T **p, size_t n) {
@@ -97,6 +128,8 @@ struct jdsg_plan {
     for (void *p : allocs) (void)hipFree(p);
     if (timing)
       for (auto &e : ev) (void)hipEventDestroy(e);
+    for (auto &v : pev)
+      for (auto &e : v) (void)hipEventDestroy(e);
   }
   int size_scratch() {
     if (cscr) {
@@ -357,6 +390,14 @@ extern "C" int jdsg_plan_chain_model(const jdsg_plan *P, float *R, float *L, flo
   return 0;
 }
 
+extern "C" int jdsg_plan_pass_timings(const jdsg_plan *P, float *ms2, int *n2) {
+  for (int w = 0; w < 2; ++w) {
+    if (ms2) ms2[w] = P->pass_ms[w];
+    if (n2) n2[w] = P->pass_n[w];
+  }
+  return P->timing ? 0 : fail("timing disabled (set JDSG_TIMING=1 before plan creation)");
+}
+
 extern "C" int jdsg_plan_timings(const jdsg_plan *P, float *ms4) {
   for (int i = 0; i < 4; ++i) ms4[i] = P->last_ms[i];
   return P->timing ? 0 : fail("timing disabled (set JDSG_TIMING=1 before plan creation)");
@@ -442,7 +483,15 @@ int run_eigen(jdsg_plan *P, const double *periods, double *c, double *u, int *st
     E.counter = P->counters + 2 + w;
     E.tv = P->tv;
     int waves = (int)std::min<long long>(P->eigen_waves, (P->nitem[w] + 63) / 64);
-    jdsg_launch_eigen(E, waves, st);
+    hipEvent_t *pev = nullptr;
+    P->pass_n[w] = 0;
+    if (P->timing && !P->capturing) {
+      int np = jdsg_eigen_pass_launches(E, waves);
+      if (P->pass_events(w, np)) return 1;
+      pev = P->pev[w].data();
+      P->pass_n[w] = np;
+    }
+    jdsg_launch_eigen(E, waves, st, pev);
     HIPCHK(hipGetLastError());
     if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[3 + w], st));
   }
@@ -464,8 +513,7 @@ extern "C" int jdsg_plan_execute(jdsg_plan *P, const float *thk, const float *vp
   double *du[4] = {duda, dudb, dudr, dudh};
   if (run_eigen(P, periods, c, u, status, dc, du, st)) return 1;
   if (P->timing && !P->capturing) {
-    HIPCHK(hipEventSynchronize(P->ev[4]));
-    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
+    if (P->finish_timing()) return 1;
   }
   return 0;
 }
@@ -1063,8 +1111,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   double *du[4] = {duda, dudb, dudr, dudh};
   if (run_eigen(P, periods, c, u, status, dc, du, st)) return 1;
   if (P->timing && !P->capturing) {
-    HIPCHK(hipEventSynchronize(P->ev[4]));
-    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&P->last_ms[i], P->ev[i], P->ev[i + 1]));
+    if (P->finish_timing()) return 1;
   }
   return 0;
 }
@@ -1321,6 +1368,7 @@ struct jdsg_surftime {
   std::vector<int> mask_h;
   int ncls[2] = {0, 0};
   bool have_mask = false;
+  double budget = 0.0;  // device bytes for the chunks' intermediate results
   ~jdsg_surftime() {
     modes[0].sets.clear();
     modes[1].sets.clear();
@@ -1554,8 +1602,15 @@ int surftime_run(jdsg_surftime *S, int mapform, const float *vs, double *vc, dou
   auto &Md = S->modes[mapform];
   if (!Md.built && build_mode(S, Md)) return 1;
   if (update_mask(S, vs, st)) return 1;
-  double budget = 12e9;
-  if (const char *b = getenv("JDSG_SURFTIME_BUDGET_GB")) budget = atof(b) * 1e9;
+  if (S->budget <= 0.0) {
+    // intermediate layer kernels of all chunks together: most of the free
+    // device memory (the whole C5 grid fits one chunk on an MI355X), or
+    // JDSG_SURFTIME_BUDGET_GB
+    size_t fr = 0, tot = 0;
+    S->budget = (hipMemGetInfo(&fr, &tot) == hipSuccess) ? 0.6 * (double)fr : 12e9;
+    if (const char *b = getenv("JDSG_SURFTIME_BUDGET_GB")) S->budget = atof(b) * 1e9;
+  }
+  const double budget = S->budget;
   for (int cls = 0; cls < 2; ++cls) {
     int nc = S->ncls[cls];
     if (nc == 0) continue;

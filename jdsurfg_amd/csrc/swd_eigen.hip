@@ -1529,18 +1529,26 @@ jdsg_eigen_combine(EigenArgs A, long long it0, long long lanes) {
   A.status[e] = status;
 }
 
+int jdsg_eigen_pass_launches(const EigenArgs &A, int nwaves) {
+  const long long lanes = (long long)nwaves * 64;
+  return (int)((A.nitem + lanes - 1) / lanes) * (A.group ? 3 : 1);
+}
+
 // Chunks of `lanes` entries: the pass(es), then the assembly.
-void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st) {
+void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st, hipEvent_t *ev) {
   const long long lanes = (long long)nwaves * 64;
   const int nv = A.group ? 3 : 1;
+  int np = 0;
   for (long long it0 = 0; it0 < A.nitem; it0 += lanes) {
     long long n = A.nitem - it0 < lanes ? A.nitem - it0 : lanes;
     dim3 g((unsigned)((n + 63) / 64));
-    for (int v = 0; v < nv; ++v) {
+    for (int v = 0; v < nv; ++v, ++np) {
+      if (ev) (void)hipEventRecord(ev[2 * np], st);
       if (A.wave == 0)
         hipLaunchKernelGGL(jdsg_eigen_pass<0>, g, dim3(64), 0, st, A, it0, lanes, v);
       else
         hipLaunchKernelGGL(jdsg_eigen_pass<1>, g, dim3(64), 0, st, A, it0, lanes, v);
+      if (ev) (void)hipEventRecord(ev[2 * np + 1], st);
     }
     if (A.wave == 0)
       hipLaunchKernelGGL(jdsg_eigen_combine<0>, g, dim3(64), 0, st, A, it0, lanes);

@@ -150,7 +150,10 @@ void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st);
 // up to `cycles` rounds (A.susp buffers set; A.susp_on is set here)
 void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st);
 size_t jdsg_chain_state_bytes();
-void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st);
+// ev (optional): 2 * jdsg_eigen_pass_launches(A, nwaves) events recorded
+// before and after each pass launch (the pass kernels' own time)
+void jdsg_launch_eigen(const EigenArgs &A, int nwaves, hipStream_t st, hipEvent_t *ev = nullptr);
+int jdsg_eigen_pass_launches(const EigenArgs &A, int nwaves);
 void jdsg_launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t st);
 void jdsg_launch_water_mask(int nxy, const float *vs, int *mask, hipStream_t st);
 void jdsg_launch_grid_to_layers(int col0, int nchunk, int nxy, int nz, int N, int water,

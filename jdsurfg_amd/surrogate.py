@@ -97,6 +97,7 @@ class MLP:
         self.gb = [torch.zeros_like(b) for b in self.b]
         self.step = 0
         self._buf = {}
+        self._consts = None   # inference constants (ext, output bias), see inference_consts
 
     def _bufs(self, M, train):
         import torch
@@ -183,6 +184,7 @@ class MLP:
     def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0, stream=None):
         L = _L()
         self.step += 1
+        self._consts = None
         for li in range(len(self.dims)):
             _chk(L.jdsg_adam(self.W[li].numel(), _p(self.W[li]), _p(self.mW[li]),
                              _p(self.vW[li]), _p(self.gW[li]), lr, b1, b2, eps, self.step, wd,
@@ -208,7 +210,20 @@ class MLP:
             m.Wbf[i] = wb.contiguous()
             m.W[i].copy_(wb.float())
             m.b[i].copy_(torch.from_numpy(z[f"b{i}"]))
+        m._consts = None
         return m
+
+    def inference_consts(self):
+        """The weight columns of the per-entry inputs with the first bias
+        (8 x hid fp32) and the output bias as a host float, formed once per
+        set of weights (the first call reads the bias back; later calls do
+        no device work)."""
+        import torch
+        if self._consts is None:
+            W1 = self.Wbf[0].float()
+            ext = torch.cat([W1[:, 144:151].t(), self.b[0][None, :]]).contiguous()
+            self._consts = (ext, float(self.b[4][0].item()))
+        return self._consts
 
 
 def training_columns(ncol, seed, K=40):
@@ -368,13 +383,17 @@ def first_layer_parts(mlp, feats, stream=None):
                    stream=stream)
     ncol, H = feats.plan.ncol, mlp.hid
     ncp = (ncol + 127) // 128 * 128
-    Xc = torch.zeros(ncp, FEAT, dtype=torch.bfloat16, device="cuda")
+    key = (ncp, H)
+    if getattr(feats, "_fl_key", None) != key:
+        # rows past ncol only feed rows of acol nothing reads
+        feats._xc = torch.empty(ncp, FEAT, dtype=torch.bfloat16, device="cuda")
+        feats._acol = torch.empty(ncp, H, device="cuda")
+        feats._fl_key = key
+    Xc, acol = feats._xc, feats._acol
     _chk(L.jdsg_mlp_column_rows(ncol, _p(feats.colf), _p(Xc), stream), "column rows")
-    acol = torch.empty(ncp, H, device="cuda")
     _chk(L.jdsg_gemm_bf16(0, 1, ncp, H, FEAT, _p(Xc), FEAT, _p(mlp.Wbf[0]), FEAT, _p(acol), H,
                           0, None, None, 0, stream), "gemm")
-    W1 = mlp.Wbf[0].float()
-    ext = torch.cat([W1[:, 144:151].t(), mlp.b[0][None, :]]).contiguous()
+    ext, _ = mlp.inference_consts()
     return acol, ext
 
 
@@ -392,13 +411,16 @@ def predict(mlp, feats, rows=None, e0=0, n=None, chunk=1 << 20, split=True, fuse
     if fused is None:
         fused = mlp.hid == 256
     out = torch.empty(n, dtype=torch.float64, device="cuda")
-    b4 = float(mlp.b[4][0].item())   # bias of the output used (the rest is padding)
+    _, b4 = mlp.inference_consts()   # bias of the output used (the rest is padding)
     if split and fused:
         if mlp.hid != 256:
             raise ValueError("the fused network kernel is for hidden width 256")
         acol, ext = first_layer_parts(mlp, feats)
         nw = int(L.jdsg_mlp_fused_work(n))
-        work = torch.empty(max(nw, 1), dtype=torch.int32, device="cuda")
+        if getattr(feats, "_work_n", None) != nw:
+            feats._work = torch.empty(max(nw, 1), dtype=torch.int32, device="cuda")
+            feats._work_n = nw
+        work = feats._work
         _chk(L.jdsg_plan_mlp_fused(feats.plan.h, _p(feats.inp[4]), e0, n, _p(acol), mlp.hid,
                                    _p(ext), *[_p(mlp.Wbf[i]) for i in (1, 2, 3)],
                                    *[_p(mlp.b[i]) for i in (1, 2, 3)], _p(mlp.Wbf[4]), b4,

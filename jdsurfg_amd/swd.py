@@ -184,6 +184,18 @@ class Plan:
             raise RuntimeError(last_error())
         return h.reshape(nvar, self.nent)
 
+    def pass_timings(self):
+        """The eigen pass kernels' own milliseconds and launch counts of the
+        last execute, per wave type (Rayleigh, Love), when JDSG_TIMING=1 was
+        set before the plan was created."""
+        ms = np.zeros(2, np.float32)
+        n = np.zeros(2, np.int32)
+        L = lib()
+        L.jdsg_plan_pass_timings.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        if L.jdsg_plan_pass_timings(self.h, ms.ctypes.data, n.ctypes.data):
+            return None
+        return ms, n
+
     def timings(self):
         """Per-stage milliseconds of the last execute (prep, chains, eigen R,
         eigen L) when JDSG_TIMING=1 was set before the plan was created."""
