@@ -468,6 +468,36 @@ def main():
         })
         dom_status = sout["status"].cpu().numpy()
         dom_ms, dom_n = float(sm[5]), int(sp.pass_timings()[1][0])
+        # the same iteration as device work only, captured into a graph and
+        # replayed on the same model (a copy of the inputs); the tests replay
+        # it with a 1 % Vs update per iteration against eager solves
+        gp = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
+        gp.set_tolerance(1e-10)
+        S.bind_network(gp, mlp)
+        ginp = tuple(t.clone() for t in inp)
+        gout = gp.alloc_outputs(kernels=kern)
+        gst = torch.cuda.Stream()
+        with torch.cuda.stream(gst):
+            S.surrogate_solve_net(gp, ginp, gout, stream=gst.cuda_stream)
+            gst.synchronize()
+            graph = S.capture_surrogate(gp, ginp, gout, gst, perturb_amp=0.0)
+            gev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            graph.launch(gst)
+            gst.synchronize()
+            gev[0].record(gst)
+            for _ in range(args.steps):
+                graph.launch(gst)
+            gev[1].record(gst)
+            gst.synchronize()
+        g_ms = gev[0].elapsed_time(gev[1]) / args.steps
+        line["surrogate"]["graph_replay"] = {
+            "ms_per_iteration": float(g_ms), "value": float(cps / (g_ms * 1e-3)),
+            "unit": "column·periods/s", "replays": args.steps,
+            "iteration": "network + bracket/polish + fallback chains + eigen stage, one graph "
+                         "launch",
+            "identical_to_eager": bool(all(torch.equal(gout[n], sout[n])
+                                           for n in ("c", "u", "status") + kern))}
+        del gp, ginp, gout, graph
     else:
         line.update({"value": line["plain"]["value"], "ms_per_step": float(ms_plain),
                      "metric": "column-periods/s (c, U, dc/dVs, dU/dVs), plain FP64 method"})

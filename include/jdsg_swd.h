@@ -297,6 +297,32 @@ int jdsg_plan_execute_surrogate(jdsg_plan *plan, const float *thk, const float *
                                 int *status, double *dcda, double *dcdb, double *dcdr,
                                 double *dcdh, double *duda, double *dudb, double *dudr,
                                 double *dudh, void *stream);
+/* The whole surrogate iteration as device work only.  jdsg_plan_set_network
+ * binds a network to the plan (W[0..4] bf16 row-major out x in: hid x 192,
+ * three hid x hid, 64 x hid; b[0..4] fp32; the caller keeps them; the
+ * prediction is exp(out * scale + shift)) and allocates the inference
+ * buffers; jdsg_plan_execute_surrogate_net then predicts every root
+ * (column features, first-layer GEMM per column, hidden layers on MFMA, the
+ * head) and runs jdsg_plan_execute_surrogate on the predictions, with no
+ * host synchronisation; jdsg_plan_capture_surrogate records the optional
+ * model update (as jdsg_plan_capture) and that iteration into a graph that
+ * jdsg_graph_launch replays.  jdsg_plan_network_pred copies the last
+ * predictions (nent doubles) to the host. */
+int jdsg_plan_set_network(jdsg_plan *plan, int hid, const void *const W[5],
+                          const float *const b[5], double scale, double shift);
+int jdsg_plan_execute_surrogate_net(jdsg_plan *plan, const float *thk, const float *vp,
+                                    const float *vs, const float *rho, const double *periods,
+                                    double half_width, double *c, double *u, int *status,
+                                    double *dcda, double *dcdb, double *dcdr, double *dcdh,
+                                    double *duda, double *dudb, double *dudr, double *dudh,
+                                    void *stream);
+int jdsg_plan_capture_surrogate(jdsg_plan *plan, float *thk, float *vp, float *vs, float *rho,
+                                const double *periods, double half_width, double *c, double *u,
+                                int *status, double *dcda, double *dcdb, double *dcdr,
+                                double *dcdh, double *duda, double *dudb, double *dudr,
+                                double *dudh, float perturb_amp, unsigned seed, void *stream,
+                                jdsg_graph **graph);
+int jdsg_plan_network_pred(const jdsg_plan *plan, double *host);
 /* The polished roots the identity checks kept after the last surrogate
  * execute (nvar*nent doubles, 0 where the chain took over) and the signs of
  * the period equation at bracket low, high and the chain floor (3*nvar*nent

@@ -80,6 +80,21 @@ struct jdsg_plan {
   ScanJob *susp_jobs = nullptr;
   unsigned *susp_njobs = nullptr;
   const double *known_in = nullptr;
+  // the surrogate network bound by jdsg_plan_set_network (weights are the
+  // caller's; the buffers are the plan's)
+  struct Net {
+    int hid = 0;
+    const uint16_t *W[5] = {};
+    const float *b[5] = {};
+    float b5 = 0.0f;
+    double scale = 0.0, shift = 0.0;
+    float *ext = nullptr, *colf = nullptr, *acol = nullptr;
+    uint16_t *Xc = nullptr, *Y[2] = {nullptr, nullptr};
+    int *work = nullptr;
+    long long work_len = 0, chunk = 0;
+    double *pred = nullptr;
+  };
+  std::unique_ptr<Net> net;
   // explicit per-entry periods for the eigen stage (per-column ABI only)
   double *tv = nullptr;
   double *eig[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1116,6 +1131,109 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   return 0;
 }
 
+// ---- the whole surrogate iteration in device work only (network + solve)
+extern "C" int jdsg_plan_set_network(jdsg_plan *P, int hid, const void *const W[5],
+                                     const float *const b[5], double scale, double shift) {
+  if (hid <= 0 || hid % 128) return fail("network width must be a positive multiple of 128");
+  for (int i = 0; i < 5; ++i)
+    if (!W[i] || !b[i] || (reinterpret_cast<uintptr_t>(W[i]) & 15) ||
+        (reinterpret_cast<uintptr_t>(b[i]) & 15))
+      return fail("network: null or unaligned weight or bias");
+  if (jdsg_plan_enable_surrogate(P)) return 1;
+  auto N = std::make_unique<jdsg_plan::Net>();
+  N->hid = hid;
+  for (int i = 0; i < 5; ++i) {
+    N->W[i] = (const uint16_t *)W[i];
+    N->b[i] = b[i];
+  }
+  N->scale = scale;
+  N->shift = shift;
+  // the first layer's weight columns of the per-entry inputs (ln T, two wave
+  // and four mode one-hots) and its bias, as jdsg_plan_mlp_expand takes them
+  std::vector<uint16_t> w0((size_t)hid * 192);
+  std::vector<float> b0(hid), ext((size_t)8 * hid);
+  HIPCHK(hipMemcpy(w0.data(), W[0], sizeof(uint16_t) * w0.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(b0.data(), b[0], sizeof(float) * hid, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&N->b5, b[4], sizeof(float), hipMemcpyDeviceToHost));
+  for (int r = 0; r < 7; ++r)
+    for (int h = 0; h < hid; ++h) {
+      uint32_t bits = (uint32_t)w0[(size_t)h * 192 + 144 + r] << 16;
+      float f;
+      memcpy(&f, &bits, sizeof(f));
+      ext[(size_t)r * hid + h] = f;
+    }
+  for (int h = 0; h < hid; ++h) ext[(size_t)7 * hid + h] = b0[h];
+  size_t ncp = (size_t)(P->ncol + 127) / 128 * 128;
+  if (P->dalloc(&N->ext, ext.size()) || P->dalloc(&N->colf, (size_t)P->ncol * 148) ||
+      P->dalloc(&N->Xc, ncp * 192) || P->dalloc(&N->acol, ncp * hid) ||
+      P->dalloc(&N->pred, P->nent))
+    return 1;
+  HIPCHK(hipMemcpy(N->ext, ext.data(), sizeof(float) * ext.size(), hipMemcpyHostToDevice));
+  if (hid == 256) {
+    N->work_len = jdsg_mlp_fused_work_len(P->nent);
+    if (P->dalloc(&N->work, std::max<long long>(N->work_len, 1))) return 1;
+  } else {
+    N->chunk = std::min<long long>(1 << 20, (P->nent + 63) / 64 * 64);
+    for (int q = 0; q < 2; ++q)
+      if (P->dalloc(&N->Y[q], (size_t)std::max<long long>(N->chunk, 64) * hid)) return 1;
+  }
+  P->net = std::move(N);
+  return 0;
+}
+
+namespace {
+// predicted flat c of every entry into net->pred
+int run_network(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
+                const float *rho, const double *periods, hipStream_t st) {
+  jdsg_plan::Net &N = *P->net;
+  const int H = N.hid;
+  const int ncp = (P->ncol + 127) / 128 * 128;
+  jdsg_launch_column_features(P->ncol, P->layer_off_d, thk, vp, vs, rho, N.colf, st);
+  if (jdsg_mlp_column_rows(P->ncol, N.colf, N.Xc, st)) return 1;
+  if (jdsg_gemm_bf16(0, 1, ncp, H, 192, N.Xc, 192, N.W[0], 192, N.acol, H, 0, nullptr, nullptr, 0,
+                     st))
+    return 1;
+  if (H == 256)
+    return jdsg_plan_mlp_fused(P, periods, 0, P->nent, N.acol, H, N.ext, N.W[1], N.W[2], N.W[3],
+                               N.b[1], N.b[2], N.b[3], N.W[4], N.b5, N.scale, N.shift, N.pred, 0,
+                               N.work, N.work_len, st);
+  for (long long s0 = 0; s0 < P->nent; s0 += N.chunk) {
+    long long m = std::min(N.chunk, P->nent - s0);
+    int mp = (int)((m + 63) / 64 * 64);
+    if (jdsg_plan_mlp_expand(P, periods, s0, m, N.acol, H, N.ext, H, N.Y[0], st)) return 1;
+    int a = 0;
+    for (int li = 1; li < 4; ++li, a ^= 1)
+      if (jdsg_gemm_bf16(0, 1, mp, H, H, N.Y[a], H, N.W[li], H, nullptr, H, 4, N.b[li],
+                         N.Y[a ^ 1], H, st))
+        return 1;
+    if (jdsg_mlp_head(m, H, N.Y[a], N.W[4], N.b5, N.scale, N.shift, N.pred, s0, st)) return 1;
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" int jdsg_plan_execute_surrogate_net(jdsg_plan *P, const float *thk, const float *vp,
+                                               const float *vs, const float *rho,
+                                               const double *periods, double half_width,
+                                               double *c, double *u, int *status, double *dcda,
+                                               double *dcdb, double *dcdr, double *dcdh,
+                                               double *duda, double *dudb, double *dudr,
+                                               double *dudh, void *stream) {
+  if (!P->net) return fail("call jdsg_plan_set_network first");
+  hipStream_t st = (hipStream_t)stream;
+  if (run_network(P, thk, vp, vs, rho, periods, st)) return 1;
+  return jdsg_plan_execute_surrogate(P, thk, vp, vs, rho, periods, P->net->pred, half_width, c,
+                                     u, status, dcda, dcdb, dcdr, dcdh, duda, dudb, dudr, dudh,
+                                     stream);
+}
+
+extern "C" int jdsg_plan_network_pred(const jdsg_plan *P, double *host) {
+  if (!P->net) return fail("no network bound");
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(host, P->net->pred, sizeof(double) * P->nent, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 // Diagnostics of the last surrogate execute: the polished roots that the
 // checks kept (0 where the chain took over), nvar*nent doubles, and the signs
 // of the period equation at bracket low, high and the chain floor, 3*nvar*nent
@@ -1295,6 +1413,42 @@ extern "C" int jdsg_plan_capture(jdsg_plan *P, float *thk, float *vp, float *vs,
     return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   *out = G;
+  return 0;
+}
+
+// The surrogate iteration (model update, network, bracket + polish, fallback
+// chains, eigen stage) captured like jdsg_plan_capture; the plan needs a
+// bound network (jdsg_plan_set_network) and exact-root mode.
+extern "C" int jdsg_plan_capture_surrogate(jdsg_plan *P, float *thk, float *vp, float *vs,
+                                           float *rho, const double *periods, double half_width,
+                                           double *c, double *u, int *status, double *dcda,
+                                           double *dcdb, double *dcdr, double *dcdh,
+                                           double *duda, double *dudb, double *dudr,
+                                           double *dudh, float perturb_amp, unsigned seed,
+                                           void *stream, jdsg_graph **out) {
+  hipStream_t st = (hipStream_t)stream;
+  if (!st) return fail("graph capture needs a non-null stream");
+  if (!P->net) return fail("call jdsg_plan_set_network first");
+  if (P->tol >= 1.0e-6) return fail("the surrogate path polishes to exact roots: set tol < 1e-6");
+  auto G = std::make_unique<jdsg_graph>();
+  if (hipMalloc(&G->iter, sizeof(int)) != hipSuccess ||
+      hipMemset(G->iter, 0, sizeof(int)) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    return fail("graph counter allocation failed");
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fail(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  P->capturing = true;
+  long long nlay = P->nlay_total;
+  if (perturb_amp > 0.f) jdsg_launch_perturb(nlay, vs, vp, rho, G->iter, perturb_amp, seed, st);
+  int rc = jdsg_plan_execute_surrogate_net(P, thk, vp, vs, rho, periods, half_width, c, u, status,
+                                           dcda, dcdb, dcdr, dcdh, duda, dudb, dudr, dudh, st);
+  if (perturb_amp > 0.f) jdsg_launch_bump(G->iter, st);
+  P->capturing = false;
+  e = hipStreamEndCapture(st, &G->graph);
+  if (rc) return 1;
+  if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  *out = G.release();
   return 0;
 }
 

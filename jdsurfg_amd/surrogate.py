@@ -506,6 +506,58 @@ def surrogate_solve(plan, inp, out, mlp=None, pred=None, half_width=0.005, strea
     return out
 
 
+def bind_network(plan, mlp):
+    """Bind the network to the plan for the device-only surrogate iteration
+    (jdsg_plan_set_network); the plan keeps a reference to the weights."""
+    L = _L()
+    W = (C.c_void_p * 5)(*[w.data_ptr() for w in mlp.Wbf])
+    B = (C.c_void_p * 5)(*[b.data_ptr() for b in mlp.b])
+    L.jdsg_plan_set_network.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                        C.c_double]
+    _chk(L.jdsg_plan_set_network(plan.h, mlp.hid, W, B, C_SCALE, C_SHIFT), "set network")
+    plan._net = mlp
+
+
+def surrogate_solve_net(plan, inp, out, half_width=0.005, stream=None):
+    """The surrogate iteration of a plan with a bound network, all device
+    work (no host synchronisation): network, bracket + polish, fallback,
+    eigen stage."""
+    import torch
+    L = _L()
+    L.jdsg_plan_execute_surrogate_net.argtypes = [C.c_void_p] * 6 + [C.c_double] + \
+        [C.c_void_p] * 12
+    ks = [_p(out.get(n)) for n in swd.KERNEL_NAMES]
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _chk(L.jdsg_plan_execute_surrogate_net(plan.h, *[_p(t) for t in inp], half_width,
+                                           _p(out["c"]), _p(out["u"]), _p(out["status"]), *ks,
+                                           C.c_void_p(st)), "execute_surrogate_net")
+    return out
+
+
+def capture_surrogate(plan, inp, out, stream, half_width=0.005, perturb_amp=0.0, seed=0):
+    """Record one surrogate iteration (optional in-place model update, then
+    surrogate_solve_net) on `stream` into a graph; returns a swd.Graph."""
+    L = _L()
+    L.jdsg_plan_capture_surrogate.argtypes = [C.c_void_p] * 6 + [C.c_double] + \
+        [C.c_void_p] * 11 + [C.c_float, C.c_uint, C.c_void_p, C.POINTER(C.c_void_p)]
+    g = C.c_void_p()
+    ks = [_p(out.get(n)) for n in swd.KERNEL_NAMES]
+    _chk(L.jdsg_plan_capture_surrogate(plan.h, *[_p(t) for t in inp], half_width,
+                                       _p(out["c"]), _p(out["u"]), _p(out["status"]), *ks,
+                                       float(perturb_amp), int(seed),
+                                       C.c_void_p(stream.cuda_stream), C.byref(g)),
+         "capture_surrogate")
+    return swd.Graph(g, plan)
+
+
+def network_pred(plan):
+    L = _L()
+    h = np.zeros(plan.nent, np.float64)
+    L.jdsg_plan_network_pred.argtypes = [C.c_void_p, C.c_void_p]
+    _chk(L.jdsg_plan_network_pred(plan.h, h.ctypes.data), "network pred")
+    return h
+
+
 def surrogate_stats(plan):
     L = _L()
     f, pe, ce = C.c_longlong(), C.c_longlong(), C.c_longlong()
