@@ -28,19 +28,24 @@ struct Table {
 
 thread_local std::string g_err;
 
-// spherical.cpp:9-26 delsph, in float
+// spherical.cpp:9-26 delsph: float variables, the trigonometry in double
+// (dlat * 0.5 and M_PI * 0.5 promote), results stored to float as there
 float delsph(float c1, float l1, float c2, float l2) {
   const float R = 6371.0f;
   const float dlat = c2 - c1, dlon = l2 - l1;
-  const float la1 = (float)(M_PI * 0.5 - c1), la2 = (float)(M_PI * 0.5 - c2);
-  const float a = std::sin(dlat * 0.5f) * std::sin(dlat * 0.5f) +
-                  std::sin(dlon * 0.5f) * std::sin(dlon * 0.5f) * std::cos(la1) * std::cos(la2);
-  const float c = 2.0f * std::atan2(std::sqrt(a), std::sqrt(1 - a));
+  const float la1 = (float)(M_PI * 0.5 - (double)c1), la2 = (float)(M_PI * 0.5 - (double)c2);
+  const float a = (float)(std::sin((double)dlat * 0.5) * std::sin((double)dlat * 0.5) +
+                          std::sin((double)dlon * 0.5) * std::sin((double)dlon * 0.5) *
+                              std::cos((double)la1) * std::cos((double)la2));
+  const float c = (float)(2.0 * std::atan2(std::sqrt((double)a), std::sqrt((double)(1 - a))));
   return R * c;
 }
 
-// io.cpp:57-87: "nmode k..." then nmode lines of periods (strtof, as the
-// reference parses them; missing numbers read as 0)
+// io.cpp:57-87: "nmode k..." then nmode lines of periods, parsed with
+// strtof.  Mode im's row holds its kmax[im] periods; the slots past them
+// stay -1, the padding get_nt_row stops at (SurfaceWave.cpp:157-168).  (The
+// reference reads nt numbers from every row, so strtof's 0 past the end of a
+// shorter row overwrites the padding and the table gains T = 0 entries.)
 bool read_table(std::ifstream &in, Table &T) {
   std::string line;
   if (!std::getline(in, line)) return false;
@@ -61,7 +66,7 @@ bool read_table(std::ifstream &in, Table &T) {
   for (int im = 0; im < T.nmode; ++im) {
     if (!std::getline(in, line)) return false;
     p = line.c_str();
-    for (int it = 0; it < T.nt; ++it) {
+    for (int it = 0; it < T.kmax[im]; ++it) {
       T.t[(size_t)im * T.nt + it] = std::strtof(p, &e);
       p = e;
     }
@@ -122,7 +127,8 @@ extern "C" int jdsg_read_swd_data(const char *path, jdsg_swd_data **out) {
       delete D;
       return 1;
     }
-    const float c1 = (90.0f - lat) * DEG2RAD, l1 = lon * DEG2RAD;
+    // io.cpp:190-191: (90.0 - lat) in double times the float DEG2RAD
+    const float c1 = (float)((90.0 - (double)lat) * (double)DEG2RAD), l1 = lon * DEG2RAD;
     const int code = (wavetp == 2 && veltp == 0)   ? 0
                      : (wavetp == 2 && veltp == 1) ? 1
                      : (wavetp == 1 && veltp == 0) ? 2
@@ -140,7 +146,7 @@ extern "C" int jdsg_read_swd_data(const char *path, jdsg_swd_data **out) {
         delete D;
         return 1;
       }
-      const float c2 = (90.0f - rlat) * DEG2RAD, l2 = rlon * DEG2RAD;
+      const float c2 = (float)((90.0 - (double)rlat) * (double)DEG2RAD), l2 = rlon * DEG2RAD;
       const float d = delsph(c1, l1, c2, l2);
       D->rcv_colat.push_back(c2);
       D->rcv_lon.push_back(l2);

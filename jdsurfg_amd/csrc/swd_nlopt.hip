@@ -326,6 +326,14 @@ extern "C" int jdsg_nlopt_create(int n, int m, float max_step, int iter_start, i
     g_nl_err = "nlopt_create: bad argument";
     return 1;
   }
+  // the history of earlier iterations (the reference's grad/direc/logmd/syn
+  // files) is a device ring that starts empty: a resumed run with
+  // iter_current past iter_start would read an unset history
+  if (iter_current != iter_start) {
+    g_nl_err = "nlopt_create: iter_current must equal iter_start (the history starts empty; "
+               "restart the optimiser at the current iteration)";
+    return 1;
+  }
   const long long nw = jdsg_nlopt_work_floats(n, m);
   float *owned = nullptr;
   if (!work) {
@@ -445,11 +453,21 @@ extern "C" int jdsg_nlopt_update(jdsg_nlopt *h, float *x, float *dsyn, int metho
       h->iter_ls += 1;
     } else {
       h->step_l = h->step_fac;
-      if (h->step_r != 0.)
+      const float tried = h->step_fac;
+      const bool grow = h->step_r == 0.;
+      if (!grow)
         h->step_fac = 0.5 * (h->step_l + h->step_r);
       else
         h->step_fac *= 10.0f;
       h->iter_ls += 1;
+      if (!next_model(h, x, fcost, method)) return 1;
+      // the tenfold step capped back to the step just tried: every further
+      // trial would repeat it (the reference loops on), so stop here
+      if (grow && h->step_fac == tried) {
+        g_nl_err = "nlopt: no step met the Wolfe conditions (the capped trial step repeats)";
+        return 2;
+      }
+      continue;
     }
     if (!next_model(h, x, fcost, method)) return 1;
   }

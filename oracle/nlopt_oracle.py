@@ -148,10 +148,18 @@ class NonlinOPTOracle:
                 self.ITER_LS += 1
             else:
                 self.STEP_L = self.STEP_FAC
-                if self.STEP_R != 0:
+                tried, grow = self.STEP_FAC, self.STEP_R == 0
+                if not grow:
                     self.STEP_FAC = F(0.5 * float(F(self.STEP_L + self.STEP_R)))
                 else:
                     self.STEP_FAC = F(self.STEP_FAC * F(10.0))
                 self.ITER_LS += 1
+                x1, direc = self.next_model(model, grad, x, fcost, method)
+                # the tenfold step capped back to the one just tried repeats
+                # forever in the reference (nonlinear_opt.hpp:360-380)
+                if grow and self.STEP_FAC == tried:
+                    raise RuntimeError("no step met the Wolfe conditions (the capped trial "
+                                       "step repeats)")
+                continue
             x1, direc = self.next_model(model, grad, x, fcost, method)
         return x, dsyn

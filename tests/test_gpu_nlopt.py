@@ -43,7 +43,8 @@ def test_device_optimiser_matches_the_oracle(method, max_step, iters):
 def test_capped_step_that_fails_the_curvature_condition_stops_both():
     """CG with a 3 % cap on this problem: the capped step never meets the
     curvature condition and the reference's line search would not end; the
-    oracle and the device path both stop after 100 trials."""
+    oracle and the device path both stop at the first repeat of the capped
+    trial step, after the one trial at the cap."""
     import torch
     from jdsurfg_amd.nlopt import NonlinOPT
     print("nlopt CG max_step=0.03 stuck line search", flush=True)
@@ -51,9 +52,32 @@ def test_capped_step_that_fails_the_curvature_condition_stops_both():
     o = NonlinOPTOracle(0.03, 0, 0)
     with pytest.raises(RuntimeError, match="Wolfe"):
         o.update(p, p.x0.copy(), "CG")
+    assert o.trials == 1
     dev = NonlinOPT(p.n, p.m, 0.03, 0, 0)
     xd = torch.from_numpy(p.x0).cuda()
     dd = torch.zeros(p.m, dtype=torch.float32, device="cuda")
     with pytest.raises(RuntimeError, match="Wolfe"):
         dev.update(TorchLogQuad(p), xd, dd, "CG")
+    assert dev.state()["trials"] == 1
     assert torch.equal(xd.cpu(), torch.from_numpy(p.x0))
+
+
+def test_restart_at_a_later_iteration_matches_the_oracle():
+    """A run that starts at iteration 2 (iter_start = iter_current = 2, an
+    empty history) updates as the oracle started there."""
+    import torch
+    from jdsurfg_amd.nlopt import NonlinOPT
+    print("nlopt LBFGS started at iteration 2", flush=True)
+    p = LogQuad()
+    o = NonlinOPTOracle(0.3, 2, 2)
+    dev = NonlinOPT(p.n, p.m, 0.3, 2, 2)
+    tp = TorchLogQuad(p)
+    xo = p.x0.copy()
+    xd = torch.from_numpy(p.x0).cuda()
+    dd = torch.zeros(p.m, dtype=torch.float32, device="cuda")
+    for k in range(4):
+        xo, dso = o.update(p, xo, "LBFGS")
+        st = dev.update(tp, xd, dd, "LBFGS")
+        torch.cuda.synchronize()
+        assert (st["iter"], st["iter_start"], st["trials"]) == (o.ITER, o.ITER_START, o.trials), k
+        assert np.allclose(xd.cpu().numpy(), xo, rtol=2e-5, atol=0), k

@@ -129,3 +129,20 @@ def test_mixed_water_and_land_grid(oracle_rest):
     assert np.abs(m[0][both] / a[0][both] - 1).max() <= 1e-12
     print("coastal: Love in water columns (kernel form) GPU", a[1][water][:, 2 * nR:][:2],
           "oracle", b[1][water][:, 2 * nR:][:2], flush=True)
+
+
+def test_ragged_period_table_through_surftime(tmp_path):
+    """A ragged two-mode header read by the data reader gives SurfTime the
+    period index count of the header (3 + 1), with get_nt_row stopping at
+    the -1 padding."""
+    from jdsurfg_amd import data as D, swd
+    p = tmp_path / "ragged.dat"
+    p.write_text("2 3 1\n10 20 30\n40\n0\n0\n0\n# 31 108 0 1 2 0\n33.7 105.7 3.1\n")
+    t, k = D.read_swd_data(p)["tables"]["Rc"]
+    print("case ragged table through SurfTime", flush=True)
+    dep = (2.0 * np.arange(11)).astype(np.float32)
+    S = swd.SurfTime(2, 1, 11, dep, {"Rc": t})
+    assert S.kmax == sum(k) == 4
+    vs = np.tile((2.8 + 0.04 * dep)[None, None, :], (2, 1, 1)).astype(np.float32)
+    vc, vout, st = swd.get_2d_map(vs, dep, {"Rc": t})
+    assert vc.shape == (2, 4) and (vc[:, :3] > 0).all()

@@ -97,6 +97,19 @@ def test_capped_step_that_never_meets_the_curvature_condition_stops():
     # at max_step / dmax: the reference's loop would not end
     p = LogQuad()
     o = NonlinOPTOracle(0.03, 0, 0)
-    with pytest.raises(RuntimeError, match="Wolfe"):
+    with pytest.raises(RuntimeError, match="repeats"):
         o.update(p, p.x0.copy(), "CG")
-    assert o.trials == 101 and o.STEP_FAC * np.float32(10) > 0
+    # the first trial is at the cap; its tenfold is capped back to it: stop
+    assert o.trials == 1 and o.STEP_FAC * np.float32(10) > 0
+
+
+def test_create_refuses_a_resumed_history():
+    """The history ring starts empty, so a create with iter_current past
+    iter_start is refused before any device work (an equal pair starts
+    fresh at that iteration)."""
+    import ctypes as C
+    from jdsurfg_amd import nlopt
+    L = nlopt._L()
+    h = C.c_void_p()
+    rc = L.jdsg_nlopt_create(300, 400, 0.3, 0, 3, None, None, C.byref(h))
+    assert rc != 0 and b"iter_current" in L.jdsg_nlopt_last_error()

@@ -2,8 +2,12 @@
 src/SWD/io.cpp:29-238) on the head of the reference's example data
 (tests/golden/surfdataSC_head.dat: its period header and first 25 source
 stations), against an independent tokenisation of the same file: tables,
-modes, period indices, wave codes and positions equal; distances within
-float rounding of the haversine in double; times = distance / velocity."""
+modes, period indices, wave codes and positions equal (colatitude as the
+reference forms it, (90.0 - lat) in double times the float DEG2RAD);
+distances equal to the reference's own delsph (shared/spherical.cpp built
+from source into oracle/_ref) bit for bit, and within float rounding of the
+haversine in double; times = distance / velocity.  A ragged two-mode header
+keeps the -1 padding past each mode's periods."""
 import os
 
 import numpy as np
@@ -26,7 +30,7 @@ def tokenise(path):
         for im in range(nmode):
             vals = [float(x) for x in lines[pos].split()]
             pos += 1
-            t[:len(vals[:nt]), im] = vals[:nt]
+            t[:kmax[im], im] = vals[:kmax[im]]
         tables.append((t, kmax))
     pairs, recv = [], []
     while pos < len(lines):
@@ -45,8 +49,20 @@ def tokenise(path):
 
 def colat_lon(lat, lon):
     d2r = np.float32(np.pi / 180.0)
-    return ((np.float32(90.0) - np.float32(lat)) * d2r).astype(np.float32), \
-        (np.float32(lon) * d2r).astype(np.float32)
+    lat = np.asarray(lat, np.float32).astype(np.float64)
+    return ((90.0 - lat) * np.float64(d2r)).astype(np.float32), \
+        (np.asarray(lon, np.float32) * d2r).astype(np.float32)
+
+
+def ref_delsph():
+    import ctypes as C
+    p = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libsphref.so")
+    if not os.path.exists(p):
+        return None
+    L = C.CDLL(p)
+    L.delsph.argtypes = [C.c_float] * 4
+    L.delsph.restype = C.c_float
+    return L.delsph
 
 
 def test_reader_matches_an_independent_tokenisation():
@@ -72,6 +88,11 @@ def test_reader_matches_an_independent_tokenisation():
     a = np.sin((c2 - c1) / 2) ** 2 + np.sin((l2 - l1) / 2) ** 2 * np.sin(c1) * np.sin(c2)
     d = 6371.0 * 2 * np.arctan2(np.sqrt(a), np.sqrt(1 - a))
     assert np.abs(got["dist"] - d).max() <= 1e-5 * d.max() + 1e-3
+    f = ref_delsph()
+    if f is not None:
+        ref = np.array([f(float(sc[s]), float(sl[s]), float(a_), float(b_))
+                        for s, a_, b_ in zip(src_idx, rc, rl)], np.float32)
+        assert np.array_equal(got["dist"], ref)
     assert np.array_equal(got["obst"], (got["dist"] / recv[:, 2].astype(np.float32)).astype(np.float32))
 
 
@@ -85,6 +106,19 @@ def test_reader_counts_the_full_example():
     assert got["pair_off"][-1] == len(lines) - n_src - 6   # 6 header lines (2 period rows)
     assert got["tables"]["Rc"][1].tolist() == [19] and got["tables"]["Lc"][1].tolist() == []
     assert (got["obst"] > 0).all() and (got["period"] >= 0).all()
+
+
+def test_ragged_mode_rows_keep_their_padding(tmp_path):
+    """Mode 0 with three periods and mode 1 with one: the table is 3 x 2
+    with -1 below mode 1's period (the reference's strtof would store 0
+    there, io.cpp:69-75)."""
+    from jdsurfg_amd import data as D
+    p = tmp_path / "ragged.dat"
+    p.write_text("2 3 1\n10 20 30\n40\n0\n0\n0\n# 31 108 0 1 2 0\n33.7 105.7 3.1\n")
+    got = D.read_swd_data(p)
+    t, k = got["tables"]["Rc"]
+    assert list(k) == [3, 1]
+    assert np.array_equal(t, np.array([[10, 40], [20, -1], [30, -1]], np.float32))
 
 
 def test_reader_rejects_a_malformed_file(tmp_path):
