@@ -510,6 +510,44 @@ void jdsg_nlopt_state(const jdsg_nlopt *h, int *iter, int *iter_start, int *tria
 void jdsg_nlopt_destroy(jdsg_nlopt *h);
 const char *jdsg_nlopt_last_error(void);
 
+/* FMST: first-arrival traveltimes and ray Frechet weights of the station
+ * pairs on the GPU.  Replaces the per-pair sequence fmst_init(nx, ny, lat0,
+ * lon0, dlat, dlon) / fmst_run(&vc(0,idx), srcx, srcz, rcx, rcz, nr, ttime) /
+ * fmst_reset(&vout(0,idx)) / fmst_raypath(srcx, srcz, rcx[i], rcz[i], fdm) /
+ * fmst_finalize() (fmst.f90:9, :108, :334, :350, :84) that
+ * SurfTime::travel_time, frechet_matrix and compute_grad run per pair
+ * (SurfaceWave.cpp:524-536, 612-624, 740-752).
+ * jdsg_fmst_create takes host arrays: the grid of fmst_init (degrees); per
+ * pair its period index (get_period_index), source colatitude/longitude
+ * (rad), receivers rcv_off[p] .. rcv_off[p+1]-1 of rcv_colat/rcv_lon (rad),
+ * and ray_time[p] != 0 where the receivers' times are the ray-integrated
+ * group times (Rg, Lg; may be null).
+ * jdsg_fmst_run: vc, vout the device maps (nx*ny, kmax) column-major as
+ * jdsg_surftime_map / _kernel fill them; mode 0 traces rays only for
+ * ray-time pairs (travel_time), mode 1 for every receiver and keeps the
+ * weights (frechet_matrix, compute_grad).  Device outputs: ttime[nrcv] the
+ * receiver times the reference reports, tsyn[nrcv] the ray-integrated times
+ * (0 where no ray was traced), rstatus[nrcv], pstatus[npair] (0 ok; 1 source
+ * or receiver outside the model, where the reference stops; 2 the band
+ * outgrew the reference's heap of NINT(0.5*nnx*nnz) entries; 3 the ray did
+ * not reach the source).  Synchronises the stream.
+ * jdsg_fmst_rays: the last mode-1 run's weights fdm(ix, iy) per receiver in
+ * the sparse form jdsg_frechet_rows takes: device ray_off[nrcv+1], ray_cell
+ * (n = ix + iy*nx increasing), ray_w; fails with *nnz set to the size needed
+ * when that exceeds cap.  Errors: jdsg_fmst_last_error. */
+typedef struct jdsg_fmst jdsg_fmst;
+int jdsg_fmst_create(int nx, int ny, float lat0, float lon0, float dlat, float dlon, int npair,
+                     const int *idx, const float *src_colat, const float *src_lon,
+                     const int *rcv_off, const float *rcv_colat, const float *rcv_lon,
+                     const int *ray_time, jdsg_fmst **h);
+int jdsg_fmst_sizes(const jdsg_fmst *h, int *npair, int *nrcv, int *nplane);
+int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout, int kmax, int mode,
+                  float *ttime, float *tsyn, int *rstatus, int *pstatus, void *stream);
+int jdsg_fmst_rays(jdsg_fmst *h, long long *ray_off, int *ray_cell, float *ray_w, long long cap,
+                   long long *nnz, void *stream);
+void jdsg_fmst_destroy(jdsg_fmst *h);
+const char *jdsg_fmst_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

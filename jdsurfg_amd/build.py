@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libjdsg_swd.so")
 SOURCES = ["swd_chain.hip", "swd_eigen.hip", "swd_surftime.hip", "swd_mlp.hip", "swd_surrogate.hip",
-           "swd_batch.hip", "swd_frechet.hip", "swd_lsmr.hip", "swd_smooth.hip", "swd_nlopt.hip", "swd_io.cpp", "swd_api.cpp"]
+           "swd_batch.hip", "swd_frechet.hip", "swd_lsmr.hip", "swd_smooth.hip", "swd_nlopt.hip", "swd_fmst.hip", "swd_io.cpp", "swd_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

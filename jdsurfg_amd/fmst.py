@@ -1,0 +1,108 @@
+"""FMST on the GPU: first-arrival traveltimes of the station pairs and the
+ray Fréchet weights, the per-pair fmst_init / fmst_run / fmst_reset /
+fmst_raypath sequence of SurfTime::travel_time, frechet_matrix and
+compute_grad (src/SWD/SurfaceWave.cpp:504-676, 695-781; src/SWD/fmst.f90,
+src/SWD/fmm.f90), run for every pair at once by libjdsg_swd.so.
+
+A station pair is a source (colatitude, longitude in rad) with its receivers
+and the period index of its dispersion map (get_period_index order).  The
+maps are the column-major (nx*ny, kmax) vc / vout that SurfTime.map() or
+SurfTime.kernel() leave on the GPU.  run() gives each receiver's time: the
+bilinear traveltime of the fast-marching field (srtimes), or, for pairs whose
+`ray_time` flag is set (Rg, Lg), the group time integrated along the ray;
+mode 1 also keeps every receiver's ray weights, which rays() returns in the
+sparse form jdsurfg_amd.frechet.frechet_rows takes.
+
+Statuses: 0 ok; 1 source or receiver outside the model (the reference
+stops); 2 the fast-marching band outgrew the reference's heap; 3 the ray
+did not reach the source.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import swd
+
+_f32 = np.float32
+
+
+def _L():
+    L = swd.lib()
+    vp = C.c_void_p
+    f = C.c_float
+    L.jdsg_fmst_last_error.restype = C.c_char_p
+    L.jdsg_fmst_create.argtypes = [C.c_int, C.c_int, f, f, f, f, C.c_int] + [vp] * 7 + \
+        [C.POINTER(vp)]
+    L.jdsg_fmst_sizes.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.jdsg_fmst_run.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    L.jdsg_fmst_rays.argtypes = [vp, vp, vp, vp, C.c_longlong, C.POINTER(C.c_longlong), vp]
+    L.jdsg_fmst_destroy.argtypes = [vp]
+    return L
+
+
+def last_error():
+    return _L().jdsg_fmst_last_error().decode()
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Fmst:
+    """Station pairs on one grid (fmst_init's nx, ny, lat0, lon0, dlat,
+    dlon: the model's upper-left node and spacing in degrees)."""
+
+    def __init__(self, nx, ny, lat0, lon0, dlat, dlon, idx, src_colat, src_lon, rcv_off,
+                 rcv_colat, rcv_lon, ray_time=None):
+        L = _L()
+        self.nx, self.ny = nx, ny
+        a = lambda x, t: np.ascontiguousarray(x, t)
+        self._keep = [a(idx, np.int32), a(src_colat, _f32), a(src_lon, _f32),
+                      a(rcv_off, np.int32), a(rcv_colat, _f32), a(rcv_lon, _f32),
+                      a(np.zeros(len(idx)) if ray_time is None else ray_time, np.int32)]
+        self.npair = len(idx)
+        self.nrcv = int(self._keep[3][-1]) if self.npair else 0
+        h = C.c_void_p()
+        if L.jdsg_fmst_create(nx, ny, lat0, lon0, dlat, dlon, self.npair,
+                              *[C.c_void_p(k.ctypes.data) for k in self._keep], C.byref(h)):
+            raise RuntimeError("jdsg_fmst_create: " + last_error())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and swd._lib is not None:
+            swd._lib.jdsg_fmst_destroy(self.h)
+            self.h = None
+
+    def run(self, vc, vout, kmax, mode=0, stream=None):
+        """vc, vout: CUDA float64 maps, plane idx at [idx*nx*ny, (idx+1)*nx*ny).
+        Returns ttime, tsyn (float32 per receiver), rstatus, pstatus."""
+        import torch
+        L = _L()
+        dev = vc.device
+        o = {"ttime": torch.zeros(max(self.nrcv, 1), dtype=torch.float32, device=dev),
+             "tsyn": torch.zeros(max(self.nrcv, 1), dtype=torch.float32, device=dev),
+             "rstatus": torch.zeros(max(self.nrcv, 1), dtype=torch.int32, device=dev),
+             "pstatus": torch.zeros(max(self.npair, 1), dtype=torch.int32, device=dev)}
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        if L.jdsg_fmst_run(self.h, _p(vc), _p(vout), kmax, mode, _p(o["ttime"]), _p(o["tsyn"]),
+                           _p(o["rstatus"]), _p(o["pstatus"]), C.c_void_p(st)):
+            raise RuntimeError("jdsg_fmst_run: " + last_error())
+        return {k: v[:self.nrcv] if k != "pstatus" else v[:self.npair] for k, v in o.items()}
+
+    def rays(self, stream=None):
+        """The sparse ray weights of the last mode-1 run: ray_off (int64,
+        nrcv + 1), ray_cell (int32, n = ix + iy*nx increasing), ray_w."""
+        import torch
+        L = _L()
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        nnz = C.c_longlong()
+        off = torch.zeros(self.nrcv + 1, dtype=torch.int64, device="cuda")
+        if L.jdsg_fmst_rays(self.h, _p(off), None, None, 0, C.byref(nnz), C.c_void_p(st)):
+            if nnz.value <= 0:
+                raise RuntimeError("jdsg_fmst_rays: " + last_error())
+        n = max(int(nnz.value), 1)
+        cell = torch.zeros(n, dtype=torch.int32, device="cuda")
+        w = torch.zeros(n, dtype=torch.float32, device="cuda")
+        if L.jdsg_fmst_rays(self.h, _p(off), _p(cell), _p(w), n, C.byref(nnz), C.c_void_p(st)):
+            raise RuntimeError("jdsg_fmst_rays: " + last_error())
+        return off, cell[:nnz.value], w[:nnz.value]

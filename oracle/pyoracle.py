@@ -466,3 +466,87 @@ def smooth_sph_conv(grad, dx, dy, dz, lat0, lon0, z0, sigma_h, sigma_v):
     nz, ny, nx = g.shape
     _smooth_lib().or_smooth_sph_conv(g, nx, ny, nz, dx, dy, dz, lat0, lon0, z0, sigma_h, sigma_v)
     return g
+
+
+# ---------------------------------------------------------------- FMST
+LIB_FMST_REF = os.path.join(HERE, "_ref", "libfmstref.so")
+
+
+def _fmst_rest():
+    L = C.CDLL(LIB_REST)
+    fp, dp, ip = _fp, _dp, _ip
+    f = C.c_float
+    L.fo_pair.argtypes = [C.c_int, C.c_int, f, f, f, f, dp, dp, f, f, C.c_int, fp, fp, fp, fp,
+                          C.c_void_p]
+    L.fo_pair.restype = C.c_int
+    L.fo_pairs.argtypes = [C.c_int, C.c_int, f, f, f, f, C.c_int, dp, dp, C.c_int, ip, fp, fp,
+                           ip, fp, fp, fp, fp, C.c_void_p, ip, C.c_int]
+    L.fo_pairs.restype = C.c_int
+    return L
+
+
+def fmst_pairs(grid, vc, vout, idx, srcx, srcz, roff, rcx, rcz, want_fdm=True, nthreads=1):
+    """The restated FMST over station pairs, as SurfTime::frechet_matrix
+    drives it (SurfaceWave.cpp:612-624), OpenMP over pairs.
+
+    grid = (nx, ny, lat0, lon0, dlat, dlon); vc, vout: (kmax, nx*ny) maps
+    (column n = ix + iy*nx); idx[p] the pair's period index; srcx/srcz the
+    source colatitude/longitude in rad; roff[p]:roff[p+1] its receivers.
+    Returns ttime (srtimes), tsyn (ray-integrated), fdm (nrcv, nx*ny) or
+    None, status per pair."""
+    nx, ny, lat0, lon0, dlat, dlon = grid
+    L = _fmst_rest()
+    vc = np.ascontiguousarray(vc, _d)
+    vout = np.ascontiguousarray(vout, _d)
+    npair = len(idx)
+    nrcv = int(roff[-1])
+    tt = np.zeros(nrcv, _f)
+    ts = np.zeros(nrcv, _f)
+    fdm = np.zeros((nrcv, nx * ny), _f) if want_fdm else None
+    st = np.zeros(npair, np.int32)
+    L.fo_pairs(nx, ny, lat0, lon0, dlat, dlon, vc.shape[0], vc, vout, npair,
+               np.ascontiguousarray(idx, np.int32), np.ascontiguousarray(srcx, _f),
+               np.ascontiguousarray(srcz, _f), np.ascontiguousarray(roff, np.int32),
+               np.ascontiguousarray(rcx, _f), np.ascontiguousarray(rcz, _f), tt, ts,
+               None if fdm is None else fdm.ctypes.data, st, nthreads)
+    return tt, ts, fdm, st
+
+
+class FmstRef:
+    """The reference fmst_init / fmst_run / fmst_reset / fmst_raypath /
+    fmst_finalize (SWD/fmst.f90) compiled from source, called in
+    SurfTime::frechet_matrix's order.  Its module state is per thread, so
+    one instance is used from one thread."""
+
+    def __init__(self):
+        self.lib = L = C.CDLL(LIB_FMST_REF)
+        f = C.c_float
+        L.fmst_init.argtypes = [C.c_int, C.c_int, f, f, f, f]
+        L.fmst_run.argtypes = [_dp, f, f, _fp, _fp, C.c_int, _fp]
+        L.fmst_reset.argtypes = [_dp]
+        L.fmst_raypath.argtypes = [f, f, f, f, _fp]
+        L.fmst_raypath.restype = f
+        L.fmst_finalize.argtypes = []
+
+    @staticmethod
+    def available():
+        return os.path.exists(LIB_FMST_REF)
+
+    def pair(self, grid, vc_col, vout_col, srcx, srcz, rcx, rcz):
+        nx, ny, lat0, lon0, dlat, dlon = grid
+        L = self.lib
+        rcx = np.ascontiguousarray(rcx, _f)
+        rcz = np.ascontiguousarray(rcz, _f)
+        nr = len(rcx)
+        tt = np.zeros(nr, _f)
+        ts = np.zeros(nr, _f)
+        fdm = np.zeros((nr, nx * ny), _f)
+        L.fmst_init(nx, ny, lat0, lon0, dlat, dlon)
+        L.fmst_run(np.ascontiguousarray(vc_col, _d), srcx, srcz, rcx, rcz, nr, tt)
+        L.fmst_reset(np.ascontiguousarray(vout_col, _d))
+        for i in range(nr):
+            row = np.zeros(nx * ny, _f)
+            ts[i] = L.fmst_raypath(srcx, srcz, rcx[i], rcz[i], row)
+            fdm[i] = row
+        L.fmst_finalize()
+        return tt, ts, fdm
