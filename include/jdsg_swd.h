@@ -174,6 +174,9 @@ int jdsg_surftime_create(int nx, int ny, int nz, const float *dep, const double 
                          int ntLc, int nmLc, const double *tLg, int ntLg, int nmLg,
                          jdsg_surftime **st);
 int jdsg_surftime_kmax(const jdsg_surftime *st, int *kmax);
+/* the grid and period-index count of a handle (SurfTime's nx, ny, depth
+ * size and kmax); any pointer may be null */
+int jdsg_surftime_dims(const jdsg_surftime *st, int *nx, int *ny, int *nz, int *kmax);
 /* get_1d_kernel.  Device pointers: vs (nx,ny,nz) column-major float; vc,
  * vout, status (nx*ny, kmax) and kernel (nx*ny, nz, kmax) column-major
  * (kernel may be null).  Rc/Lc entries hold the spherical phase velocity in
@@ -547,6 +550,28 @@ int jdsg_fmst_rays(jdsg_fmst *h, long long *ray_off, int *ray_cell, float *ray_w
                    long long *nnz, void *stream);
 void jdsg_fmst_destroy(jdsg_fmst *h);
 const char *jdsg_fmst_last_error(void);
+/* The SurfTime methods that run FMST, one call each on the device model vs
+ * (nx,ny,nz) column-major float, with the grid/tables handle and the pairs'
+ * FMST handle (same nx, ny); data[nrcv], rstatus[nrcv] device arrays in the
+ * order of the pairs' receivers (the data vector's order); errors through
+ * jdsg_fmst_last_error.
+ * jdsg_surftime_travel_time   SurfTime::travel_time (SurfaceWave.cpp:509-541):
+ *   get_2d_map, then the receiver times (ray times for ray-time pairs).
+ * jdsg_surftime_frechet       SurfTime::frechet_matrix (:551-676): get_1d_kernel,
+ *   every receiver's ray, data, and the rows in CSR as jdsg_frechet_rows
+ *   writes them (row_off[nrcv+1], col, val, capacity cap, *nnz on the host);
+ *   jdsg_write_frechet then gives the reference's file.
+ * jdsg_surftime_compute_grad  SurfTime::compute_grad (:695-781): as frechet,
+ *   then grad[(nx-2)(ny-2)(nz-1)] = sum of rows times (data - obst), times
+ *   vs at the cell. */
+int jdsg_surftime_travel_time(jdsg_surftime *st, jdsg_fmst *fm, const float *vs, float *data,
+                              int *rstatus, void *stream);
+int jdsg_surftime_frechet(jdsg_surftime *st, jdsg_fmst *fm, const float *vs, float *data,
+                          int *rstatus, long long *row_off, int *col, float *val, long long cap,
+                          long long *nnz, void *stream);
+int jdsg_surftime_compute_grad(jdsg_surftime *st, jdsg_fmst *fm, const float *vs,
+                               const float *obst, float *data, float *grad, int *rstatus,
+                               void *stream);
 
 #ifdef __cplusplus
 }

@@ -1852,6 +1852,15 @@ extern "C" int jdsg_surftime_kmax(const jdsg_surftime *S, int *kmax) {
   return 0;
 }
 
+extern "C" int jdsg_surftime_dims(const jdsg_surftime *S, int *nx, int *ny, int *nz, int *kmax) {
+  if (!S) return fail("surftime_dims: null handle");
+  if (nx) *nx = S->nx;
+  if (ny) *ny = S->ny;
+  if (nz) *nz = S->nz;
+  if (kmax) *kmax = S->kmax;
+  return 0;
+}
+
 extern "C" void jdsg_surftime_destroy(jdsg_surftime *S) { delete S; }
 
 extern "C" int jdsg_surftime_kernel(jdsg_surftime *S, const float *vs, double *vc, double *vout,

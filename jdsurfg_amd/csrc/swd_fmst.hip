@@ -37,6 +37,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/jdsg_swd.h"
+
 #pragma clang fp contract(off)
 
 namespace {
@@ -1321,3 +1323,145 @@ extern "C" int jdsg_fmst_rays(jdsg_fmst *h, long long *ray_off, int *ray_cell, f
 }
 
 extern "C" void jdsg_fmst_destroy(jdsg_fmst *h) { delete h; }
+
+// ---------------------------------------------------------------- SurfTime
+// The three SurfTime methods that run FMST per station pair, as one call
+// each on device buffers: the dispersion maps (and depth kernels) of the
+// model, the pairs' traveltimes and rays, then the Fréchet rows or the
+// misfit gradient.
+namespace {
+
+struct StDims {
+  int nx, ny, nz, kmax;
+};
+
+int st_check(jdsg_surftime *st, jdsg_fmst *fm, StDims *d) {
+  if (!st || !fm) return fail("surftime/fmst: null handle");
+  if (jdsg_surftime_dims(st, &d->nx, &d->ny, &d->nz, &d->kmax))
+    return fail(std::string("surftime: ") + jdsg_last_error());
+  if (d->nx != fm->g.nx || d->ny != fm->g.ny)
+    return fail("surftime/fmst: the FMST grid is not the model's nx, ny");
+  for (int p : fm->plane_idx)
+    if (p >= d->kmax) return fail("surftime/fmst: a pair's period index is not below kmax");
+  return 0;
+}
+
+struct DBuf {
+  std::vector<void *> p;
+  template <class T>
+  T *get(size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return (T *)q;
+  }
+  ~DBuf() {
+    for (void *q : p) (void)hipFree(q);
+  }
+};
+
+// period index of every receiver's row (the pair's), on the device
+int *row_index(jdsg_fmst *fm, DBuf &B) {
+  std::vector<int> ri(fm->nrcv);
+  for (int q = 0; q < fm->nrcv; ++q) ri[q] = fm->pairs[fm->rpair[q]].idx;
+  int *d = B.get<int>(ri.size());
+  if (d && !ri.empty() && hipMemcpy(d, ri.data(), sizeof(int) * ri.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return nullptr;
+  return d;
+}
+
+}  // namespace
+
+// SurfTime::travel_time (SurfaceWave.cpp:509-541): get_2d_map, then each
+// pair's receiver times (ray times for Rg/Lg pairs) into data[nrcv].
+extern "C" int jdsg_surftime_travel_time(jdsg_surftime *st, jdsg_fmst *fm, const float *vs,
+                                         float *data, int *rstatus, void *stream) {
+  StDims d;
+  if (st_check(st, fm, &d)) return 1;
+  DBuf B;
+  const size_t nk = (size_t)d.nx * d.ny * d.kmax;
+  double *vc = B.get<double>(nk), *vout = B.get<double>(nk);
+  int *mst = B.get<int>(nk), *pst = B.get<int>(fm->npair);
+  float *tsyn = B.get<float>(fm->nrcv);
+  if (!vc || !vout || !mst || !pst || !tsyn) return fail("surftime_travel_time: device allocation");
+  if (jdsg_surftime_map(st, vs, vc, vout, mst, stream))
+    return fail(std::string("surftime_map: ") + jdsg_last_error());
+  return jdsg_fmst_run(fm, vc, vout, d.kmax, 0, data, tsyn, rstatus, pst, stream);
+}
+
+// SurfTime::frechet_matrix (SurfaceWave.cpp:551-676): get_1d_kernel, every
+// receiver's ray, data[nrcv] and the rows in CSR (device row_off[nrcv+1],
+// col, val: the layout jdsg_frechet_rows writes, rows in data order).
+extern "C" int jdsg_surftime_frechet(jdsg_surftime *st, jdsg_fmst *fm, const float *vs,
+                                     float *data, int *rstatus, long long *row_off, int *col,
+                                     float *val, long long cap, long long *nnz, void *stream) {
+  StDims d;
+  if (st_check(st, fm, &d)) return 1;
+  DBuf B;
+  const size_t nk = (size_t)d.nx * d.ny * d.kmax;
+  double *vc = B.get<double>(nk), *vout = B.get<double>(nk), *ker = B.get<double>(nk * d.nz);
+  int *mst = B.get<int>(nk), *pst = B.get<int>(fm->npair);
+  float *tsyn = B.get<float>(fm->nrcv);
+  if (!vc || !vout || !ker || !mst || !pst || !tsyn)
+    return fail("surftime_frechet: device allocation");
+  if (hipMemsetAsync(ker, 0, sizeof(double) * nk * d.nz, (hipStream_t)stream) != hipSuccess)
+    return fail("surftime_frechet: memset");
+  if (jdsg_surftime_kernel(st, vs, vc, vout, ker, mst, stream))
+    return fail(std::string("surftime_kernel: ") + jdsg_last_error());
+  if (jdsg_fmst_run(fm, vc, vout, d.kmax, 1, data, tsyn, rstatus, pst, stream)) return 1;
+  long long nray = 0;
+  jdsg_fmst_rays(fm, nullptr, nullptr, nullptr, 0, &nray, stream);
+  long long *roff = B.get<long long>((size_t)fm->nrcv + 1);
+  int *cell = B.get<int>((size_t)nray);
+  float *w = B.get<float>((size_t)nray);
+  int *ridx = row_index(fm, B);
+  if (!roff || !cell || !w || !ridx) return fail("surftime_frechet: device allocation");
+  if (jdsg_fmst_rays(fm, roff, cell, w, nray, &nray, stream)) return 1;
+  if (jdsg_frechet_rows(d.nx, d.ny, d.nz, d.kmax, ker, fm->nrcv, ridx, roff, cell, w, row_off, col,
+                        val, cap, nnz, stream))
+    return fail(std::string("frechet_rows: ") + jdsg_last_error());
+  return 0;
+}
+
+// SurfTime::compute_grad (SurfaceWave.cpp:695-781): get_1d_kernel, every
+// receiver's ray, data[nrcv], and grad[(nx-2)(ny-2)(nz-1)] = sum over rows
+// of row * (data - obst), times vs at the cell.
+extern "C" int jdsg_surftime_compute_grad(jdsg_surftime *st, jdsg_fmst *fm, const float *vs,
+                                          const float *obst, float *data, float *grad,
+                                          int *rstatus, void *stream) {
+  StDims d;
+  if (st_check(st, fm, &d)) return 1;
+  DBuf B;
+  const size_t nk = (size_t)d.nx * d.ny * d.kmax;
+  double *vc = B.get<double>(nk), *vout = B.get<double>(nk), *ker = B.get<double>(nk * d.nz);
+  int *mst = B.get<int>(nk), *pst = B.get<int>(fm->npair);
+  float *tsyn = B.get<float>(fm->nrcv), *resid = B.get<float>(fm->nrcv);
+  if (!vc || !vout || !ker || !mst || !pst || !tsyn || !resid)
+    return fail("surftime_compute_grad: device allocation");
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemsetAsync(ker, 0, sizeof(double) * nk * d.nz, s) != hipSuccess)
+    return fail("surftime_compute_grad: memset");
+  if (jdsg_surftime_kernel(st, vs, vc, vout, ker, mst, stream))
+    return fail(std::string("surftime_kernel: ") + jdsg_last_error());
+  if (jdsg_fmst_run(fm, vc, vout, d.kmax, 1, data, tsyn, rstatus, pst, stream)) return 1;
+  long long nray = 0;
+  jdsg_fmst_rays(fm, nullptr, nullptr, nullptr, 0, &nray, stream);
+  long long *roff = B.get<long long>((size_t)fm->nrcv + 1);
+  int *cell = B.get<int>((size_t)nray);
+  float *w = B.get<float>((size_t)nray);
+  int *ridx = row_index(fm, B);
+  if (!roff || !cell || !w || !ridx) return fail("surftime_compute_grad: device allocation");
+  if (jdsg_fmst_rays(fm, roff, cell, w, nray, &nray, stream)) return 1;
+  // resid = data - obst, in float as the reference forms it (:758)
+  {
+    std::vector<float> a(fm->nrcv), b(fm->nrcv);
+    FCHK(hipMemcpy(a.data(), data, sizeof(float) * fm->nrcv, hipMemcpyDeviceToHost));
+    FCHK(hipMemcpy(b.data(), obst, sizeof(float) * fm->nrcv, hipMemcpyDeviceToHost));
+    for (int q = 0; q < fm->nrcv; ++q) a[q] = a[q] - b[q];
+    FCHK(hipMemcpy(resid, a.data(), sizeof(float) * fm->nrcv, hipMemcpyHostToDevice));
+  }
+  if (jdsg_frechet_grad(d.nx, d.ny, d.nz, d.kmax, ker, fm->nrcv, ridx, roff, cell, w, resid, vs,
+                        grad, stream))
+    return fail(std::string("frechet_grad: ") + jdsg_last_error());
+  return 0;
+}

@@ -37,6 +37,9 @@ def _L():
     L.jdsg_fmst_run.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
     L.jdsg_fmst_rays.argtypes = [vp, vp, vp, vp, C.c_longlong, C.POINTER(C.c_longlong), vp]
     L.jdsg_fmst_destroy.argtypes = [vp]
+    L.jdsg_surftime_travel_time.argtypes = [vp] * 6
+    L.jdsg_surftime_frechet.argtypes = [vp] * 8 + [C.c_longlong, C.POINTER(C.c_longlong), vp]
+    L.jdsg_surftime_compute_grad.argtypes = [vp] * 8
     return L
 
 
@@ -106,3 +109,80 @@ class Fmst:
         if L.jdsg_fmst_rays(self.h, _p(off), _p(cell), _p(w), n, C.byref(nnz), C.c_void_p(st)):
             raise RuntimeError("jdsg_fmst_rays: " + last_error())
         return off, cell[:nnz.value], w[:nnz.value]
+
+
+def _stream(stream):
+    import torch
+    return C.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
+
+
+def travel_time(S, F, vs, stream=None):
+    """SurfTime::travel_time on the GPU: S a swd.SurfTime, F an Fmst on the
+    same grid, vs the CUDA float32 model ((nx, ny, nz) column-major flat).
+    Returns data (float32 per receiver) and rstatus."""
+    import torch
+    L = _L()
+    data = torch.zeros(max(F.nrcv, 1), dtype=torch.float32, device="cuda")
+    rst = torch.zeros(max(F.nrcv, 1), dtype=torch.int32, device="cuda")
+    if L.jdsg_surftime_travel_time(S.h, F.h, _p(vs), _p(data), _p(rst), _stream(stream)):
+        raise RuntimeError("jdsg_surftime_travel_time: " + last_error())
+    return data[:F.nrcv], rst[:F.nrcv]
+
+
+def frechet_matrix(S, F, vs, cap=None, stream=None):
+    """SurfTime::frechet_matrix on the GPU: data, rstatus and the rows in CSR
+    (row_off int64, col int32, val float32), rows in data order."""
+    import torch
+    L = _L()
+    data = torch.zeros(max(F.nrcv, 1), dtype=torch.float32, device="cuda")
+    rst = torch.zeros(max(F.nrcv, 1), dtype=torch.int32, device="cuda")
+    off = torch.zeros(F.nrcv + 1, dtype=torch.int64, device="cuda")
+    nnz = C.c_longlong()
+    if cap is None:
+        cap = F.nrcv * (S.nx - 2) * (S.ny - 2) * (S.nz - 1) // 8 + 1
+    col = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    val = torch.zeros(cap, dtype=torch.float32, device="cuda")
+    if L.jdsg_surftime_frechet(S.h, F.h, _p(vs), _p(data), _p(rst), _p(off), _p(col), _p(val), cap,
+                               C.byref(nnz), _stream(stream)):
+        if nnz.value > cap:
+            return frechet_matrix(S, F, vs, cap=nnz.value, stream=stream)
+        raise RuntimeError("jdsg_surftime_frechet: " + last_error())
+    n = nnz.value
+    return data[:F.nrcv], rst[:F.nrcv], off, col[:n], val[:n]
+
+
+def compute_grad(S, F, vs, obst, stream=None):
+    """SurfTime::compute_grad on the GPU: data, grad ((nx-2)(ny-2)(nz-1)
+    float32), rstatus; obst the observed times (CUDA float32 per receiver)."""
+    import torch
+    L = _L()
+    data = torch.zeros(max(F.nrcv, 1), dtype=torch.float32, device="cuda")
+    rst = torch.zeros(max(F.nrcv, 1), dtype=torch.int32, device="cuda")
+    grad = torch.zeros((S.nx - 2) * (S.ny - 2) * (S.nz - 1), dtype=torch.float32, device="cuda")
+    if L.jdsg_surftime_compute_grad(S.h, F.h, _p(vs), _p(obst), _p(data), _p(grad), _p(rst),
+                                    _stream(stream)):
+        raise RuntimeError("jdsg_surftime_compute_grad: " + last_error())
+    return data[:F.nrcv], grad, rst[:F.nrcv]
+
+
+def period_index(kmax_per_mode, wave, mode, period):
+    """SurfTime::get_period_index (SurfaceWave.cpp:14-48): the map column of
+    a (wave, mode, period) triple.  kmax_per_mode: four arrays (Rc, Rg, Lc,
+    Lg) of the period count of each mode; wave 0 Rc, 1 Rg, 2 Lc, 3 Lg."""
+    base = 0
+    for w in range(wave):
+        base += int(np.sum(kmax_per_mode[w]))
+    return base + int(np.sum(kmax_per_mode[wave][:mode])) + int(period)
+
+
+def from_data(d, nx, ny, lat0, lon0, dlat, dlon):
+    """An Fmst for every station pair of a data file read by
+    jdsurfg_amd.data.read_swd_data, on the model grid (fmst_init's lat0,
+    lon0 = the model's upper-left node, dlat, dlon its spacing); Rg and Lg
+    pairs report the ray time, as SurfTime does."""
+    kpm = [np.asarray(d["tables"][w][1]) for w in ("Rc", "Rg", "Lc", "Lg")]
+    idx = np.array([period_index(kpm, w, m, t) for w, m, t in
+                    zip(d["wave"], d["mode"], d["period"])], np.int32)
+    ray_time = np.isin(d["wave"], (1, 3)).astype(np.int32)
+    return Fmst(nx, ny, lat0, lon0, dlat, dlon, idx, d["src_colat"], d["src_lon"], d["pair_off"],
+                d["rcv_colat"], d["rcv_lon"], ray_time)
