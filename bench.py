@@ -41,14 +41,17 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="grid256",
                     help="grid256 (the default line), grid128, uneven (the 50,000-column "
-                         "batch, replayed graph), lsmr (LSMR on Frechet rows) or smooth "
-                         "(gradient smoothing)")
+                         "batch, replayed graph), lsmr (LSMR on Frechet rows), smooth "
+                         "(gradient smoothing) or fmst (traveltimes and rays of 3040 station "
+                         "pairs)")
     ap.add_argument("--cpu-sample", type=int, default=384,
                     help="columns in the CPU baseline sample (0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surftime-steps", type=int, default=2,
                     help="timed calls of the drop-in SurfTime leg (0 skips it)")
+    ap.add_argument("--fmst-steps", type=int, default=2,
+                    help="timed calls of the FMST leg (0 skips it)")
     return ap.parse_args()
 
 
@@ -335,6 +338,89 @@ def surftime_leg(steps):
             "status_counts": {str(i): int(c) for i, c in enumerate(np.bincount(st)) if c}}
 
 
+def fmst_leg(steps, cpu_pairs, nthreads):
+    """FMST at the reference example's scale (jdsurfg_amd.workloads.
+    station_pairs: 33 x 36 x 41 model, 3040 station pairs, 41,033 receivers,
+    Rayleigh phase and group at 19 periods): the phase/group maps and depth
+    kernels of the model (get_1d_kernel), then every pair's fast-marching
+    traveltimes and every receiver's ray and weights (the FMST loop of
+    SurfTime::frechet_matrix, SurfaceWave.cpp:598-672), inputs resident in
+    HBM.  Timed: the FMST pass alone (jdsg_fmst_run mode 1 + jdsg_fmst_rays)
+    and the whole frechet_matrix call (kernels, FMST, Fréchet rows).  CPU: the
+    reference FMST compiled from source in the reference's OpenMP pair loop
+    (oracle/_ref, all host threads) on the first `cpu_pairs` pairs."""
+    import numpy as np
+    import torch
+    from jdsurfg_amd import fmst, swd
+    from jdsurfg_amd import workloads as W
+    d = W.station_pairs()
+    nx, ny, lat0, lon0, dlat, dlon = d["grid"]
+    S = swd.SurfTime(nx, ny, len(d["dep"]), d["dep"], {w: d["tables"][w][0] for w in ("Rc", "Rg")})
+    vs = torch.from_numpy(np.ascontiguousarray(d["vs"].ravel(order="F"))).cuda()
+    F = fmst.from_data(d, nx, ny, lat0, lon0, dlat, dlon)
+    out = S.alloc(kernel=True)
+    S.kernel(vs, out)
+    torch.cuda.synchronize()
+    res = {}
+
+    def run_fmst():
+        res["o"] = F.run(out["vc"], out["vout"], S.kmax, mode=1)
+        res["r"] = F.rays()
+    ms, _ = timed(run_fmst, steps, 1)
+    nz = int(res["r"][1].numel())
+
+    def run_frechet():
+        res["f"] = fmst.frechet_matrix(S, F, vs)
+    ms_f, _ = timed(run_frechet, steps, 1)
+    st = res["o"]["rstatus"].cpu().numpy()
+    line = {"value": float(F.npair / (ms * 1e-3)), "unit": "station pairs/s",
+            "ms_per_step": float(ms), "steps": steps,
+            "receivers_per_s": float(F.nrcv / (ms * 1e-3)),
+            "frechet_matrix_ms": float(ms_f),
+            "frechet_rows_nonzeros": int(res["f"][3].numel()),
+            "ray_weights": nz,
+            "config": f"FMST, {nx}x{ny} nodes (propagation grid {(nx - 3) * 8 + 1}x{(ny - 3) * 8 + 1}, "
+                      f"source boxes refined 8x), {F.npair} station pairs, {F.nrcv} receivers, "
+                      "38 period maps; times, rays and weights for every receiver",
+            "status_counts": {str(i): int(c) for i, c in enumerate(np.bincount(st)) if c}}
+    if cpu_pairs > 0:
+        from oracle import pyoracle as P
+        n = min(cpu_pairs, F.npair)
+        k = F._keep
+        vc = out["vc"].cpu().numpy().reshape(S.kmax, nx * ny)
+        vo = out["vout"].cpu().numpy().reshape(S.kmax, nx * ny)
+        sub = (k[0][:n], k[1][:n], k[2][:n], k[3][:n + 1], k[4][:k[3][n]], k[5][:k[3][n]])
+        kind = "reference" if P.FmstRef.available() else "port"
+        fn = P.fmst_pairs_ref if kind == "reference" else P.fmst_pairs
+        t0 = time.perf_counter()
+        r = fn(d["grid"], vc, vo, *sub, want_fdm=False, nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        gt = res["o"]["ttime"].cpu().numpy()[:k[3][n]]
+        exp = r[0].copy()
+        for p in range(n):      # Rg pairs report the ray time (SurfaceWave.cpp:616-618)
+            if k[6][p]:
+                exp[k[3][p]:k[3][p + 1]] = r[1][k[3][p]:k[3][p + 1]]
+        line["cpu_baseline"] = {"value": float(n / dt), "unit": "station pairs/s", "cores": nthreads,
+                                "kind": kind,
+                                "sample": f"the first {n} pairs ({k[3][n]} receivers), {dt:.1f} s, "
+                                          f"OpenMP over pairs on {nthreads} host threads",
+                                "gpu_times_identical": bool(np.array_equal(gt, exp))}
+    return line
+
+
+def fmst_main(args):
+    nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    leg = fmst_leg(args.steps, 0 if args.no_cpu else 256, nthr)
+    line = {"metric": "station pairs/s (FMST traveltimes + ray weights, example scale)",
+            "value": leg["value"], "unit": "station pairs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "ms_per_step": leg["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": leg["config"]}, "detail": leg}
+    if "cpu_baseline" in leg:
+        line["cpu_baseline"] = leg.pop("cpu_baseline")
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -348,6 +434,8 @@ def main():
         return smooth_main(args)
     if args.workload == "uneven":
         return uneven_main(args)
+    if args.workload == "fmst":
+        return fmst_main(args)
     import numpy as np
     import torch
     from jdsurfg_amd import swd
@@ -523,6 +611,9 @@ def main():
         line["roofline"]["hw_frac"] = hw / peak
     if args.surftime_steps > 0:
         line["surftime"] = surftime_leg(args.surftime_steps)
+    if args.fmst_steps > 0:
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        line["fmst"] = fmst_leg(args.fmst_steps, 0 if args.no_cpu else 96, nthr)
     if not args.no_cpu and args.cpu_sample > 0:
         nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         res, ncols, ncp = cpu_baseline(w, args.cpu_sample, nthr)

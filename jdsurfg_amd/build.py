@@ -17,20 +17,23 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
 def build(verbose=True):
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     objs = []
-    for src in SOURCES:
-        s = os.path.join(CSRC, src)
-        o = os.path.join(HERE, "lib", src + ".o")
-        lang = ["-x", "hip"]
-        cmd = [HIPCC] + FLAGS + lang + ["-c", s, "-o", o]
-        if verbose:
-            print(" ".join(cmd), flush=True)
+    try:
+        for src in SOURCES:
+            s = os.path.join(CSRC, src)
+            o = os.path.join(HERE, "lib", src + ".o")
+            lang = ["-x", "hip"]
+            cmd = [HIPCC] + FLAGS + lang + ["-c", s, "-o", o]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            objs.append(o)
+            subprocess.run(cmd, check=True)
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
         subprocess.run(cmd, check=True)
-        objs.append(o)
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
-    subprocess.run(cmd, check=True)
-    os.replace(LIB + ".tmp", LIB)
-    for o in objs:
-        os.remove(o)
+        os.replace(LIB + ".tmp", LIB)
+    finally:
+        for o in objs:
+            if os.path.exists(o):
+                os.remove(o)
     return LIB
 
 

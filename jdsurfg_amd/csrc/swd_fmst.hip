@@ -55,7 +55,7 @@ int fail(const std::string &m) {
   } while (0)
 
 constexpr float kPi = 3.1415926535898f;  // globalp: REAL(sp), PARAMETER :: pi
-constexpr int kHeapLds = 2048;            // heap entries held in LDS per pair
+constexpr int kHeapLds = 1536;            // heap entries held in LDS per pair
 constexpr int kRmax = 129;                // refined nodes per axis: 16 cells x 8 + 1
 
 struct __attribute__((aligned(8))) Node {
@@ -128,19 +128,30 @@ fmst_gridder(FGrid g, int nplane, const int *plane_idx, const double *map, long 
 }
 
 // ---------------------------------------------------------------- march
-struct Heap {
-  HEnt *lds;  // entries 1..kHeapLds
-  HEnt *glb;  // entries kHeapLds+1..cap
+// The heap band.  LdsHeap holds it in LDS (the fast path: the band of a
+// 241 x 265 grid stays within a few hundred nodes); a pair whose band
+// outgrows that is marched again with GlobalHeap, in global memory up to
+// the reference's own capacity.
+typedef __attribute__((address_space(3))) HEnt LdsHEnt;  // LDS-typed: ds_* access
+struct LdsHeap {
+  LdsHEnt *e;  // entries 1..cap
   int ntr, cap;
   __device__ __forceinline__ HEnt get(int k) const {
-    return k <= kHeapLds ? lds[k - 1] : glb[k - 1 - kHeapLds];
+    HEnt r;
+    r.v = e[k - 1].v;
+    r.n = e[k - 1].n;
+    return r;
   }
-  __device__ __forceinline__ void put(int k, HEnt e) {
-    if (k <= kHeapLds)
-      lds[k - 1] = e;
-    else
-      glb[k - 1 - kHeapLds] = e;
+  __device__ __forceinline__ void put(int k, HEnt v) {
+    e[k - 1].v = v.v;
+    e[k - 1].n = v.n;
   }
+};
+struct GlobalHeap {
+  HEnt *e;
+  int ntr, cap;
+  __device__ __forceinline__ HEnt get(int k) const { return e[k - 1]; }
+  __device__ __forceinline__ void put(int k, HEnt v) { e[k - 1] = v; }
 };
 
 // a grid being marched: nodes (iz, ix) at nd[(iz-1) + (ix-1)*nnz]
@@ -172,6 +183,7 @@ __device__ __forceinline__ void set_pos(const MGrid &G, int node, int k) {
 }
 
 // addtree (fmm.f90:616-653): the node's traveltime t is already in place
+template <class Heap>
 __device__ bool heap_add(Heap &H, const MGrid &G, int node, float t, Pending &pd) {
   if (H.ntr + 1 > H.cap) return false;
   H.ntr += 1;
@@ -195,6 +207,7 @@ __device__ bool heap_add(Heap &H, const MGrid &G, int node, float t, Pending &pd
 }
 
 // updtree (fmm.f90:745-772) from position k, new traveltime t
+template <class Heap>
 __device__ void heap_up(Heap &H, const MGrid &G, int node, float t, int k, Pending &pd) {
   while (k > 1) {
     const int p = k / 2;
@@ -214,6 +227,7 @@ __device__ void heap_up(Heap &H, const MGrid &G, int node, float t, int k, Pendi
 }
 
 // downtree (fmm.f90:662-738) after the root was taken
+template <class Heap>
 __device__ void heap_pop(Heap &H, const MGrid &G) {
   if (H.ntr == 1) {
     H.ntr = 0;
@@ -394,7 +408,7 @@ __device__ __forceinline__ float fouds2(const MGrid &G, const Node *P, int izp, 
 }
 
 // one neighbour of the accepted node: trial time, then into the heap
-template <int DZ, int DX>
+template <int DZ, int DX, class Heap>
 __device__ __forceinline__ bool visit(Heap &H, const MGrid &G, const Node *P, const float *vn,
                                       const float *rs, int izp, int ixp, int q, Pending &pd) {
   const int iz = izp + DZ, ix = ixp + DX;
@@ -412,6 +426,7 @@ __device__ __forceinline__ bool visit(Heap &H, const MGrid &G, const Node *P, co
 // the fast-marching loop of travel (fmm.f90:210-328); urg = 1 adds the
 // refined grid's exit test (:216-237, comparing the box bounds with the
 // refined node counts, as written).  Returns false on heap overflow.
+template <class Heap>
 __device__ bool march(Heap &H, const MGrid &G, int urg, int vnl, int vnr, int vnt, int vnb) {
   while (H.ntr > 0) {
     const HEnt r = H.get(1);
@@ -502,19 +517,36 @@ struct MarchArgs {
   Node *cnodes;           // per slot nnx*nnz
   Node *rnodes;           // per slot kRmax*kRmax
   float *rvel;            // per slot kRmax*kRmax
-  HEnt *hov;              // per slot hov_stride
+  HEnt *hov;              // GH = true: hov_stride entries per listed pair
   long long hov_stride;
   const float *rcx, *rcz;
   float *ttime;
   int *rstat;
   int *pstat;
+  const int *list;        // GH = true: the chunk slots to march
+  int lds_cap;            // GH = false: heap entries used in LDS (<= kHeapLds)
 };
 
 // fmst_run (fmst.f90:108-332) for one station pair per workgroup.
+template <bool GH>
+using HeapOf = typename std::conditional<GH, GlobalHeap, LdsHeap>::type;
+template <bool GH>
+__device__ __forceinline__ HeapOf<GH> make_heap(HEnt *lheap, const MarchArgs &A, int sl, int maxbt) {
+  if constexpr (GH) {
+    return GlobalHeap{A.hov + (long long)blockIdx.x * A.hov_stride, 0, maxbt};
+  } else {
+    return LdsHeap{(LdsHEnt *)lheap, 0, min(maxbt, A.lds_cap)};
+  }
+}
+
+// GH = false: the heap in LDS, a pair whose band outgrows it ends with
+// status 9 (march it again); GH = true: the heap in global memory, over the
+// pairs listed in A.list.
+template <bool GH>
 __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
-  __shared__ HEnt lheap[kHeapLds];
+  __shared__ HEnt lheap[GH ? 1 : kHeapLds];
   __shared__ int ok;
-  const int sl = blockIdx.x;
+  const int sl = GH ? A.list[blockIdx.x] : (int)blockIdx.x;
   if (sl >= A.npair) return;
   const FPair P = A.pairs[A.p0 + sl];
   const FGrid &g = A.g;
@@ -563,7 +595,7 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
     ok = 1;
     MGrid G{rn, rv, A.risti_r + (long long)(A.p0 + sl) * kRmax, P.nrnx, P.nrnz,
             P.gorx, P.gorz, P.drnx, P.drnz, g.earth};
-    Heap H{lheap, A.hov + (long long)sl * A.hov_stride, 0, P.maxbt};
+    auto H = make_heap<GH>(lheap, A, sl, P.maxbt);
     // the source: bilinear velocity and straight-line times to the four
     // corners of its refined cell (fmm.f90:182-207)
     const int isx = P.isxr, isz = P.iszr;
@@ -587,9 +619,12 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
   }
   __syncthreads();
   if (!ok) {
-    if (lane == 0) A.pstat[A.p0 + sl] = 2;
+    // the reference's heap overflows (2), or only the LDS one (9: again
+    // with the heap in global memory)
+    const int code = (GH || P.maxbt <= A.lds_cap) ? 2 : 9;
+    if (lane == 0) A.pstat[A.p0 + sl] = code;
     for (int r = lane; r < P.nr; r += 64) {
-      A.rstat[P.r0 + r] = 2;
+      A.rstat[P.r0 + r] = code;
       A.ttime[P.r0 + r] = 0.0f;
     }
     return;
@@ -627,7 +662,7 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
   if (lane == 0) {
     MGrid G{cn, A.veln + (long long)P.plane * ncn, A.risti_c, g.nnx, g.nnz,
             g.gox, g.goz, g.dnx, g.dnz, g.earth};
-    Heap H{lheap, A.hov + (long long)sl * A.hov_stride, 0, P.maxbt};
+    auto H = make_heap<GH>(lheap, A, sl, P.maxbt);
     Pending pd{{-1, -1, -1, -1}, {0, 0, 0, 0}};
     // the band from the close nodes, x outer, z inner (fmm.f90:188-197);
     // only box nodes can be close
@@ -640,9 +675,12 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
   }
   __syncthreads();
   if (!ok) {
-    if (lane == 0) A.pstat[A.p0 + sl] = 2;
+    // the reference's heap overflows (2), or only the LDS one (9: again
+    // with the heap in global memory)
+    const int code = (GH || P.maxbt <= A.lds_cap) ? 2 : 9;
+    if (lane == 0) A.pstat[A.p0 + sl] = code;
     for (int r = lane; r < P.nr; r += 64) {
-      A.rstat[P.r0 + r] = 2;
+      A.rstat[P.r0 + r] = code;
       A.ttime[P.r0 + r] = 0.0f;
     }
     return;
@@ -1011,7 +1049,7 @@ struct jdsg_fmst {
   float *d_rcx = nullptr, *d_rcz = nullptr;
   float *d_risti_c = nullptr, *d_risti_r = nullptr;
   int *d_plane_idx = nullptr;
-  long long hov_max = 0;                  // heap overflow entries per slot
+  long long hov_max = 0;                  // largest heap past the LDS capacity (0: none)
   // results of the last run
   std::vector<long long> ray_cnt;         // per receiver nonzero weights (rays kept)
   bool have_rays = false;
@@ -1149,7 +1187,7 @@ extern "C" int jdsg_fmst_create(int nx, int ny, float lat0, float lon0, float dl
     const size_t before = risti_r.size();
     make_pair(h->g, P.srcx, P.srcz, P, risti_r);
     risti_r.resize(before + kRmax, 0.0f);
-    if (P.maxbt > h->hov_max + kHeapLds) h->hov_max = P.maxbt - kHeapLds;
+    if (P.maxbt > kHeapLds) h->hov_max = std::max<long long>(h->hov_max, P.maxbt);
     auto it = std::find(h->plane_idx.begin(), h->plane_idx.end(), P.idx);
     if (it == h->plane_idx.end()) {
       P.plane = (int)h->plane_idx.size();
@@ -1225,8 +1263,7 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
   }
   int maxnr = 0;
   for (const auto &P : h->pairs) maxnr = std::max(maxnr, P.nr);
-  const double slot_bytes = (double)ncn * sizeof(Node) + (double)kRmax * kRmax * (sizeof(Node) + 4) +
-                            (double)h->hov_max * sizeof(HEnt);
+  const double slot_bytes = (double)ncn * sizeof(Node) + (double)kRmax * kRmax * (sizeof(Node) + 4);
   const double rcv_bytes = (double)ncell * 4 + 16;
   size_t fre = 0, tot = 0;
   double budget = (hipMemGetInfo(&fre, &tot) == hipSuccess) ? 0.5 * (double)fre : 4e9;
@@ -1243,7 +1280,6 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
   int chunk = (int)std::min<double>(h->npair, std::max(1.0, budget / per_pair));
   Node *d_cn = nullptr, *d_rn = nullptr;
   float *d_rv = nullptr, *d_fdm = nullptr;
-  HEnt *d_hov = nullptr;
   int *d_rpair = nullptr;
   FCHK(hipMalloc((void **)&d_cn, sizeof(Node) * ncn * chunk));
   fr.p.push_back(d_cn);
@@ -1251,8 +1287,7 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
   fr.p.push_back(d_rn);
   FCHK(hipMalloc((void **)&d_rv, sizeof(float) * kRmax * kRmax * chunk));
   fr.p.push_back(d_rv);
-  FCHK(hipMalloc((void **)&d_hov, sizeof(HEnt) * std::max<long long>(h->hov_max, 1) * chunk));
-  fr.p.push_back(d_hov);
+
   if (!keep) {
     FCHK(hipMalloc((void **)&d_fdm, sizeof(float) * ncell * std::max(1, maxnr) * chunk));
     fr.p.push_back(d_fdm);
@@ -1263,10 +1298,38 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
   for (int p0 = 0; p0 < h->npair; p0 += chunk) {
     const int np = std::min(chunk, h->npair - p0);
     MarchArgs M{g, h->d_pairs, p0, np, vc, nxy, d_veln, h->d_risti_c, h->d_risti_r, d_cn, d_rn,
-                d_rv, d_hov, std::max<long long>(h->hov_max, 1), h->d_rcx, h->d_rcz, ttime,
-                rstatus, pstatus};
-    hipLaunchKernelGGL(fmst_march, dim3(np), dim3(64), 0, st, M);
+                d_rv, nullptr, 0, h->d_rcx, h->d_rcz, ttime, rstatus, pstatus, nullptr, kHeapLds};
+    // a smaller LDS heap sends more pairs to the global-memory heap (tests)
+    if (const char *e = getenv("JDSG_FMST_LDS_HEAP")) M.lds_cap = std::max(4, std::min(kHeapLds, atoi(e)));
+    hipLaunchKernelGGL(fmst_march<false>, dim3(np), dim3(64), 0, st, M);
     FCHK(hipGetLastError());
+    // pairs whose band outgrew the LDS heap: again with the heap in global memory
+    if (h->hov_max > 0 || M.lds_cap < kHeapLds) {
+      std::vector<int> ps(np);
+      FCHK(hipMemcpyAsync(ps.data(), pstatus + p0, sizeof(int) * np, hipMemcpyDeviceToHost, st));
+      FCHK(hipStreamSynchronize(st));
+      std::vector<int> again;
+      for (int i = 0; i < np; ++i)
+        if (ps[i] == 9) again.push_back(i);
+      if (!again.empty()) {
+        long long stride = 1;
+        for (int i : again) stride = std::max<long long>(stride, h->pairs[p0 + i].maxbt);
+        HEnt *d_hov = nullptr;
+        int *d_list = nullptr;
+        FCHK(hipMalloc((void **)&d_hov, sizeof(HEnt) * stride * again.size()));
+        fr.p.push_back(d_hov);
+        FCHK(hipMalloc((void **)&d_list, sizeof(int) * again.size()));
+        fr.p.push_back(d_list);
+        FCHK(hipMemcpyAsync(d_list, again.data(), sizeof(int) * again.size(), hipMemcpyHostToDevice,
+                            st));
+        MarchArgs MG = M;
+        MG.hov = d_hov;
+        MG.hov_stride = stride;
+        MG.list = d_list;
+        hipLaunchKernelGGL(fmst_march<true>, dim3((unsigned)again.size()), dim3(64), 0, st, MG);
+        FCHK(hipGetLastError());
+      }
+    }
     // the chunk's traced receivers, contiguous from the first pair's
     const int q0 = h->pairs[p0].r0;
     const int q1 = h->pairs[p0 + np - 1].r0 + h->pairs[p0 + np - 1].nr;

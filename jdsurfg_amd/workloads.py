@@ -224,3 +224,50 @@ def random_columns(ncol, nlay, seed, nmode=(1, 1), group=True, periods=None,
 
 
 BUILDERS = {"grid20": grid20, "grid128": grid128, "grid256": grid256, "uneven": uneven}
+
+
+def station_pairs(seed=3046, nx=33, ny=36, lat0=35.3, lon0=99.7, dlat=0.3, dlon=0.3, nz=41,
+                  nsrc=80, nsta=240, nrcv=(13, 14)):
+    """The travel-time workload at the reference example's scale
+    (example/MOD: 33 x 36 nodes from 35.3 N, 99.7 E at 0.3 deg, 41 depth
+    nodes; example/surfdataSC.dat: 3046 station pairs, 41,068 times, Rayleigh
+    phase and group at 19 periods 4-40 s).  A seeded synthetic model (depth
+    trend plus smooth lateral variation), 240 stations inside the grid,
+    `nsrc` sources each paired with 13-14 receivers at every one of the 38
+    period indices (Rc periods 0-18, then Rg).  Returns a dict in the form
+    jdsurfg_amd.data.read_swd_data gives, plus grid, dep and vs."""
+    rng = np.random.default_rng(seed)
+    dep = np.linspace(0.0, 80.0, nz).astype(np.float32)
+    ix = np.arange(nx)[:, None, None] / nx
+    iy = np.arange(ny)[None, :, None] / ny
+    z = dep[None, None, :]
+    vs = (2.9 + 0.02 * z + 0.12 * np.sin(2 * np.pi * 1.2 * ix + 0.3) * np.cos(2 * np.pi * 0.9 * iy) *
+          np.exp(-z / 60.0)).astype(np.float32)
+    per = np.arange(4.0, 41.0, 2.0)
+    tables = {"Rc": (per[:, None].copy(), np.array([len(per)], np.int32)),
+              "Rg": (per[:, None].copy(), np.array([len(per)], np.int32)),
+              "Lc": (np.zeros((0, 0)), np.zeros(0, np.int32)),
+              "Lg": (np.zeros((0, 0)), np.zeros(0, np.int32))}
+    d2r = np.pi / 180.0
+    lat = rng.uniform(lat0 - dlat * (nx - 2) + 0.05, lat0 - dlat - 0.05, nsta)
+    lon = rng.uniform(lon0 + dlon + 0.05, lon0 + dlon * (ny - 2) - 0.05, nsta)
+    colat = ((90.0 - lat) * d2r).astype(np.float32)
+    lonr = (lon * d2r).astype(np.float32)
+    wave, mode, period, sc, sl, rc, rl, off = [], [], [], [], [], [], [], [0]
+    for s in range(nsrc):
+        for k in range(2 * len(per)):
+            n = int(rng.integers(nrcv[0], nrcv[1] + 1))
+            others = rng.choice(np.delete(np.arange(nsta), s), n, replace=False)
+            wave.append(0 if k < len(per) else 1)
+            mode.append(0)
+            period.append(k % len(per))
+            sc.append(colat[s])
+            sl.append(lonr[s])
+            rc += list(colat[others])
+            rl += list(lonr[others])
+            off.append(off[-1] + n)
+    return {"grid": (nx, ny, lat0, lon0, dlat, dlon), "dep": dep, "vs": vs, "tables": tables,
+            "pair_off": np.array(off, np.int32), "wave": np.array(wave, np.int32),
+            "mode": np.array(mode, np.int32), "period": np.array(period, np.int32),
+            "src_colat": np.array(sc, np.float32), "src_lon": np.array(sl, np.float32),
+            "rcv_colat": np.array(rc, np.float32), "rcv_lon": np.array(rl, np.float32)}

@@ -550,3 +550,28 @@ class FmstRef:
             fdm[i] = row
         L.fmst_finalize()
         return tt, ts, fdm
+
+
+def fmst_pairs_ref(grid, vc, vout, idx, srcx, srcz, roff, rcx, rcz, want_fdm=True, nthreads=1):
+    """fmst_pairs with the reference FMST compiled from source, driven by the
+    OpenMP pair loop of SurfTime::frechet_matrix (oracle/fmst_ref_driver.c);
+    same arguments and results (statuses are not reported: the reference
+    stops on a station outside the model)."""
+    nx, ny, lat0, lon0, dlat, dlon = grid
+    L = C.CDLL(LIB_FMST_REF)
+    f = C.c_float
+    L.fr_pairs.argtypes = [C.c_int, C.c_int, f, f, f, f, _dp, _dp, C.c_int, _ip, _fp, _fp, _ip,
+                           _fp, _fp, _fp, _fp, C.c_void_p, C.c_void_p, C.c_int]
+    vc = np.ascontiguousarray(vc, _d)
+    vout = np.ascontiguousarray(vout, _d)
+    nrcv = int(roff[-1])
+    tt = np.zeros(nrcv, _f)
+    ts = np.zeros(nrcv, _f)
+    fdm = np.zeros((nrcv, nx * ny), _f) if want_fdm else None
+    scratch = np.zeros(max(nthreads, 1) * nx * ny, _f)
+    L.fr_pairs(nx, ny, lat0, lon0, dlat, dlon, vc, vout, len(idx),
+               np.ascontiguousarray(idx, np.int32), np.ascontiguousarray(srcx, _f),
+               np.ascontiguousarray(srcz, _f), np.ascontiguousarray(roff, np.int32),
+               np.ascontiguousarray(rcx, _f), np.ascontiguousarray(rcz, _f), tt, ts,
+               None if fdm is None else fdm.ctypes.data, scratch.ctypes.data, nthreads)
+    return tt, ts, fdm

@@ -161,3 +161,15 @@ def test_outside_source_and_receiver_report_status():
     ok[0] = False
     assert (g["rstatus"][ok] == 0).all()
     assert np.array_equal(g["ttime"][ok], c["ttime"][ok])
+
+
+def test_global_memory_heap_matches(monkeypatch):
+    """Pairs whose fast-marching band outgrows the heap kept in LDS are
+    marched again with the heap in global memory; forcing a 64-entry LDS
+    heap sends every pair there, and the results stay the reference's."""
+    monkeypatch.setenv("JDSG_FMST_LDS_HEAP", "64")
+    for name in CASES:
+        c = _case(name)
+        print(f"case fmst {name} with the global-memory heap", flush=True)
+        g = _gpu(c, mode=1)
+        _compare(name + " (global heap)", g, c, int(c["nx"]) * int(c["ny"]))
