@@ -168,6 +168,22 @@ __device__ __forceinline__ long long nid(const MGrid &G, int iz, int ix) {
 }
 __device__ __forceinline__ int pack(int iz, int ix) { return (ix << 16) | iz; }
 
+// The march runs on the whole wavefront with wave-uniform control: heap
+// entries read from LDS are made uniform (readfirstlane), so the heap's
+// sifts branch on scalars; every lane performs the same stores, so each
+// lane's later loads follow its own stores in program order.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+template <class Heap>
+__device__ __forceinline__ HEnt hget(const Heap &H, int k) {
+  HEnt e = H.get(k);
+  e.v = uni(e.v);
+  e.n = uni(e.n);
+  return e;
+}
+
 // the up to four neighbours whose heap positions a sift must keep current
 struct Pending {
   int n[4], pos[4];
@@ -190,7 +206,7 @@ __device__ bool heap_add(Heap &H, const MGrid &G, int node, float t, Pending &pd
   int k = H.ntr;
   while (k > 1) {
     const int p = k / 2;
-    const HEnt e = H.get(p);
+    const HEnt e = hget(H, p);
     if (t < e.v) {
       H.put(k, e);
       set_pos(G, e.n, k);
@@ -211,7 +227,7 @@ template <class Heap>
 __device__ void heap_up(Heap &H, const MGrid &G, int node, float t, int k, Pending &pd) {
   while (k > 1) {
     const int p = k / 2;
-    const HEnt e = H.get(p);
+    const HEnt e = hget(H, p);
     if (t < e.v) {
       H.put(k, e);
       set_pos(G, e.n, k);
@@ -233,13 +249,13 @@ __device__ void heap_pop(Heap &H, const MGrid &G) {
     H.ntr = 0;
     return;
   }
-  const HEnt x = H.get(H.ntr);
+  const HEnt x = hget(H, H.ntr);
   H.ntr -= 1;
   const int ntr = H.ntr;
   int hole = 1, c = 2;
   while (c < ntr) {
-    HEnt a = H.get(c);
-    const HEnt b = H.get(c + 1);
+    HEnt a = hget(H, c);
+    const HEnt b = hget(H, c + 1);
     if (a.v > b.v) {
       c = c + 1;
       a = b;
@@ -254,7 +270,7 @@ __device__ void heap_pop(Heap &H, const MGrid &G) {
     }
   }
   if (c == ntr) {
-    const HEnt a = H.get(c);
+    const HEnt a = hget(H, c);
     if (a.v < x.v) {
       H.put(hole, a);
       set_pos(G, a.n, hole);
@@ -265,171 +281,118 @@ __device__ void heap_pop(Heap &H, const MGrid &G) {
   set_pos(G, x.n, hole);
 }
 
-// the update stencil of the four neighbours of an accepted node (izp, ixp):
-// the 25 nodes with |dz| + |dx| <= 3, slot(dz, dx) row by row
-__host__ __device__ constexpr int slot(int dz, int dx) {
-  return (dz == -3 ? 0 : dz == -2 ? 1 : dz == -1 ? 4 : dz == 0 ? 9 : dz == 1 ? 16 : dz == 2 ? 21 : 24) +
-         dx + (3 - (dz < 0 ? -dz : dz));
-}
-
-// fouds2 (fmm.f90:437-608) at the neighbour (izp + DZ, ixp + DX), reading
-// the stencil P; the j loop runs over x, the k loop over z, as written
-template <int DZ, int DX>
-__device__ __forceinline__ float fouds2(const MGrid &G, const Node *P, int izp, int ixp,
-                                        float vel, float risti) {
-  const int iz = izp + DZ, ix = ixp + DX;
-  int tsw1 = 0;
-  float travm = 0.0f;
+// One quadrant (j = ix + sj, k = iz + sk) of fouds2 (fmm.f90:437-608) at
+// node (iz, ix): the trial time of its solution, or +inf where the quadrant
+// gives none (out of the grid, or no alive upwind node).  The reference's
+// minimum over the quadrants does not depend on their order.
+__device__ __forceinline__ float fouds2_quadrant(const MGrid &G, int iz, int ix, int sj, int sk) {
+  const float kInf = __int_as_float(0x7f800000);
+  const int j = ix + sj, k = iz + sk;
+  if (!(j >= 1 && j <= G.nnx) || !(k >= 1 && k <= G.nnz)) return kInf;
+  const int j2 = j + sj, k2 = k + sk;
+  const bool j2in = sj < 0 ? (j2 >= 1) : (j2 <= G.nnx);
+  const bool k2in = sk < 0 ? (k2 >= 1) : (k2 <= G.nnz);
+  const Node nj = G.nd[nid(G, iz, j)];
+  const Node nk = G.nd[nid(G, k, ix)];
+  const Node nj2 = j2in ? G.nd[nid(G, iz, j2)] : Node{0.0f, -2};
+  const Node nk2 = k2in ? G.nd[nid(G, k2, ix)] : Node{0.0f, -2};
+  const float vel = G.vel[nid(G, iz, ix)];
+  const float risti = G.risti[ix - 1];
   const float slown = 1.0f / vel;
   const float ri = G.earth;
-#pragma unroll
-  for (int sj = -1; sj <= 1; sj += 2) {
-    const int j = ix + sj;
-    if (!(j >= 1 && j <= G.nnx)) continue;
-    const Node nj = P[slot(DZ, DX + sj)];
-    const int j2 = j + sj;
-    const Node nj2 = P[slot(DZ, DX + 2 * sj)];
-    int swj = -1;
-    if (sj < 0 ? (j2 >= 1) : (j2 <= G.nnx))
-      if (nj2.s == 0) swj = 0;
-    if (nj.s == 0 && swj == 0) {
-      swj = -1;
-      if (nj.t > nj2.t) swj = 0;
+  const int swj = (nj.s == 0 && j2in && nj2.s == 0 && nj.t > nj2.t) ? 0 : -1;
+  const int swk = (nk.s == 0 && k2in && nk2.s == 0 && nk.t > nk2.t) ? 0 : -1;
+  float a, b, c, u, v, em, tref, tdiv;
+  if (swj == 0) {
+    if (swk == 0) {
+      u = 2.0f * ri * G.dnx;
+      v = 2.0f * risti * G.dnz;
+      em = 4.0f * nj.t - nj2.t - 4.0f * nk.t;
+      em = em + nk2.t;
+      a = v * v + u * u;
+      b = 2.0f * em * (u * u);
+      c = (u * u) * (em * em - (slown * slown) * (v * v));
+      tref = 4.0f * nj.t - nj2.t;
+      tdiv = 3.0f;
+    } else if (nk.s == 0) {
+      u = risti * G.dnz;
+      v = 2.0f * ri * G.dnx;
+      em = 3.0f * nk.t - 4.0f * nj.t + nj2.t;
+      a = v * v + 9.0f * (u * u);
+      b = 6.0f * em * (u * u);
+      c = (u * u) * (em * em - (slown * slown) * (v * v));
+      tref = nk.t;
+      tdiv = 1.0f;
     } else {
-      swj = -1;
+      u = 2.0f * ri * G.dnx;
+      a = 1.0f;
+      b = 0.0f;
+      c = -((u * u) * (slown * slown));
+      tref = 4.0f * nj.t - nj2.t;
+      tdiv = 3.0f;
     }
-#pragma unroll
-    for (int sk = -1; sk <= 1; sk += 2) {
-      const int k = iz + sk;
-      if (!(k >= 1 && k <= G.nnz)) continue;
-      const Node nk = P[slot(DZ + sk, DX)];
-      const int k2 = k + sk;
-      const Node nk2 = P[slot(DZ + 2 * sk, DX)];
-      int swk = -1;
-      if (sk < 0 ? (k2 >= 1) : (k2 <= G.nnz))
-        if (nk2.s == 0) swk = 0;
-      if (nk.s == 0 && swk == 0) {
-        swk = -1;
-        if (nk.t > nk2.t) swk = 0;
-      } else {
-        swk = -1;
-      }
-      int swsol = 0;
-      float a = 0.0f, b = 0.0f, c = 0.0f, u, v, em, tref = 0.0f, tdiv = 1.0f;
-      if (swj == 0) {
-        swsol = 1;
-        if (swk == 0) {
-          u = 2.0f * ri * G.dnx;
-          v = 2.0f * risti * G.dnz;
-          em = 4.0f * nj.t - nj2.t - 4.0f * nk.t;
-          em = em + nk2.t;
-          a = v * v + u * u;
-          b = 2.0f * em * (u * u);
-          c = (u * u) * (em * em - (slown * slown) * (v * v));
-          tref = 4.0f * nj.t - nj2.t;
-          tdiv = 3.0f;
-        } else if (nk.s == 0) {
-          u = risti * G.dnz;
-          v = 2.0f * ri * G.dnx;
-          em = 3.0f * nk.t - 4.0f * nj.t + nj2.t;
-          a = v * v + 9.0f * (u * u);
-          b = 6.0f * em * (u * u);
-          c = (u * u) * (em * em - (slown * slown) * (v * v));
-          tref = nk.t;
-          tdiv = 1.0f;
-        } else {
-          u = 2.0f * ri * G.dnx;
-          a = 1.0f;
-          b = 0.0f;
-          c = -((u * u) * (slown * slown));
-          tref = 4.0f * nj.t - nj2.t;
-          tdiv = 3.0f;
-        }
-      } else if (nj.s == 0) {
-        swsol = 1;
-        if (swk == 0) {
-          u = ri * G.dnx;
-          v = 2.0f * risti * G.dnz;
-          em = 3.0f * nj.t - 4.0f * nk.t + nk2.t;
-          a = v * v + 9.0f * (u * u);
-          b = 6.0f * em * (u * u);
-          c = (u * u) * (em * em - (v * v) * (slown * slown));
-          tref = nj.t;
-          tdiv = 1.0f;
-        } else if (nk.s == 0) {
-          u = ri * G.dnx;
-          v = risti * G.dnz;
-          em = nk.t - nj.t;
-          a = u * u + v * v;
-          b = -(2.0f * (u * u) * em);
-          c = (u * u) * (em * em - (v * v) * (slown * slown));
-          tref = nj.t;
-          tdiv = 1.0f;
-        } else {
-          a = 1.0f;
-          b = 0.0f;
-          c = -((slown * slown) * (ri * ri) * (G.dnx * G.dnx));
-          tref = nj.t;
-          tdiv = 1.0f;
-        }
-      } else {
-        if (swk == 0) {
-          swsol = 1;
-          u = 2.0f * risti * G.dnz;
-          a = 1.0f;
-          b = 0.0f;
-          c = -((u * u) * (slown * slown));
-          tref = 4.0f * nk.t - nk2.t;
-          tdiv = 3.0f;
-        } else if (nk.s == 0) {
-          swsol = 1;
-          a = 1.0f;
-          b = 0.0f;
-          c = -((slown * slown) * (risti * risti) * (G.dnz * G.dnz));
-          tref = nk.t;
-          tdiv = 1.0f;
-        }
-      }
-      if (swsol == 1) {
-        float rd1 = b * b - 4.0f * a * c;
-        if (rd1 < 0.0f) rd1 = 0.0f;
-        const float tdsh = (-b + sqrtf(rd1)) / (2.0f * a);
-        const float trav = (tref + tdsh) / tdiv;
-        if (tsw1 == 1) {
-          travm = (trav < travm) ? trav : travm;
-        } else {
-          travm = trav;
-          tsw1 = 1;
-        }
-      }
+  } else if (nj.s == 0) {
+    if (swk == 0) {
+      u = ri * G.dnx;
+      v = 2.0f * risti * G.dnz;
+      em = 3.0f * nj.t - 4.0f * nk.t + nk2.t;
+      a = v * v + 9.0f * (u * u);
+      b = 6.0f * em * (u * u);
+      c = (u * u) * (em * em - (v * v) * (slown * slown));
+      tref = nj.t;
+      tdiv = 1.0f;
+    } else if (nk.s == 0) {
+      u = ri * G.dnx;
+      v = risti * G.dnz;
+      em = nk.t - nj.t;
+      a = u * u + v * v;
+      b = -(2.0f * (u * u) * em);
+      c = (u * u) * (em * em - (v * v) * (slown * slown));
+      tref = nj.t;
+      tdiv = 1.0f;
+    } else {
+      a = 1.0f;
+      b = 0.0f;
+      c = -((slown * slown) * (ri * ri) * (G.dnx * G.dnx));
+      tref = nj.t;
+      tdiv = 1.0f;
     }
+  } else if (swk == 0) {
+    u = 2.0f * risti * G.dnz;
+    a = 1.0f;
+    b = 0.0f;
+    c = -((u * u) * (slown * slown));
+    tref = 4.0f * nk.t - nk2.t;
+    tdiv = 3.0f;
+  } else if (nk.s == 0) {
+    a = 1.0f;
+    b = 0.0f;
+    c = -((slown * slown) * (risti * risti) * (G.dnz * G.dnz));
+    tref = nk.t;
+    tdiv = 1.0f;
+  } else {
+    return kInf;
   }
-  return travm;
-}
-
-// one neighbour of the accepted node: trial time, then into the heap
-template <int DZ, int DX, class Heap>
-__device__ __forceinline__ bool visit(Heap &H, const MGrid &G, const Node *P, const float *vn,
-                                      const float *rs, int izp, int ixp, int q, Pending &pd) {
-  const int iz = izp + DZ, ix = ixp + DX;
-  if (DX != 0 && !(ix >= 1 && ix <= G.nnx)) return true;
-  if (DZ != 0 && !(iz >= 1 && iz <= G.nnz)) return true;
-  const Node me = P[slot(DZ, DX)];
-  if (me.s == 0) return true;
-  const float t = fouds2<DZ, DX>(G, P, izp, ixp, vn[q], rs[DX + 1]);
-  G.nd[nid(G, iz, ix)].t = t;
-  if (me.s == -1) return heap_add(H, G, pack(iz, ix), t, pd);
-  heap_up(H, G, pack(iz, ix), t, pd.pos[q], pd);
-  return true;
+  float rd1 = b * b - 4.0f * a * c;
+  if (rd1 < 0.0f) rd1 = 0.0f;
+  const float tdsh = (-b + sqrtf(rd1)) / (2.0f * a);
+  return (tref + tdsh) / tdiv;
 }
 
 // the fast-marching loop of travel (fmm.f90:210-328); urg = 1 adds the
 // refined grid's exit test (:216-237, comparing the box bounds with the
-// refined node counts, as written).  Returns false on heap overflow.
+// refined node counts, as written).  Each accepted node's four neighbours
+// take their trial times from 16 lanes, one quadrant each, reduced to the
+// minimum per neighbour; the heap updates follow in the reference's order
+// (x - 1, x + 1, z - 1, z + 1).  Returns false on heap overflow.
 template <class Heap>
 __device__ bool march(Heap &H, const MGrid &G, int urg, int vnl, int vnr, int vnt, int vnb) {
+  const int lane = threadIdx.x & 15;
+  const int q = lane >> 2;                                  // this lane's neighbour
+  const int sj = (lane & 1) ? 1 : -1, sk = (lane & 2) ? 1 : -1;  // and quadrant
+  const int qdz = (q == 2) ? -1 : (q == 3) ? 1 : 0, qdx = (q == 0) ? -1 : (q == 1) ? 1 : 0;
   while (H.ntr > 0) {
-    const HEnt r = H.get(1);
+    const HEnt r = hget(H, 1);
     const int ix = r.n >> 16, iz = r.n & 0xffff;
     if (urg == 1) {
       int swrg = 0;
@@ -444,47 +407,44 @@ __device__ bool march(Heap &H, const MGrid &G, int urg, int vnl, int vnr, int vn
     }
     G.nd[nid(G, iz, ix)].s = 0;
     heap_pop(H, G);
-    // the stencil of the four neighbours, read in one batch
-    Node P[25];
-#pragma unroll
-    for (int dz = -3; dz <= 3; ++dz) {
-      const int w = 3 - (dz < 0 ? -dz : dz);
-#pragma unroll
-      for (int dx = -w; dx <= w; ++dx) {
-        const int jz = iz + dz, jx = ix + dx;
-        Node v{0.0f, -2};
-        if (jz >= 1 && jz <= G.nnz && jx >= 1 && jx <= G.nnx) v = G.nd[nid(G, jz, jx)];
-        P[slot(dz, dx)] = v;
-      }
+    // each lane: its neighbour's state and one quadrant's trial time
+    const int nz = iz + qdz, nx = ix + qdx;
+    const bool in = nz >= 1 && nz <= G.nnz && nx >= 1 && nx <= G.nnx;
+    int st = -2;
+    float cand = __int_as_float(0x7f800000);
+    if (in) {
+      st = G.nd[nid(G, nz, nx)].s;
+      if (st != 0) cand = fouds2_quadrant(G, nz, nx, sj, sk);
     }
-    P[slot(0, 0)].s = 0;
-    float vn[4], rs[3];
     {
-      const int jx[4] = {ix - 1, ix + 1, ix, ix}, jz[4] = {iz, iz, iz - 1, iz + 1};
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const bool in = jz[q] >= 1 && jz[q] <= G.nnz && jx[q] >= 1 && jx[q] <= G.nnx;
-        vn[q] = in ? G.vel[nid(G, jz[q], jx[q])] : 1.0f;
-      }
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int c = ix - 1 + q;
-        rs[q] = (c >= 1 && c <= G.nnx) ? G.risti[c - 1] : 0.0f;
-      }
+      float o = __shfl_xor(cand, 1);
+      cand = (o < cand) ? o : cand;
+      o = __shfl_xor(cand, 2);
+      cand = (o < cand) ? o : cand;
     }
     Pending pd;
+    float tq[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      tq[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cand), 4 * i));
+      pd.pos[i] = __builtin_amdgcn_readlane(st, 4 * i);
+    }
     pd.n[0] = pack(iz, ix - 1);
     pd.n[1] = pack(iz, ix + 1);
     pd.n[2] = pack(iz - 1, ix);
     pd.n[3] = pack(iz + 1, ix);
-    pd.pos[0] = P[slot(0, -1)].s;
-    pd.pos[1] = P[slot(0, 1)].s;
-    pd.pos[2] = P[slot(-1, 0)].s;
-    pd.pos[3] = P[slot(1, 0)].s;
-    if (!visit<0, -1>(H, G, P, vn, rs, iz, ix, 0, pd)) return false;
-    if (!visit<0, 1>(H, G, P, vn, rs, iz, ix, 1, pd)) return false;
-    if (!visit<-1, 0>(H, G, P, vn, rs, iz, ix, 2, pd)) return false;
-    if (!visit<1, 0>(H, G, P, vn, rs, iz, ix, 3, pd)) return false;
+    const int sti[4] = {pd.pos[0], pd.pos[1], pd.pos[2], pd.pos[3]};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (sti[i] == 0 || sti[i] == -2) continue;
+      const int niz = pd.n[i] & 0xffff, nix = pd.n[i] >> 16;
+      G.nd[nid(G, niz, nix)].t = tq[i];
+      if (sti[i] == -1) {
+        if (!heap_add(H, G, pd.n[i], tq[i], pd)) return false;
+      } else {
+        heap_up(H, G, pd.n[i], tq[i], pd.pos[i], pd);
+      }
+    }
   }
   return true;
 }
@@ -590,9 +550,10 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
       rn[q] = Node{0.0f, -1};
     }
   }
+  if (lane == 0) ok = 1;
   __syncthreads();
-  if (lane == 0) {
-    ok = 1;
+  // the marches run on the whole wavefront with uniform control (see march)
+  {
     MGrid G{rn, rv, A.risti_r + (long long)(A.p0 + sl) * kRmax, P.nrnx, P.nrnz,
             P.gorx, P.gorz, P.drnx, P.drnz, g.earth};
     auto H = make_heap<GH>(lheap, A, sl, P.maxbt);
@@ -602,7 +563,7 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
     const float scx = P.srcx, scz = P.srcz;
     float vss[2][2];
     for (int i = 1; i <= 2; ++i)
-      for (int j = 1; j <= 2; ++j) vss[i - 1][j - 1] = rv[nid(G, isz - 1 + j, isx - 1 + i)];
+      for (int j = 1; j <= 2; ++j) vss[i - 1][j - 1] = uni(rv[nid(G, isz - 1 + j, isx - 1 + i)]);
     const float dsx = (scx - G.gox) - (float)(isx - 1) * G.dnx;
     const float dsz = (scz - G.goz) - (float)(isz - 1) * G.dnz;
     const float vsrc = bilin(vss[0][0], vss[0][1], vss[1][0], vss[1][1], dsx, dsz, G.dnx, G.dnz);
@@ -659,7 +620,7 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
     }
   }
   __syncthreads();
-  if (lane == 0) {
+  {
     MGrid G{cn, A.veln + (long long)P.plane * ncn, A.risti_c, g.nnx, g.nnz,
             g.gox, g.goz, g.dnx, g.dnz, g.earth};
     auto H = make_heap<GH>(lheap, A, sl, P.maxbt);
@@ -669,7 +630,7 @@ __global__ void __launch_bounds__(64) fmst_march(MarchArgs A) {
     for (int i = P.vnl; i <= P.vnr && ok; ++i)
       for (int j = P.vnt; j <= P.vnb && ok; ++j) {
         const Node c = cn[nid(G, j, i)];
-        if (c.s > 0 && !heap_add(H, G, pack(j, i), c.t, pd)) ok = 0;
+        if (uni(c.s) > 0 && !heap_add(H, G, pack(j, i), uni(c.t), pd)) ok = 0;
       }
     if (ok && !march(H, G, 2, 0, 0, 0, 0)) ok = 0;
   }
