@@ -282,26 +282,43 @@ __device__ void heap_pop(Heap &H, const MGrid &G) {
 }
 
 // One quadrant (j = ix + sj, k = iz + sk) of fouds2 (fmm.f90:437-608) at
-// node (iz, ix): the trial time of its solution, or +inf where the quadrant
-// gives none (out of the grid, or no alive upwind node).  The reference's
-// minimum over the quadrants does not depend on their order.
-__device__ __forceinline__ float fouds2_quadrant(const MGrid &G, int iz, int ix, int sj, int sk) {
+// node (iz, ix): quad_load reads its nodes (bounds-checked, before the
+// node's own status is known, so the loads of a step go out together);
+// quad_eval gives the trial time of its solution, or +inf where the
+// quadrant gives none (out of the grid, or no alive upwind node).  The
+// reference's minimum over the quadrants does not depend on their order.
+struct QLoad {
+  Node nj, nj2, nk, nk2;
+  float vel, risti;
+  bool jin, kin, j2in, k2in;
+};
+
+__device__ __forceinline__ QLoad quad_load(const MGrid &G, int iz, int ix, int sj, int sk, bool in) {
+  QLoad L;
+  const int j = ix + sj, k = iz + sk, j2 = j + sj, k2 = k + sk;
+  L.jin = in && j >= 1 && j <= G.nnx;
+  L.kin = in && k >= 1 && k <= G.nnz;
+  L.j2in = L.jin && (sj < 0 ? (j2 >= 1) : (j2 <= G.nnx));
+  L.k2in = L.kin && (sk < 0 ? (k2 >= 1) : (k2 <= G.nnz));
+  const Node none{0.0f, -2};
+  L.nj = L.jin ? G.nd[nid(G, iz, j)] : none;
+  L.nk = L.kin ? G.nd[nid(G, k, ix)] : none;
+  L.nj2 = L.j2in ? G.nd[nid(G, iz, j2)] : none;
+  L.nk2 = L.k2in ? G.nd[nid(G, k2, ix)] : none;
+  L.vel = in ? G.vel[nid(G, iz, ix)] : 1.0f;
+  L.risti = in ? G.risti[ix - 1] : 0.0f;
+  return L;
+}
+
+__device__ __forceinline__ float quad_eval(const MGrid &G, const QLoad &L) {
   const float kInf = __int_as_float(0x7f800000);
-  const int j = ix + sj, k = iz + sk;
-  if (!(j >= 1 && j <= G.nnx) || !(k >= 1 && k <= G.nnz)) return kInf;
-  const int j2 = j + sj, k2 = k + sk;
-  const bool j2in = sj < 0 ? (j2 >= 1) : (j2 <= G.nnx);
-  const bool k2in = sk < 0 ? (k2 >= 1) : (k2 <= G.nnz);
-  const Node nj = G.nd[nid(G, iz, j)];
-  const Node nk = G.nd[nid(G, k, ix)];
-  const Node nj2 = j2in ? G.nd[nid(G, iz, j2)] : Node{0.0f, -2};
-  const Node nk2 = k2in ? G.nd[nid(G, k2, ix)] : Node{0.0f, -2};
-  const float vel = G.vel[nid(G, iz, ix)];
-  const float risti = G.risti[ix - 1];
-  const float slown = 1.0f / vel;
+  if (!L.jin || !L.kin) return kInf;
+  const Node nj = L.nj, nj2 = L.nj2, nk = L.nk, nk2 = L.nk2;
+  const float risti = L.risti;
+  const float slown = 1.0f / L.vel;
   const float ri = G.earth;
-  const int swj = (nj.s == 0 && j2in && nj2.s == 0 && nj.t > nj2.t) ? 0 : -1;
-  const int swk = (nk.s == 0 && k2in && nk2.s == 0 && nk.t > nk2.t) ? 0 : -1;
+  const int swj = (nj.s == 0 && L.j2in && nj2.s == 0 && nj.t > nj2.t) ? 0 : -1;
+  const int swk = (nk.s == 0 && L.k2in && nk2.s == 0 && nk.t > nk2.t) ? 0 : -1;
   float a, b, c, u, v, em, tref, tdiv;
   if (swj == 0) {
     if (swk == 0) {
@@ -410,12 +427,10 @@ __device__ bool march(Heap &H, const MGrid &G, int urg, int vnl, int vnr, int vn
     // each lane: its neighbour's state and one quadrant's trial time
     const int nz = iz + qdz, nx = ix + qdx;
     const bool in = nz >= 1 && nz <= G.nnz && nx >= 1 && nx <= G.nnx;
-    int st = -2;
+    const int st = in ? G.nd[nid(G, nz, nx)].s : -2;
+    const QLoad ql = quad_load(G, nz, nx, sj, sk, in);
     float cand = __int_as_float(0x7f800000);
-    if (in) {
-      st = G.nd[nid(G, nz, nx)].s;
-      if (st != 0) cand = fouds2_quadrant(G, nz, nx, sj, sk);
-    }
+    if (in && st != 0) cand = quad_eval(G, ql);
     {
       float o = __shfl_xor(cand, 1);
       cand = (o < cand) ? o : cand;
