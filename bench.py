@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--surftime-steps", type=int, default=2,
                     help="timed calls of the drop-in SurfTime leg (0 skips it)")
+    ap.add_argument("--weights", default=None,
+                    help="surrogate network (.npz); default: the 4 x 512 network for grid128 "
+                         "(the C4 configuration), the 4 x 256 one otherwise")
     ap.add_argument("--fmst-steps", type=int, default=2,
                     help="timed calls of the FMST leg (0 skips it)")
     return ap.parse_args()
@@ -470,8 +473,12 @@ def main():
         "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{w.name}: {w.ncol} columns x {int(N)} layer entries, "
-                               f"{int(np.diff(w.per_off)[0])} periods 1-80 s, "
-                               "Rayleigh modes 0-3 + Love modes 0-1, phase+group+kernels",
+                               f"{int(np.diff(w.per_off)[0])} periods {w.periods.min():g}-"
+                               f"{w.periods.max():g} s, Rayleigh modes 0-"
+                               f"{int(np.asarray(w.nmode).reshape(w.ncol, -1)[:, 0].max()) - 1} + "
+                               f"Love modes 0-"
+                               f"{int(np.asarray(w.nmode).reshape(w.ncol, -1)[:, 1].max()) - 1}, "
+                               "phase+group+kernels",
                    "columns": w.ncol, "column_periods": cps, "parallelism": "single GPU"},
         "plain": {"value": float(cps / (ms_plain * 1e-3)), "ms_per_step": float(ms_plain),
                   "stage_ms": {"prep": float(sm_plain[0]), "chains": float(sm_plain[1]),
@@ -493,22 +500,27 @@ def main():
     dom_status, dom_ms, dom_n = status, float(sm_plain[4]), int(plain.pass_timings()[1][0])
     if not args.no_surrogate:
         from jdsurfg_amd import surrogate as S
-        mlp = S.MLP.load(S.DEFAULT_WEIGHTS)
+        wpath = args.weights or (os.path.join(os.path.dirname(S.DEFAULT_WEIGHTS),
+                                              "surrogate_mlp_512.npz")
+                                 if args.workload == "grid128" else S.DEFAULT_WEIGHTS)
+        mlp = S.MLP.load(wpath)
         sp = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
         sp.set_tolerance(1e-10)
         sout = sp.alloc_outputs(kernels=kern)
-        feats = S.Features(sp, inp)
+        # the whole iteration in the library (network, bracket/polish,
+        # fallback, eigen stage) with no host work between its launches
+        S.bind_network(sp, mlp)
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
 
         def run_sur():
             ev[0].record()
-            feats.first = True
-            pred = S.predict(mlp, feats, e0=0, n=sp.nent)
+            S.surrogate_solve_net(sp, inp, sout)
             ev[1].record()
-            S.surrogate_solve(sp, inp, sout, pred=pred)
-            t = sp.timings()
-            return [ev[0].elapsed_time(ev[1])] + [float(x) for x in t] + \
-                [float(sp.pass_timings()[0][0])]
+            ev[1].synchronize()
+            t = [float(x) for x in sp.timings()]
+            # the network runs first, before the plan's own stage events
+            net = ev[0].elapsed_time(ev[1]) - sum(t)
+            return [net] + t + [float(sp.pass_timings()[0][0])]
         ms_sur, st_sur = timed(run_sur, args.steps, args.warmup)
         sm = np.mean(np.array(st_sur), axis=0)  # predict, prep+polish, chains, eigR, eigL, passes
         stats = S.surrogate_stats(sp)
@@ -552,7 +564,7 @@ def main():
                                                               .item()),
                           "mlp_tflops": float(mlp_flops / (sm[0] * 1e-3) / 1e12),
                           "mlp_mfma_frac_bf16": float(mlp_flops / (sm[0] * 1e-3) / 2.5e15),
-                          "weights": os.path.relpath(S.DEFAULT_WEIGHTS, ROOT)},
+                          "weights": os.path.relpath(wpath, ROOT)},
         })
         dom_status = sout["status"].cpu().numpy()
         dom_ms, dom_n = float(sm[5]), int(sp.pass_timings()[1][0])
