@@ -304,6 +304,15 @@ def uneven_main(args):
             "detail": {k: rep[k] for k in ("eager_ms", "autotune_ms", "chain_waves",
                                             "eigen_waves", "graph_ms", "status_counts",
                                             "max_vs_drift") if k in rep}}
+    # the same iterations with the learned bracket (exact-root mode), captured
+    # with the network in the library
+    sr = B.run_surrogate_iterations(w, iters=max(2, args.steps * 2), amp=0.01,
+                                    log=lambda m: print(m, file=sys.stderr, flush=True))
+    line["surrogate"] = {"value": sr["column_periods_per_s"], "unit": "column·periods/s",
+                         "ms_per_step": sr["graph_ms_mean"], "graph_ms": sr["graph_ms"],
+                         "eager_ms": sr["eager_ms"], "fallback_rate": sr["fallback_rate"],
+                         "network_width": sr["network_width"],
+                         "speedup_vs_plain": rep["graph_ms_mean"] / sr["graph_ms_mean"]}
     print(json.dumps(line), flush=True)
 
 

@@ -68,6 +68,53 @@ def run_iterations(w, iters=10, amp=0.01, seed=1, kernels=("dcdb", "dudb"), tune
     return rep
 
 
+def run_surrogate_iterations(w, iters=10, amp=0.01, seed=1, kernels=("dcdb", "dudb"), mlp=None,
+                             log=None):
+    """The same iterations with the learned bracket: the plan in exact-root
+    mode with the network bound to it, each iteration (1 % Vs update, network,
+    bracket/polish, fallback chains, eigen stage) captured once and replayed.
+    Returns a report dict (times in ms) with the fallback rate."""
+    import torch
+    from . import surrogate as S
+    mlp = mlp or S.MLP.load(S.DEFAULT_WEIGHTS)
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    plan.set_tolerance(1e-10)
+    S.bind_network(plan, mlp)
+    inp = swd.upload(w)
+    out = plan.alloc_outputs(kernels)
+    st = torch.cuda.Stream()
+    rep = {"workload": w.name, "columns": w.ncol, "column_periods": w.column_periods(),
+           "iterations": iters, "perturbation": amp, "network_width": mlp.hid}
+    with torch.cuda.stream(st):
+        S.surrogate_solve_net(plan, inp, out, stream=st.cuda_stream)
+        st.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(st)
+        S.surrogate_solve_net(plan, inp, out, stream=st.cuda_stream)
+        ev[1].record(st)
+        st.synchronize()
+        rep["eager_ms"] = ev[0].elapsed_time(ev[1])
+        g = S.capture_surrogate(plan, inp, out, st, perturb_amp=amp, seed=seed)
+        times = []
+        for i in range(iters):
+            ev[0].record(st)
+            g.launch(st)
+            ev[1].record(st)
+            st.synchronize()
+            times.append(ev[0].elapsed_time(ev[1]))
+            if log:
+                log(f"surrogate iteration {i} {times[-1]:.1f} ms")
+    stats = S.surrogate_stats(plan)
+    nvar = 3 if w.group else 1
+    rep["graph_ms"] = times
+    rep["graph_ms_mean"] = float(np.mean(times))
+    rep["column_periods_per_s"] = w.column_periods() / (rep["graph_ms_mean"] * 1e-3)
+    rep["fallback_rate"] = stats["roots_left_to_chain"] / (nvar * plan.nent)
+    rep["status_counts"] = {str(i): int(c) for i, c in
+                            enumerate(np.bincount(out["status"].cpu().numpy())) if c}
+    return rep
+
+
 def main(argv=None):
     """python -m jdsurfg_amd.batch [--ncol N] [--iters K]: the uneven batch
     (50,000 columns, 16-128 layers, 8-64 periods, 1-3 modes per wave type)
