@@ -1030,14 +1030,28 @@ struct jdsg_fmst {
   std::vector<long long> ray_cnt;         // per receiver nonzero weights (rays kept)
   bool have_rays = false;
   float *d_fdm_all = nullptr;             // dense weights of every receiver (rays kept)
+  // device work buffers kept between runs (an inversion calls run every
+  // iteration): slot -> (pointer, bytes)
+  std::vector<std::pair<void *, size_t>> bufs;
+  void *buf(int slot, size_t bytes) {
+    if ((int)bufs.size() <= slot) bufs.resize(slot + 1, {nullptr, 0});
+    auto &b = bufs[slot];
+    if (b.second < bytes) {
+      (void)hipFree(b.first);
+      b = {nullptr, 0};
+      if (hipMalloc(&b.first, std::max<size_t>(bytes, 1)) != hipSuccess) return nullptr;
+      b.second = bytes;
+    }
+    return b.first;
+  }
   ~jdsg_fmst() {
+    for (auto &b : bufs) (void)hipFree(b.first);
     (void)hipFree(d_pairs);
     (void)hipFree(d_rcx);
     (void)hipFree(d_rcz);
     (void)hipFree(d_risti_c);
     (void)hipFree(d_risti_r);
     (void)hipFree(d_plane_idx);
-    (void)hipFree(d_fdm_all);
   }
 };
 
@@ -1209,7 +1223,6 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
     if (p >= kmax) return fail("fmst_run: a pair's period index is not below kmax");
   hipStream_t st = (hipStream_t)stream;
   h->have_rays = false;
-  (void)hipFree(h->d_fdm_all);
   h->d_fdm_all = nullptr;
   if (h->npair == 0) return 0;
   if (!vc || !vout || !ttime || !tsyn || !rstatus || !pstatus)
@@ -1218,17 +1231,15 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
   const int nplane = (int)h->plane_idx.size();
   const long long ncell = (long long)(g.nvx + 2) * (g.nvz + 2);
   // device memory: planes, then as many pair slots as 0.5 of the free memory takes
-  float *d_veln = nullptr, *d_velnew = nullptr;
-  FCHK(hipMalloc((void **)&d_veln, sizeof(float) * ncn * nplane));
-  FCHK(hipMalloc((void **)&d_velnew, sizeof(float) * ncn * nplane));
+  float *d_veln = (float *)h->buf(0, sizeof(float) * ncn * nplane);
+  float *d_velnew = (float *)h->buf(1, sizeof(float) * ncn * nplane);
+  if (!d_veln || !d_velnew) return fail("fmst_run: device allocation");
   struct Free {
     std::vector<void *> p;
     ~Free() {
       for (void *q : p) (void)hipFree(q);
     }
   } fr;
-  fr.p.push_back(d_veln);
-  fr.p.push_back(d_velnew);
   {
     dim3 grid((unsigned)((ncn + 255) / 256), (unsigned)nplane);
     hipLaunchKernelGGL(fmst_gridder, grid, dim3(256), 0, st, g, nplane, h->d_plane_idx, vc, nxy,
@@ -1249,7 +1260,8 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
     // every traced receiver's dense weights stay for jdsg_fmst_rays
     const double need = (double)h->nrcv * ncell * 4;
     if (need > 0.5 * budget) return fail("fmst_run: dense ray weights exceed the memory budget");
-    FCHK(hipMalloc((void **)&h->d_fdm_all, sizeof(float) * (size_t)(h->nrcv * ncell)));
+    h->d_fdm_all = (float *)h->buf(2, sizeof(float) * (size_t)h->nrcv * ncell);
+    if (!h->d_fdm_all) return fail("fmst_run: device allocation");
     budget -= need;
   }
   const double per_pair = slot_bytes + (keep ? 0.0 : maxnr * rcv_bytes);
@@ -1257,19 +1269,17 @@ extern "C" int jdsg_fmst_run(jdsg_fmst *h, const double *vc, const double *vout,
   Node *d_cn = nullptr, *d_rn = nullptr;
   float *d_rv = nullptr, *d_fdm = nullptr;
   int *d_rpair = nullptr;
-  FCHK(hipMalloc((void **)&d_cn, sizeof(Node) * ncn * chunk));
-  fr.p.push_back(d_cn);
-  FCHK(hipMalloc((void **)&d_rn, sizeof(Node) * kRmax * kRmax * chunk));
-  fr.p.push_back(d_rn);
-  FCHK(hipMalloc((void **)&d_rv, sizeof(float) * kRmax * kRmax * chunk));
-  fr.p.push_back(d_rv);
+  d_cn = (Node *)h->buf(3, sizeof(Node) * ncn * chunk);
+  d_rn = (Node *)h->buf(4, sizeof(Node) * kRmax * kRmax * chunk);
+  d_rv = (float *)h->buf(5, sizeof(float) * kRmax * kRmax * chunk);
+  if (!d_cn || !d_rn || !d_rv) return fail("fmst_run: device allocation");
 
   if (!keep) {
-    FCHK(hipMalloc((void **)&d_fdm, sizeof(float) * ncell * std::max(1, maxnr) * chunk));
-    fr.p.push_back(d_fdm);
+    d_fdm = (float *)h->buf(6, sizeof(float) * ncell * std::max(1, maxnr) * chunk);
+    if (!d_fdm) return fail("fmst_run: device allocation");
   }
-  FCHK(hipMalloc((void **)&d_rpair, sizeof(int) * std::max(1, h->nrcv)));
-  fr.p.push_back(d_rpair);
+  d_rpair = (int *)h->buf(7, sizeof(int) * std::max(1, h->nrcv));
+  if (!d_rpair) return fail("fmst_run: device allocation");
   std::vector<int> rp_local(h->nrcv);
   for (int p0 = 0; p0 < h->npair; p0 += chunk) {
     const int np = std::min(chunk, h->npair - p0);
