@@ -17,7 +17,25 @@
 /* smoothing.cpp:84-110: central coefficients of the deriv-th derivative on
  * acc+1 points, from sum_j c_j (j-n)^i / i! = delta(i, deriv); solved in
  * double by Gaussian elimination with partial pivoting, then rounded */
+/* test hook: coefficient sets used in place of the rationals by the
+ * smoothers (acc 8 / deriv 2, acc 6 / deriv 2, acc 6 / deriv 1; null pointers
+ * restore the rationals), so a test can run the reference's float-LU
+ * coefficients through the same smoother */
+static const float *g_fd_override[3];
+void or_fdcoefs_override(const float *c82, const float *c62, const float *c61) {
+  g_fd_override[0] = c82;
+  g_fd_override[1] = c62;
+  g_fd_override[2] = c61;
+}
+
 void or_fdcoefs(int acc, int deriv, float *coefs) {
+  const float *ov = acc == 8 && deriv == 2 ? g_fd_override[0]
+                    : acc == 6 && deriv == 2 ? g_fd_override[1]
+                    : acc == 6 && deriv == 1 ? g_fd_override[2] : NULL;
+  if (ov) {
+    memcpy(coefs, ov, sizeof(float) * (acc + 1));
+    return;
+  }
   const int N = acc + 1, n = acc / 2;
   double A[17][18];
   for (int i = 0; i < N; ++i) {
