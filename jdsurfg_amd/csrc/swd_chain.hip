@@ -180,7 +180,8 @@ enum Phase : int {
   PH_DONE,
   PH_SUSPEND,   // a long upward scan handed to a scan job; the chain is saved
   PH_SCAN_HIT,  // resumed: the scan job found the sign change
-  PH_SCAN_END   // resumed: the scan job ran to its end without one
+  PH_SCAN_END,  // resumed: the scan job ran to its end without one
+  PH_FP         // exact-root mode: false-position polish of the bracket
 };
 
 struct ChainState {
@@ -216,6 +217,21 @@ struct ChainStateSusp : ChainState {
 
 
 
+// Exact-root mode polishes a bracket [c1, c2] (del1, del2 of opposite sign)
+// by false position with the Anderson-Bjorck weight instead of nevill's
+// Neville/bisection hybrid, which bisects most of the way down to a 1e-10
+// bracket (about 25 evaluations a root against 5-8).  The iterate is c3, the
+// previous one del3 (0 before the second), the step count nctrl.
+JDSG_DEV void fp_begin(ChainState &S) {
+  S.nctrl = 0;
+  S.del3 = 0.0;
+  double c = (S.c1 * S.del2 - S.c2 * S.del1) / (S.del2 - S.del1);
+  if (!(c > fmin(S.c1, S.c2) && c < fmax(S.c1, S.c2))) c = 0.5 * (S.c1 + S.c2);
+  S.c3 = c;
+  S.ec = c;
+  S.phase = PH_FP;
+}
+
 JDSG_DEV double period_at(const ChainArgs &A, const ChainState &S, int kk) {
   double T = A.periods[A.per_off[S.col] + kk];
   return (S.variant == 0) ? T : T * S.tscale;
@@ -246,7 +262,9 @@ JDSG_DEV void begin_pass(ChainState &S, double *cs, int kpad) {
 
 // Returns true with S.ec set when a period-equation value is needed at
 // phase velocity S.ec (period S.t1); false when the chain is complete.
-template <bool SUSP>
+// EXACT: exact-root mode (tolerance below 1e-6), brackets polished by
+// false position.
+template <bool SUSP, bool EXACT>
 __device__ __attribute__((noinline)) bool advance(
     typename std::conditional<SUSP, ChainStateSusp, ChainState>::type &S, const ChainArgs &A,
     double *cs, double F) {
@@ -373,6 +391,10 @@ __device__ __attribute__((noinline)) bool advance(
         S.del2 = F;
         if constexpr (SUSP) S.nscan++;
         if (sign1(S.del1) != sign1(S.del2)) {
+          if constexpr (EXACT) {
+            fp_begin(S);
+            return true;
+          }
           S.c3 = 0.5 * (S.c1 + S.c2);
           S.ec = S.c3;
           S.phase = PH_NV_INIT;
@@ -386,6 +408,10 @@ __device__ __attribute__((noinline)) bool advance(
       }
       case PH_SCAN_HIT:  // as PH_GS_SCAN on a sign change between c1 and c2
         if constexpr (SUSP) {
+          if constexpr (EXACT) {
+            fp_begin(S);
+            return true;
+          }
           S.c3 = 0.5 * (S.c1 + S.c2);
           S.ec = S.c3;
           S.phase = PH_NV_INIT;
@@ -485,6 +511,48 @@ __device__ __attribute__((noinline)) bool advance(
         S.del3 = F;
         S.phase = PH_NV_LOOP;
         continue;
+      }
+      case PH_FP: {
+        if constexpr (!EXACT) return false;
+        // c1 the retained end, c2 the newest; the retained end's value is
+        // scaled by 1 - F/del2 (0.5 when that is <= 0) while it stays
+        const double fc = F;
+        if (fc == 0.0) {
+          S.c1 = S.c3;
+          S.c2 = S.c3;
+        } else {
+          if (sign1(fc) != sign1(S.del2)) {
+            S.c1 = S.c2;
+            S.del1 = S.del2;
+          } else {
+            const double m = 1.0 - fc / S.del2;
+            S.del1 = ((m > 0.0) ? m : 0.5) * S.del1;
+          }
+          S.c2 = S.c3;
+          S.del2 = fc;
+        }
+        ++S.nctrl;
+        bool done = fc == 0.0 || fabs(S.c2 - S.c1) <= A.tol * fabs(S.c2) || S.nctrl >= 80;
+        double root = S.c2;
+        if (!done) {
+          double cn = (S.c1 * S.del2 - S.c2 * S.del1) / (S.del2 - S.del1);
+          if (!(cn > fmin(S.c1, S.c2) && cn < fmax(S.c1, S.c2))) cn = 0.5 * (S.c1 + S.c2);
+          // the corrected value's error is about d1^2 / d0 for a linearly
+          // converging step and less once convergence is superlinear
+          const double d1 = fabs(cn - S.c3), d0 = fabs(S.c3 - S.del3), cc = fabs(S.c3);
+          if ((d1 <= A.tol * cc && d0 <= 1.0e-4 * cc) ||
+              (S.del3 != 0.0 && d0 <= 1.0e-3 * cc && d1 * d1 <= 0.1 * A.tol * cc * d0)) {
+            done = true;
+            root = cn;
+          } else {
+            S.del3 = S.c3;
+            S.c3 = cn;
+            S.ec = cn;
+            return true;
+          }
+        }
+        S.c3 = root;
+        goto nv_done;
       }
       case PH_RETRY_NEXT: {
         // surfdisp.cpp:95-109: each missing period is re-solved alone; a
@@ -661,7 +729,7 @@ jdsg_root_chains(ChainArgs A) {
     g = nlanes + (long long)atomicAdd(counter, 1u);
   }
   double F = 0.0;
-  bool need = live ? advance<false>(S, A, cs, F) : false;
+  bool need = live ? advance<false, FAST>(S, A, cs, F) : false;
   const float4 *base = (W == 0) ? A.cm.chainR : A.cm.chainL;
   while (live) {
     if (need) {
@@ -677,7 +745,7 @@ jdsg_root_chains(ChainArgs A) {
                      : dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       ++nev;
       ++cev;
-      need = advance<false>(S, A, cs, F);
+      need = advance<false, FAST>(S, A, cs, F);
       continue;
     }
     finish_chain(S, A, g, W);
@@ -690,7 +758,7 @@ jdsg_root_chains(ChainArgs A) {
       if (setup_chain(S, A, g, cs, W)) { live = true; break; }
       if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
     }
-    if (live) need = advance<false>(S, A, cs, 0.0);
+    if (live) need = advance<false, FAST>(S, A, cs, 0.0);
   }
   if (A.nevals && nev) atomicAdd(A.nevals, nev);
 }
@@ -757,7 +825,7 @@ jdsg_root_chains_susp(ChainArgs A) {
     item = nlanes + (long long)atomicAdd(counter, 1u);
   }
   double F = 0.0;
-  bool need = live ? advance<true>(S, A, cs, F) : false;
+  bool need = live ? advance<true, true>(S, A, cs, F) : false;
   const float4 *base = (W == 0) ? A.cm.chainR : A.cm.chainL;
   while (live) {
     if (need) {
@@ -769,7 +837,7 @@ jdsg_root_chains_susp(ChainArgs A) {
                    : jdsg_pf::dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       ++nev;
       ++S.evals;
-      need = advance<true>(S, A, cs, F);
+      need = advance<true, true>(S, A, cs, F);
       continue;
     }
     if (S.phase == PH_SUSPEND) {
@@ -799,7 +867,7 @@ jdsg_root_chains_susp(ChainArgs A) {
       if (item >= nitems) break;
       if (start_item(item)) { live = true; break; }
     }
-    if (live) need = advance<true>(S, A, cs, 0.0);
+    if (live) need = advance<true, true>(S, A, cs, 0.0);
   }
   if (A.nevals && nev) atomicAdd(A.nevals, nev);
 }
