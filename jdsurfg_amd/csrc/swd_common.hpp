@@ -38,8 +38,12 @@ JDSG_DEV cplx operator/(cplx a, cplx b) {
 JDSG_DEV cplx operator/(double s, cplx b) { return cplx{s, 0.0} / b; }
 JDSG_DEV cplx cconj(cplx a) { return cplx{a.re, -a.im}; }
 // principal square root of a real number seen as complex (x + 0i)
+// (one square root and two scalar selects: a select between two aggregates
+// becomes a select between their addresses, which puts them in private memory)
 JDSG_DEV cplx csqrt_real(double x) {
-  return (x >= 0.0) ? cplx{sqrt(x), 0.0} : cplx{0.0, sqrt(-x)};
+  const bool pos = x >= 0.0;
+  const double s = sqrt(pos ? x : -x);
+  return cplx{pos ? s : 0.0, pos ? 0.0 : s};
 }
 // exp of a complex number; a real or imaginary argument takes only the one
 // function it needs (exp(0) = 1, sincos(0) = (0, 1) exactly, so the value is

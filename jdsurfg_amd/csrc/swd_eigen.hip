@@ -102,27 +102,37 @@ JDSG_DEV LayerWaves layer_waves(const Layer &L, double omega, double wvno2) {
   w.pi = w.ra.im * L.zd;
   w.qr = w.rb.re * L.zd;
   w.qi = w.rb.im * L.zd;
+  // The branches form scalars and the fields are assigned once after them:
+  // stores to different fields in the two branches are merged by the
+  // compiler into stores through selected addresses, in private memory.
+  double ea_re, ea_im, ma;
   if (w.pi != 0.0) {
     double s, c;
     sincos(w.pi, &s, &c);
-    w.Ea = cmk(c, -s);
-    w.ma = 1.0;
+    ea_re = c;
+    ea_im = -s;
+    ma = 1.0;
   } else {
-    w.ma = (w.pr == 0.0) ? 1.0 : exp(-w.pr);
-    w.Ea = cmk(w.ma, 0.0);
+    ma = (w.pr == 0.0) ? 1.0 : exp(-w.pr);
+    ea_re = ma;
+    ea_im = 0.0;
   }
-  w.Eb = cmk(0.0, 0.0);
-  w.mb = 1.0;
+  double eb_re = 0.0, eb_im = 0.0, mb = 1.0;
   if (L.iwat == 0) {
     if (w.qi != 0.0) {
       double s, c;
       sincos(w.qi, &s, &c);
-      w.Eb = cmk(c, -s);
+      eb_re = c;
+      eb_im = -s;
     } else {
-      w.mb = (w.qr == 0.0) ? 1.0 : exp(-w.qr);
-      w.Eb = cmk(w.mb, 0.0);
+      mb = (w.qr == 0.0) ? 1.0 : exp(-w.qr);
+      eb_re = mb;
     }
   }
+  w.Ea = cmk(ea_re, ea_im);
+  w.ma = ma;
+  w.Eb = cmk(eb_re, eb_im);
+  w.mb = mb;
   return w;
 }
 
@@ -187,30 +197,29 @@ JDSG_DEV double normalize_pow2(double (&v)[K], double floor = 1.0e-40) {
   return (double)e * 0.69314718055994530942;
 }
 
-// sregn96.f90:491-647 dnka; returns the 5x5 matrix row-major in ca
-JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, double cossv,
-                     double rsinsv, double sinsvr, float rho, float b, int iwat,
-                     const LayerWaves &w, const Wk &K) {
+// sregn96.f90:491-647 dnka, applied: ee = cd * ca (the up-sweep's row vector
+// times the layer's 5x5 Dunkin matrix, summed over j in increasing order).
+// Each branch forms the matrix entries as scalars and the product itself, so
+// no 25-element array is merged across the fluid/solid branches (which the
+// compiler keeps in private memory, a scratch round trip per layer step).
+JDSG_DEV void dnka_apply(const double (&cd)[5], double (&ee)[5], double cosp, double rsinp,
+                         double sinpr, double cossv, double rsinsv, double sinsvr, float rho,
+                         float b, int iwat, const LayerWaves &w, const Wk &K) {
   const double wvno = K.k, wvno2 = K.k2, om2 = K.om2;
   // exp(-ex) and exp(-exa) of the reference (ex = Re(ra) d, exa = ex +
   // Re(rb) d) are the layer magnitudes ma and ma*mb
   const double ex = w.pr, exa = w.pr + w.qr;
   if (iwat == 1) {
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-#pragma unroll
-      for (int i = 0; i < 5; ++i) ca[i][j] = 0.0;
-    double dfac = (ex > 35.0) ? 0.0 : w.ma;
-    ca[2][2] = dfac;
-    ca[0][0] = cosp;
-    ca[4][4] = cosp;
+    // nonzero entries: 00 11 33 44 = cosp, 22 = dfac, 01 = 34, 10 = 43
+    const double dfac = (ex > 35.0) ? 0.0 : w.ma;
     const double irom = frcp((double)rho * om2);
-    ca[0][1] = -rsinp * irom;
-    ca[1][0] = -(double)rho * sinpr * om2;
-    ca[1][1] = cosp;
-    ca[3][3] = cosp;
-    ca[3][4] = ca[0][1];
-    ca[4][3] = ca[1][0];
+    const double c01 = -rsinp * irom;
+    const double c10 = -(double)rho * sinpr * om2;
+    ee[0] = cd[0] * cosp + cd[1] * c10;
+    ee[1] = cd[0] * c01 + cd[1] * cosp;
+    ee[2] = cd[2] * dfac;
+    ee[3] = cd[3] * cosp + cd[4] * c10;
+    ee[4] = cd[3] * c01 + cd[4] * cosp;
     return;
   }
   double a0 = (exa < 60.0) ? w.ma * w.mb : 0.0;
@@ -237,55 +246,60 @@ JDSG_DEV void dnka_r(double ca[5][5], double cosp, double rsinp, double sinpr, d
   double rom = (double)rho * om2;
   const double irom = frcp(rom);
   double temp = a0c * gg1 + g2xz2 + g2wy2;
-  ca[2][2] = a0 + temp + temp;
-  ca[0][0] = cpcq - temp;
-  ca[0][1] = (-cqx + wvno2 * cpy) * irom;
+  const double c22 = a0 + temp + temp;
+  const double c00 = cpcq - temp;
+  const double c01 = (-cqx + wvno2 * cpy) * irom;
   temp = 0.5 * a0cgg1 + gxz2 + g1wy2;
-  ca[0][2] = wvno * temp * irom;
-  ca[0][3] = (-cqww2 + cpz) * irom;
+  const double c02 = wvno * temp * irom;
+  const double c03 = (-cqww2 + cpz) * irom;
   temp = wvno2 * (a0c + wy2) + xz;
-  ca[0][4] = -temp * frcp((double)rho2 * om2 * om2);
-  ca[1][0] = (-gamm2 * cqw + gam2 * cpz * K.ik2) * (double)rho * om2;
-  ca[1][1] = cpcq;
-  ca[1][2] = (gamm1 * cqww2 - gam * cpz) * K.ik;
-  ca[1][3] = -wz;
-  ca[1][4] = ca[0][3];
+  const double c04 = -temp * frcp((double)rho2 * om2 * om2);
+  const double c10 = (-gamm2 * cqw + gam2 * cpz * K.ik2) * (double)rho * om2;
+  const double c11 = cpcq;
+  const double c12 = (gamm1 * cqww2 - gam * cpz) * K.ik;
+  const double c13 = -wz;
+  const double c14 = c03;
   temp = 0.5 * a0cgg1 * gg1 + gam2 * gxz2 + gamm2 * g1wy2;
-  ca[2][0] = -2.0 * temp * (double)rho * om2 * K.ik;
-  ca[2][1] = -wvno * (gam * cqxw2 - gamm1 * cpy) * 2.0;
-  ca[2][3] = -2.0 * ca[1][2];
-  ca[2][4] = -2.0 * ca[0][2];
-  ca[3][0] = (-gam2 * cqxw2 + gamm2 * cpy) * (double)rho * om2;
-  ca[3][1] = -xy;
-  ca[3][2] = -ca[2][1] / 2.0;
-  ca[3][3] = ca[1][1];
-  ca[3][4] = ca[0][1];
+  const double c20 = -2.0 * temp * (double)rho * om2 * K.ik;
+  const double c21 = -wvno * (gam * cqxw2 - gamm1 * cpy) * 2.0;
+  const double c23 = -2.0 * c12;
+  const double c24 = -2.0 * c02;
+  const double c30 = (-gam2 * cqxw2 + gamm2 * cpy) * (double)rho * om2;
+  const double c31 = -xy;
+  const double c32 = -c21 / 2.0;
+  const double c33 = c11;
+  const double c34 = c01;
   temp = gamm2 * (a0c * gam2 + g2wy2) + gam2 * g2xz2;
-  ca[4][0] = -(double)rho2 * om2 * om2 * temp * K.ik2;
-  ca[4][1] = ca[3][0];
-  ca[4][2] = -ca[2][0] / 2.0;
-  ca[4][3] = ca[1][0];
-  ca[4][4] = ca[0][0];
+  const double c40 = -(double)rho2 * om2 * om2 * temp * K.ik2;
+  const double c41 = c30;
+  const double c42 = -c20 / 2.0;
+  const double c43 = c10;
+  const double c44 = c00;
+  ee[0] = cd[0] * c00 + cd[1] * c10 + cd[2] * c20 + cd[3] * c30 + cd[4] * c40;
+  ee[1] = cd[0] * c01 + cd[1] * c11 + cd[2] * c21 + cd[3] * c31 + cd[4] * c41;
+  ee[2] = cd[0] * c02 + cd[1] * c12 + cd[2] * c22 + cd[3] * c32 + cd[4] * c42;
+  ee[3] = cd[0] * c03 + cd[1] * c13 + cd[2] * c23 + cd[3] * c33 + cd[4] * c43;
+  ee[4] = cd[0] * c04 + cd[1] * c14 + cd[2] * c24 + cd[3] * c34 + cd[4] * c44;
 }
 
-// sregn96.f90:913-987 hska
-JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, double tcossv,
-                   double trsinsv, double tsinsvr, float rho, float b, int iwat,
-                   const LayerWaves &w, const Wk &K) {
+// sregn96.f90:913-987 hska, applied: out = AA * vv (the down-sweep's column
+// vector times the layer's 4x4 Haskell matrix, summed over j in increasing
+// order), the entries formed as scalars in each branch (no array merged
+// across the fluid/solid branches, which would live in private memory).
+JDSG_DEV void hska_apply(const double (&vv)[4], double (&out)[4], double cosp, double rsinp,
+                         double sinpr, double tcossv, double trsinsv, double tsinsvr, float rho,
+                         float b, int iwat, const LayerWaves &w, const Wk &K) {
   const double wvno = K.k, wvno2 = K.k2, om2 = K.om2;
   const double pex = w.pr, svex = (iwat == 1) ? 0.0 : w.qr;
   if (iwat == 1) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) AA[i][j] = 0.0;
-    double dfac = (pex > 35.0) ? 0.0 : w.ma;
-    AA[0][0] = dfac;
-    AA[3][3] = dfac;
-    AA[1][1] = cosp;
-    AA[2][2] = cosp;
-    AA[1][2] = -rsinp * frcp((double)rho * om2);
-    AA[2][1] = -(double)rho * om2 * sinpr;
+    // nonzero entries: 00 33 = dfac, 11 22 = cosp, 12, 21
+    const double dfac = (pex > 35.0) ? 0.0 : w.ma;
+    const double a12 = -rsinp * frcp((double)rho * om2);
+    const double a21 = -(double)rho * om2 * sinpr;
+    out[0] = dfac * vv[0];
+    out[1] = cosp * vv[1] + a12 * vv[2];
+    out[2] = a21 * vv[1] + cosp * vv[2];
+    out[3] = dfac * vv[3];
     return;
   }
   // exp(svex - pex): with a propagating S wave (svex = 0) that is ma
@@ -298,22 +312,26 @@ JDSG_DEV void hska(double AA[4][4], double cosp, double rsinp, double sinpr, dou
   double gamm1 = gam - 1.0;
   double rom = (double)rho * om2;
   const double irom = frcp(rom);
-  AA[0][0] = cossv + gam * (cosp - cossv);
-  AA[0][1] = -wvno * gamm1 * sinpr + gam * rsinsv * K.ik;
-  AA[0][2] = -wvno * (cosp - cossv) * irom;
-  AA[0][3] = (wvno2 * sinpr - rsinsv) * irom;
-  AA[1][0] = gam * rsinp * K.ik - wvno * gamm1 * sinsvr;
-  AA[1][1] = cosp - gam * (cosp - cossv);
-  AA[1][2] = (-rsinp + wvno2 * sinsvr) * irom;
-  AA[1][3] = -AA[0][2];
-  AA[2][0] = rom * gam * gamm1 * (cosp - cossv) * K.ik;
-  AA[2][1] = rom * (-gamm1 * gamm1 * sinpr + gam * gam * rsinsv * K.ik2);
-  AA[2][2] = AA[1][1];
-  AA[2][3] = -AA[0][1];
-  AA[3][0] = rom * (gam * gam * rsinp * K.ik2 - gamm1 * gamm1 * sinsvr);
-  AA[3][1] = -AA[2][0];
-  AA[3][2] = -AA[1][0];
-  AA[3][3] = AA[0][0];
+  const double a00 = cossv + gam * (cosp - cossv);
+  const double a01 = -wvno * gamm1 * sinpr + gam * rsinsv * K.ik;
+  const double a02 = -wvno * (cosp - cossv) * irom;
+  const double a03 = (wvno2 * sinpr - rsinsv) * irom;
+  const double a10 = gam * rsinp * K.ik - wvno * gamm1 * sinsvr;
+  const double a11 = cosp - gam * (cosp - cossv);
+  const double a12 = (-rsinp + wvno2 * sinsvr) * irom;
+  const double a13 = -a02;
+  const double a20 = rom * gam * gamm1 * (cosp - cossv) * K.ik;
+  const double a21 = rom * (-gamm1 * gamm1 * sinpr + gam * gam * rsinsv * K.ik2);
+  const double a22 = a11;
+  const double a23 = -a01;
+  const double a30 = rom * (gam * gam * rsinp * K.ik2 - gamm1 * gamm1 * sinsvr);
+  const double a31 = -a20;
+  const double a32 = -a10;
+  const double a33 = a00;
+  out[0] = a00 * vv[0] + a01 * vv[1] + a02 * vv[2] + a03 * vv[3];
+  out[1] = a10 * vv[0] + a11 * vv[1] + a12 * vv[2] + a13 * vv[3];
+  out[2] = a20 * vv[0] + a21 * vv[1] + a22 * vv[2] + a23 * vv[3];
+  out[3] = a30 * vv[0] + a31 * vv[1] + a32 * vv[2] + a33 * vv[3];
 }
 
 // sregn96.f90:649-825 evalg (jbdry = 0)
@@ -788,17 +806,9 @@ JDSG_DEV void dunkin_step(const EigenArgs &A, int col, int m, double omega, doub
   const LayerWaves lw = layer_waves(L, omega, wvno2);
   double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
   varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
-  double ca[5][5];
-  dnka_r(ca, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat, lw,
-         K);
   double ee[5];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    double cr = 0.0;
-#pragma unroll
-    for (int j = 0; j < 5; ++j) cr = cr + cd[j] * ca[j][i];
-    ee[i] = cr;
-  }
+  dnka_apply(cd, ee, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb,
+             L.iwat, lw, K);
   double exn = normalize_pow2(ee);
 #pragma unroll
   for (int i = 0; i < 5; ++i) cd[i] = ee[i];
@@ -884,17 +894,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     const LayerWaves lw = layer_waves(L, omega, wvno2);
     double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
     varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
-    double AA[4][4];
-    hska(AA, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb, L.iwat, lw,
-         K);
     double aa0[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      double cc = 0.0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) cc = cc + AA[i][j] * vv[j];
-      aa0[i] = cc;
-    }
+    hska_apply(vv, aa0, cosp, rsinp, sinpr, cossv, rsinsv, sinsvr, (float)L.zrho, (float)L.zb,
+               L.iwat, lw, K);
     double ex2 = normalize_pow2(aa0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) vv[i] = aa0[i];
@@ -959,7 +961,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     Layer Ln = load_layer(A, col, i1, true);
     double xmu_n = Ln.zrho * (Ln.zb * Ln.zb);
     double xlam_n = Ln.zrho * (Ln.za * Ln.za) - 2 * xmu_n;
-    double ixmu_n = 1.0 / xmu_n, ixl2_n = 1.0 / (xlam_n + xmu_n + xmu_n);
+    // reciprocals by the refined hardware estimate (lambda + 2 mu = rho a^2 > 0;
+    // mu is zero in a fluid layer, where dcdh_terms does not read 1/mu)
+    double ixmu_n = (xmu_n != 0.0) ? frcp(xmu_n) : 0.0, ixl2_n = frcp(xlam_n + xmu_n + xmu_n);
     if (D.p[3]) {
       double g = dcdh_terms(i1, Ln, L, xmu_n, xlam_n, xmu_prev, xlam_prev, ixl2_n, ixmu_n,
                             ixl2_prev, ixmu_prev, un, om2, wvno, wvno2);
