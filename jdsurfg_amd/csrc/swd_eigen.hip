@@ -1012,29 +1012,38 @@ JDSG_DEV VarL varl(const Layer &L, double omega, double wvno, double dpth) {
   v.rb = sqrt(wvnop * wvnom);
   double q = v.rb * dpth;
   v.mu = L.zrho * L.zb * L.zb;
-  v.eexl = 0.0;
-  v.sinq = 0.0;
   v.irb = (v.rb > 0.0) ? frcp(v.rb) : 0.0;
+  // scalars per branch, the fields assigned once after them (stores to
+  // different fields in the branches would be merged into stores through
+  // selected addresses, in private memory)
+  double eexl = 0.0, sinq = 0.0, cosq, yl, zl, e1re, e1im = 0.0;
   if (wvno < v.xkb) {
-    sincos(q, &v.sinq, &v.cosq);
-    v.yl = v.sinq * v.irb;
-    v.zl = -v.rb * v.sinq;
-    v.E1 = cmk(v.cosq, -v.sinq);
+    sincos(q, &sinq, &cosq);
+    yl = sinq * v.irb;
+    zl = -v.rb * sinq;
+    e1re = cosq;
+    e1im = -sinq;
   } else if (wvno == v.xkb) {
-    v.cosq = 1.0;
-    v.yl = dpth;
-    v.zl = 0.0;
-    v.E1 = cmk(1.0, 0.0);
+    cosq = 1.0;
+    yl = dpth;
+    zl = 0.0;
+    e1re = 1.0;
   } else {
-    v.eexl = q;
+    eexl = q;
     const double e1 = exp(-q);
-    v.E1 = cmk(e1, 0.0);
+    e1re = e1;
     double fac = (q < 18.0) ? e1 * e1 : 0.0;
-    v.cosq = (1.0 + fac) * 0.5;
-    v.sinq = (1.0 - fac) * 0.5;
-    v.yl = v.sinq * v.irb;
-    v.zl = v.rb * v.sinq;
+    cosq = (1.0 + fac) * 0.5;
+    sinq = (1.0 - fac) * 0.5;
+    yl = sinq * v.irb;
+    zl = v.rb * sinq;
   }
+  v.eexl = eexl;
+  v.sinq = sinq;
+  v.cosq = cosq;
+  v.yl = yl;
+  v.zl = zl;
+  v.E1 = cmk(e1re, e1im);
   return v;
 }
 
@@ -1084,7 +1093,7 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
         continue;
       }
       VarL v = varl(L, omega, wvno, L.zd);
-      double a11 = v.cosq, a22 = v.cosq, a12 = -(v.yl / v.mu), a21 = -(v.zl * v.mu);
+      double a11 = v.cosq, a22 = v.cosq, a12 = -(v.yl * frcp(v.mu)), a21 = -(v.zl * v.mu);
       double at[2] = {a11 * u1 + a12 * (t1v), a21 * u1 + a22 * (t1v)};
       EXL(k) = normalize_pow2(at, 1.0e-30) + v.eexl;
       u1 = at[0];
@@ -1120,10 +1129,10 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
   bool solid0 = love_solid(A, col, 0, pu);
   double u_k = (norm && solid0) ? UU(0) * iumax : UU(0);
   double t_k = 0.0;
-  const double ik2 = 1.0 / wvno2;
+  const double ik2 = 1.0 / wvno2, hik = 0.5 / wvno;
   double ext = 0.0;
   int llflag = 0;
-  double zr_prev = 0.0, xmu_prev = 0.0;
+  double zr_prev = 0.0, xmu_prev = 0.0, ixmu_prev = 0.0;
   for (int k = 0; k < N; ++k) {
     Layer L = load_layer_love(A, col, k, pu);
     long long o = D.base + (long long)k * D.stride;
@@ -1139,6 +1148,8 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
       }
     }
     double xmu = L.zrho * (L.zb * L.zb);
+    // 1/mu by the refined hardware reciprocal, carried to the next layer
+    const double ixmu = (xmu != 0.0) ? frcp(xmu) : 0.0;
     if (L.iwat != 0) {
       if (D.p[1]) D.p[1][o] = 0.0;
       if (D.p[2]) D.p[2][o] = 0.0;
@@ -1160,10 +1171,10 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
         // nub is i rb or rb, so 1/nub and exp(-nub d) (from varl, unless rb
         // was raised to the floor above) need no complex division
         const bool prop = wvno < v.xkb;
-        const double irb = (v.rb >= 1.0e-10) ? v.irb : 1.0 / rb;
+        const double irb = (v.rb >= 1.0e-10) ? v.irb : 1.0 / 1.0e-10;  // rb at its floor
         const cplx inub = prop ? cmk(0.0, -irb) : cmk(irb, 0.0);
-        const cplx ixn = (0.5 / (wvno * dmu)) * inub;  // 0.5 / (wvno xnub)
-        const double hk = 0.5 / wvno;
+        const cplx ixn = (hik * ixmu) * inub;  // 0.5 / (wvno xnub), dmu = mu
+        const double hk = hik;
         const cplx km1dn = hk * uk - ixn * t_k;
         const cplx kmup = hk * u_n + ixn * t_n;
         const double f3re = prop ? 0.0 : rb * dpth;
@@ -1198,7 +1209,7 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
         } else {
           drh = L.zrho - zr_prev;
           dm = xmu - xmu_prev;
-          dvdz = t_k * t_k * (1.0 / xmu - 1.0 / xmu_prev);
+          dvdz = t_k * t_k * (ixmu - ixmu_prev);
         }
         D.p[3][o] = uk * uk * (omega2 * drh - wvno2 * dm) + dvdz;
       }
@@ -1206,6 +1217,7 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
     }
     zr_prev = L.zrho;
     xmu_prev = xmu;
+    ixmu_prev = ixmu;
     if (want_eig) {
       A.eig[0][ebase + k] = u_k;
       A.eig[1][ebase + k] = t_k;
