@@ -766,15 +766,18 @@ JDSG_DEV void decode_item(const EigenArgs &A, long long it, int &col, int &mode,
   mode = (A.wave == 0) ? wm : wm - A.nmode[2 * col];
 }
 
-JDSG_DEV Layer load_layer(const EigenArgs &A, int col, int i, bool rayleigh) {
+// Rayleigh layer i of a column: the flattened velocities and thickness and
+// the density with its Rayleigh flattening factor (1 for a flat model, so
+// the product is the density itself).  Five loads with no branch between
+// them, so they are in flight together.
+JDSG_DEV Layer load_layer(const EigenArgs &A, int col, int i) {
   size_t q = (size_t)i * A.cm.ncol_pad + col;
   Layer L;
   L.za = A.cm.za[q];
   L.zb = A.cm.zb[q];
   L.zd = A.cm.zd[q];
-  double rt = (A.iflsph > 0) ? (rayleigh ? A.cm.rtpR[q] : A.cm.rtpL[q]) : 1.0;
-  L.zrho = (A.iflsph > 0) ? A.cm.rho0[q] * rt : A.cm.rho0[q];
-  L.iwat = rayleigh ? (L.zb > 0.0 ? 0 : 1) : 0;
+  L.zrho = A.cm.rho0[q] * A.cm.rtpR[q];
+  L.iwat = L.zb > 0.0 ? 0 : 1;
   return L;
 }
 
@@ -798,11 +801,10 @@ JDSG_DEV int up_slot(int m, int N) {
   return (m == N - 1) ? (N - 2) / kUpSeg + 1 : m / kUpSeg;
 }
 
-// One step of the up-sweep (sregn96.f90:455-487): cd, the vector at the top
-// of layer m + 1, becomes the vector at the top of layer m.
-JDSG_DEV void dunkin_step(const EigenArgs &A, int col, int m, double omega, double wvno2,
-                          const Wk &K, double (&cd)[5], double &exsum) {
-  Layer L = load_layer(A, col, m, true);
+// One step of the up-sweep (sregn96.f90:455-487) through layer L: cd, the
+// vector at the top of the layer below, becomes the vector at the top of L.
+JDSG_DEV void dunkin_step(const Layer &L, double omega, double wvno2, const Wk &K,
+                          double (&cd)[5], double &exsum) {
   const LayerWaves lw = layer_waves(L, omega, wvno2);
   double cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex;
   varsv_w(lw, L, cosp, cossv, rsinp, rsinsv, sinpr, sinsvr, pex, svex);
@@ -838,7 +840,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   // ---- up: Dunkin vectors from the half-space (sregn96.f90:401-489),
   // checkpoints only
   {
-    Layer H = load_layer(A, col, N - 1, true);
+    Layer H = load_layer(A, col, N - 1);
     EMat M;
     double g[5];
     evalg0(H, allfluid, wvno, omega, om2, wvno2, M, g);
@@ -850,8 +852,13 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     }
     ck(N - 1, 5) = 0.0;
     double exsum = 0.0;
+    // layer m - 1 is loaded while layer m is swept (its loads are in flight
+    // during the step instead of stalling the next one)
+    Layer Lc = load_layer(A, col, max(N - 2, 0));
     for (int m = N - 2; m >= 0; --m) {
-      dunkin_step(A, col, m, omega, wvno2, K, cd, exsum);
+      const Layer Ln = load_layer(A, col, max(m - 1, 0));
+      dunkin_step(Lc, omega, wvno2, K, cd, exsum);
+      Lc = Ln;
       if (m % kUpSeg == 0) {
 #pragma unroll
         for (int j = 0; j < 5; ++j) ck(m, j) = cd[j];
@@ -875,7 +882,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   double exsum = 0.0;
   double c = omega / wvno;
   EnergySums S = {0.0, 0.0, 0.0, 0.0};
-  Layer Lprev = load_layer(A, col, 0, true);
+  Layer Lprev = load_layer(A, col, 0);
   double xmu_prev = Lprev.zrho * (Lprev.zb * Lprev.zb);
   double xlam_prev = Lprev.zrho * (Lprev.za * Lprev.za) - 2 * xmu_prev;
   double ixmu_prev = 1.0 / xmu_prev, ixl2_prev = 1.0 / (xlam_prev + xmu_prev + xmu_prev);
@@ -917,7 +924,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
       for (int j = 0; j < 5; ++j) sb[j * 64] = cu[j];
       sb[5 * 64] = eu;
       for (int mm = bot - 1; mm >= top; --mm) {
-        dunkin_step(A, col, mm, omega, wvno2, K, cu, eu);
+        dunkin_step(load_layer(A, col, mm), omega, wvno2, K, cu, eu);
         double *sm = seg + (long long)(mm - top) * 6 * 64;
 #pragma unroll
         for (int j = 0; j < 5; ++j) sm[j * 64] = cu[j];
@@ -958,7 +965,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
       A.eig[3][ebase + m] = uprev.tz;
     }
     // getdcdh of layer m+1
-    Layer Ln = load_layer(A, col, i1, true);
+    Layer Ln = load_layer(A, col, i1);
     double xmu_n = Ln.zrho * (Ln.zb * Ln.zb);
     double xlam_n = Ln.zrho * (Ln.za * Ln.za) - 2 * xmu_n;
     // reciprocals by the refined hardware estimate (lambda + 2 mu = rho a^2 > 0;
