@@ -915,6 +915,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
       // recompute the up-sweep vectors of layers top..bot from the
       // checkpoint at bot (a multiple of kUpSeg, or the half-space)
       const int bot = min(top + kUpSeg - 1, N - 1);
+      // layer mm - 1 is loaded while layer mm is swept (the first one with
+      // the checkpoint)
+      Layer Lc = load_layer(A, col, max(bot - 1, top));
       double cu[5], eu;
 #pragma unroll
       for (int j = 0; j < 5; ++j) cu[j] = ck(bot, j);
@@ -924,7 +927,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
       for (int j = 0; j < 5; ++j) sb[j * 64] = cu[j];
       sb[5 * 64] = eu;
       for (int mm = bot - 1; mm >= top; --mm) {
-        dunkin_step(load_layer(A, col, mm), omega, wvno2, K, cu, eu);
+        const Layer Ln = load_layer(A, col, max(mm - 1, top));
+        dunkin_step(Lc, omega, wvno2, K, cu, eu);
+        Lc = Ln;
         double *sm = seg + (long long)(mm - top) * 6 * 64;
 #pragma unroll
         for (int j = 0; j < 5; ++j) sm[j * 64] = cu[j];
@@ -1091,8 +1096,11 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
     }
     EXL(N - 1) = 0.0;
     double u1 = UU(N - 1), t1v = TT(N - 1);
+    // layer k - 1 is loaded while layer k is swept
+    Layer Lc = load_layer_love(A, col, max(N - 2, 0), pu);
     for (int k = N - 2; k >= 0; --k) {
-      Layer L = load_layer_love(A, col, k, pu);
+      const Layer L = Lc;
+      Lc = load_layer_love(A, col, max(k - 1, 0), pu);
       if (L.iwat != 0) {
         UU(k) = 0.0;
         TT(k) = 0.0;
@@ -1140,18 +1148,33 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
   double ext = 0.0;
   int llflag = 0;
   double zr_prev = 0.0, xmu_prev = 0.0, ixmu_prev = 0.0;
+  // Layer k + 1 and the sweep values that layer k reads (uu, tt at k + 1,
+  // exl at k) come from loads issued one layer earlier, so they are in
+  // flight during the previous layer's work (indices clamped to the stack).
+  Layer Lk = load_layer_love(A, col, 0, pu);
+  Layer Ln = load_layer_love(A, col, min(1, N - 1), pu);
+  double uu_n = UU(min(1, N - 1)), tt_n = TT(min(1, N - 1)), exl_k = EXL(0);
   for (int k = 0; k < N; ++k) {
-    Layer L = load_layer_love(A, col, k, pu);
+    const Layer L = Lk, L1 = Ln;
+    const double uu1 = uu_n, tt1 = tt_n, exl = exl_k;
+    {
+      const int k2 = min(k + 2, N - 1);
+      Lk = L1;
+      Ln = load_layer_love(A, col, k2, pu);
+      uu_n = UU(k2);
+      tt_n = TT(k2);
+      exl_k = EXL(min(k + 1, N - 1));
+    }
     long long o = D.base + (long long)k * D.stride;
     // normalised amplitudes at the top of layer k+1
     double u_n = 0.0, t_n = 0.0;
     if (k + 1 < N) {
-      if (love_solid(A, col, k + 1, pu)) {
-        ext = ext + EXL(k);
+      if (L1.iwat == 0) {  // layer k + 1 solid
+        ext = ext + exl;
         double fact = (ext < 80.0) ? exp(-ext) : 0.0;
         if (norm) fact = fact * iumax;
-        u_n = UU(k + 1) * fact;
-        t_n = TT(k + 1) * fact;
+        u_n = uu1 * fact;
+        t_n = tt1 * fact;
       }
     }
     double xmu = L.zrho * (L.zb * L.zb);
@@ -1335,7 +1358,7 @@ JDSG_DEV double entry_period(const EigenArgs &A, const EntryRef &r, int v) {
 }  // namespace
 
 template <int W>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W == 0 ? 2 : 3)))
 jdsg_eigen_pass(EigenArgs A, long long it0, long long lanes, int v) {
   const long long lane = chunk_lane(A, it0, lanes);
   const long long it = it0 + lane;
