@@ -48,11 +48,25 @@ struct PLane {
   long long e;          // entry
   int col, variant, phase, nit;
   double omega, cm, betmx, a, b, fa, fb, c;
+  double fat;           // F(a) itself (fa carries the Anderson-Bjorck weight)
   double cp;            // the previous iterate
+  double cen;           // the bracket's centre
   double a2, b2;        // the widened bracket ends (three half widths)
   double lo, hi;        // the bracket the root was taken from
   signed char s_floor, s_lo;
 };
+
+JDSG_DEV void polish_finish(const PolishArgs &A, const PLane &S, double root, signed char s_hi,
+                            unsigned &nf) {
+  long long q = (long long)S.variant * A.nent + S.e;
+  A.known[q] = root;
+  A.sg[q * 3 + 0] = S.s_lo;
+  A.sg[q * 3 + 1] = s_hi;
+  A.sg[q * 3 + 2] = S.s_floor;
+  A.brk[2 * q] = S.lo;
+  A.brk[2 * q + 1] = S.hi;
+  if (root == 0.0) ++nf;
+}
 
 // Decode item `it` into the lane; false when `it` is past the last item.
 // Items whose centre is unusable are finished here without an evaluation.
@@ -101,6 +115,7 @@ JDSG_DEV bool polish_begin(const PolishArgs &A, PLane &S, long long &it, long lo
     S.nit = 0;
     S.cp = 0.0;
     double cen = A.centers[v][e];
+    S.cen = cen;
     // a root above betmx is rejected, so a centre whose widest bracket starts
     // above betmx cannot give one: no evaluation is spent on it
     bool usable = cen > 0.0 && cen * (1.0 - 3.0 * A.hw[v]) < S.betmx;
@@ -112,11 +127,26 @@ JDSG_DEV bool polish_begin(const PolishArgs &A, PLane &S, long long &it, long lo
     } else {
       S.lo = S.hi = 0.0;
     }
-    if (mode == 0) {
+    // The floor sign of a T(1 +- 0.005) entry is its T entry's: cm lies
+    // below every root at every period (the chain starts its scans there), so
+    // F(cm, omega) has no zero between the three frequencies and keeps its
+    // sign.  Only where the T pass left no sign is it evaluated again.
+    const signed char s0 = (mode == 0 && v > 0) ? A.sg[e * 3 + 2] : (signed char)0;
+    if (mode == 0 && s0 == 0) {
       S.phase = PP_FLOOR;
       S.c = S.cm;
       if (!usable) S.a = S.b = 0.0;  // floor sign still wanted by mode order
       return true;
+    }
+    if (mode == 0) {
+      S.s_floor = s0;
+      if (usable) {
+        S.phase = PP_LO;
+        S.c = S.a;
+        return true;
+      }
+      polish_finish(A, S, 0.0, 0, nf);
+      continue;
     }
     if (usable) {
       S.phase = PP_LO;
@@ -135,18 +165,6 @@ JDSG_DEV bool polish_begin(const PolishArgs &A, PLane &S, long long &it, long lo
   return false;
 }
 
-JDSG_DEV void polish_finish(const PolishArgs &A, const PLane &S, double root, signed char s_hi,
-                            unsigned &nf) {
-  long long q = (long long)S.variant * A.nent + S.e;
-  A.known[q] = root;
-  A.sg[q * 3 + 0] = S.s_lo;
-  A.sg[q * 3 + 1] = s_hi;
-  A.sg[q * 3 + 2] = S.s_floor;
-  A.brk[2 * q] = S.lo;
-  A.brk[2 * q + 1] = S.hi;
-  if (root == 0.0) ++nf;
-}
-
 // Feed F(S.c); true when the lane's root is finished (results written).
 JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf) {
   switch (S.phase) {
@@ -161,6 +179,7 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
       return true;
     case PP_LO:
       S.fa = F;
+      S.fat = F;
       S.s_lo = (signed char)sign1(F);
       S.phase = PP_HI;
       S.c = S.b;
@@ -189,9 +208,10 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
     case PP_LO2:
       if (sign1(F) != sign1(S.fa)) {  // root in [a2, a]
         S.b = S.a;
-        S.fb = S.fa;
+        S.fb = S.fat;
         S.a = S.a2;
         S.fa = F;
+        S.fat = F;
         S.lo = S.a2;
         S.hi = S.b;
         S.s_lo = (signed char)sign1(F);
@@ -210,6 +230,7 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
       if (sign1(F) != sign1(S.fb)) {  // root in [b, b2]
         S.a = S.b;
         S.fa = S.fb;
+        S.fat = S.fb;
         S.b = S.b2;
         S.fb = F;
         S.lo = S.a;
@@ -222,13 +243,19 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
       return true;
     default: {  // false position step with F = F(S.c)
       double fc = F;
+      double xo = 0.0, fo = 0.0;  // the point that stops being an end
       if (fc == 0.0) {
         S.a = S.b = S.c;
       } else {
         if (sign1(fc) != sign1(S.fb)) {
+          xo = S.a;
+          fo = S.fat;
           S.a = S.b;
           S.fa = S.fb;
+          S.fat = S.fb;
         } else {
+          xo = S.b;
+          fo = S.fb;
           // the retained end keeps its place: Anderson-Bjorck scales its
           // value by 1 - fc/fb (0.5, the Illinois factor, when that is <= 0)
           double m = 1.0 - fc / S.fb;
@@ -243,6 +270,16 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
       if (!done) {
         double cn = (S.a * S.fb - S.b * S.fa) / (S.fb - S.fa);
         if (!(cn > fmin(S.a, S.b) && cn < fmax(S.a, S.b))) cn = 0.5 * (S.a + S.b);
+        // inverse quadratic interpolation through F at the two ends and at
+        // the point just dropped (as in Brent's method), taken when it falls
+        // inside the bracket: from a close centre or a first interpolation
+        // it reaches the tolerance one evaluation earlier than false position
+        if (fo != S.fat && fo != S.fb && S.fat != S.fb) {
+          const double x = xo * S.fat * S.fb / ((fo - S.fat) * (fo - S.fb)) +
+                           S.a * fo * S.fb / ((S.fat - fo) * (S.fat - S.fb)) +
+                           S.b * fo * S.fat / ((S.fb - fo) * (S.fb - S.fat));
+          if (x > fmin(S.a, S.b) && x < fmax(S.a, S.b)) cn = x;
+        }
         // the corrected value's error is about dc^2 / dc_prev for a linearly
         // converging step and less once convergence is superlinear: accept
         // it when that estimate is a tenth of the tolerance
@@ -266,6 +303,13 @@ JDSG_DEV bool polish_step(const PolishArgs &A, PLane &S, double F, unsigned &nf)
   }
   double c = (S.a * S.fb - S.b * S.fa) / (S.fb - S.fa);
   if (!(c > fmin(S.a, S.b) && c < fmax(S.a, S.b))) c = 0.5 * (S.a + S.b);
+  // A T(1 +- 0.005) centre, the polished T root moved along the slope of
+  // the neighbouring periods' roots, lies about 1e-6 from its root, much
+  // closer than the interpolation between the ends of its bracket (which is
+  // kept as wide as the chain's scan step, so that a close pair of roots
+  // shows as no sign change, as the chain sees it): false position starts
+  // at the centre when it lies inside the bracket (not after a widening)
+  if (S.variant > 0 && S.cen > fmin(S.a, S.b) && S.cen < fmax(S.a, S.b)) c = S.cen;
   S.c = c;
   return false;
 }
@@ -336,6 +380,7 @@ JDSG_DEV bool continue_begin(const PolishArgs &A, PLane &S, long long e, int col
   S.nit = 0;
   S.cp = 0.0;
   S.s_lo = 0;
+  S.cen = cen;
   if (!(cen > 0.0 && cen * (1.0 - 3.0 * A.hw[v]) < S.betmx)) return false;
   bracket_ends(cen, A.hw[v], S.cm, &S.a, &S.b);
   bracket_ends(cen, 3.0 * A.hw[v], S.cm, &S.a2, &S.b2);
