@@ -202,6 +202,7 @@ struct ChainState {
   int idir, ifirst;
   // nevill
   double c3, del3;
+  double del1t;  // exact-root polish: F(c1) itself (del1 carries the weight)
   int nev, nctrl, m;
   int phase;
   double ec;  // requested evaluation point
@@ -218,13 +219,15 @@ struct ChainStateSusp : ChainState {
 
 
 // Exact-root mode polishes a bracket [c1, c2] (del1, del2 of opposite sign)
-// by false position with the Anderson-Bjorck weight instead of nevill's
+// by false position with the Anderson-Bjorck weight (and the inverse quadratic
+// step where it falls inside) instead of nevill's
 // Neville/bisection hybrid, which bisects most of the way down to a 1e-10
 // bracket (about 25 evaluations a root against 5-8).  The iterate is c3, the
 // previous one del3 (0 before the second), the step count nctrl.
 JDSG_DEV void fp_begin(ChainState &S) {
   S.nctrl = 0;
   S.del3 = 0.0;
+  S.del1t = S.del1;
   double c = (S.c1 * S.del2 - S.c2 * S.del1) / (S.del2 - S.del1);
   if (!(c > fmin(S.c1, S.c2) && c < fmax(S.c1, S.c2))) c = 0.5 * (S.c1 + S.c2);
   S.c3 = c;
@@ -517,14 +520,20 @@ __device__ __attribute__((noinline)) bool advance(
         // c1 the retained end, c2 the newest; the retained end's value is
         // scaled by 1 - F/del2 (0.5 when that is <= 0) while it stays
         const double fc = F;
+        double xo = 0.0, fo = 0.0;  // the point that stops being an end
         if (fc == 0.0) {
           S.c1 = S.c3;
           S.c2 = S.c3;
         } else {
           if (sign1(fc) != sign1(S.del2)) {
+            xo = S.c1;
+            fo = S.del1t;
             S.c1 = S.c2;
             S.del1 = S.del2;
+            S.del1t = S.del2;
           } else {
+            xo = S.c2;
+            fo = S.del2;
             const double m = 1.0 - fc / S.del2;
             S.del1 = ((m > 0.0) ? m : 0.5) * S.del1;
           }
@@ -537,6 +546,14 @@ __device__ __attribute__((noinline)) bool advance(
         if (!done) {
           double cn = (S.c1 * S.del2 - S.c2 * S.del1) / (S.del2 - S.del1);
           if (!(cn > fmin(S.c1, S.c2) && cn < fmax(S.c1, S.c2))) cn = 0.5 * (S.c1 + S.c2);
+          // inverse quadratic interpolation through F at both ends and at
+          // the point just dropped (Brent's step), when it falls inside
+          if (fo != S.del1t && fo != S.del2 && S.del1t != S.del2) {
+            const double x = xo * S.del1t * S.del2 / ((fo - S.del1t) * (fo - S.del2)) +
+                             S.c1 * fo * S.del2 / ((S.del1t - fo) * (S.del1t - S.del2)) +
+                             S.c2 * fo * S.del1t / ((S.del2 - fo) * (S.del2 - S.del1t));
+            if (x > fmin(S.c1, S.c2) && x < fmax(S.c1, S.c2)) cn = x;
+          }
           // the corrected value's error is about d1^2 / d0 for a linearly
           // converging step and less once convergence is superlinear
           const double d1 = fabs(cn - S.c3), d0 = fabs(S.c3 - S.del3), cc = fabs(S.c3);
