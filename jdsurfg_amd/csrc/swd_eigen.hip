@@ -8,9 +8,10 @@
 // with the eigenfunction assembly (svfunc :193), the energy integrals
 // (energy :1061, intijr :1198 -- E/E^-1 and the layer functions are formed
 // once per layer instead of once per integral) and the dc/dh terms (getdcdh
-// :1430).  With group kernels the lane runs the three periods T, T(1+.005),
-// T(1-.005) and assembles dU/dm with the reference's formula, which reads the
-// dc/dm left by the T(1-.005) pass (sregn96.f90:1831-1834).
+// :1430).  With group kernels a pass kernel runs each of the periods T,
+// T(1+.005), T(1-.005) and a combine kernel assembles dU/dm with the
+// reference's formula, which reads the dc/dm left by the T(1-.005) pass
+// (sregn96.f90:1831-1834).
 //
 // Outputs per entry: spherical c and U, and per layer dc/d{a,b,rho,h} and
 // dU/d{a,b,rho,h} (h: layer thickness, half-space 0), at
@@ -789,17 +790,14 @@ struct Dst {
   long long base, stride;
 };
 
-// The Dunkin up-sweep is kept as checkpoints: its vector (5 entries and the
-// exponent sum) at layers 0, S, 2S, ... and at the half-space, in the lane's
-// global scratch; the down-sweep recomputes each segment of S layers from
-// the checkpoint below it into LDS (S x 6 doubles per lane) and reads it
-// there.  The recomputation repeats the first sweep's operations from the
-// same stored values, so every vector is the one the full sweep gives; the
-// global traffic per lane drops from 6 doubles per layer to 6 per S layers.
-constexpr int kUpSeg = 6;
-JDSG_DEV int up_slot(int m, int N) {
-  return (m == N - 1) ? (N - 2) / kUpSeg + 1 : m / kUpSeg;
-}
+// The Dunkin up-sweep's vector (5 entries and the exponent sum) at the top
+// of every layer goes to the lane's global scratch (lane-strided, so each
+// store of a wave is one 512-byte line); the down-sweep reads interface i + 1
+// while it works on interface i.  Measured against keeping every sixth
+// vector and recomputing the segments into LDS: 120 ms against 136 ms for
+// the C5 Rayleigh stage (the recomputation cost a second Dunkin sweep; the
+// stored vectors add about 6 GB of HBM traffic per launch, well inside the
+// bandwidth left beside the sweeps' arithmetic).
 
 // One step of the up-sweep (sregn96.f90:455-487) through layer L: cd, the
 // vector at the top of the layer below, becomes the vector at the top of L.
@@ -819,8 +817,7 @@ JDSG_DEV void dunkin_step(const Layer &L, double omega, double wvno2, const Wk &
 
 JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, double wvno,
                             double *scr, long long sstride, const Dst &D, double &ugr,
-                            double &sumi0, double &fac, double *eig, long long ebase,
-                            double *seg) {
+                            double &sumi0, double &fac, double *eig, long long ebase) {
   double om2 = omega * omega, wvno2 = wvno * wvno;
   const Wk K = make_wk(omega, wvno);
   bool allfluid = true;
@@ -835,10 +832,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
   }
   if (allfluid) jwat = 0;
   auto ck = [&](int m, int q) -> double & {
-    return scr[((long long)up_slot(m, N) * 6 + q) * sstride];
+    return scr[((long long)m * 6 + q) * sstride];
   };
-  // ---- up: Dunkin vectors from the half-space (sregn96.f90:401-489),
-  // checkpoints only
+  // ---- up: Dunkin vectors from the half-space (sregn96.f90:401-489)
   {
     Layer H = load_layer(A, col, N - 1);
     EMat M;
@@ -859,11 +855,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
       const Layer Ln = load_layer(A, col, max(m - 1, 0));
       dunkin_step(Lc, omega, wvno2, K, cd, exsum);
       Lc = Ln;
-      if (m % kUpSeg == 0) {
 #pragma unroll
-        for (int j = 0; j < 5; ++j) ck(m, j) = cd[j];
-        ck(m, 5) = exsum;
-      }
+      for (int j = 0; j < 5; ++j) ck(m, j) = cd[j];
+      ck(m, 5) = exsum;
     }
   }
   // ---- down + svfunc + energy + getdcdh, top to bottom
@@ -895,6 +889,9 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     }
   }
   Layer Lm1 = Lprev;  // layer m-1 props for getdcdh
+  double nx[6];       // the stored up-sweep vector of the next interface
+#pragma unroll
+  for (int j = 0; j < 6; ++j) nx[j] = ck(min(1, N - 1), j);
   double xmu_m1 = xmu_prev, xlam_m1 = xlam_prev;
   for (int m = 0; m < N - 1; ++m) {
     Layer L = Lprev;  // layer m
@@ -910,35 +907,14 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     exsum = exsum + pex + ex2;
     // eigenfunction at the top of layer m+1 (svfunc)
     int i1 = m + 1;
-    const int top = ((i1 - 1) / kUpSeg) * kUpSeg + 1;
-    if (i1 == top) {
-      // recompute the up-sweep vectors of layers top..bot from the
-      // checkpoint at bot (a multiple of kUpSeg, or the half-space)
-      const int bot = min(top + kUpSeg - 1, N - 1);
-      // layer mm - 1 is loaded while layer mm is swept (the first one with
-      // the checkpoint)
-      Layer Lc = load_layer(A, col, max(bot - 1, top));
-      double cu[5], eu;
+    // the up-sweep vector there (loaded one interface ahead)
+    const double cd1 = nx[0], cd2 = nx[1], cd3 = nx[2], cd4 = -cd3, cd5 = nx[3], cd6 = nx[4],
+                 exe_i = nx[5];
+    {
+      const int i2 = min(i1 + 1, N - 1);
 #pragma unroll
-      for (int j = 0; j < 5; ++j) cu[j] = ck(bot, j);
-      eu = ck(bot, 5);
-      double *sb = seg + (long long)(bot - top) * 6 * 64;
-#pragma unroll
-      for (int j = 0; j < 5; ++j) sb[j * 64] = cu[j];
-      sb[5 * 64] = eu;
-      for (int mm = bot - 1; mm >= top; --mm) {
-        const Layer Ln = load_layer(A, col, max(mm - 1, top));
-        dunkin_step(Lc, omega, wvno2, K, cu, eu);
-        Lc = Ln;
-        double *sm = seg + (long long)(mm - top) * 6 * 64;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) sm[j * 64] = cu[j];
-        sm[5 * 64] = eu;
-      }
+      for (int j = 0; j < 6; ++j) nx[j] = ck(i2, j);
     }
-    const double *sv = seg + (long long)(i1 - top) * 6 * 64;
-    double cd1 = sv[0], cd2 = sv[64], cd3 = sv[2 * 64], cd4 = -cd3, cd5 = sv[3 * 64],
-           cd6 = sv[4 * 64], exe_i = sv[5 * 64];
     double tz1 = -vv[3], tz2 = -vv[2], tz3 = vv[1], tz4 = vv[0];
     double uu1 = tz2 * cd6 - tz3 * cd5 + tz4 * cd4;
     double uu2 = -tz1 * cd6 + tz3 * cd3 - tz4 * cd2;
@@ -1382,9 +1358,8 @@ jdsg_eigen_pass(EigenArgs A, long long it0, long long lanes, int v) {
   long long ebase = (A.eig[0] && A.kern_off) ? A.kern_off[r.col] + r.el * r.N : 0;
   double ugr, s0, fac;
   if constexpr (W == 0) {
-    __shared__ double seg[kUpSeg * 6 * 64];
     rayleigh_pass(A, r.col, r.N, omega, wvno, scr, ss, D, ugr, s0, fac,
-                  (v == 0 && A.eig[0]) ? A.eig[0] : nullptr, ebase, seg + threadIdx.x);
+                  (v == 0 && A.eig[0]) ? A.eig[0] : nullptr, ebase);
   } else {
     love_pass(A, r.col, r.N, omega, wvno, A.group != 0, scr, ss, D, ugr, s0, fac,
               v == 0 && A.eig[0] != nullptr, ebase);
