@@ -851,6 +851,7 @@ JDSG_DEV void rayleigh_pass(const EigenArgs &A, int col, int N, double omega, do
     // layer m - 1 is loaded while layer m is swept (its loads are in flight
     // during the step instead of stalling the next one)
     Layer Lc = load_layer(A, col, max(N - 2, 0));
+#pragma unroll 4
     for (int m = N - 2; m >= 0; --m) {
       const Layer Ln = load_layer(A, col, max(m - 1, 0));
       dunkin_step(Lc, omega, wvno2, K, cd, exsum);
@@ -1074,6 +1075,7 @@ JDSG_DEV void love_pass(const EigenArgs &A, int col, int N, double omega, double
     double u1 = UU(N - 1), t1v = TT(N - 1);
     // layer k - 1 is loaded while layer k is swept
     Layer Lc = load_layer_love(A, col, max(N - 2, 0), pu);
+#pragma unroll 4
     for (int k = N - 2; k >= 0; --k) {
       const Layer L = Lc;
       Lc = load_layer_love(A, col, max(k - 1, 0), pu);
