@@ -311,7 +311,7 @@ def uneven_main(args):
     line["surrogate"] = {"value": sr["column_periods_per_s"], "unit": "column·periods/s",
                          "ms_per_step": sr["graph_ms_mean"], "graph_ms": sr["graph_ms"],
                          "eager_ms": sr["eager_ms"], "fallback_rate": sr["fallback_rate"],
-                         "network_width": sr["network_width"],
+                         "network_width": sr["network_width"], "weights": sr["weights"],
                          "speedup_vs_plain": rep["graph_ms_mean"] / sr["graph_ms_mean"]}
     print(json.dumps(line), flush=True)
 
@@ -511,7 +511,7 @@ def main():
         from jdsurfg_amd import surrogate as S
         wpath = args.weights or (os.path.join(os.path.dirname(S.DEFAULT_WEIGHTS),
                                               "surrogate_mlp_512.npz")
-                                 if args.workload == "grid128" else S.DEFAULT_WEIGHTS)
+                                 if args.workload == "grid128" else S.weights_for(w.nmode))
         mlp = S.MLP.load(wpath)
         sp = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, layout)
         sp.set_tolerance(1e-10)

@@ -10,6 +10,7 @@ SurfTime::get_1d_kernel once per inversion iteration (surftomo.cpp:27-138);
 the graph turns that into one launch per iteration.
 """
 import json
+import os
 import time
 
 import numpy as np
@@ -76,7 +77,8 @@ def run_surrogate_iterations(w, iters=10, amp=0.01, seed=1, kernels=("dcdb", "du
     Returns a report dict (times in ms) with the fallback rate."""
     import torch
     from . import surrogate as S
-    mlp = mlp or S.MLP.load(S.DEFAULT_WEIGHTS)
+    wpath = S.weights_for(w.nmode) if mlp is None else "given"
+    mlp = mlp or S.MLP.load(wpath)
     plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
     plan.set_tolerance(1e-10)
     S.bind_network(plan, mlp)
@@ -84,7 +86,9 @@ def run_surrogate_iterations(w, iters=10, amp=0.01, seed=1, kernels=("dcdb", "du
     out = plan.alloc_outputs(kernels)
     st = torch.cuda.Stream()
     rep = {"workload": w.name, "columns": w.ncol, "column_periods": w.column_periods(),
-           "iterations": iters, "perturbation": amp, "network_width": mlp.hid}
+           "iterations": iters, "perturbation": amp, "network_width": mlp.hid,
+           "weights": os.path.relpath(wpath, os.path.dirname(os.path.dirname(
+               os.path.abspath(__file__)))) if wpath != "given" else wpath}
     with torch.cuda.stream(st):
         S.surrogate_solve_net(plan, inp, out, stream=st.cuda_stream)
         st.synchronize()

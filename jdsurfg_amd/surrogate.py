@@ -226,7 +226,7 @@ class MLP:
         return self._consts
 
 
-def training_columns(ncol, seed, K=40):
+def training_columns(ncol, seed, K=40, love_modes=3):
     """Seeded random layered columns spanning the workload families, built
     vectorised over all columns.  Each column draws
       - 16-128 finite layers: uniform thickness 0.8-2 km (grid models) or
@@ -238,7 +238,8 @@ def training_columns(ncol, seed, K=40):
         with probability 1/2; per-layer noise (sd 0.075) for uneven stacks;
         and a soft sediment top (0-6 km, 0.7-2.0 km/s) with probability 1/2;
         Vs kept in 0.6-5.6 km/s;
-      - K log-uniform periods in 1-80 s; Rayleigh modes 0-3, Love modes 0-1.
+      - K log-uniform periods in 1-80 s; Rayleigh modes 0-3, Love modes 0 to
+        love_modes - 1 (the batched workloads ask for up to three of each).
     Vp and rho follow Brocher's relations as in the workloads."""
     rng = np.random.default_rng(seed)
     uneven = rng.random(ncol) < 0.3
@@ -296,7 +297,7 @@ def training_columns(ncol, seed, K=40):
     vp, rho = W.brocher(vs)
     return W.Workload("train", layer_off, thk, vp, vs, rho,
                       np.arange(ncol + 1) * K, periods.ravel(),
-                      np.tile(np.array([4, 2], np.int32), (ncol, 1)), group=False)
+                      np.tile(np.array([4, love_modes], np.int32), (ncol, 1)), group=False)
 
 
 def make_dataset(w):
@@ -332,12 +333,12 @@ class Features:
 
 
 def train(ncol=200000, steps=3000, batch=8192, lr=1e-3, seed=7, log=print, valid_cols=4000,
-          huber=0.02, hid=HID):
+          huber=0.02, hid=HID, love_modes=3):
     """Generate data with the plain GPU solver and train the MLP on the GPU.
     Returns (mlp, report)."""
     import torch
     t0 = time.time()
-    w = training_columns(ncol + valid_cols, seed)
+    w = training_columns(ncol + valid_cols, seed, love_modes=love_modes)
     plan, inp, roots = make_dataset(w)
     t_data = time.time() - t0
     ok = torch.nonzero(roots > 0).squeeze(1)
@@ -570,6 +571,19 @@ def surrogate_stats(plan):
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
                                "surrogate_mlp.npz")
+# the same width and training, with Love modes 0-2 in the training set
+L3_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights",
+                          "surrogate_mlp_l3.npz")
+
+
+def weights_for(nmode):
+    """The shipped width-256 network for a batch's modes: the one trained on
+    Rayleigh modes 0-3 and Love modes 0-1, which is the more accurate on
+    those, unless a column asks for a third Love mode, which only the network
+    trained on Love modes 0-2 as well predicts (the other one never saw that
+    mode's input and misses most of its roots)."""
+    nm = np.asarray(nmode).reshape(-1, 2)
+    return L3_WEIGHTS if nm.size and int(nm[:, 1].max()) > 2 else DEFAULT_WEIGHTS
 
 
 def _timed(fn, reps=1):
@@ -651,23 +665,25 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=7)
     t.add_argument("--hid", type=int, default=HID, help="hidden width (multiple of 128)")
+    t.add_argument("--love-modes", type=int, default=3, help="Love modes in the training set")
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     e = sub.add_parser("eval")
-    e.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    e.add_argument("--weights", default=None, help="default: the network the workload's "
+                   "modes call for (weights_for)")
     e.add_argument("--workload", default="grid128")
     a = ap.parse_args(argv)
     if not torch.cuda.is_available():
         raise SystemExit("the surrogate runs on the GPU")
     if a.cmd == "train":
         mlp, rep = train(ncol=a.ncol, steps=a.steps, batch=a.batch, lr=a.lr, seed=a.seed,
-                         log=lambda m: print(m, flush=True), hid=a.hid)
+                         log=lambda m: print(m, flush=True), hid=a.hid, love_modes=a.love_modes)
         os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
         mlp.save(a.out)
         rep["weights"] = a.out
         print(json.dumps(rep), flush=True)
     else:
-        mlp = MLP.load(a.weights)
         w = W.BUILDERS[a.workload]()
+        mlp = MLP.load(a.weights or weights_for(w.nmode))
         print(json.dumps(compare_solves(w, mlp)), flush=True)
 
 

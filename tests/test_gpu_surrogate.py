@@ -484,3 +484,42 @@ def test_split_first_layer_matches_full_rows():
     assert np.all(np.isfinite(split.cpu().numpy()))
     assert rel.max() <= 1e-3
     assert np.median(rel) <= 1e-5
+
+
+def test_love_mode2_network_gain():
+    """The network trained with Love modes 0-2 (weights_for picks it when a
+    column asks for a third Love mode) brackets Love mode-2 roots that the
+    network trained on Love modes 0-1 misses: on 1,500 columns of the uneven
+    batch, the share of rooted Love mode-2 T entries whose prediction lies
+    inside the +-0.5 % bracket, and the same share over the other modes."""
+    import torch
+    from jdsurfg_amd import swd, surrogate as S
+    w = W.uneven(ncol=1500)
+    assert S.weights_for(w.nmode) == S.L3_WEIGHTS
+    print(f"case Love mode-2 network, uneven ncol={w.ncol}", flush=True)
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    plan.set_tolerance(EXACT_TOL)
+    inp = swd.upload(w)
+    out = plan.alloc_outputs()
+    plan.execute(*inp, out)
+    torch.cuda.synchronize()
+    roots = plan.flat_roots()[0]
+    feats = S.Features(plan, inp)
+    nm = np.asarray(w.nmode).reshape(w.ncol, 2)
+    K = np.diff(np.asarray(w.per_off))
+    wave = np.concatenate([np.repeat(np.repeat([0, 1], nm[c]), K[c]) for c in range(w.ncol)])
+    mode = np.concatenate([np.tile(np.repeat(np.concatenate([np.arange(nm[c, 0]),
+                                                             np.arange(nm[c, 1])]), K[c]), 1)
+                           for c in range(w.ncol)])
+    l2 = (wave == 1) & (mode == 2) & (roots > 0)
+    rest = (roots > 0) & ~((wave == 1) & (mode == 2))
+    share = {}
+    for name, path in (("l01", S.DEFAULT_WEIGHTS), ("l012", S.L3_WEIGHTS)):
+        pred = S.predict(S.MLP.load(path), feats, e0=0, n=plan.nent).cpu().numpy()
+        inside = np.abs(pred - roots) <= 0.005 * np.where(roots > 0, roots, 1.0)
+        share[name] = (float(inside[l2].mean()), float(inside[rest].mean()))
+    print(f"Love mode-2 / other rooted entries inside the bracket: {share}, "
+          f"{int(l2.sum())} Love mode-2 roots", flush=True)
+    assert l2.sum() > 1000
+    assert share["l012"][0] >= 0.9 and share["l01"][0] <= 0.6
+    assert share["l012"][1] >= 0.9
