@@ -479,22 +479,35 @@ __device__ __forceinline__ uint4 first_layer8(const EntryIn &in, const float (&a
                     pack_bf2(z[6], z[7]));
 }
 
-// Y(row, j) = first layer of entry e0 + row (row < nrow).  One thread per
-// 8 outputs of a row (one 16-byte store).
+// Y(row, j) = first layer of entry e0 + row (row < nrow).  A workgroup takes
+// a tile of kExpandRows rows; two of its threads find the columns of the
+// tile's first and last rows (binary searches over the columns, in two
+// waves at once), and every row then searches only between those two: the
+// entries of a column are contiguous, so that is one or two columns.  A
+// thread writes eight outputs of a row (one 16-byte store) per step.
+constexpr int kExpandRows = 64;
 __global__ void __launch_bounds__(256)
 jdsg_mlp_expand(long long e0, long long nrow, int ncol, const long long *__restrict__ ent_off,
                 const int *__restrict__ per_off, const double *__restrict__ periods,
                 const int *__restrict__ nmode, const float *__restrict__ acol, int lda,
                 const float *__restrict__ ext, int H, uint16_t *Y) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int s_lo, s_hi;
+  const long long r0 = (long long)blockIdx.x * kExpandRows;
+  const long long r1 = (r0 + kExpandRows < nrow ? r0 + kExpandRows : nrow) - 1;
+  if (threadIdx.x == 0) s_lo = entry_column(e0 + r0, 0, ncol - 1, ent_off);
+  if (threadIdx.x == 64) s_hi = entry_column(e0 + r1, 0, ncol - 1, ent_off);
+  __syncthreads();
+  const int lo = s_lo, hi = s_hi;
   const int groups = H >> 3;
-  const long long row = t / groups;
-  if (row >= nrow) return;
-  const int j0 = (int)(t % groups) * 8;
-  const EntryIn in = entry_inputs(e0 + row, 0, ncol - 1, ent_off, per_off, periods, nmode);
-  float a[8];
-  load8(acol + (size_t)in.col * lda + j0, a);
-  *reinterpret_cast<uint4 *>(Y + row * H + j0) = first_layer8(in, a, ext, H, j0);
+  const int ntask = (int)(r1 - r0 + 1) * groups;
+  for (int task = threadIdx.x; task < ntask; task += blockDim.x) {
+    const long long row = r0 + task / groups;
+    const int j0 = (task % groups) * 8;
+    const EntryIn in = entry_inputs(e0 + row, lo, hi, ent_off, per_off, periods, nmode);
+    float a[8];
+    load8(acol + (size_t)in.col * lda + j0, a);
+    *reinterpret_cast<uint4 *>(Y + row * H + j0) = first_layer8(in, a, ext, H, j0);
+  }
 }
 
 // The whole network at inference for 128 entries per workgroup, hidden width
@@ -660,9 +673,9 @@ void jdsg_launch_mlp_expand(long long e0, long long nrow, int ncol, const long l
                             const int *per_off, const double *periods, const int *nmode,
                             const float *acol, int lda, const float *ext, int H, uint16_t *Y,
                             hipStream_t st) {
-  const long long n = nrow * (H / 8);
-  if (n <= 0) return;
-  hipLaunchKernelGGL(jdsg_mlp_expand, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, e0,
+  if (nrow <= 0 || H <= 0) return;
+  hipLaunchKernelGGL(jdsg_mlp_expand, dim3((unsigned)((nrow + kExpandRows - 1) / kExpandRows)),
+                     dim3(256), 0, st, e0,
                      nrow, ncol, ent_off, per_off, periods, nmode, acol, lda, ext, H, Y);
 }
 
