@@ -267,7 +267,8 @@ int jdsg_plan_features(jdsg_plan *plan, const float *thk, const float *vp, const
  * first weights gives acol (fp32 [ncol][lda]).  jdsg_plan_mlp_expand then
  * writes Y = gelu(acol + bf16(lnT) ext[0] + ext[1+wave] + ext[3+mode] +
  * ext[7]) (bf16 [nrow][H]) for entries e0..e0+nrow-1, ext (fp32 [8][H])
- * holding the weight columns 144..150 and the bias. */
+ * holding the weight columns 144..150 and the bias; H a multiple of 8 up to
+ * 1024, lda >= H a multiple of 4, acol, ext and Y 16-byte aligned. */
 int jdsg_mlp_column_rows(int ncol, const float *colf, void *X, void *stream);
 int jdsg_plan_mlp_expand(jdsg_plan *plan, const double *periods, long long e0, long long nrow,
                          const float *acol, int lda, const float *ext, int H, void *Y,

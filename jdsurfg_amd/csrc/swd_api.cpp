@@ -933,8 +933,11 @@ extern "C" int jdsg_plan_mlp_expand(jdsg_plan *P, const double *periods, long lo
                                     long long nrow, const float *acol, int lda, const float *ext,
                                     int H, void *Y, void *stream) {
   if (e0 < 0 || nrow < 0 || e0 + nrow > P->nent) return fail("expand rows out of range");
-  if (H <= 0 || H % 8 || lda < H || lda % 4)
-    return fail("expand: H must be a positive multiple of 8 <= lda, lda a multiple of 4");
+  if (H <= 0 || H % 8 || H > 1024 || lda < H || lda % 4)
+    return fail("expand: H must be a positive multiple of 8, at most 1024 and <= lda, lda a "
+                "multiple of 4");
+  if (((uintptr_t)acol | (uintptr_t)ext | (uintptr_t)Y) & 15)
+    return fail("expand: acol, ext and Y must be 16-byte aligned");
   jdsg_launch_mlp_expand(e0, nrow, P->ncol, P->ent_off_d, P->per_off_d, periods, P->nmode_d, acol,
                          lda, ext, H, (uint16_t *)Y, (hipStream_t)stream);
   HIPCHK(hipGetLastError());

@@ -440,11 +440,6 @@ struct EntryIn {
   int col, wave, mode;
   float lt;
 };
-__device__ __forceinline__ void load8(const float *p, float (&a)[8]) {
-  const float4 u = reinterpret_cast<const float4 *>(p)[0], v = reinterpret_cast<const float4 *>(p)[1];
-  a[0] = u.x; a[1] = u.y; a[2] = u.z; a[3] = u.w;
-  a[4] = v.x; a[5] = v.y; a[6] = v.z; a[7] = v.w;
-}
 __device__ __forceinline__ EntryIn entry_inputs(long long e, int lo, int hi,
                                                 const long long *__restrict__ ent_off,
                                                 const int *__restrict__ per_off,
@@ -462,51 +457,78 @@ __device__ __forceinline__ EntryIn entry_inputs(long long e, int lo, int hi,
   r.lt = bf2f(f2bf((float)((log(periods[per_off[lo] + k]) - 2.0) / 1.5)));
   return r;
 }
-// Eight first-layer outputs j0..j0+7 of an entry as packed bf16 from its
-// column's pre-activations a[0..7] = acol(col, j0..j0+7):
-// gelu(a + lt ext[0] + ext[1 + wave] + ext[3 + mode] + ext[7]); modes above
-// 3 have no one-hot input, so no mode term.
-__device__ __forceinline__ uint4 first_layer8(const EntryIn &in, const float (&a)[8],
-                                              const float *ext, int H, int j0) {
-  float z[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int j = j0 + q;
-    const float m = in.mode < 4 ? ext[(3 + in.mode) * H + j] : 0.f;
-    z[q] = gelu(a[q] + in.lt * ext[j] + ext[(1 + in.wave) * H + j] + m + ext[7 * H + j]);
-  }
-  return make_uint4(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]), pack_bf2(z[4], z[5]),
-                    pack_bf2(z[6], z[7]));
+// Four first-layer outputs j..j+3 of an entry as packed bf16 from its
+// column's pre-activations a = acol(col, j..j+3):
+// gelu(a + lt ext[0] + ext[1 + wave] + ext[3 + mode] + ext[7]).  ext is the
+// LDS copy of the eight input rows with a ninth row of zeros, the mode term
+// of modes above 3 (they have no one-hot input); j is a multiple of 4, so
+// every row is read as one 16-byte vector.
+__device__ __forceinline__ uint2 first_layer4(const EntryIn &in, const float4 a,
+                                              const float *ext, int H, int j) {
+  const int mrow = in.mode < 4 ? 3 + in.mode : 8;
+  const float4 e0 = *reinterpret_cast<const float4 *>(ext + j);
+  const float4 ew = *reinterpret_cast<const float4 *>(ext + (1 + in.wave) * H + j);
+  const float4 em = *reinterpret_cast<const float4 *>(ext + mrow * H + j);
+  const float4 e7 = *reinterpret_cast<const float4 *>(ext + 7 * H + j);
+  uint2 r;
+  r.x = pack_bf2(gelu(a.x + in.lt * e0.x + ew.x + em.x + e7.x),
+                 gelu(a.y + in.lt * e0.y + ew.y + em.y + e7.y));
+  r.y = pack_bf2(gelu(a.z + in.lt * e0.z + ew.z + em.z + e7.z),
+                 gelu(a.w + in.lt * e0.w + ew.w + em.w + e7.w));
+  return r;
+}
+// ext (8 x H, global) into LDS rows 0..7 and zeros into row 8, by the
+// workgroup's nt threads.
+__device__ __forceinline__ void stage_ext(float *sx, const float *__restrict__ ext, int H, int tid,
+                                          int nt) {
+  const int n4 = 2 * H;  // float4s in eight rows
+  for (int q = tid; q < n4; q += nt)
+    reinterpret_cast<float4 *>(sx)[q] = reinterpret_cast<const float4 *>(ext)[q];
+  for (int q = tid; q < H / 4; q += nt)
+    reinterpret_cast<float4 *>(sx)[n4 + q] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Y(row, j) = first layer of entry e0 + row (row < nrow).  A workgroup takes
-// a tile of kExpandRows rows; two of its threads find the columns of the
-// tile's first and last rows (binary searches over the columns, in two
-// waves at once), and every row then searches only between those two: the
-// entries of a column are contiguous, so that is one or two columns.  A
-// thread writes eight outputs of a row (one 16-byte store) per step.
-constexpr int kExpandRows = 64;
+// a tile of kExpandRows rows: it stages ext in LDS (dynamic, 36 H bytes),
+// two of its threads find the columns of the tile's first and last rows
+// (binary searches over the columns, in two waves at once), one thread per
+// row then forms that row's inputs (column, wave, mode, ln T; its column
+// searched only between the tile's two, since the entries of a column are
+// contiguous) into LDS, and a thread then writes features 4g..4g+3 and
+// H/2+4g..H/2+4g+3 of a row per step, so that the lanes of a wave read and
+// write contiguous 16- and 8-byte pieces.
+constexpr int kExpandRows = 256;
 __global__ void __launch_bounds__(256)
 jdsg_mlp_expand(long long e0, long long nrow, int ncol, const long long *__restrict__ ent_off,
                 const int *__restrict__ per_off, const double *__restrict__ periods,
                 const int *__restrict__ nmode, const float *__restrict__ acol, int lda,
                 const float *__restrict__ ext, int H, uint16_t *Y) {
+  extern __shared__ __attribute__((aligned(16))) float sx[];
   __shared__ int s_lo, s_hi;
+  __shared__ EntryIn s_in[kExpandRows];
+  const int tid = threadIdx.x;
   const long long r0 = (long long)blockIdx.x * kExpandRows;
   const long long r1 = (r0 + kExpandRows < nrow ? r0 + kExpandRows : nrow) - 1;
-  if (threadIdx.x == 0) s_lo = entry_column(e0 + r0, 0, ncol - 1, ent_off);
-  if (threadIdx.x == 64) s_hi = entry_column(e0 + r1, 0, ncol - 1, ent_off);
+  if (tid == 0) s_lo = entry_column(e0 + r0, 0, ncol - 1, ent_off);
+  if (tid == 64) s_hi = entry_column(e0 + r1, 0, ncol - 1, ent_off);
+  stage_ext(sx, ext, H, tid, blockDim.x);
   __syncthreads();
-  const int lo = s_lo, hi = s_hi;
-  const int groups = H >> 3;
-  const int ntask = (int)(r1 - r0 + 1) * groups;
-  for (int task = threadIdx.x; task < ntask; task += blockDim.x) {
-    const long long row = r0 + task / groups;
-    const int j0 = (task % groups) * 8;
-    const EntryIn in = entry_inputs(e0 + row, lo, hi, ent_off, per_off, periods, nmode);
-    float a[8];
-    load8(acol + (size_t)in.col * lda + j0, a);
-    *reinterpret_cast<uint4 *>(Y + row * H + j0) = first_layer8(in, a, ext, H, j0);
+  const int nr = (int)(r1 - r0 + 1);
+  if (tid < nr)
+    s_in[tid] = entry_inputs(e0 + r0 + tid, s_lo, s_hi, ent_off, per_off, periods, nmode);
+  __syncthreads();
+  const int groups = H >> 3, half = H >> 1;
+  const int ntask = nr * groups;
+  for (int task = tid; task < ntask; task += blockDim.x) {
+    const int r = task / groups;
+    const int j = (task - r * groups) * 4;
+    const EntryIn in = s_in[r];
+    const float *ac = acol + (size_t)in.col * lda;
+    const float4 a0 = *reinterpret_cast<const float4 *>(ac + j);
+    const float4 a1 = *reinterpret_cast<const float4 *>(ac + half + j);
+    uint16_t *y = Y + (r0 + r) * H;
+    *reinterpret_cast<uint2 *>(y + j) = first_layer4(in, a0, sx, H, j);
+    *reinterpret_cast<uint2 *>(y + half + j) = first_layer4(in, a1, sx, H, half + j);
   }
 }
 
@@ -549,14 +571,11 @@ jdsg_mlp_fused(long long e0, long long nrow, int ncol, const long long *__restri
                double scale, double shift, double *pred, long long p0,
                const int *__restrict__ tcol) {
   __shared__ __attribute__((aligned(16))) uint16_t X[FR][FLD];
-  __shared__ __attribute__((aligned(16))) float sext[8 * FH];
+  __shared__ __attribute__((aligned(16))) float sext[9 * FH];
   __shared__ EntryIn sin_[FR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long long r0 = (long long)blockIdx.x * FR;
-#pragma unroll
-  for (int q = 0; q < 2; ++q)
-    reinterpret_cast<float4 *>(sext)[q * 256 + tid] =
-        reinterpret_cast<const float4 *>(ext)[q * 256 + tid];
+  stage_ext(sext, ext, FH, tid, 256);
   if (tid < FR) {
     long long row = r0 + tid;
     if (row >= nrow) row = nrow - 1;  // tail rows repeat the last entry, not stored
@@ -576,19 +595,23 @@ jdsg_mlp_fused(long long e0, long long nrow, int ncol, const long long *__restri
   }
   __syncthreads();
   {
-    // a thread keeps features j0..j0+7 and walks rows tid/32 + 8i; rows of
-    // one column share acol, loaded again only when the column changes
-    const int j0 = (tid & 31) * 8;
+    // a thread keeps features j..j+3 and FH/2+j..FH/2+j+3 and walks rows
+    // tid/32 + 8i; rows of one column share acol, loaded again only when
+    // the column changes
+    const int j = (tid & 31) * 4;
     int pc = -1;
-    float a[8];
+    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
     for (int i = 0; i < FR / 8; ++i) {
       const int row = i * 8 + (tid >> 5);
       const EntryIn in = sin_[row];
       if (in.col != pc) {
-        load8(acol + (size_t)in.col * lda + j0, a);
+        const float *ac = acol + (size_t)in.col * lda;
+        a0 = *reinterpret_cast<const float4 *>(ac + j);
+        a1 = *reinterpret_cast<const float4 *>(ac + FH / 2 + j);
         pc = in.col;
       }
-      *reinterpret_cast<uint4 *>(&X[row][j0]) = first_layer8(in, a, sext, FH, j0);
+      *reinterpret_cast<uint2 *>(&X[row][j]) = first_layer4(in, a0, sext, FH, j);
+      *reinterpret_cast<uint2 *>(&X[row][FH / 2 + j]) = first_layer4(in, a1, sext, FH, FH / 2 + j);
     }
   }
   __syncthreads();
@@ -675,7 +698,7 @@ void jdsg_launch_mlp_expand(long long e0, long long nrow, int ncol, const long l
                             hipStream_t st) {
   if (nrow <= 0 || H <= 0) return;
   hipLaunchKernelGGL(jdsg_mlp_expand, dim3((unsigned)((nrow + kExpandRows - 1) / kExpandRows)),
-                     dim3(256), 0, st, e0,
+                     dim3(256), (size_t)36 * H, st, e0,
                      nrow, ncol, ent_off, per_off, periods, nmode, acol, lda, ext, H, Y);
 }
 
