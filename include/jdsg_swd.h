@@ -110,6 +110,13 @@ void jdsg_plan_destroy(jdsg_plan *plan);
  * the reference's (surfdisp96.f:634) and returns float-rounded roots like it;
  * tol < 1e-6 is the exact-root mode (roots kept in double). */
 int jdsg_plan_set_tolerance(jdsg_plan *plan, double tol);
+/* How the surrogate-bracketed solve runs the chains of the roots its
+ * brackets left open (all forms give the same roots):
+ * 0  one lane per chain, each chain to its end;
+ * 1  one lane per chain, long upward scans suspended into parallel scan jobs;
+ * 2  as 1, but a chain resumed after a suspended scan runs one chain per
+ *    wave (the wave forms each period equation together) -- the default. */
+int jdsg_plan_set_fallback(jdsg_plan *plan, int form);
 /* Number of period-equation evaluations the last execute performed
  * (synchronises; for the flop model). */
 int jdsg_plan_eval_count(const jdsg_plan *plan, long long *nevals);

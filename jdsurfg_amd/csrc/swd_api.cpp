@@ -68,6 +68,7 @@ struct jdsg_plan {
   unsigned long long *nevals = nullptr;
   int chain_waves = 0, eigen_waves = 0;
   double tol = 1.0e-6;
+  int fallback = 2;  // chains of the open roots: see jdsg_plan_set_fallback
   // surrogate path buffers (allocated by jdsg_plan_enable_surrogate)
   double *known = nullptr, *cen = nullptr;
   signed char *sg = nullptr;
@@ -355,6 +356,12 @@ extern "C" void jdsg_plan_destroy(jdsg_plan *P) { delete P; }
 extern "C" int jdsg_plan_set_tolerance(jdsg_plan *P, double tol) {
   if (!(tol > 0.0) || tol > 1.0e-2) return fail("tolerance must be in (0, 1e-2]");
   P->tol = tol;
+  return 0;
+}
+
+extern "C" int jdsg_plan_set_fallback(jdsg_plan *P, int form) {
+  if (form < 0 || form > 2) return fail("fallback form must be 0, 1 or 2");
+  P->fallback = form;
   return 0;
 }
 
@@ -1119,7 +1126,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     A.susp.total = P->susp_njobs + 4;
     jdsg_launch_fill_u32(A.susp.total, 2, 0u, st);
     A.susp.min_steps = 2;
-    jdsg_launch_chains_susp(A, P->chain_waves, 64, st);
+    jdsg_launch_chains_susp(A, P->chain_waves, P->fallback == 0 ? 0 : 64, P->fallback == 2, st);
     rc = hipGetLastError() != hipSuccess;
   }
   P->known_in = nullptr;

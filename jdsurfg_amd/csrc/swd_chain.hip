@@ -784,13 +784,23 @@ jdsg_root_chains(ChainArgs A) {
 // is saved and leaves a scan job (PH_SUSPEND); a later launch resumes the
 // chains of a job list from their saved state once jdsg_scan_jobs has run
 // the scans.  Each chain then keeps its scratch in a slot of its own.
-template <int W>
+// COOP: one chain per wave instead of one per lane.  Every lane then holds
+// the same chain state and takes the same branches, the period equation is
+// formed by the whole wave (the layers' coefficients in parallel, the sweep
+// in every lane: the one-lane value in a fraction of its latency, for about
+// ten times the instructions), and lane 0 alone takes work and job slots
+// and posts the saved state.  It serves the resumed chains, a tail of long
+// walks whose latency sets the time of each cycle.
+template <int W, bool COOP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
 jdsg_root_chains_susp(ChainArgs A) {
-  const long long nlanes = (long long)gridDim.x * blockDim.x;
-  const long long lane_id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double cf[COOP ? 64 * (W == 0 ? jdsg_pf::kCoef4 : jdsg_pf::kCoef1) : 1];
+  const long long nlanes = COOP ? (long long)gridDim.x : (long long)gridDim.x * blockDim.x;
+  const long long lane_id =
+      COOP ? (long long)blockIdx.x : (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool lead = !COOP || threadIdx.x == 0;
   const long long cstride = 2 * A.kmax_pad + 22;
-  double *cs = A.cscr + lane_id * cstride;
+  double *cs = nullptr;  // the chain's own slot, set by start_item
   unsigned *counter = A.counter + W;
   const long long nch = (long long)A.cm.ncol * A.nvar;
   // work items: every chain, or the chains of a job list to resume
@@ -835,11 +845,18 @@ jdsg_root_chains_susp(ChainArgs A) {
     if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
     return false;
   };
+  // the next work item of this lane (COOP: of this wave)
+  auto next_item = [&]() -> long long {
+    unsigned v = 0;
+    if (lead) v = atomicAdd(counter, 1u);
+    if (COOP) v = __shfl(v, 0);
+    return nlanes + (long long)v;
+  };
   long long item = lane_id;
   bool live = false;
   while (item < nitems) {
     if (start_item(item)) { live = true; break; }
-    item = nlanes + (long long)atomicAdd(counter, 1u);
+    item = next_item();
   }
   double F = 0.0;
   bool need = live ? advance<true, true>(S, A, cs, F) : false;
@@ -850,8 +867,12 @@ jdsg_root_chains_susp(ChainArgs A) {
       int mmax = A.cm.nlay[S.col];
       int llw = A.cm.llw[S.col];
       double wvno = S.omega / S.ec;
-      F = (W == 0) ? jdsg_pf::dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega)
-                   : jdsg_pf::dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
+      if (COOP)
+        F = (W == 0) ? jdsg_pf::dltar4_wave(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega, cf)
+                     : jdsg_pf::dltar1_wave(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega, cf);
+      else
+        F = (W == 0) ? jdsg_pf::dltar4(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega)
+                     : jdsg_pf::dltar1(st, A.cm.ncol_pad, mmax, llw, wvno, S.omega);
       ++nev;
       ++S.evals;
       need = advance<true, true>(S, A, cs, F);
@@ -859,8 +880,12 @@ jdsg_root_chains_susp(ChainArgs A) {
     }
     if (S.phase == PH_SUSPEND) {
       // at most one job per chain and list, so the slot stays below nch
-      const unsigned slot = atomicAdd(&A.susp.njobs[W * 2 + A.susp.list_out], 1u);
-      atomicAdd(&A.susp.total[W], 1u);
+      unsigned slot = 0;
+      if (lead) {
+        slot = atomicAdd(&A.susp.njobs[W * 2 + A.susp.list_out], 1u);
+        atomicAdd(&A.susp.total[W], 1u);
+      }
+      if (COOP) slot = __shfl(slot, 0);
       ScanJob J;
       J.g = g;
       J.c0 = S.c1;
@@ -872,21 +897,23 @@ jdsg_root_chains_susp(ChainArgs A) {
       J.found = 0;
       J.evals = 0;
       J.c_prev = J.f_prev = J.c_hit = J.f_hit = 0.0;
-      A.susp.jobs[W][A.susp.list_out][slot] = J;
-      saved[g] = S;
+      if (lead) {
+        A.susp.jobs[W][A.susp.list_out][slot] = J;
+        saved[g] = S;
+      }
     } else {
       finish_chain(S, A, g, W);
       if (A.chain_evals) A.chain_evals[(long long)W * nch + g] = S.evals;
     }
     live = false;
     for (;;) {
-      item = nlanes + (long long)atomicAdd(counter, 1u);
+      item = next_item();
       if (item >= nitems) break;
       if (start_item(item)) { live = true; break; }
     }
     if (live) need = advance<true, true>(S, A, cs, 0.0);
   }
-  if (A.nevals && nev) atomicAdd(A.nevals, nev);
+  if (lead && A.nevals && nev) atomicAdd(A.nevals, nev);
 }
 
 // Scan jobs of suspended chains, one wave per job: each round the 64 lanes
@@ -963,8 +990,10 @@ template __global__ void jdsg_root_chains<0, false>(ChainArgs);
 template __global__ void jdsg_root_chains<1, false>(ChainArgs);
 template __global__ void jdsg_root_chains<0, true>(ChainArgs);
 template __global__ void jdsg_root_chains<1, true>(ChainArgs);
-template __global__ void jdsg_root_chains_susp<0>(ChainArgs);
-template __global__ void jdsg_root_chains_susp<1>(ChainArgs);
+template __global__ void jdsg_root_chains_susp<0, false>(ChainArgs);
+template __global__ void jdsg_root_chains_susp<1, false>(ChainArgs);
+template __global__ void jdsg_root_chains_susp<0, true>(ChainArgs);
+template __global__ void jdsg_root_chains_susp<1, true>(ChainArgs);
 
 size_t jdsg_chain_state_bytes() { return sizeof(ChainStateSusp); }
 
@@ -996,13 +1025,32 @@ void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st) {
   }
 }
 
-void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st) {
+void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, bool coop, hipStream_t st) {
   A.susp_on = 1;
   const long long nch = (long long)A.cm.ncol * A.nvar;
   const unsigned scan_blocks = (unsigned)std::min<long long>(nch, 2048);
   // after a few cycles only the long chains are left: a smaller grid keeps
   // the near-empty resume launches cheap (lanes still take every job)
   const int tail_waves = std::max(64, nwaves / 8);
+  // coop: the first launch (every chain, most with little or no work) runs
+  // a chain per lane, for throughput; the resumed chains -- a tail of long
+  // walks -- a chain per wave, three waves on each SIMD for the first
+  // cycles and one after them (a launch without jobs then costs what the
+  // lane form's small grid does)
+  auto chains = [&](int w, int gw, bool first, bool early) {
+    if (coop && !first) {
+      const unsigned cw = (unsigned)std::min<long long>(nch, early ? 3072 : 1024);
+      if (w == 0)
+        hipLaunchKernelGGL((jdsg_root_chains_susp<0, true>), dim3(cw), dim3(64), 0, st, A);
+      else
+        hipLaunchKernelGGL((jdsg_root_chains_susp<1, true>), dim3(cw), dim3(64), 0, st, A);
+    } else {
+      if (w == 0)
+        hipLaunchKernelGGL((jdsg_root_chains_susp<0, false>), dim3(gw), dim3(64), 0, st, A);
+      else
+        hipLaunchKernelGGL((jdsg_root_chains_susp<1, false>), dim3(gw), dim3(64), 0, st, A);
+    }
+  };
   for (int w = 0; w < 2; ++w) {
     if (!A.any_wave[w]) continue;
     A.susp.list_in = -1;
@@ -1010,10 +1058,7 @@ void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st
     A.susp.allow = cycles > 0;
     jdsg_launch_fill_u32(A.counter + w, 1, 0u, st);
     jdsg_launch_fill_u32(A.susp.njobs + 2 * w, 2, 0u, st);
-    if (w == 0)
-      hipLaunchKernelGGL((jdsg_root_chains_susp<0>), dim3(nwaves), dim3(64), 0, st, A);
-    else
-      hipLaunchKernelGGL((jdsg_root_chains_susp<1>), dim3(nwaves), dim3(64), 0, st, A);
+    chains(w, nwaves, true, true);
     for (int c = 0; c < cycles; ++c) {
       const int lin = c & 1, lout = lin ^ 1;
       if (w == 0)
@@ -1023,11 +1068,7 @@ void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st
       A.susp.list_in = lin;
       A.susp.list_out = lout;
       A.susp.allow = c + 1 < cycles;  // the last pass finishes every chain
-      const int gw = c < 4 ? nwaves : std::min(nwaves, tail_waves);
-      if (w == 0)
-        hipLaunchKernelGGL((jdsg_root_chains_susp<0>), dim3(gw), dim3(64), 0, st, A);
-      else
-        hipLaunchKernelGGL((jdsg_root_chains_susp<1>), dim3(gw), dim3(64), 0, st, A);
+      chains(w, c < 4 ? nwaves : std::min(nwaves, tail_waves), false, c < 4);
     }
   }
 }

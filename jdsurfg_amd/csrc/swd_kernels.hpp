@@ -148,7 +148,7 @@ void jdsg_launch_prep(int ncol, int ncol_pad, const int *layer_off, const float 
 void jdsg_launch_chains(const ChainArgs &A, int nwaves, hipStream_t st);
 // exact-root fallback with long scans suspended into parallel scan jobs for
 // up to `cycles` rounds (A.susp buffers set; A.susp_on is set here)
-void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, hipStream_t st);
+void jdsg_launch_chains_susp(ChainArgs A, int nwaves, int cycles, bool coop, hipStream_t st);
 size_t jdsg_chain_state_bytes();
 // ev (optional): 2 * jdsg_eigen_pass_launches(A, nwaves) events recorded
 // before and after each pass launch (the pass kernels' own time)

@@ -168,6 +168,13 @@ class Plan:
         if lib().jdsg_plan_set_tolerance(self.h, float(tol)):
             raise RuntimeError(last_error())
 
+    def set_fallback(self, form):
+        """Chains of the roots the surrogate's brackets left open: 0 one lane
+        per chain, 1 the same with suspended scans, 2 as 1 with the resumed
+        chains one per wave (the default)."""
+        if lib().jdsg_plan_set_fallback(self.h, int(form)):
+            raise RuntimeError(last_error())
+
     def eval_count(self):
         v = C.c_longlong()
         if lib().jdsg_plan_eval_count(self.h, C.byref(v)):

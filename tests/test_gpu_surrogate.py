@@ -523,3 +523,46 @@ def test_love_mode2_network_gain():
     assert l2.sum() > 1000
     assert share["l012"][0] >= 0.9 and share["l01"][0] <= 0.6
     assert share["l012"][1] >= 0.9
+
+
+@pytest.mark.parametrize("name", ["grid128", "uneven"])
+def test_fallback_forms_agree(name):
+    """The three forms of the fallback chains -- one lane per chain to its
+    end, one lane with the long scans suspended into parallel scan jobs, and
+    the resumed chains one per wave forming each period equation together --
+    take the same
+    steps: roots, statuses and U bit for bit, and the two suspended forms the
+    same evaluations and scan jobs.  grid128: 2,048 columns of the sediment
+    grid (its fallback walks modes past cut-off); uneven: 1,500 columns of
+    16-128 layers (more than 64 steps a sweep, so the wave form takes its
+    layers in two or more blocks)."""
+    import torch
+    from jdsurfg_amd import swd, surrogate as S
+    if name == "grid128":
+        w = W.grid128().subset(np.arange(0, 16384, 8))
+    else:
+        w = W.uneven(seed=7, ncol=1500)
+    mlp = S.MLP.load(S.weights_for(w.nmode))
+    plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
+    plan.set_tolerance(EXACT_TOL)
+    inp = swd.upload(w)
+    res = []
+    for form in (0, 1, 2):
+        print(f"case fallback forms {name} ncol={w.ncol} form={form}", flush=True)
+        plan.set_fallback(form)
+        out = plan.alloc_outputs()
+        S.surrogate_solve(plan, inp, out, mlp=mlp)
+        torch.cuda.synchronize()
+        st = S.surrogate_stats(plan)
+        print(form, st, flush=True)
+        res.append((plan.flat_roots().copy(), out["status"].cpu().numpy(),
+                    out["u"].cpu().numpy(), st))
+    assert res[0][3]["roots_left_to_chain"] > 0
+    assert res[1][3]["scan_jobs"] > 0
+    for f in (1, 2):
+        assert np.array_equal(res[0][0], res[f][0]), \
+            f"form {f}: roots differ at {np.nonzero(res[0][0] != res[f][0])[:2]}"
+        assert np.array_equal(res[0][1], res[f][1])
+        assert np.array_equal(res[0][2], res[f][2])
+    assert res[1][3] == res[2][3]
+    plan.set_fallback(2)
