@@ -525,7 +525,7 @@ def test_love_mode2_network_gain():
     assert share["l012"][1] >= 0.9
 
 
-@pytest.mark.parametrize("name", ["grid128", "uneven"])
+@pytest.mark.parametrize("name", ["grid128", "uneven", "water"])
 def test_fallback_forms_agree(name):
     """The three forms of the fallback chains -- one lane per chain to its
     end, one lane with the long scans suspended into parallel scan jobs, and
@@ -535,23 +535,33 @@ def test_fallback_forms_agree(name):
     same evaluations and scan jobs.  grid128: 2,048 columns of the sediment
     grid (its fallback walks modes past cut-off); uneven: 1,500 columns of
     16-128 layers (more than 64 steps a sweep, so the wave form takes its
-    layers in two or more blocks)."""
+    layers in two or more blocks); water: 64 columns under a water layer
+    (the top-layer step of the Rayleigh equation), with centres 3 % off the
+    roots so that most roots go to the chains."""
     import torch
     from jdsurfg_amd import swd, surrogate as S
     if name == "grid128":
         w = W.grid128().subset(np.arange(0, 16384, 8))
-    else:
+    elif name == "uneven":
         w = W.uneven(seed=7, ncol=1500)
-    mlp = S.MLP.load(S.weights_for(w.nmode))
+    else:
+        w = W.random_columns(64, 40, 5, nmode=(3, 1), group=True, water=True)
     plan = swd.Plan(w.layer_off, w.per_off, w.nmode, w.group, "csr")
     plan.set_tolerance(EXACT_TOL)
     inp = swd.upload(w)
+    if name == "water":
+        roots = S.plain_roots(plan, inp)[:plan.nent]
+        rng = np.random.default_rng(6)
+        noise = torch.from_numpy(rng.uniform(-0.03, 0.03, plan.nent)).cuda()
+        kw = {"pred": (roots * (1 + noise)).contiguous()}
+    else:
+        kw = {"mlp": S.MLP.load(S.weights_for(w.nmode))}
     res = []
     for form in (0, 1, 2):
         print(f"case fallback forms {name} ncol={w.ncol} form={form}", flush=True)
         plan.set_fallback(form)
         out = plan.alloc_outputs()
-        S.surrogate_solve(plan, inp, out, mlp=mlp)
+        S.surrogate_solve(plan, inp, out, **kw)
         torch.cuda.synchronize()
         st = S.surrogate_stats(plan)
         print(form, st, flush=True)
