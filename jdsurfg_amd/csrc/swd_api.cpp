@@ -1065,7 +1065,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
   // T(1 +- 0.005) chain misses a root near cutoff)
   auto cont = [&](int v) {
     for (int w = 0; w < 2; ++w)
-      if (P->nitem[w] > 0) jdsg_launch_continue(Q, w, v, JDSG_MAXMODE, st);
+      if (P->nitem[w] > 0) jdsg_launch_continue(Q, w, v, JDSG_MAXMODE, P->fallback == 2, st);
   };
   polish(0, 1);
   order(0);

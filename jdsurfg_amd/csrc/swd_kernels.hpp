@@ -197,7 +197,8 @@ void jdsg_launch_frechet_grad(const RowCtx &R, const double *kernel, int nrows,
                               const float *ray_w, const float *resid, const float *vs,
                               float *grad, hipStream_t st);
 void jdsg_launch_mode_order(const PolishArgs &A, int kmax, hipStream_t st);
-void jdsg_launch_continue(const PolishArgs &A, int wave, int v, int mmax_modes, hipStream_t st);
+void jdsg_launch_continue(const PolishArgs &A, int wave, int v, int mmax_modes, bool coop,
+                          hipStream_t st);
 void jdsg_launch_count_open(const double *known, long long n, unsigned *out, hipStream_t st);
 void jdsg_launch_pm_centers(const long long *ent_off, const int *per_off, const double *periods,
                             int ncol, long long nent, const double *known0, const double *pred,

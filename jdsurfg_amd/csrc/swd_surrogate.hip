@@ -394,9 +394,14 @@ JDSG_DEV bool continue_begin(const PolishArgs &A, PLane &S, long long e, int col
   return true;
 }
 
-template <int W>
+// COOP: one (column, mode) walk per wave: every lane holds the same lane
+// state and the wave forms each period equation together (the one-lane
+// value in a fraction of its latency); the walks are few and long.
+template <int W, bool COOP>
 __global__ void __launch_bounds__(64) jdsg_continue_w(PolishArgs A, int v, int mmax_modes) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double cf[COOP ? 64 * (W == 0 ? jdsg_pf::kCoef4 : jdsg_pf::kCoef1) : 1];
+  const long long t =
+      COOP ? (long long)blockIdx.x : (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int ncol = A.cm.ncol;
   if (t >= (long long)ncol * mmax_modes) return;
   const int col = (int)(t / mmax_modes), m = (int)(t % mmax_modes);
@@ -410,6 +415,18 @@ __global__ void __launch_bounds__(64) jdsg_continue_w(PolishArgs A, int v, int m
   const int mmax = A.cm.nlay[col], llw = A.cm.llw[col], stride = A.cm.ncol_pad;
   unsigned long long nev = 0;
   unsigned nf = 0;
+  if (COOP) {
+    // the walk continues only open entries whose prediction lies, with its
+    // widened bracket, below betmx: the lanes look for one at once, and a
+    // walk without any ends here instead of visiting every period
+    bool cand = false;
+    for (int k = threadIdx.x; k < K; k += 64) {
+      const double p1 = pred[e0 + k];
+      cand = cand || (known[e0 + k] == 0.0 && p1 > 0.0 &&
+                      p1 * (1.0 + 3.0 * A.hw[0]) < (double)A.cm.betmx[col]);
+    }
+    if (!__any(cand)) return;
+  }
   for (int dir = 0; dir < 2; ++dir) {
     for (int i = 1; i < K; ++i) {
       const int k = dir == 0 ? i : K - 1 - i;
@@ -424,8 +441,13 @@ __global__ void __launch_bounds__(64) jdsg_continue_w(PolishArgs A, int v, int m
       if (!continue_begin<W>(A, S, e0 + k, col, m, k, v, cen)) continue;
       for (;;) {
         const double wv = S.omega / S.c;
-        const double F = (W == 0) ? jdsg_pf::dltar4(st, stride, mmax, llw, wv, S.omega)
-                                  : jdsg_pf::dltar1(st, stride, mmax, llw, wv, S.omega);
+        double F;
+        if (COOP)
+          F = (W == 0) ? jdsg_pf::dltar4_wave(st, stride, mmax, llw, wv, S.omega, cf)
+                       : jdsg_pf::dltar1_wave(st, stride, mmax, llw, wv, S.omega, cf);
+        else
+          F = (W == 0) ? jdsg_pf::dltar4(st, stride, mmax, llw, wv, S.omega)
+                       : jdsg_pf::dltar1(st, stride, mmax, llw, wv, S.omega);
         ++nev;
         if (polish_step(A, S, F, nf)) break;
       }
@@ -433,7 +455,7 @@ __global__ void __launch_bounds__(64) jdsg_continue_w(PolishArgs A, int v, int m
       if (*kp > 0.0 && fabs(*kp - p1) > 0.05 * p1) *kp = 0.0;
     }
   }
-  if (A.nevals && nev) atomicAdd(A.nevals, nev);
+  if ((!COOP || threadIdx.x == 0) && A.nevals && nev) atomicAdd(A.nevals, nev);
 }
 
 // Entries still open after the surrogate stages (known = 0): the roots the
@@ -678,14 +700,23 @@ void jdsg_launch_polish(const PolishArgs &A, int wave, int nwaves, hipStream_t s
   else
     hipLaunchKernelGGL(jdsg_polish_w<1>, dim3(g), dim3(64), 0, st, A);
 }
-void jdsg_launch_continue(const PolishArgs &A, int wave, int v, int mmax_modes, hipStream_t st) {
+void jdsg_launch_continue(const PolishArgs &A, int wave, int v, int mmax_modes, bool coop,
+                          hipStream_t st) {
   const long long n = (long long)A.cm.ncol * mmax_modes;
   if (n <= 0) return;
+  if (coop) {
+    const dim3 g((unsigned)n);
+    if (wave == 0)
+      hipLaunchKernelGGL((jdsg_continue_w<0, true>), g, dim3(64), 0, st, A, v, mmax_modes);
+    else
+      hipLaunchKernelGGL((jdsg_continue_w<1, true>), g, dim3(64), 0, st, A, v, mmax_modes);
+    return;
+  }
   const dim3 g((unsigned)((n + 63) / 64));
   if (wave == 0)
-    hipLaunchKernelGGL(jdsg_continue_w<0>, g, dim3(64), 0, st, A, v, mmax_modes);
+    hipLaunchKernelGGL((jdsg_continue_w<0, false>), g, dim3(64), 0, st, A, v, mmax_modes);
   else
-    hipLaunchKernelGGL(jdsg_continue_w<1>, g, dim3(64), 0, st, A, v, mmax_modes);
+    hipLaunchKernelGGL((jdsg_continue_w<1, false>), g, dim3(64), 0, st, A, v, mmax_modes);
 }
 void jdsg_launch_count_open(const double *known, long long n, unsigned *out, hipStream_t st) {
   if (n <= 0) return;
