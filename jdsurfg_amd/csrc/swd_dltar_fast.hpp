@@ -1,11 +1,16 @@
 // swd_dltar_fast.hpp -- the period equations of surfdisp96.f:712-898 for the
-// exact-root polish of the learned-bracket path.  Same mathematics as
+// exact-root polish of the learned-bracket path (polish, continuation,
+// exact-root chains and their scan jobs).  Same mathematics as
 // swd_dltar.hpp, arranged for throughput instead of bit-faithful rounding:
 // FMA contraction, reciprocals from the hardware estimate refined by two
 // Newton steps (shared where one denominator serves several quotients), and
 // the three exponentials of a layer reduced to two (exp(-2p) = exp(-p)^2,
 // exp(-p-q) = exp(-p) exp(-q)).  The values differ from the faithful form in
 // the last bits only; a root polished to 1e-10 relative does not move.
+// Each layer step is the layer's coefficients times the sweep vector; the
+// one-lane forms (dltar1, dltar4) and the wave forms (dltar1_wave,
+// dltar4_wave: a wave evaluates one equation, its lanes forming the layers'
+// coefficients in parallel) share those two halves and return the same bits.
 #pragma once
 #include "swd_common.hpp"
 
