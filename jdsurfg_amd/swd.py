@@ -45,6 +45,7 @@ def lib():
     L.jdsg_plan_timings.argtypes = [vp, vp]
     L.jdsg_plan_eval_count.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.jdsg_plan_set_tolerance.argtypes = [vp, C.c_double]
+    L.jdsg_plan_set_fallback.argtypes = [vp, C.c_int]
     L.jdsg_surfdisp.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_int,
                                 C.c_int, C.c_int]
     _lib = L
