@@ -52,7 +52,7 @@ def parse():
                     help="timed calls of the drop-in SurfTime leg (0 skips it)")
     ap.add_argument("--weights", default=None,
                     help="surrogate network (.npz); default: the 4 x 512 network for grid128 "
-                         "(the C4 configuration), the 4 x 256 one otherwise")
+                         "(its 128 x 128 sediment columns), the 4 x 256 one otherwise")
     ap.add_argument("--fmst-steps", type=int, default=2,
                     help="timed calls of the FMST leg (0 skips it)")
     return ap.parse_args()
@@ -318,8 +318,8 @@ def uneven_main(args):
 
 def surftime_leg(steps):
     """The drop-in SurfTime form (get_1d_kernel, SurfaceWave.cpp:458-502) on a
-    C5-equivalent grid: 256 x 256 columns of 17 nodes 0-64 km (4 km apart,
-    refined to 65 layer entries), the C5 Vs (trend with the +-8 % checkerboard
+    grid like grid256: 256 x 256 columns of 17 nodes 0-64 km (4 km apart,
+    refined to 65 layer entries), the grid256 Vs (trend with the +-8 % checkerboard
     at the node depths), Rg tables of 4 modes and Lg tables of 2 modes at the
     60 periods 1-80 s; Vp and rho by Brocher inside, the a/b/rho kernels,
     the chain rule and the grid epilogue; Vs and outputs resident in HBM."""

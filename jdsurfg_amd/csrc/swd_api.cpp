@@ -993,7 +993,7 @@ extern "C" int jdsg_plan_enable_surrogate(jdsg_plan *P) {
       P->dalloc(&P->brk, 2 * (size_t)P->nvar * P->nent) || P->dalloc(&P->sfail, 2))
     return 1;
   // per chain of each wave type: its saved state, its own scratch, a job slot
-  // in each of two lists (C5: about 0.7 GB)
+  // in each of two lists (256x256 columns x 60 periods: about 0.7 GB)
   const size_t nch = (size_t)P->ncol * P->nvar;
   if (P->dalloc(&P->susp_saved, 2 * nch * jdsg_chain_state_bytes()) ||
       P->dalloc(&P->susp_cs, 2 * nch * (2 * (size_t)P->kpad + 22)) ||
@@ -1768,7 +1768,7 @@ int surftime_run(jdsg_surftime *S, int mapform, const float *vs, double *vc, dou
   if (update_mask(S, vs, st)) return 1;
   if (S->budget <= 0.0) {
     // intermediate layer kernels of all chunks together: most of the free
-    // device memory (the whole C5 grid fits one chunk on an MI355X), or
+    // device memory (a whole 256x256-column grid fits one chunk on an MI355X), or
     // JDSG_SURFTIME_BUDGET_GB
     size_t fr = 0, tot = 0;
     S->budget = (hipMemGetInfo(&fr, &tot) == hipSuccess) ? 0.6 * (double)fr : 12e9;

@@ -795,9 +795,10 @@ struct Dst {
 // store of a wave is one 512-byte line); the down-sweep reads interface i + 1
 // while it works on interface i.  Measured against keeping every sixth
 // vector and recomputing the segments into LDS: 120 ms against 136 ms for
-// the C5 Rayleigh stage (the recomputation cost a second Dunkin sweep; the
-// stored vectors add about 6 GB of HBM traffic per launch, well inside the
-// bandwidth left beside the sweeps' arithmetic).
+// the Rayleigh stage of 256x256 columns x 60 periods (the recomputation
+// cost a second Dunkin sweep; the stored vectors add about 6 GB of HBM
+// traffic per launch, well inside the bandwidth left beside the sweeps'
+// arithmetic).
 
 // One step of the up-sweep (sregn96.f90:455-487) through layer L: cd, the
 // vector at the top of the layer below, becomes the vector at the top of L.

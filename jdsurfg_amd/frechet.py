@@ -172,7 +172,7 @@ def read_frechet(path):
 
 
 def bench(nx=256, ny=256, nz=17, kmax=120, nrows=20000, steps=5, seed=0):
-    """Rows and gradient for synthetic straight rays over a C5-sized grid
+    """Rows and gradient for synthetic straight rays over a grid256-sized grid
     (256 x 256 columns, 17 depth nodes = 65 layer entries, kmax period
     indices) with random kernels resident in HBM; returns a report dict."""
     import torch
@@ -215,7 +215,7 @@ def bench(nx=256, ny=256, nz=17, kmax=120, nrows=20000, steps=5, seed=0):
 
 def main(argv=None):
     """python -m jdsurfg_amd.frechet [--rows N]: Fréchet rows and gradient of
-    synthetic rays over a C5-sized grid; prints one JSON report."""
+    synthetic rays over a grid256-sized grid; prints one JSON report."""
     import argparse
     ap = argparse.ArgumentParser(prog="python -m jdsurfg_amd.frechet")
     ap.add_argument("--rows", type=int, default=20000)

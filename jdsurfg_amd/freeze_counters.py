@@ -1,4 +1,4 @@
-"""Freeze rocprofv3 counter runs of the C5 bench into the per-kernel traffic
+"""Freeze rocprofv3 counter runs of the grid256 bench into the per-kernel traffic
 and hardware-FP64 table that bench.py's roofline reads
 (jdsurfg_amd/traffic_model.json), and write the Markdown summary of the same
 runs.
@@ -83,7 +83,7 @@ def freeze_main(trace, sq, fetch, write, cfetch, cwrite, out_md):
     ff, fw = _copy_factor(cfetch, "FETCH_SIZE"), _copy_factor(cwrite, "WRITE_SIZE")
     src = (f"{os.path.relpath(out_md, os.path.dirname(HERE))} (rocprofv3 FETCH_SIZE x{ff:.2f}, "
            f"WRITE_SIZE x{fw:.2f} against a 768 MiB device copy; SQ_INSTS_VALU_*_F64 x64 lanes, "
-           "FMA x2; C5 bench)")
+           "FMA x2; grid256 bench)")
     model = {"source": src, "kernels": {}}
     rows = []
     for k, (calls, ms) in sorted(st.items(), key=lambda kv: -kv[1][1]):

@@ -1,13 +1,13 @@
 """The benchmark configurations on the GPU against the oracle, on strided
 column subsets sized so the CPU oracle finishes in seconds.
 
-* C5 (256x256 columns x 64 layers of 1 km with the +-8 % checkerboard, 60
+* grid256 (256x256 columns x 64 layers of 1 km with the +-8 % checkerboard, 60
   periods 1-80 s, Rayleigh modes 0-3 + Love 0-1, group kernels): the plain
   path at the reference tolerance, the plain path in exact-root mode, and the
   shipped network's surrogate solve, each against the oracle.
-* C4 (128x128 columns x 48 layers of 1.5 km with sediments, 40 periods
+* grid128 (128x128 columns x 48 layers of 1.5 km with sediments, 40 periods
   1-60 s, Rayleigh 0-2 + Love 0-1, group kernels): the same three, the
-  surrogate with the 4 x 512 network the configuration names, reporting its
+  surrogate with the 4 x 512 network configured for it, reporting its
   fallback rate.
 
 Bars: status identical; c and U within 1e-6 and kernels within 1e-4 (of
@@ -26,7 +26,7 @@ from tests.compare import (KERNELS, differing_root_report, gpu_solve, kernel_err
 
 pytestmark = pytest.mark.gpu
 # exact-root stop rule: at 1e-10 the oracle's own dU/dm still moves by
-# 2e-4 on the C4 sediment columns when the tolerance is tightened (the
+# 2e-4 on the grid128 sediment columns when the tolerance is tightened (the
 # difference quotient over T(1+-0.005) amplifies the root's last digits near
 # mode osculations); at 1e-12 it moves by at most 5e-5
 EXACT_TOL = 1e-12

@@ -381,7 +381,7 @@ def test_training_learns_roots():
 
 
 def test_trained_network_gain():
-    """What the shipped network buys on a C2/C4-like grid (2,048 columns of
+    """What the shipped network buys on a grid128-like grid (2,048 columns of
     the 128x128x48 sediment model, 40 periods, Rayleigh 0-2 + Love 0-1,
     group kernels): the surrogate solve evaluates the period equation far
     less often than the plain exact-root solve and returns the same roots."""
