@@ -227,10 +227,19 @@ int plan_host(jdsg_plan *P, int ncol, const int *layer_off, const int *per_off,
   P->eigen_waves = std::max(1, P->eigen_waves);
   P->nlay_total = layer_off[ncol];
   if (!uni) {
+    // columns deepest first (stable): a wave's lanes then work on columns of
+    // about the same depth, whose layer loops end together, and the longest
+    // work starts first; every kernel that walks the list treats each entry
+    // on its own, so the order changes no result
+    std::vector<int> corder(ncol);
+    for (int c = 0; c < ncol; ++c) corder[c] = c;
+    std::stable_sort(corder.begin(), corder.end(), [&](int a, int b) {
+      return layer_off[a + 1] - layer_off[a] > layer_off[b + 1] - layer_off[b];
+    });
     for (int w = 0; w < 2; ++w) {
       items[w].clear();
       items[w].reserve(P->nitem[w]);
-      for (int c = 0; c < ncol; ++c) {
+      for (int c : corder) {
         int K = per_off[c + 1] - per_off[c];
         int nR = nmode[2 * c], nL = nmode[2 * c + 1];
         int M = w == 0 ? nR : nL, off = w == 0 ? 0 : nR;
