@@ -60,6 +60,7 @@ struct jdsg_plan {
   int *layer_off_d = nullptr, *per_off_d = nullptr, *nmode_d = nullptr;
   long long nlay_total = 0;  // layer entries of all columns
   long long *ent_off_d = nullptr, *kern_off_d = nullptr, *items_d[2] = {nullptr, nullptr};
+  int *corder_d = nullptr;  // uneven batches: columns deepest first, the order chains are taken
   PrepOut prep{};
   ColumnModel cm{};
   double *cflat = nullptr, *cscr = nullptr, *escr = nullptr;
@@ -330,6 +331,15 @@ extern "C" int jdsg_plan_create(int ncol, const int *layer_off, const int *per_o
       Q->size_scratch())
     return 1;
   if (plan_upload(Q, layer_off, per_off, nmode, items, nullptr)) return 1;
+  if (!Q->uniform) {
+    std::vector<int> corder(ncol);
+    for (int c = 0; c < ncol; ++c) corder[c] = c;
+    std::stable_sort(corder.begin(), corder.end(), [&](int a, int b) {
+      return layer_off[a + 1] - layer_off[a] > layer_off[b + 1] - layer_off[b];
+    });
+    if (Q->dalloc(&Q->corder_d, (size_t)ncol)) return 1;
+    HIPCHK(hipMemcpy(Q->corder_d, corder.data(), sizeof(int) * ncol, hipMemcpyHostToDevice));
+  }
   const char *tenv = getenv("JDSG_TIMING");
   if (tenv && tenv[0] == '1') {
     P->timing = true;
@@ -447,6 +457,7 @@ int run_front(jdsg_plan *P, const float *thk, const float *vp, const float *vs,
   jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, st);
   if (P->timing && !P->capturing) HIPCHK(hipEventRecord(P->ev[1], st));
   ChainArgs A{};
+  A.corder = P->corder_d;
   A.cm = P->cm;
   A.ent_off = P->ent_off_d;
   A.per_off = P->per_off_d;
@@ -1099,6 +1110,7 @@ extern "C" int jdsg_plan_execute_surrogate(jdsg_plan *P, const float *thk, const
     jdsg_launch_fill_u32((unsigned *)P->nevals, 2, 0u, st);
     jdsg_launch_fill_u32((unsigned *)P->cstat, (long long)P->nvar * P->nent, 0u, st);
     ChainArgs A{};
+    A.corder = P->corder_d;
     A.cm = P->cm;
     A.ent_off = P->ent_off_d;
     A.per_off = P->per_off_d;

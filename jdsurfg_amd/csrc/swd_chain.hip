@@ -668,6 +668,17 @@ __device__ __attribute__((noinline)) bool advance(
   }
 }
 
+// The chain taken i-th: period variant i / ncol, column corder[i % ncol] (the
+// columns deepest first, so a wave's lanes hold chains of about one depth,
+// whose layer loops end together), or chain i without an order.  Chain ids
+// keep their meaning (column g % ncol, variant g / ncol) for every array
+// indexed by chain.
+JDSG_DEV long long chain_at(const ChainArgs &A, long long i) {
+  if (!A.corder) return i;
+  const long long nc = A.cm.ncol;
+  return (i / nc) * nc + A.corder[i % nc];
+}
+
 // Decode chain id -> job; returns false when the chain has no work.
 JDSG_DEV bool setup_chain(ChainState &S, const ChainArgs &A, long long g, double *cs, int w) {
   int ncol = A.cm.ncol;
@@ -738,12 +749,13 @@ jdsg_root_chains(ChainArgs A) {
   unsigned long long nev = 0;
   unsigned cev = 0;  // evaluations of the current chain
   ChainState S;
-  long long g = lane_id;
+  long long i = lane_id, g = 0;
   bool live = false;
-  while (g < nch) {
+  while (i < nch) {
+    g = chain_at(A, i);
     if (setup_chain(S, A, g, cs, W)) { live = true; break; }
     if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
-    g = nlanes + (long long)atomicAdd(counter, 1u);
+    i = nlanes + (long long)atomicAdd(counter, 1u);
   }
   double F = 0.0;
   bool need = live ? advance<false, FAST>(S, A, cs, F) : false;
@@ -770,8 +782,9 @@ jdsg_root_chains(ChainArgs A) {
     cev = 0;
     live = false;
     for (;;) {
-      g = nlanes + (long long)atomicAdd(counter, 1u);
-      if (g >= nch) break;
+      i = nlanes + (long long)atomicAdd(counter, 1u);
+      if (i >= nch) break;
+      g = chain_at(A, i);
       if (setup_chain(S, A, g, cs, W)) { live = true; break; }
       if (A.ierr_out) A.ierr_out[(long long)W * nch + g] = 0;
     }
@@ -835,7 +848,7 @@ jdsg_root_chains_susp(ChainArgs A) {
       S.evals += J.evals;
       return true;
     }
-    g = i;
+    g = chain_at(A, i);
     cs = A.susp.cs + ((long long)W * nch + g) * cstride;
     if (setup_chain(S, A, g, cs, W)) {
       S.nscan = 0;

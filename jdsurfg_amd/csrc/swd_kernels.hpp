@@ -57,6 +57,7 @@ struct ChainArgs {
   int any_wave[2];            // some column asks for Rayleigh / Love modes
   long long nchain;           // ncol * 2 * nvar
   unsigned *chain_evals;  // optional [2*ncol*nvar] evaluations per chain
+  const int *corder;      // optional [ncol]: columns in the order chains are taken (deepest first)
   int susp_on;            // exact-root fallback: long scans run as parallel scan jobs
   ChainSusp susp;
 };
